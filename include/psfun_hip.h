@@ -1,0 +1,158 @@
+/*
+ * psfun_hip.h -- C-ABI of the MI355X (gfx950) f(A)x library.
+ *
+ * This is the drop-in boundary under PSFUN's Fortran API.  Every entry point
+ * takes plain pointers and sizes, returns an int status (PSFUN_SUCCESS == 0)
+ * and never stops or aborts the process.  Vector arguments are DEVICE
+ * pointers (FP64, or interleaved (re,im) FP64 pairs for complex data) unless
+ * the name says "host".  Work is ordered on the context's HIP stream; calls
+ * that return a scalar to the host synchronise that stream.
+ *
+ * Reference interfaces replaced (paths relative to the PSFUN source tree):
+ *   psb_spmm / psb_gedot / psb_genrm2 / psb_geamax / psb_geaxpby / v%scal
+ *       -- PSBLAS primitives called from krylov/polynomial/psfun_d_lanczos.f90:183-377,
+ *          krylov/polynomial/psfun_d_arnoldi.f90:183-329, krylov/shifted/psfun_dcg.F90:259-326
+ *   psfun_d_lanczos  -- krylov/polynomial/psfun_d_lanczos.f90:37-400
+ *   psfun_d_arnoldi  -- krylov/polynomial/psfun_d_arnoldi.f90:37-352
+ *   psfun_dcg_vect   -- krylov/shifted/psfun_dcg.F90:129-373
+ *   psfun_d_serial%apply (f(H) step, EXP family) -- serial/psfun_d_serial_mod.F90:172-523
+ *   hhtmethod1       -- quadratures/psfun_z_quadrules_mod.f90:40-85
+ *   ellipkkp, ellipj -- utils/psfun_d_utils_mod.f90:37-99, utils/psfun_z_utils_mod.f90:39-107
+ */
+#ifndef PSFUN_HIP_H
+#define PSFUN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* The Fortran layer maps these onto the PSBLAS error names used by the
+ * reference (psb_success_, psb_err_invalid_istop_, ...). */
+enum {
+  PSFUN_SUCCESS = 0,
+  PSFUN_ERR_INVALID_ARG = 1,          /* bad size / null pointer / bad option value */
+  PSFUN_ERR_INVALID_ISTOP = 2,        /* psb_err_invalid_istop_ (psfun_d_lanczos.f90:108-113) */
+  PSFUN_ERR_INVALID_VECT_STATE = 3,   /* psb_err_invalid_vect_state_ (psfun_d_lanczos.f90:133-142) */
+  PSFUN_ERR_INVALID_ARGS_COMBINATION = 4, /* psb_err_invalid_args_combination_ */
+  PSFUN_ERR_NOT_SQUARE = 5,           /* psfun_d_serial_mod.F90:206-210 */
+  PSFUN_ERR_UNSUPPORTED = 6,          /* function/variant without a device implementation */
+  PSFUN_ERR_ALLOC = 7,                /* device or pinned allocation failed */
+  PSFUN_ERR_HIP = 8,                  /* a HIP runtime call failed */
+  PSFUN_ERR_NO_DEVICE = 9,
+  PSFUN_ERR_BREAKDOWN = 10            /* solver breakdown reported as an error (CG/COCG only) */
+};
+
+/* f(H) step selection: function names follow psfun_d_serial%fname ('EXP'),
+ * variants follow psfun_d_serial%variant (serial/psfun_d_serial_mod.F90:214-390). */
+enum { PSFUN_FN_EXP = 0 };
+enum {
+  PSFUN_VAR_CHBSYM = 0,   /* symmetric H (Lanczos) */
+  PSFUN_VAR_CHBHES = 1,   /* Hessenberg H (Arnoldi) */
+  PSFUN_VAR_CHBGEN = 2,
+  PSFUN_VAR_GENPADE = 3,
+  PSFUN_VAR_TAYLOR = 4,
+  PSFUN_VAR_SASQ = 5
+};
+
+typedef struct psfun_ctx *psfun_ctx_t;
+typedef struct psfun_csr *psfun_csr_t;
+
+/* Memory space of pointers handed to psfun_csr_create. */
+enum { PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1 };
+
+/* ---- context / errors --------------------------------------------------- */
+const char *psfun_strerror(int status);
+/* Create a context on HIP device `device`.  The context owns a default-priority
+ * stream, a pinned host staging area and a grow-only device workspace. */
+int psfun_ctx_create(int device, psfun_ctx_t *ctx);
+int psfun_ctx_destroy(psfun_ctx_t ctx);
+/* Use an external hipStream_t (e.g. the caller's current stream); NULL restores
+ * the context's own stream. */
+int psfun_ctx_set_stream(psfun_ctx_t ctx, void *hip_stream);
+int psfun_ctx_synchronize(psfun_ctx_t ctx);
+/* Number of compute units of the context's device (for launch-shape tuning). */
+int psfun_ctx_num_cus(psfun_ctx_t ctx, int *num_cus);
+
+/* ---- per-kernel timing (HIP events around each launch of a hot kernel) --- */
+/* Kernel ids for psfun_prof_read. */
+enum {
+  PSFUN_PROF_LZ_SPMV = 0,    /* Lanczos SpMV + alpha dot (fused)          */
+  PSFUN_PROF_LZ_UPDATE = 1,  /* Lanczos 3-term update + beta nrm2 + f(T)  */
+  PSFUN_PROF_LZ_ASSEMBLE = 2,/* y = ||x|| V yk                           */
+  PSFUN_PROF_SPMV = 3,       /* plain CSR SpMV (psfun_spmv)              */
+  PSFUN_PROF_MS_SPMV = 4,    /* multi-shift CG seed SpMV + dots          */
+  PSFUN_PROF_MS_UPDATE = 5,  /* multi-shift CG per-shift updates         */
+  PSFUN_PROF_CG = 6,         /* shifted CG fused kernels                 */
+  PSFUN_PROF_BATCH = 7,      /* batched Lanczos step kernels             */
+  PSFUN_PROF_SDDMM = 8       /* gradient SDDMM                           */
+};
+/* Turn timing on/off (off by default); reset clears the totals. */
+int psfun_prof_enable(psfun_ctx_t ctx, int on);
+int psfun_prof_reset(psfun_ctx_t ctx);
+/* Totals since the last reset: milliseconds, launches, algorithmic bytes. */
+int psfun_prof_read(psfun_ctx_t ctx, int kernel_id, double *total_ms, int64_t *launches, double *bytes);
+
+/* ---- CSR matrix handle (psb_dspmat_type, CSR storage) ------------------- */
+/* Host arrays (PSFUN_MEM_HOST) are copied to the device and converted to
+ * 0-based indices; device arrays (PSFUN_MEM_DEVICE) must be 0-based and are
+ * borrowed (the caller keeps them alive).  index_base is 0 or 1 (PSBLAS CSR is
+ * 1-based).  Square matrices only. */
+int psfun_csr_create(psfun_ctx_t ctx, int64_t n, int64_t nnz, const int32_t *rowptr,
+                     const int32_t *colind, const double *val, int index_base, int mem,
+                     psfun_csr_t *A);
+int psfun_csr_destroy(psfun_csr_t A);
+/* Replace the nonzero values (same pattern); `val` is a device pointer. */
+int psfun_csr_set_values(psfun_csr_t A, const double *val);
+/* Force the SpMV lanes-per-row (0 = automatic; 1..64 power of two). */
+int psfun_csr_set_lanes(psfun_csr_t A, int lanes_per_row);
+int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
+
+/* ---- device vectors ------------------------------------------------------ */
+int psfun_vec_alloc(psfun_ctx_t ctx, int64_t n, double **dptr);
+int psfun_vec_free(psfun_ctx_t ctx, double *dptr);
+int psfun_vec_upload(psfun_ctx_t ctx, double *dptr, const double *host, int64_t n);
+int psfun_vec_download(psfun_ctx_t ctx, double *host, const double *dptr, int64_t n);
+
+/* ---- PSBLAS primitives --------------------------------------------------- */
+/* y <- alpha*A*x + beta*y  (psb_spmm, psfun_d_lanczos.f90:198) */
+int psfun_spmv(psfun_ctx_t ctx, psfun_csr_t A, double alpha, const double *x, double beta,
+               double *y);
+/* Y <- alpha*A*X + beta*Y for k column-major right-hand sides with leading
+ * dimension ld (psb_spmm on a multivector). */
+int psfun_spmm(psfun_ctx_t ctx, psfun_csr_t A, int k, double alpha, const double *X, int64_t ldx,
+               double beta, double *Y, int64_t ldy);
+/* Scalar results are returned on the host (psb_gedot / psb_genrm2 / psb_geamax). */
+int psfun_dot(psfun_ctx_t ctx, int64_t n, const double *x, const double *y, double *result);
+int psfun_nrm2(psfun_ctx_t ctx, int64_t n, const double *x, double *result);
+int psfun_amax(psfun_ctx_t ctx, int64_t n, const double *x, double *result);
+/* y <- alpha*x + beta*y  (psb_geaxpby) */
+int psfun_axpby(psfun_ctx_t ctx, int64_t n, double alpha, const double *x, double beta, double *y);
+/* x <- alpha*x  (v%scal) */
+int psfun_scal(psfun_ctx_t ctx, int64_t n, double alpha, double *x);
+
+/* ---- f(H) step on a small dense matrix (psfun_d_serial%apply) ------------ */
+/* y = f(scaling*H) x for a k-by-k column-major host matrix H (k <= 256),
+ * evaluated on the device.  fname = PSFUN_FN_EXP.  Symmetric variants
+ * (CHBSYM) require a symmetric tridiagonal H; the others accept any H. */
+int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scaling, int k,
+                            const double *H, const double *x, double *y);
+
+/* ---- Krylov f(A)x (psfun_d_krylov%apply) --------------------------------- */
+/* y = ||x|| V_k f(fscale*T_k) e1 by Lanczos with the reference's stopping rule,
+ * one-step-lagged a-posteriori estimate and assembly (psfun_d_lanczos.f90:183-377).
+ * The Krylov operator is opscale*A (the reference uses opscale = 1, ignoring
+ * fun%scaling, psfun_d_lanczos.f90:172-173,198,239).  itmax >= 2, istop in {1,2}.
+ * Outputs: *iter, *err; res (host, length >= itmax+1) receives res(1:itx+1)
+ * and *nres = itx+1.  x and y are device vectors of length n. */
+int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
+                        const double *x, double *y, double eps, int itmax, int istop, int *iter,
+                        double *err, double *res, int *nres);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSFUN_HIP_H */

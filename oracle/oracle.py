@@ -1,0 +1,227 @@
+"""ctypes front-end of the CPU oracle (oracle/psfun_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the CPU baseline,
+never as the thing measured or shipped.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsfun_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+ZFUN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from oracle import build as _b  # noqa: F401  (build on demand in the container)
+            _b.build(verbose=False)
+        L = C.CDLL(LIB_PATH)
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_max_threads.restype = C.c_int
+        L.or_spmv.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, _dp, C.c_double, _dp]
+        L.or_dot.argtypes = [C.c_int, _dp, _dp]
+        L.or_dot.restype = C.c_double
+        L.or_nrm2.argtypes = [C.c_int, _dp]
+        L.or_nrm2.restype = C.c_double
+        L.or_amax.argtypes = [C.c_int, _dp]
+        L.or_amax.restype = C.c_double
+        L.or_axpby.argtypes = [C.c_int, C.c_double, _dp, C.c_double, _dp]
+        L.or_tridiag_eig.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.or_tridiag_eig.restype = C.c_int
+        L.or_exp_tridiag.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
+        L.or_exp_tridiag.restype = C.c_int
+        L.or_dense_expm.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.or_dense_expm.restype = C.c_int
+        kry = [C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_int, C.c_int,
+               C.POINTER(C.c_int), C.POINTER(C.c_double), _dp, C.POINTER(C.c_int)]
+        L.or_lanczos.argtypes = kry
+        L.or_lanczos.restype = C.c_int
+        L.or_arnoldi.argtypes = kry
+        L.or_arnoldi.restype = C.c_int
+        L.or_cg_shifted.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, C.c_double,
+                                    C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.or_cg_shifted.restype = C.c_int
+        L.or_mscg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                              np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), _dp]
+        L.or_mscg.restype = C.c_int
+        L.or_ellipkkp.argtypes = [C.c_double, _dp]
+        L.or_ellipj.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L.or_horner.argtypes = [C.c_int, _dp, C.c_double]
+        L.or_horner.restype = C.c_double
+        L.or_hhtmethod1.argtypes = [ZFUN, C.c_int, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.or_hhtmethod1.restype = C.c_int
+        L.or_talbot.argtypes = [C.c_int, C.c_double, _dp, _dp]
+        L.or_talbot.restype = C.c_int
+        L.or_quad_apply.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp,
+                                    C.c_double, C.c_int, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        L.or_quad_apply.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def set_threads(t):
+    lib().or_set_threads(int(t))
+
+
+def max_threads():
+    return lib().or_get_max_threads()
+
+
+def _csr(A):
+    rp, ci, va = A
+    return (np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+            np.ascontiguousarray(va, np.float64))
+
+
+def spmv(A, x, alpha=1.0, beta=0.0, y=None):
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    y = np.zeros(n) if y is None else np.ascontiguousarray(y, np.float64).copy()
+    lib().or_spmv(n, rp, ci, va, alpha, np.ascontiguousarray(x, np.float64), beta, y)
+    return y
+
+
+def exp_tridiag(d, e, s, x):
+    d = np.ascontiguousarray(d, np.float64)
+    k = len(d)
+    e = np.ascontiguousarray(np.append(e, 0.0), np.float64)
+    y = np.zeros(k)
+    rc = lib().or_exp_tridiag(k, d, e, s, np.ascontiguousarray(x, np.float64), y)
+    if rc:
+        raise RuntimeError("tridiagonal QL did not converge")
+    return y
+
+
+def tridiag_eig(d, e):
+    d = np.ascontiguousarray(d, np.float64).copy()
+    k = len(d)
+    Z = np.zeros(k * k)
+    rc = lib().or_tridiag_eig(k, d, np.ascontiguousarray(np.append(e, 0.0), np.float64), Z)
+    if rc:
+        raise RuntimeError("tridiagonal QL did not converge")
+    return d, Z.reshape(k, k).T
+
+
+def dense_expm(H, s=1.0):
+    H = np.asfortranarray(H, dtype=np.float64)
+    k = H.shape[0]
+    E = np.zeros(k * k)
+    rc = lib().or_dense_expm(k, np.ascontiguousarray(H.reshape(-1, order="F")), s, E)
+    if rc:
+        raise RuntimeError("Pade solve failed")
+    return E.reshape(k, k, order="F")
+
+
+def _krylov(fn, A, x, eps, itmax, istop, fscale, opscale):
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    y = np.zeros(n)
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    nres = C.c_int(0)
+    res = np.zeros(max(itmax, 2) + 2)
+    rc = fn(n, rp, ci, va, fscale, opscale, np.ascontiguousarray(x, np.float64), y, eps, itmax, istop,
+            C.byref(it), C.byref(err), res, C.byref(nres))
+    return dict(info=rc, y=y, iter=it.value, err=err.value, res=res[:nres.value].copy())
+
+
+def lanczos(A, x, eps, itmax, istop=1, fscale=1.0, opscale=1.0):
+    """Reference-semantics Lanczos (krylov/polynomial/psfun_d_lanczos.f90)."""
+    return _krylov(lib().or_lanczos, A, x, eps, itmax, istop, fscale, opscale)
+
+
+def arnoldi(A, x, eps, itmax, istop=1, fscale=1.0, scaling=1.0):
+    """Reference-semantics Arnoldi (krylov/polynomial/psfun_d_arnoldi.f90)."""
+    return _krylov(lib().or_arnoldi, A, x, eps, itmax, istop, fscale, scaling)
+
+
+def cg_shifted(A, b, eta, zeta, x0=None, eps=1e-8, itmax=1000, istop=2, prec=0, want_cond=False):
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    cond = C.c_double(0.0)
+    rc = lib().or_cg_shifted(n, rp, ci, va, prec, np.ascontiguousarray(b, np.float64), eta, zeta, x, eps,
+                             itmax, istop, C.byref(it), C.byref(err), C.byref(cond) if want_cond else None)
+    return dict(info=rc, x=x, iter=it.value, err=err.value, cond=cond.value if want_cond else None)
+
+
+def mscg(A, b, shifts, eps=1e-8, itmax=1000):
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    shifts = np.asarray(shifts, dtype=np.complex128)
+    ns = len(shifts)
+    sg = np.ascontiguousarray(np.stack([shifts.real, shifts.imag], 1).reshape(-1))
+    X = np.zeros(2 * ns * n)
+    iters = np.zeros(ns, dtype=np.int32)
+    resid = np.zeros(ns)
+    lib().or_mscg(n, rp, ci, va, ns, sg, np.ascontiguousarray(b, np.float64), X, eps, itmax, iters, resid)
+    Xc = X.reshape(ns, n, 2)
+    return dict(X=Xc[..., 0] + 1j * Xc[..., 1], iters=iters, resid=resid)
+
+
+def ellipkkp(L):
+    K = np.zeros(2)
+    lib().or_ellipkkp(L, K)
+    return K
+
+
+def ellipj(u, L):
+    out = np.zeros(6)
+    u = complex(u)
+    lib().or_ellipj(u.real, u.imag, L, out)
+    return complex(out[0], out[1]), complex(out[2], out[3]), complex(out[4], out[5])
+
+
+def horner(coeffs, x):
+    c = np.ascontiguousarray(coeffs, np.float64)
+    return lib().or_horner(len(c), c, x)
+
+
+def hhtmethod1(zfun, N, rparams):
+    def cb(re, im, ore, oim):
+        v = complex(zfun(complex(re, im)))
+        ore[0] = v.real
+        oim[0] = v.imag
+    cfun = ZFUN(cb)
+    xi = np.zeros(2 * N)
+    c = np.zeros(2 * N)
+    eta = C.c_double(0)
+    sign = C.c_double(0)
+    lib().or_hhtmethod1(cfun, N, rparams[0], rparams[1], xi, c, C.byref(eta), C.byref(sign))
+    return xi[0::2] + 1j * xi[1::2], c[0::2] + 1j * c[1::2], eta.value, sign.value
+
+
+def talbot(N, t=1.0):
+    s = np.zeros(2 * N)
+    w = np.zeros(2 * N)
+    lib().or_talbot(N, t, s, w)
+    return s[0::2] + 1j * s[1::2], w[0::2] + 1j * w[1::2]
+
+
+def quad_apply(A, shifts, weights, gamma, form, x, eps=1e-8, itmax=2000):
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    shifts = np.asarray(shifts, np.complex128)
+    weights = np.asarray(weights, np.complex128)
+    ns = len(shifts)
+    sg = np.ascontiguousarray(np.stack([shifts.real, shifts.imag], 1).reshape(-1))
+    ww = np.ascontiguousarray(np.stack([weights.real, weights.imag], 1).reshape(-1))
+    y = np.zeros(n)
+    iters = np.zeros(ns, dtype=np.int32)
+    lib().or_quad_apply(n, rp, ci, va, ns, sg, ww, gamma, form, np.ascontiguousarray(x, np.float64), y, eps,
+                        itmax, iters)
+    return dict(y=y, iters=iters)

@@ -1,0 +1,857 @@
+/*
+ * psfun_oracle.c -- CPU restatement of PSFUN's f(A)x path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product library (psfun_amd/libpsfun_hip.so) never calls it.
+ *
+ * Every routine follows a reference file/line range (paths relative to the
+ * PSFUN tree) and runs unfused, one pass per PSBLAS-like primitive, exactly in
+ * the reference's operation order.  PSBLAS itself is not part of the reference
+ * tree; its primitives are restated from their textbook definitions (CSR row
+ * sums in storage order, BLAS-1 with left-to-right accumulation).
+ *
+ * Pinning (what checks each routine; see tests/):
+ *   or_tridiag_eig / or_exp_tridiag   -- scipy.linalg.expm fixtures, closed-form 2x2
+ *   or_dense_expm                      -- scipy.linalg.expm fixtures, diagonal/nilpotent cases
+ *   or_lanczos                         -- DST closed form exp(-tA) for Dirichlet Laplacians,
+ *                                         scipy dense expm on small matrices, breakdown cases
+ *   or_arnoldi                         -- same oracles as or_lanczos
+ *   or_cg_shifted                      -- dense solve, residual check, PSBLAS stop rules
+ *   or_mscg                            -- dense complex solve per shift
+ *   or_ellipkkp / or_ellipj / horner  -- the control values of examples/utils/utiltest.f90,
+ *                                         mpmath elliptic functions
+ *   or_hhtmethod1 / or_quad_apply      -- scipy sqrtm / expm on small SPD matrices
+ *   or_talbot                          -- scalar exp(-x) on a grid (max error at N=16)
+ */
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zdouble;
+
+static int g_threads = 1;
+void or_set_threads(int t) {
+  g_threads = t < 1 ? 1 : t;
+#ifdef _OPENMP
+  omp_set_num_threads(g_threads);
+#endif
+}
+int or_get_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* PSBLAS-like primitives (0-based CSR)                                */
+/* ------------------------------------------------------------------ */
+/* y = alpha*A*x + beta*y  (psb_spmm) */
+void or_spmv(int n, const int *rp, const int *ci, const double *va, double alpha, const double *x,
+             double beta, double *y) {
+#pragma omp parallel for schedule(static) if (g_threads > 1)
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) s += va[j] * x[ci[j]];
+    y[i] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[i];
+  }
+}
+
+double or_dot(int n, const double *x, const double *y) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static) if (g_threads > 1)
+  for (int i = 0; i < n; ++i) s += x[i] * y[i];
+  return s;
+}
+
+double or_nrm2(int n, const double *x) { return sqrt(or_dot(n, x, x)); }
+
+double or_amax(int n, const double *x) {
+  double m = 0.0;
+#pragma omp parallel for reduction(max : m) schedule(static) if (g_threads > 1)
+  for (int i = 0; i < n; ++i) m = fmax(m, fabs(x[i]));
+  return m;
+}
+
+/* y = alpha*x + beta*y  (psb_geaxpby) */
+void or_axpby(int n, double alpha, const double *x, double beta, double *y) {
+#pragma omp parallel for schedule(static) if (g_threads > 1)
+  for (int i = 0; i < n; ++i) y[i] = (beta == 0.0) ? alpha * x[i] : alpha * x[i] + beta * y[i];
+}
+
+/* max_i sum_j |a_ij|  (psb_spnrmi) */
+double or_spnrmi(int n, const int *rp, const int *ci, const double *va) {
+  (void)ci;
+  double m = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) s += fabs(va[j]);
+    m = fmax(m, s);
+  }
+  return m;
+}
+
+/* ------------------------------------------------------------------ */
+/* Small dense f(H) step                                               */
+/* ------------------------------------------------------------------ */
+/* Symmetric tridiagonal eigen-decomposition by implicit QL with Wilkinson-type
+ * shifts (Golub & Van Loan, Matrix Computations, sec. 8.3; the classical tql2
+ * scheme).  d[k] diagonal (overwritten by eigenvalues), e[k-1] off-diagonal,
+ * Z[k*k] column-major eigenvectors.  Returns 0 on success. */
+int or_tridiag_eig(int k, double *d, const double *e_in, double *Z) {
+  double *e = (double *)calloc((size_t)k + 1, sizeof(double));
+  for (int i = 0; i + 1 < k; ++i) e[i] = e_in[i];
+  for (int i = 0; i < k * k; ++i) Z[i] = 0.0;
+  for (int i = 0; i < k; ++i) Z[i + (size_t)i * k] = 1.0;
+  for (int l = 0; l < k; ++l) {
+    int iter = 0, m;
+    for (;;) {
+      for (m = l; m < k - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 100) {
+        free(e);
+        return 1;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i;
+      int early = 0;
+      for (i = m - 1; i >= l; --i) {
+        double f = s * e[i], b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          early = 1;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int q = 0; q < k; ++q) {
+          double *zi = Z + (size_t)i * k, *zi1 = Z + (size_t)(i + 1) * k;
+          f = zi1[q];
+          zi1[q] = s * zi[q] + c * f;
+          zi[q] = c * zi[q] - s * f;
+        }
+      }
+      if (early) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  free(e);
+  return 0;
+}
+
+/* y = exp(s*T) x for symmetric tridiagonal T (d: k, e: k-1), via the
+ * eigen-decomposition T = Z diag(lambda) Z^T. */
+int or_exp_tridiag(int k, const double *d, const double *e, double s, const double *x, double *y) {
+  double *lam = (double *)malloc(sizeof(double) * (size_t)k);
+  double *Z = (double *)malloc(sizeof(double) * (size_t)k * k);
+  double *t = (double *)malloc(sizeof(double) * (size_t)k);
+  memcpy(lam, d, sizeof(double) * (size_t)k);
+  int rc = or_tridiag_eig(k, lam, e, Z);
+  if (rc == 0) {
+    for (int j = 0; j < k; ++j) {
+      double a = 0.0;
+      for (int i = 0; i < k; ++i) a += Z[i + (size_t)j * k] * x[i];
+      t[j] = a * exp(s * lam[j]);
+    }
+    for (int i = 0; i < k; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < k; ++j) a += Z[i + (size_t)j * k] * t[j];
+      y[i] = a;
+    }
+  }
+  free(lam);
+  free(Z);
+  free(t);
+  return rc;
+}
+
+/* ---- dense helpers (column-major) ---- */
+static void mm(int k, const double *A, const double *B, double *C) {
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) {
+      double s = 0.0;
+      for (int l = 0; l < k; ++l) s += A[i + (size_t)l * k] * B[l + (size_t)j * k];
+      C[i + (size_t)j * k] = s;
+    }
+}
+
+/* Solve A X = B (k x k, nrhs columns) by Gaussian elimination with partial
+ * pivoting; A and B are overwritten. */
+static int gesv(int k, int nrhs, double *A, double *B) {
+  for (int c = 0; c < k; ++c) {
+    int p = c;
+    for (int i = c + 1; i < k; ++i)
+      if (fabs(A[i + (size_t)c * k]) > fabs(A[p + (size_t)c * k])) p = i;
+    if (A[p + (size_t)c * k] == 0.0) return 1;
+    if (p != c) {
+      for (int j = 0; j < k; ++j) {
+        double t = A[c + (size_t)j * k];
+        A[c + (size_t)j * k] = A[p + (size_t)j * k];
+        A[p + (size_t)j * k] = t;
+      }
+      for (int j = 0; j < nrhs; ++j) {
+        double t = B[c + (size_t)j * k];
+        B[c + (size_t)j * k] = B[p + (size_t)j * k];
+        B[p + (size_t)j * k] = t;
+      }
+    }
+    const double piv = A[c + (size_t)c * k];
+    for (int i = c + 1; i < k; ++i) {
+      const double f = A[i + (size_t)c * k] / piv;
+      if (f == 0.0) continue;
+      for (int j = c; j < k; ++j) A[i + (size_t)j * k] -= f * A[c + (size_t)j * k];
+      for (int j = 0; j < nrhs; ++j) B[i + (size_t)j * k] -= f * B[c + (size_t)j * k];
+    }
+  }
+  for (int j = 0; j < nrhs; ++j)
+    for (int i = k - 1; i >= 0; --i) {
+      double s = B[i + (size_t)j * k];
+      for (int l = i + 1; l < k; ++l) s -= A[i + (size_t)l * k] * B[l + (size_t)j * k];
+      B[i + (size_t)j * k] = s / A[i + (size_t)i * k];
+    }
+  return 0;
+}
+
+/* E = exp(s*H), H k x k column-major: [13/13] Pade approximant with scaling
+ * and squaring (Higham, SIAM J. Matrix Anal. Appl. 26 (2005) 1179-1193). */
+int or_dense_expm(int k, const double *H, double s, double *E) {
+  static const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0,
+                               1187353796428800.0,  129060195264000.0,   10559470521600.0,
+                               670442572800.0,      33522128640.0,       1323241920.0,
+                               40840800.0,          960960.0,            16380.0,
+                               182.0,               1.0};
+  const size_t kk = (size_t)k * k;
+  double *A = (double *)malloc(sizeof(double) * kk), *A2 = (double *)malloc(sizeof(double) * kk),
+         *A4 = (double *)malloc(sizeof(double) * kk), *A6 = (double *)malloc(sizeof(double) * kk),
+         *U = (double *)malloc(sizeof(double) * kk), *V = (double *)malloc(sizeof(double) * kk),
+         *T = (double *)malloc(sizeof(double) * kk), *P = (double *)malloc(sizeof(double) * kk);
+  double nrm1 = 0.0;
+  for (int j = 0; j < k; ++j) {
+    double cs = 0.0;
+    for (int i = 0; i < k; ++i) {
+      A[i + (size_t)j * k] = s * H[i + (size_t)j * k];
+      cs += fabs(A[i + (size_t)j * k]);
+    }
+    nrm1 = fmax(nrm1, cs);
+  }
+  int sq = 0;
+  const double theta13 = 5.371920351148152;
+  if (nrm1 > theta13) sq = (int)ceil(log2(nrm1 / theta13));
+  if (sq < 0) sq = 0;
+  const double sc = ldexp(1.0, -sq);
+  for (size_t i = 0; i < kk; ++i) A[i] *= sc;
+  mm(k, A, A, A2);
+  mm(k, A2, A2, A4);
+  mm(k, A4, A2, A6);
+  /* U = A [A6 (b13 A6 + b11 A4 + b9 A2) + b7 A6 + b5 A4 + b3 A2 + b1 I] */
+  for (size_t i = 0; i < kk; ++i) T[i] = b[13] * A6[i] + b[11] * A4[i] + b[9] * A2[i];
+  mm(k, A6, T, P);
+  for (size_t i = 0; i < kk; ++i) P[i] += b[7] * A6[i] + b[5] * A4[i] + b[3] * A2[i];
+  for (int i = 0; i < k; ++i) P[i + (size_t)i * k] += b[1];
+  mm(k, A, P, U);
+  /* V = A6 (b12 A6 + b10 A4 + b8 A2) + b6 A6 + b4 A4 + b2 A2 + b0 I */
+  for (size_t i = 0; i < kk; ++i) T[i] = b[12] * A6[i] + b[10] * A4[i] + b[8] * A2[i];
+  mm(k, A6, T, V);
+  for (size_t i = 0; i < kk; ++i) V[i] += b[6] * A6[i] + b[4] * A4[i] + b[2] * A2[i];
+  for (int i = 0; i < k; ++i) V[i + (size_t)i * k] += b[0];
+  /* (V - U) E = (V + U) */
+  for (size_t i = 0; i < kk; ++i) {
+    T[i] = V[i] - U[i];
+    E[i] = V[i] + U[i];
+  }
+  int rc = gesv(k, k, T, E);
+  for (int q = 0; q < sq && rc == 0; ++q) {
+    mm(k, E, E, T);
+    memcpy(E, T, sizeof(double) * kk);
+  }
+  free(A); free(A2); free(A4); free(A6); free(U); free(V); free(T); free(P);
+  return rc;
+}
+
+/* y = exp(s*H) x for a dense k x k column-major H. */
+int or_dense_expm_apply(int k, const double *H, double s, const double *x, double *y) {
+  double *E = (double *)malloc(sizeof(double) * (size_t)k * k);
+  int rc = or_dense_expm(k, H, s, E);
+  if (rc == 0)
+    for (int i = 0; i < k; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < k; ++j) a += E[i + (size_t)j * k] * x[j];
+      y[i] = a;
+    }
+  free(E);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Lanczos f(A)x  (krylov/polynomial/psfun_d_lanczos.f90:97-386)       */
+/* ------------------------------------------------------------------ */
+/* f = exp(fscale * .), Krylov operator opscale*A (the reference passes 1).
+ * res must hold itmax+1 doubles.  Returns 0 or an error code:
+ * 2 = invalid istop (lines 108-113), 1 = other invalid argument. */
+int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fscale, double opscale,
+               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+               double *res, int *nres) {
+  if (istop < 1 || istop > 2) return 2;
+  if (itmax < 2) return 1;
+  const int nl = itmax;
+  double *V = (double *)malloc(sizeof(double) * (size_t)n * (nl + 1));
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *T = (double *)calloc((size_t)(nl + 1) * (nl + 1), sizeof(double)); /* T(i,j) -> T[(i-1)+(j-1)*(nl+1)] */
+  double *rs = (double *)calloc((size_t)nl + 1, sizeof(double));
+  double *yk = NULL;
+  int nyk = 0;
+#define TT(i, j) T[((i)-1) + (size_t)((j)-1) * (nl + 1)]
+#define VV(i) (V + (size_t)((i)-1) * n)
+  double errnum = 0.0, errden = 1.0;
+  /* v(1) = x; rs(1) = ||v(1)||; v(1) *= 1/rs(1)   (183-196) */
+  or_axpby(n, 1.0, x, 0.0, VV(1));
+  rs[0] = or_nrm2(n, VV(1));
+  or_axpby(n, 1.0 / rs[0], VV(1), 0.0, VV(1));
+  /* w = A v(1); t(1,1) = w.v(1); w -= t(1,1) v(1)   (198-217) */
+  or_spmv(n, rp, ci, va, opscale, VV(1), 0.0, w);
+  TT(1, 1) = or_dot(n, w, VV(1));
+  or_axpby(n, -TT(1, 1), VV(1), 1.0, w);
+  int itx = 0;
+  for (int i = 2; i <= nl; ++i) { /* 219-326 */
+    itx++;
+    TT(i - 1, i) = or_nrm2(n, w);
+    TT(i, i - 1) = TT(i - 1, i);
+    if (fabs(TT(i - 1, i)) < 1e-16) break;
+    or_axpby(n, 1.0 / TT(i - 1, i), w, 0.0, VV(i));
+    or_spmv(n, rp, ci, va, opscale, VV(i), 0.0, w);
+    TT(i, i) = or_dot(n, w, VV(i));
+    or_axpby(n, -TT(i, i), VV(i), 1.0, w);
+    or_axpby(n, -TT(i - 1, i), VV(i - 1), 1.0, w);
+    if (istop == 1) {
+      const int k = i - 1;
+      double *d = (double *)malloc(sizeof(double) * k), *e = (double *)malloc(sizeof(double) * (k + 1));
+      double *e1 = (double *)calloc((size_t)k, sizeof(double));
+      free(yk);
+      yk = (double *)malloc(sizeof(double) * k);
+      nyk = k;
+      for (int q = 1; q <= k; ++q) d[q - 1] = TT(q, q);
+      for (int q = 1; q < k; ++q) e[q - 1] = TT(q, q + 1);
+      e1[0] = 1.0;
+      or_exp_tridiag(k, d, e, fscale, e1, yk); /* fun%apply(t(1:i-1,1:i-1),yk,e)  (292) */
+      free(d); free(e); free(e1);
+      errnum = rs[0] * TT(i, i - 1) * fabs(yk[i - 2]);
+      double s2 = 0.0;
+      for (int q = 0; q < k; ++q) s2 += yk[q] * yk[q];
+      errden = sqrt(s2);
+      rs[i - 1] = errnum / errden;
+      if (rs[i - 1] < eps) break;
+      free(yk);
+      yk = NULL;
+      nyk = 0;
+    }
+  }
+  if (!yk) { /* 330-362: yk = f(T(1:itx-1,1:itx-1)) e1; itx-1 == 0 is undefined in
+                the reference and is taken as the 1x1 block here */
+    int k = itx - 1;
+    if (k < 1) k = 1;
+    double *d = (double *)malloc(sizeof(double) * k), *e = (double *)malloc(sizeof(double) * (k + 1));
+    double *e1 = (double *)calloc((size_t)k, sizeof(double));
+    yk = (double *)malloc(sizeof(double) * k);
+    nyk = k;
+    for (int q = 1; q <= k; ++q) d[q - 1] = TT(q, q);
+    for (int q = 1; q < k; ++q) e[q - 1] = TT(q, q + 1);
+    e1[0] = 1.0;
+    or_exp_tridiag(k, d, e, fscale, e1, yk);
+    free(d); free(e); free(e1);
+  }
+  /* log_end: err = errnum/errden, iter = itx   (365-371) */
+  const double derr = (errden == 0.0) ? errnum : errnum / errden;
+  if (iter) *iter = itx;
+  if (err) *err = derr;
+  if (res) {
+    for (int q = 0; q < itx; ++q) res[q] = rs[q];
+    res[itx] = derr;
+  }
+  if (nres) *nres = itx + 1;
+  /* y = rs(1) (yk(1) v(1) + sum_{i=2}^{itx-1} yk(i) v(i))   (373-377) */
+  or_axpby(n, yk[0], VV(1), 0.0, y);
+  for (int q = 2; q <= itx - 1 && q <= nyk; ++q) or_axpby(n, yk[q - 1], VV(q), 1.0, y);
+  or_axpby(n, rs[0], y, 0.0, y);
+#undef TT
+#undef VV
+  free(V); free(w); free(T); free(rs); free(yk);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Arnoldi f(A)x  (krylov/polynomial/psfun_d_arnoldi.f90:172-329)      */
+/* ------------------------------------------------------------------ */
+/* The Krylov operator is scaling*A (line 202) and f(H) uses scaling 1
+ * (line 174); fscale multiplies H inside f for callers that want it. */
+int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double scaling,
+               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+               double *res, int *nres) {
+  if (istop < 1 || istop > 2) return 2;
+  if (itmax < 2) return 1;
+  const int nl = itmax;
+  double *V = (double *)malloc(sizeof(double) * (size_t)n * (nl + 1));
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *Hm = (double *)calloc((size_t)(nl + 1) * (nl + 1), sizeof(double));
+  double *rs = (double *)calloc((size_t)nl + 1, sizeof(double));
+  double *yk = NULL;
+  int nyk = 0;
+#define HH(i, j) Hm[((i)-1) + (size_t)((j)-1) * (nl + 1)]
+#define VV(i) (V + (size_t)((i)-1) * n)
+  double errnum = 0.0, errden = 1.0;
+  or_axpby(n, 1.0, x, 0.0, VV(1));
+  rs[0] = or_nrm2(n, VV(1));
+  or_axpby(n, 1.0 / rs[0], VV(1), 0.0, VV(1));
+  int itx = 0;
+  for (int i = 2; i <= nl; ++i) {
+    itx++;
+    or_spmv(n, rp, ci, va, scaling, VV(i - 1), 0.0, w);
+    for (int k = 1; k <= i - 1; ++k) {
+      HH(k, i - 1) = or_dot(n, VV(k), w);
+      or_axpby(n, -HH(k, i - 1), VV(k), 1.0, w);
+    }
+    HH(i, i - 1) = or_nrm2(n, w);
+    or_axpby(n, 1.0 / HH(i, i - 1), w, 0.0, VV(i));
+    if (istop == 1) {
+      const int k = i - 1;
+      double *Hk = (double *)malloc(sizeof(double) * k * k), *e1 = (double *)calloc((size_t)k, sizeof(double));
+      for (int c = 1; c <= k; ++c)
+        for (int r = 1; r <= k; ++r) Hk[(r - 1) + (size_t)(c - 1) * k] = HH(r, c);
+      e1[0] = 1.0;
+      free(yk);
+      yk = (double *)malloc(sizeof(double) * k);
+      nyk = k;
+      or_dense_expm_apply(k, Hk, fscale, e1, yk);
+      free(Hk); free(e1);
+      errnum = rs[0] * HH(i, i - 1) * fabs(yk[i - 2]);
+      double s2 = 0.0;
+      for (int q = 0; q < k; ++q) s2 += yk[q] * yk[q];
+      errden = sqrt(s2);
+      rs[i - 1] = errnum / errden;
+      if (rs[i - 1] < eps) break;
+      free(yk);
+      yk = NULL;
+      nyk = 0;
+    }
+  }
+  if (!yk) {
+    int k = itx - 1;
+    if (k < 1) k = 1;
+    double *Hk = (double *)malloc(sizeof(double) * k * k), *e1 = (double *)calloc((size_t)k, sizeof(double));
+    for (int c = 1; c <= k; ++c)
+      for (int r = 1; r <= k; ++r) Hk[(r - 1) + (size_t)(c - 1) * k] = HH(r, c);
+    e1[0] = 1.0;
+    yk = (double *)malloc(sizeof(double) * k);
+    nyk = k;
+    or_dense_expm_apply(k, Hk, fscale, e1, yk);
+    free(Hk); free(e1);
+  }
+  const double derr = (errden == 0.0) ? errnum : errnum / errden;
+  if (iter) *iter = itx;
+  if (err) *err = derr;
+  if (res) {
+    for (int q = 0; q < itx; ++q) res[q] = rs[q];
+    res[itx] = derr;
+  }
+  if (nres) *nres = itx + 1;
+  or_axpby(n, yk[0], VV(1), 0.0, y);
+  for (int q = 2; q <= itx - 1 && q <= nyk; ++q) or_axpby(n, yk[q - 1], VV(q), 1.0, y);
+  or_axpby(n, rs[0], y, 0.0, y);
+#undef HH
+#undef VV
+  free(V); free(w); free(Hm); free(rs); free(yk);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shifted PCG (krylov/shifted/psfun_dcg.F90:249-357)                  */
+/* ------------------------------------------------------------------ */
+/* (eta*A + zeta*I) x = b, x holds the initial guess.  prec: 0 = none,
+ * 1 = diagonal of the unshifted A (the DIAG preconditioner built on A).
+ * Stopping follows PSBLAS psb_init_conv/psb_check_conv/psb_end_conv:
+ * istop 1: ||r||_inf <= eps (||A||_inf ||x||_inf + ||b||_inf);
+ * istop 2: ||r||_2 <= eps ||b||_2; and it >= itmax stops.
+ * cond (if non-null) is the ratio of extreme eigenvalues of the CG
+ * tridiagonal (dstebz in the reference, lines 314-354). */
+int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int prec, const double *b, double eta,
+                  double zeta, double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond) {
+  if (istop < 1 || istop > 2) return 2;
+  double *p = (double *)calloc((size_t)n, sizeof(double)), *q = (double *)calloc((size_t)n, sizeof(double)),
+         *r = (double *)calloc((size_t)n, sizeof(double)), *z = (double *)calloc((size_t)n, sizeof(double)),
+         *dinv = NULL;
+  if (prec == 1) {
+    dinv = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) {
+      double dv = 0.0;
+      for (int j = rp[i]; j < rp[i + 1]; ++j)
+        if (ci[j] == i) dv += va[j];
+      dinv[i] = (dv != 0.0) ? 1.0 / dv : 1.0;
+    }
+  }
+  double *td = cond ? (double *)calloc((size_t)itmax + 2, sizeof(double)) : NULL;
+  double *tu = cond ? (double *)calloc((size_t)itmax + 2, sizeof(double)) : NULL;
+  int istebz = 0;
+  int itx = 0;
+  double alpha = 0.0, beta = 0.0, rho = 0.0, rho_old, sigma, alpha_old;
+  double ani = 0.0, bni = 0.0, bn2 = 0.0, errnum = 0.0, errden = 1.0;
+  int conv = 0;
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    int it = 0;
+    or_axpby(n, 1.0, b, 0.0, r);
+    or_spmv(n, rp, ci, va, -eta, x, 1.0, r);
+    or_axpby(n, -zeta, x, 1.0, r);
+    rho = 0.0;
+    /* psb_init_conv (uses ||A|| of the unshifted A, line 272) */
+    if (istop == 1) {
+      ani = or_spnrmi(n, rp, ci, va);
+      bni = or_amax(n, b);
+    } else {
+      bn2 = or_nrm2(n, b);
+    }
+    for (;;) { /* iteration */
+      it++;
+      itx++;
+      if (prec == 1)
+        for (int i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
+      else
+        or_axpby(n, 1.0, r, 0.0, z);
+      rho_old = rho;
+      rho = or_dot(n, r, z);
+      if (it == 1) {
+        or_axpby(n, 1.0, z, 0.0, p);
+      } else {
+        if (rho_old == 0.0) break;
+        beta = rho / rho_old;
+        or_axpby(n, 1.0, z, beta, p);
+      }
+      or_spmv(n, rp, ci, va, eta, p, 0.0, q);
+      or_axpby(n, zeta, p, 1.0, q);
+      sigma = or_dot(n, p, q);
+      if (sigma == 0.0) break;
+      alpha_old = alpha;
+      alpha = rho / sigma;
+      if (cond) {
+        istebz++;
+        if (istebz == 1)
+          td[0] = 1.0 / alpha;
+        else {
+          td[istebz - 1] = 1.0 / alpha + beta / alpha_old;
+          tu[istebz - 2] = sqrt(beta) / alpha_old;
+        }
+      }
+      or_axpby(n, alpha, p, 1.0, x);
+      or_axpby(n, -alpha, q, 1.0, r);
+      /* psb_check_conv */
+      if (istop == 1) {
+        errnum = or_amax(n, r);
+        errden = ani * or_amax(n, x) + bni;
+      } else {
+        errnum = or_nrm2(n, r);
+        errden = bn2;
+      }
+      const int ok = (errden == 0.0) ? (errnum <= eps) : (errnum <= eps * errden);
+      if (ok || itx >= itmax) {
+        conv = 1;
+        break;
+      }
+    }
+    if (conv) break;
+  }
+  if (cond) {
+    *cond = 0.0;
+    if (istebz > 0) {
+      double *Z = (double *)malloc(sizeof(double) * (size_t)istebz * istebz);
+      if (or_tridiag_eig(istebz, td, tu, Z) == 0) {
+        double mn = td[0], mx = td[0];
+        for (int i = 1; i < istebz; ++i) {
+          mn = fmin(mn, td[i]);
+          mx = fmax(mx, td[i]);
+        }
+        *cond = mx / mn;
+      }
+      free(Z);
+    }
+    free(td);
+    free(tu);
+  }
+  /* psb_end_conv */
+  if (err) *err = (errden == 0.0) ? errnum : errnum / errden;
+  if (iter) *iter = itx;
+  free(p); free(q); free(r); free(z); free(dinv);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Multi-shift CG for (A + sigma_j I) x_j = b, complex sigma_j         */
+/* (no reference counterpart: psfun_dcg.F90 is a single real shift).   */
+/* ------------------------------------------------------------------ */
+/* Seed: real CG on the SPD matrix A with x0 = 0.  Shifted iterates follow the
+ * collinear-residual recurrences r_k^s = zeta_k^s r_k (Jegerlehner 1996;
+ * Frommer 2003):
+ *   zeta_{k+1} = zeta_k zeta_{k-1} a_{k-1} /
+ *                (a_k b_{k-1} (zeta_{k-1} - zeta_k) + zeta_{k-1} a_{k-1} (1 + s a_k))
+ *   a_k^s = a_k zeta_{k+1}/zeta_k,  b_k^s = b_k (zeta_{k+1}/zeta_k)^2
+ *   x^s += a_k^s p^s,  p^s = zeta_{k+1} r_{k+1} + b_k^s p^s.
+ * A shift stops updating once |zeta_k^s| ||r_k|| <= eps ||b||.  X is
+ * nshift x n complex (interleaved re/im), row j = shift j.  iters[j] is the
+ * iteration at which shift j converged (or itmax). */
+int or_mscg(int n, const int *rp, const int *ci, const double *va, int nshift, const double *shifts,
+            const double *b, double *X, double eps, int itmax, int *iters, double *resid) {
+  zdouble *sg = (zdouble *)malloc(sizeof(zdouble) * nshift);
+  for (int s = 0; s < nshift; ++s) sg[s] = shifts[2 * s] + I * shifts[2 * s + 1];
+  zdouble *Xs = (zdouble *)X;
+  zdouble *Ps = (zdouble *)malloc(sizeof(zdouble) * (size_t)nshift * n);
+  double *r = (double *)malloc(sizeof(double) * n), *p = (double *)malloc(sizeof(double) * n),
+         *q = (double *)malloc(sizeof(double) * n);
+  zdouble *zc = (zdouble *)malloc(sizeof(zdouble) * nshift), *zp = (zdouble *)malloc(sizeof(zdouble) * nshift);
+  int *act = (int *)malloc(sizeof(int) * nshift);
+  const double bn = or_nrm2(n, b);
+  for (int s = 0; s < nshift; ++s) {
+    zc[s] = 1.0;
+    zp[s] = 1.0;
+    act[s] = 1;
+    iters[s] = 0;
+    for (int i = 0; i < n; ++i) {
+      Xs[(size_t)s * n + i] = 0.0;
+      Ps[(size_t)s * n + i] = b[i];
+    }
+  }
+  memcpy(r, b, sizeof(double) * n);
+  memcpy(p, b, sizeof(double) * n);
+  double rr = or_dot(n, r, r);
+  double a_prev = 1.0, b_prev = 0.0;
+  int nact = nshift;
+  if (bn == 0.0) nact = 0;
+  for (int k = 0; k < itmax && nact > 0; ++k) {
+    or_spmv(n, rp, ci, va, 1.0, p, 0.0, q);
+    const double pq = or_dot(n, p, q);
+    const double a = rr / pq;
+    for (int i = 0; i < n; ++i) r[i] -= a * q[i];
+    const double rr_new = or_dot(n, r, r);
+    const double bk = rr_new / rr;
+    const double rnorm = sqrt(rr_new);
+    for (int s = 0; s < nshift; ++s) {
+      if (!act[s]) continue;
+      const zdouble zn = zc[s] * zp[s] * a_prev /
+                         (a * b_prev * (zp[s] - zc[s]) + zp[s] * a_prev * (1.0 + sg[s] * a));
+      const zdouble as = a * zn / zc[s];
+      const zdouble bs = bk * (zn / zc[s]) * (zn / zc[s]);
+      zdouble *xs = Xs + (size_t)s * n, *ps = Ps + (size_t)s * n;
+      for (int i = 0; i < n; ++i) {
+        xs[i] += as * ps[i];
+        ps[i] = zn * r[i] + bs * ps[i];
+      }
+      zp[s] = zc[s];
+      zc[s] = zn;
+      iters[s] = k + 1;
+      if (resid) resid[s] = cabs(zn) * rnorm / bn;
+      if (cabs(zn) * rnorm <= eps * bn) {
+        act[s] = 0;
+        nact--;
+      }
+    }
+    for (int i = 0; i < n; ++i) p[i] = r[i] + bk * p[i];
+    a_prev = a;
+    b_prev = bk;
+    rr = rr_new;
+    if (rr == 0.0) break;
+  }
+  free(sg); free(Ps); free(r); free(p); free(q); free(zc); free(zp); free(act);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Special functions for contour rules                                 */
+/* ------------------------------------------------------------------ */
+/* Complete elliptic K(M), K(1-M) at M = exp(-2 pi L) by the AGM
+ * (utils/psfun_d_utils_mod.f90:37-99, including the L > 10 asymptotic). */
+void or_ellipkkp(double L, double *K) {
+  const double pi = 4.0 * atan(1.0);
+  if (L > 10.0) {
+    K[0] = pi / 2.0;
+    K[1] = pi * L + log(4.0);
+    return;
+  }
+  const double M = exp(-2.0 * pi * L);
+  for (int pass = 0; pass < 2; ++pass) {
+    double a0 = 1.0, b0 = pass == 0 ? sqrt(1.0 - M) : sqrt(M), a1 = a0, mm = 1.0;
+    int i1 = 0;
+    while (mm > DBL_EPSILON) {
+      a1 = (a0 + b0) / 2.0;
+      const double b1 = sqrt(a0 * b0);
+      const double c1 = (a0 - b0) / 2.0;
+      i1++;
+      mm = ldexp(1.0, i1) * c1 * c1;
+      a0 = a1;
+      b0 = b1;
+    }
+    K[pass] = pi / (2.0 * a1);
+  }
+}
+
+/* Horner evaluation sum_i c_i x^(i-1)  (utils/psfun_d_utils_mod.f90:119-138) */
+double or_horner(int nc, const double *c, double x) {
+  double r = 0.0;
+  for (int i = nc - 1; i >= 0; --i) r = r * x + c[i];
+  return r;
+}
+
+/* Jacobi sn, cn, dn at complex u, parameter m = exp(-2 pi L) (flag = 1) or
+ * m = L directly (flag = 0), by descending Landen transformations
+ * (utils/psfun_z_utils_mod.f90:39-107).  As in the reference, the argument
+ * reflected for Im u > K'/2 is not used afterwards (lines 76-77, 88). */
+static void z_ellipj(zdouble u, double L, zdouble *sn, zdouble *cn, zdouble *dn, int flag) {
+  const double pi = 4.0 * atan(1.0);
+  static const double coeffs[7] = {0.0, 1.0, 2.0, 5.0, 14.0, 42.0, 132.0};
+  double m, K[2];
+  int high;
+  zdouble ucp = u;
+  if (flag) {
+    or_ellipkkp(L, K);
+    high = cimag(ucp) > K[1] / 2.0;
+    if (high) ucp = I * K[1] - ucp;
+    m = exp(-2.0 * pi * L);
+  } else {
+    high = 0;
+    m = L;
+  }
+  (void)ucp;
+  if (m < 4.0 * DBL_EPSILON) {
+    const zdouble s = csin(u), c = ccos(u);
+    *sn = s + m / 4.0 * (s * c - u) * c;
+    *cn = c + m / 4.0 * (-s * c + u) * s;
+    *dn = 1.0 + m / 4.0 * (c * c - s * s - 1.0);
+  } else {
+    double kappa;
+    if (m > 1e-3)
+      kappa = (1.0 - sqrt(1.0 - m)) / (1.0 + sqrt(1.0 - m));
+    else
+      kappa = or_horner(7, coeffs, m / 4.0);
+    const double mu = kappa * kappa;
+    const zdouble v = u / (1.0 + kappa);
+    zdouble sn1, cn1, dn1;
+    z_ellipj(v, mu, &sn1, &cn1, &dn1, 0);
+    const zdouble den = 1.0 + kappa * sn1 * sn1;
+    *sn = (1.0 + kappa) * sn1 / den;
+    *cn = cn1 * dn1 / den;
+    *dn = (1.0 - kappa * sn1 * sn1) / den;
+  }
+  if (high) {
+    const zdouble snh = *sn, cnh = *cn, dnh = *dn;
+    *sn = -1.0 / (sqrt(m) * snh);
+    *cn = I * dnh / (sqrt(m) * snh);
+    *dn = I * cnh / snh;
+  }
+}
+
+void or_ellipj(double ure, double uim, double L, double *out /* sn, cn, dn as re,im pairs */) {
+  zdouble sn, cn, dn;
+  z_ellipj(ure + I * uim, L, &sn, &cn, &dn, 1);
+  out[0] = creal(sn); out[1] = cimag(sn);
+  out[2] = creal(cn); out[3] = cimag(cn);
+  out[4] = creal(dn); out[5] = cimag(dn);
+}
+
+typedef void (*zfun_t)(double re, double im, double *out_re, double *out_im);
+
+/* HHT 2008 Method 1 (quadratures/psfun_z_quadrules_mod.f90:63-83).
+ * xi, c: N complex (interleaved). */
+int or_hhtmethod1(zfun_t zfun, int N, double r1, double r2, double *xi, double *c, double *eta, double *sign) {
+  const double pi = 4.0 * atan(1.0);
+  const double mineig = fmin(r1, r2), maxeig = fmax(r1, r2);
+  const double cnd = (sqrt(maxeig / mineig) - 1.0) / (sqrt(maxeig / mineig) + 1.0);
+  const double L = -log(cnd) / pi;
+  double K[2];
+  or_ellipkkp(L, K);
+  for (int j = 1; j <= N; ++j) {
+    const zdouble t = 0.5 * I * K[1] - K[0] + (2.0 * (j - 1) + 1.0) * K[0] / N;
+    zdouble u, cn, dn;
+    z_ellipj(t, L, &u, &cn, &dn, 1);
+    const zdouble x = sqrt(mineig * maxeig) * ((1.0 / cnd + u) / (1.0 / cnd - u));
+    double fr, fi;
+    zfun(creal(x), cimag(x), &fr, &fi);
+    const zdouble cc = ((fr + I * fi) / x) * (cn * dn) / ((1.0 / cnd - u) * (1.0 / cnd - u));
+    xi[2 * (j - 1)] = creal(x);
+    xi[2 * (j - 1) + 1] = cimag(x);
+    c[2 * (j - 1)] = creal(cc);
+    c[2 * (j - 1) + 1] = cimag(cc);
+  }
+  *eta = -(4.0 * K[0] * sqrt(mineig * maxeig)) / (cnd * pi * N);
+  *sign = -1.0;
+  return 0;
+}
+
+/* Modified Talbot contour for exp(-t x), x >= 0 (Trefethen, Weideman &
+ * Schmelzer, BIT 46 (2006) 653-670):  z(th) = N (0.5017 th cot(0.6407 th)
+ * - 0.6122 + 0.2645 i th), midpoints th_j = -pi + (j - 1/2) 2 pi / N,
+ * exp(-t x) ~ Re sum_j w_j (z_j/t + x)^{-1}, w_j = e^{z_j} z'(th_j)/(i N t).
+ * Returned as shifts sigma_j = z_j/t (solve (A + sigma_j I) v = x) and
+ * weights w_j. */
+int or_talbot(int N, double t, double *sigma, double *w) {
+  const double pi = 4.0 * atan(1.0);
+  for (int j = 1; j <= N; ++j) {
+    const double th = -pi + (j - 0.5) * 2.0 * pi / N;
+    const double a = 0.6407 * th;
+    const double cot = cos(a) / sin(a);
+    const zdouble z = N * (0.5017 * th * cot - 0.6122 + 0.2645 * I * th);
+    const double sn = sin(a);
+    const zdouble dz = N * (0.5017 * cot - 0.5017 * 0.6407 * th / (sn * sn) + 0.2645 * I);
+    const zdouble ww = cexp(z) * dz / (I * N * t);
+    sigma[2 * (j - 1)] = creal(z) / t;
+    sigma[2 * (j - 1) + 1] = cimag(z) / t;
+    w[2 * (j - 1)] = creal(ww);
+    w[2 * (j - 1) + 1] = cimag(ww);
+  }
+  return 0;
+}
+
+/* Quadrature apply.  v_j = (A + sigma_j I)^{-1} x by or_mscg, then
+ *   form 0 (resolvent sum): y = gamma * Re( sum_j w_j v_j )
+ *   form 1 (HHT method 1):  y = gamma * A * Im( sum_j w_j v_j ). */
+int or_quad_apply(int n, const int *rp, const int *ci, const double *va, int nshift, const double *sigma,
+                  const double *w, double gamma, int form, const double *x, double *y, double eps, int itmax,
+                  int *iters) {
+  double *X = (double *)malloc(sizeof(double) * 2 * (size_t)nshift * n);
+  or_mscg(n, rp, ci, va, nshift, sigma, x, X, eps, itmax, iters, NULL);
+  double *acc = (double *)calloc((size_t)n, sizeof(double));
+  for (int s = 0; s < nshift; ++s) {
+    const zdouble ws = w[2 * s] + I * w[2 * s + 1];
+    const zdouble *xs = (const zdouble *)X + (size_t)s * n;
+    for (int i = 0; i < n; ++i) {
+      const zdouble t = ws * xs[i];
+      acc[i] += (form == 0) ? creal(t) : cimag(t);
+    }
+  }
+  if (form == 0) {
+    for (int i = 0; i < n; ++i) y[i] = gamma * acc[i];
+  } else {
+    or_spmv(n, rp, ci, va, gamma, acc, 0.0, y);
+  }
+  free(X);
+  free(acc);
+  return 0;
+}

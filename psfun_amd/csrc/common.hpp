@@ -1,0 +1,213 @@
+// Shared definitions for the gfx950 f(A)x library: context, workspace,
+// status handling, wave/block/grid reductions and the XCD-aware work split.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <math.h>
+#include <mutex>
+#include <vector>
+
+#include "../../include/psfun_hip.h"
+
+#define PSFUN_BS 256           // threads per block for the streaming kernels
+#define PSFUN_MAX_GRID 4096    // upper bound on reduction grids (partials slots)
+#define PSFUN_NCOUNTERS 64     // grid-reduction completion counters
+#define PSFUN_PINNED_BYTES (1 << 20)
+#define PSFUN_PROF_NK 16       // kernel ids for psfun_prof_*
+#define PSFUN_PROF_MAXPAIRS 8192
+
+#define PSFUN_HIP_TRY(call)                               \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) {                               \
+      psfun_record_hip_error(e_, #call, __FILE__, __LINE__); \
+      return PSFUN_ERR_HIP;                               \
+    }                                                     \
+  } while (0)
+
+#define PSFUN_TRY(call)           \
+  do {                            \
+    int s_ = (call);              \
+    if (s_ != PSFUN_SUCCESS) return s_; \
+  } while (0)
+
+void psfun_record_hip_error(hipError_t e, const char *what, const char *file, int line);
+
+// Counter slots: each kernel family owns one so that back-to-back kernels on
+// one stream never share a live counter.  The last block of a grid resets its
+// counter to zero, so no memset is ever needed between launches.
+enum CounterSlot {
+  CNT_REDUCE = 0,
+  CNT_LZ_SPMV = 1,
+  CNT_LZ_UPD = 2,
+  CNT_CG_A = 3,
+  CNT_CG_B = 4,
+  CNT_MS_A = 5,
+  CNT_MS_B = 6,
+  CNT_AR = 7,
+  CNT_AR2 = 8,
+  CNT_GEN = 9,
+};
+
+struct psfun_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  double *d_partials = nullptr;   // PSFUN_MAX_GRID * 8 doubles
+  unsigned *d_counters = nullptr; // PSFUN_NCOUNTERS
+  double *d_scalar = nullptr;     // 64 doubles of scalar results
+  void *h_pinned = nullptr;       // pinned host staging
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // grow-only workspace
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  // optional per-launch timing of the hot kernels (psfun_prof_*)
+  int prof_on = 0;
+  std::vector<hipEvent_t> prof_ev;       // event pool, 2 per timed launch
+  std::vector<int> prof_kid;             // kernel id of each pending pair
+  std::vector<double> prof_bytes_pending;
+  int prof_used = 0;
+  double prof_ms[PSFUN_PROF_NK] = {};
+  double prof_bytes[PSFUN_PROF_NK] = {};
+  int64_t prof_count[PSFUN_PROF_NK] = {};
+};
+
+// Timing hooks: bracket one launch of kernel `kid` (with `bytes` algorithmic
+// bytes) by two events when profiling is on.  psfun_prof_collect() folds the
+// finished pairs into the per-kernel totals (call after a synchronize).
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes);
+int psfun_prof_end(psfun_ctx *c);
+int psfun_prof_collect(psfun_ctx *c);
+
+struct psfun_csr {
+  psfun_ctx *ctx = nullptr;
+  int n = 0;
+  int64_t nnz = 0;
+  int *rowptr = nullptr;
+  int *colind = nullptr;
+  double *val = nullptr;
+  bool owns = false;
+  int lanes = 8;   // lanes per row for the vector-CSR kernels
+  int max_row = 0;
+};
+
+// Device workspace: returns a pointer to at least `bytes` bytes (256-byte
+// aligned).  Grows by reallocation; previous contents are not preserved.
+int psfun_workspace(psfun_ctx *ctx, size_t bytes, void **ptr);
+
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Number of blocks for a grid-stride kernel over `chunks` work items.
+// Always a multiple of 8 (one share per XCD) and at most PSFUN_MAX_GRID.
+static inline int psfun_grid(const psfun_ctx *ctx, int64_t chunks, int per_cu = 16) {
+  int64_t g = (int64_t)ctx->num_cu * per_cu;
+  if (g > PSFUN_MAX_GRID) g = PSFUN_MAX_GRID;
+  if (chunks < g) g = chunks;
+  g = (g + 7) / 8 * 8;
+  if (g < 8) g = 8;
+  return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over a block of BS threads; the result is valid in every thread.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  constexpr int NW = BS / 64;
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) t += lds[i];
+  return t;
+}
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double *lds) {
+  constexpr int NW = BS / 64;
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double t = lds[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) t = fmax(t, lds[i]);
+  return t;
+}
+
+// Deterministic grid reduction, "last block finishes" form.  Each block
+// publishes NV partial values; the block that arrives last sums the partials
+// in block order and returns true (every thread of that block sees the totals
+// in out[0..NV-1]).  The counter is reset for the next launch.
+template <int BS, int NV, bool MAX = false>
+__device__ __forceinline__ bool grid_reduce_last(const double (&mine)[NV], double *partials,
+                                                 unsigned *counter, double (&out)[NV]) {
+  __shared__ double red_lds[BS / 64 + 1];
+  __shared__ int is_last;
+  double blk[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) blk[q] = MAX ? block_max<BS>(mine[q], red_lds) : block_sum<BS>(mine[q], red_lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) partials[(size_t)q * gridDim.x + blockIdx.x] = blk[q];
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double acc = MAX ? 0.0 : 0.0;
+    const volatile double *p = partials + (size_t)q * gridDim.x;
+    // fixed order: thread t sums blocks t, t+BS, ... then a fixed tree
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) acc = MAX ? fmax(acc, p[b]) : acc + p[b];
+    out[q] = MAX ? block_max<BS>(acc, red_lds) : block_sum<BS>(acc, red_lds);
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+// XCD-aware split of `nchunks` chunks over the grid: the 8 XCDs receive
+// contiguous eighths of the chunk range (blocks are dispatched round-robin
+// over XCDs, block b -> XCD b % 8), so neighbouring rows share an L2.
+struct XcdRange {
+  int64_t begin, end, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(int64_t nchunks) {
+  const int G = gridDim.x;          // multiple of 8
+  const int xcd = blockIdx.x & 7;
+  const int lb = blockIdx.x >> 3;
+  const int gp = G >> 3;
+  int64_t lo = nchunks * xcd / 8, hi = nchunks * (xcd + 1) / 8;
+  XcdRange r;
+  r.begin = lo + lb;
+  r.end = hi;
+  r.stride = gp;
+  return r;
+}

@@ -1,0 +1,352 @@
+// Context, workspace, CSR handles and device vectors of the C-ABI.
+#include "common.hpp"
+
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+
+static thread_local char g_last_hip_error[512];
+
+void psfun_record_hip_error(hipError_t e, const char *what, const char *file, int line) {
+  snprintf(g_last_hip_error, sizeof(g_last_hip_error), "%s: %s (%s:%d)", hipGetErrorString(e), what,
+           file, line);
+}
+
+extern "C" const char *psfun_strerror(int status) {
+  switch (status) {
+    case PSFUN_SUCCESS: return "success";
+    case PSFUN_ERR_INVALID_ARG: return "invalid argument";
+    case PSFUN_ERR_INVALID_ISTOP: return "invalid stopping criterion (istop must be 1 or 2)";
+    case PSFUN_ERR_INVALID_VECT_STATE: return "vector not allocated";
+    case PSFUN_ERR_INVALID_ARGS_COMBINATION: return "invalid combination of arguments";
+    case PSFUN_ERR_NOT_SQUARE: return "matrix is not square";
+    case PSFUN_ERR_UNSUPPORTED: return "function/variant not available on the device";
+    case PSFUN_ERR_ALLOC: return "allocation failed";
+    case PSFUN_ERR_HIP: return g_last_hip_error[0] ? g_last_hip_error : "HIP runtime error";
+    case PSFUN_ERR_NO_DEVICE: return "no HIP device";
+    case PSFUN_ERR_BREAKDOWN: return "solver breakdown";
+    default: return "unknown status";
+  }
+}
+
+extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
+  if (!out) return PSFUN_ERR_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PSFUN_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_HIP_TRY(hipSetDevice(device));
+  psfun_ctx *c = new psfun_ctx();
+  c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->num_cu = cus;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_partials, sizeof(double) * PSFUN_MAX_GRID * 8);
+  if (e == hipSuccess) e = hipMalloc(&c->d_counters, sizeof(unsigned) * PSFUN_NCOUNTERS);
+  if (e == hipSuccess) e = hipMemset(c->d_counters, 0, sizeof(unsigned) * PSFUN_NCOUNTERS);
+  if (e == hipSuccess) e = hipMalloc(&c->d_scalar, sizeof(double) * 64);
+  if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, PSFUN_PINNED_BYTES, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    psfun_record_hip_error(e, "psfun_ctx_create", __FILE__, __LINE__);
+    psfun_ctx_destroy(c);
+    return PSFUN_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
+  if (!c) return PSFUN_SUCCESS;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->own_stream) hipStreamSynchronize(c->own_stream);
+  if (c->ws) hipFree(c->ws);
+  if (c->d_partials) hipFree(c->d_partials);
+  if (c->d_counters) hipFree(c->d_counters);
+  if (c->d_scalar) hipFree(c->d_scalar);
+  if (c->h_pinned) hipHostFree(c->h_pinned);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  for (auto &e : c->prof_ev) hipEventDestroy(e);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_ctx_set_stream(psfun_ctx_t c, void *s) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_ctx_synchronize(psfun_ctx_t c) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_ctx_num_cus(psfun_ctx_t c, int *n) {
+  if (!c || !n) return PSFUN_ERR_INVALID_ARG;
+  *n = c->num_cu;
+  return PSFUN_SUCCESS;
+}
+
+int psfun_workspace(psfun_ctx *c, size_t bytes, void **ptr) {
+  bytes = align_up(bytes < 256 ? 256 : bytes, 256);
+  if (bytes > c->ws_bytes) {
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->ws) PSFUN_HIP_TRY(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    size_t want = align_up(bytes + bytes / 8, 1 << 20);
+    if (hipMalloc(&c->ws, want) != hipSuccess) {
+      (void)hipGetLastError();
+      if (hipMalloc(&c->ws, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        c->ws = nullptr;
+        return PSFUN_ERR_ALLOC;
+      }
+      want = bytes;
+    }
+    c->ws_bytes = want;
+  }
+  *ptr = c->ws;
+  return PSFUN_SUCCESS;
+}
+
+static int choose_lanes(int64_t n, int64_t nnz) {
+  double avg = n > 0 ? (double)nnz / (double)n : 1.0;
+  int l = 2;
+  while (l < avg && l < 64) l <<= 1;
+  return l;
+}
+
+extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int32_t *rowptr,
+                                const int32_t *colind, const double *val, int base, int mem,
+                                psfun_csr_t *out) {
+  if (!c || !out || n < 0 || nnz < 0 || n > INT32_MAX - 1 || nnz > INT32_MAX) return PSFUN_ERR_INVALID_ARG;
+  if (base != 0 && base != 1) return PSFUN_ERR_INVALID_ARG;
+  if ((n > 0 && !rowptr) || (nnz > 0 && (!colind || !val))) return PSFUN_ERR_INVALID_ARG;
+  *out = nullptr;
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  psfun_csr *A = new psfun_csr();
+  A->ctx = c;
+  A->n = (int)n;
+  A->nnz = nnz;
+  if (mem == PSFUN_MEM_DEVICE) {
+    if (base != 0) {
+      delete A;
+      return PSFUN_ERR_INVALID_ARG;
+    }
+    A->rowptr = const_cast<int *>(rowptr);
+    A->colind = const_cast<int *>(colind);
+    A->val = const_cast<double *>(val);
+    A->owns = false;
+    A->max_row = -1;
+  } else if (mem == PSFUN_MEM_HOST) {
+    std::vector<int> rp(rowptr, rowptr + n + 1), ci(colind, colind + nnz);
+    if (rp[0] != base || rp[n] - base != nnz) {
+      delete A;
+      return PSFUN_ERR_INVALID_ARG;
+    }
+    int mx = 0;
+    for (int64_t i = 0; i <= n; ++i) rp[i] -= base;
+    for (int64_t i = 0; i < n; ++i) {
+      int len = rp[i + 1] - rp[i];
+      if (len < 0) {
+        delete A;
+        return PSFUN_ERR_INVALID_ARG;
+      }
+      mx = std::max(mx, len);
+    }
+    for (int64_t j = 0; j < nnz; ++j) {
+      ci[j] -= base;
+      if (ci[j] < 0 || ci[j] >= n) {
+        delete A;
+        return PSFUN_ERR_INVALID_ARG;
+      }
+    }
+    A->max_row = mx;
+    hipError_t e = hipMalloc(&A->rowptr, sizeof(int) * (n + 1));
+    if (e == hipSuccess) e = hipMalloc(&A->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
+    if (e == hipSuccess) e = hipMalloc(&A->val, sizeof(double) * std::max<int64_t>(nnz, 1));
+    if (e == hipSuccess) e = hipMemcpy(A->rowptr, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(A->colind, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz) e = hipMemcpy(A->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice);
+    A->owns = true;
+    if (e != hipSuccess) {
+      psfun_record_hip_error(e, "psfun_csr_create", __FILE__, __LINE__);
+      psfun_csr_destroy(A);
+      return PSFUN_ERR_HIP;
+    }
+  } else {
+    delete A;
+    return PSFUN_ERR_INVALID_ARG;
+  }
+  A->lanes = choose_lanes(n, nnz);
+  *out = A;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_destroy(psfun_csr_t A) {
+  if (!A) return PSFUN_SUCCESS;
+  if (A->owns) {
+    hipSetDevice(A->ctx->device);
+    hipStreamSynchronize(A->ctx->stream);
+    if (A->rowptr) hipFree(A->rowptr);
+    if (A->colind) hipFree(A->colind);
+    if (A->val) hipFree(A->val);
+  }
+  delete A;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_set_values(psfun_csr_t A, const double *val) {
+  if (!A || (!val && A->nnz)) return PSFUN_ERR_INVALID_ARG;
+  if (A->owns) {
+    if (A->nnz)
+      PSFUN_HIP_TRY(hipMemcpyAsync(A->val, val, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice,
+                                   A->ctx->stream));
+  } else {
+    A->val = const_cast<double *>(val);
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_set_lanes(psfun_csr_t A, int lanes) {
+  if (!A) return PSFUN_ERR_INVALID_ARG;
+  if (lanes == 0) {
+    A->lanes = choose_lanes(A->n, A->nnz);
+    return PSFUN_SUCCESS;
+  }
+  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) return PSFUN_ERR_INVALID_ARG;
+  A->lanes = lanes;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes) {
+  if (!A) return PSFUN_ERR_INVALID_ARG;
+  if (n) *n = A->n;
+  if (nnz) *nnz = A->nnz;
+  if (lanes) *lanes = A->lanes;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_vec_alloc(psfun_ctx_t c, int64_t n, double **d) {
+  if (!c || !d || n < 0) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  if (hipMalloc((void **)d, sizeof(double) * (n > 0 ? n : 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    *d = nullptr;
+    return PSFUN_ERR_ALLOC;
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_vec_free(psfun_ctx_t c, double *d) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  if (d) {
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    PSFUN_HIP_TRY(hipFree(d));
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_vec_upload(psfun_ctx_t c, double *d, const double *h, int64_t n) {
+  if (!c || n < 0 || (n && (!d || !h))) return PSFUN_ERR_INVALID_ARG;
+  if (n) PSFUN_HIP_TRY(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_vec_download(psfun_ctx_t c, double *h, const double *d, int64_t n) {
+  if (!c || n < 0 || (n && (!d || !h))) return PSFUN_ERR_INVALID_ARG;
+  if (n) PSFUN_HIP_TRY(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSFUN_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Per-kernel timing
+// ---------------------------------------------------------------------------
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes) {
+  if (!c->prof_on) return PSFUN_SUCCESS;
+  if (c->prof_used + 1 >= PSFUN_PROF_MAXPAIRS) {  // pool full: fold finished pairs first
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    PSFUN_TRY(psfun_prof_collect(c));
+  }
+  const size_t need = 2 * (size_t)(c->prof_used + 1);
+  while (c->prof_ev.size() < need) {
+    hipEvent_t e;
+    PSFUN_HIP_TRY(hipEventCreate(&e));
+    c->prof_ev.push_back(e);
+  }
+  if (c->prof_kid.size() < (size_t)c->prof_used + 1) {
+    c->prof_kid.resize(c->prof_used + 1);
+    c->prof_bytes_pending.resize(c->prof_used + 1);
+  }
+  c->prof_kid[c->prof_used] = kid;
+  c->prof_bytes_pending[c->prof_used] = bytes;
+  PSFUN_HIP_TRY(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
+  return PSFUN_SUCCESS;
+}
+
+int psfun_prof_end(psfun_ctx *c) {
+  if (!c->prof_on) return PSFUN_SUCCESS;
+  PSFUN_HIP_TRY(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
+  c->prof_used++;
+  return PSFUN_SUCCESS;
+}
+
+int psfun_prof_collect(psfun_ctx *c) {
+  for (int i = 0; i < c->prof_used; ++i) {
+    float ms = 0.f;
+    PSFUN_HIP_TRY(hipEventSynchronize(c->prof_ev[2 * i + 1]));
+    PSFUN_HIP_TRY(hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+    const int k = c->prof_kid[i];
+    if (k >= 0 && k < PSFUN_PROF_NK) {
+      c->prof_ms[k] += ms;
+      c->prof_bytes[k] += c->prof_bytes_pending[i];
+      c->prof_count[k] += 1;
+    }
+  }
+  c->prof_used = 0;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_prof_enable(psfun_ctx_t c, int on) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  if (!on && c->prof_used) {
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    PSFUN_TRY(psfun_prof_collect(c));
+  }
+  c->prof_on = on ? 1 : 0;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_prof_reset(psfun_ctx_t c) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  PSFUN_TRY(psfun_prof_collect(c));
+  for (int k = 0; k < PSFUN_PROF_NK; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_bytes[k] = 0.0;
+    c->prof_count[k] = 0;
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_prof_read(psfun_ctx_t c, int kid, double *ms, int64_t *count, double *bytes) {
+  if (!c || kid < 0 || kid >= PSFUN_PROF_NK) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  PSFUN_TRY(psfun_prof_collect(c));
+  if (ms) *ms = c->prof_ms[kid];
+  if (count) *count = c->prof_count[kid];
+  if (bytes) *bytes = c->prof_bytes[kid];
+  return PSFUN_SUCCESS;
+}

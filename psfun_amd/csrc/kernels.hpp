@@ -1,0 +1,7 @@
+// Internal launch helpers shared between the translation units of the library.
+#pragma once
+#include "common.hpp"
+
+int launch_spmv(psfun_ctx *c, const psfun_csr *A, double alpha, const double *x, double beta, double *y);
+int launch_reduce(psfun_ctx *c, int op, int64_t n, const double *x, const double *y, double *d_out);
+int launch_axpby(psfun_ctx *c, int64_t n, double alpha, const double *x, double beta, double *y);

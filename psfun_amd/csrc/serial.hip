@@ -1,0 +1,64 @@
+// The f(H) step as a stand-alone call (psfun_d_serial%apply on a dense array,
+// serial/psfun_d_serial_mod.F90:172-523), evaluated on the device.
+#include "common.hpp"
+#include "fexp.hpp"
+#include "kernels.hpp"
+
+// y = exp(s*T) x for symmetric tridiagonal T (d: k, e: k-1), one wavefront.
+template <int R>
+__global__ __launch_bounds__(64) void k_serial_tridiag(int k, const double *d, const double *e, double s,
+                                                       const double *x, double *y) {
+  __shared__ double coef[PSFUN_CHEB_CAP];
+  const int lane = threadIdx.x;
+  double v[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane * R + q;
+    v[q] = (i < k) ? x[i] : 0.0;
+  }
+  wave_exp_tridiag<R>(k, d, e, s, v, coef);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane * R + q;
+    if (i < k) y[i] = v[q];
+  }
+}
+
+int launch_serial_tridiag(psfun_ctx *c, int k, const double *d, const double *e, double s, const double *x,
+                          double *y) {
+  if (k <= 64) hipLaunchKernelGGL(k_serial_tridiag<1>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
+  else if (k <= 128) hipLaunchKernelGGL(k_serial_tridiag<2>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
+  else if (k <= 256) hipLaunchKernelGGL(k_serial_tridiag<4>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
+  else hipLaunchKernelGGL(k_serial_tridiag<8>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_serial_apply_host(psfun_ctx_t c, int fname, int variant, double scaling, int k,
+                                       const double *H, const double *x, double *y) {
+  if (!c || k < 0 || (k && (!H || !x || !y))) return PSFUN_ERR_INVALID_ARG;
+  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+  if (k == 0) return PSFUN_SUCCESS;
+  if (k > 512) return PSFUN_ERR_INVALID_ARG;
+  if (variant != PSFUN_VAR_CHBSYM) return PSFUN_ERR_UNSUPPORTED;
+  // CHBSYM: H must be symmetric tridiagonal (the Lanczos T_k).
+  std::vector<double> buf(3 * (size_t)k, 0.0);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) {
+      const double h = H[(size_t)j * k + i];
+      if (std::abs(i - j) > 1 && h != 0.0) return PSFUN_ERR_INVALID_ARG;
+      if (std::abs(i - j) == 1 && h != H[(size_t)i * k + j]) return PSFUN_ERR_INVALID_ARG;
+    }
+  for (int i = 0; i < k; ++i) buf[i] = H[(size_t)i * k + i];
+  for (int i = 0; i + 1 < k; ++i) buf[k + i] = H[(size_t)i * k + i + 1];
+  for (int i = 0; i < k; ++i) buf[2 * k + i] = x[i];
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace(c, sizeof(double) * 4 * (size_t)k, &ws));
+  double *dd = (double *)ws;
+  PSFUN_HIP_TRY(hipMemcpyAsync(dd, buf.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, c->stream));
+  PSFUN_TRY(launch_serial_tridiag(c, k, dd, dd + k, scaling, dd + 2 * k, dd + 3 * k));
+  PSFUN_HIP_TRY(hipMemcpyAsync(y, dd + 3 * k, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSFUN_SUCCESS;
+}

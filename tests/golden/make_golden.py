@@ -1,0 +1,124 @@
+"""Writes the golden fixtures under tests/golden/ (run in the build container).
+
+Inputs and expected outputs only; nothing here is imported at test time.
+  * laplacian50.npz   -- the reference's example matrix
+                         (examples/polynomial/laplacian.mtx, 50x50 1-D
+                         Laplacian, diagonal -2, off-diagonals 1) as 1-based
+                         CSR, plus exp(A)*1 and the shifted-system solution of
+                         examples/shiftedkrylov/shifted.inp ((-A + I) x = 1)
+                         from scipy.linalg.
+  * expm_small.npz    -- symmetric tridiagonal and dense Hessenberg test
+                         matrices with scipy.linalg.expm(s*H) applied to e1.
+  * quad_small.npz    -- a 10x10-grid Dirichlet Laplacian with sqrtm(A) x and
+                         expm(-A) x (scipy.linalg) for the quadrature paths.
+  * frechet_small.npz -- a small anisotropic operator with the dense
+                         Frechet-derivative gradient of <g, exp(-tA) x>.
+"""
+import os
+import sys
+
+import numpy as np
+import scipy.io
+import scipy.linalg as sla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from psfun_amd import matgen  # noqa: E402
+
+REF_MTX = "/root/reference/examples/polynomial/laplacian.mtx"
+
+
+def laplacian50():
+    A = scipy.io.mmread(REF_MTX).tocsr()  # symmetric MM, text
+    A.sort_indices()
+    n = A.shape[0]
+    ones = np.ones(n)
+    Ad = A.toarray()
+    y_exp = sla.expm(Ad) @ ones
+    # shifted.inp: eta = -1, zeta = 1, b = 1  ->  (-A + I) x = b
+    x_shift = np.linalg.solve(-Ad + np.eye(n), ones)
+    np.savez(os.path.join(HERE, "laplacian50.npz"), rowptr=A.indptr.astype(np.int32) + 1,
+             colind=A.indices.astype(np.int32) + 1, val=A.data.astype(np.float64), y_exp=y_exp,
+             x_shift=x_shift)
+
+
+def _mp_exp_tridiag_e1(d, e, s, dps=80):
+    """exp(s T) e1 for symmetric tridiagonal T by its Taylor series in
+    `dps`-digit arithmetic (mpmath): every component, including the tiny
+    trailing ones, correct to double precision."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    k = len(d)
+    dm = [mp.mpf(float(v)) * s for v in d]
+    em = [mp.mpf(float(v)) * s for v in e]
+    v = [mp.mpf(0)] * k
+    v[0] = mp.mpf(1)
+    acc = list(v)
+    tol = mp.mpf(10) ** (-(dps - 10))
+    for m in range(1, 4000):
+        w = []
+        for i in range(k):
+            t = dm[i] * v[i]
+            if i > 0:
+                t += em[i - 1] * v[i - 1]
+            if i < k - 1:
+                t += em[i] * v[i + 1]
+            w.append(t / m)
+        v = w
+        acc = [a + b for a, b in zip(acc, v)]
+        if max(abs(t) for t in v) < tol:
+            break
+    return np.array([float(a) for a in acc])
+
+
+def expm_small():
+    rng = np.random.default_rng(20240601)
+    out = {}
+    for k in (1, 2, 5, 17, 40, 64, 65, 130):
+        d = rng.uniform(-8.0, 0.0, k)
+        e = rng.uniform(0.1, 2.0, max(k - 1, 0))
+        for s in (1.0, -0.5):
+            out[f"tri_{k}_{s}_d"] = d
+            out[f"tri_{k}_{s}_e"] = e
+            out[f"tri_{k}_{s}_y"] = _mp_exp_tridiag_e1(d, e, s)
+    for k in (3, 12, 30):
+        H = np.triu(rng.normal(size=(k, k)), -1)
+        E = sla.expm(H)
+        out[f"hess_{k}_H"] = H
+        out[f"hess_{k}_E"] = E
+    np.savez(os.path.join(HERE, "expm_small.npz"), **out)
+
+
+def quad_small():
+    A = matgen.laplacian((10, 10))
+    Ad = matgen.to_scipy(A).toarray()
+    x = matgen.uniform_pm1(7, 100)
+    np.savez(os.path.join(HERE, "quad_small.npz"), rowptr=A[0], colind=A[1], val=A[2], x=x,
+             y_sqrt=np.real(sla.sqrtm(Ad)) @ x, y_expm=sla.expm(-Ad) @ x)
+
+
+def frechet_small():
+    m, t = 8, 0.1
+    A = matgen.anisotropic_2d(m, seed=11)
+    Ad = matgen.to_scipy(A).toarray()
+    n = m * m
+    x = matgen.normal(12, n)
+    g = matgen.normal(13, n)
+    # L = <g, exp(-tA) x>;  dL/dA = -t * L_exp(-t A^T, g x^T)  (Frechet adjoint),
+    # sampled on the sparsity pattern; dL/dx = exp(-t A^T) g; dL/dt = -<g, A exp(-tA) x>
+    _, Lf = sla.expm_frechet(-t * Ad.T, np.outer(g, x))
+    gA = -t * Lf
+    rp, ci, _ = A
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    y = sla.expm(-t * Ad) @ x
+    np.savez(os.path.join(HERE, "frechet_small.npz"), rowptr=A[0], colind=A[1], val=A[2], x=x, g=g, t=t,
+             y=y, grad_val=gA[rows, ci], grad_x=sla.expm(-t * Ad.T) @ g, grad_t=-g @ (Ad @ y))
+
+
+if __name__ == "__main__":
+    laplacian50()
+    expm_small()
+    quad_small()
+    frechet_small()
+    print("fixtures written to", HERE)

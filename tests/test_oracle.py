@@ -1,0 +1,290 @@
+"""Pins the CPU oracle (oracle/psfun_oracle.c) before it is trusted as the
+checker: reference known-answer values, closed forms, scipy dense references
+and brute force, each chosen so that a dropped term or wrong index fails."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import load_golden
+from oracle import oracle as O
+from psfun_amd import matgen
+
+
+# ---- examples/utils/utiltest.f90:63-87 control values ----------------------
+def test_horner_kat():
+    assert O.horner([132.0, 42.0, 14.0, 5.0, 2.0, 1.0, 0.0], 1.0) == 196.0
+    # brute-force polynomial on another point
+    c = np.array([3.0, -1.0, 0.5, 2.0])
+    x = 1.7
+    assert abs(O.horner(c, x) - sum(c[i] * x ** i for i in range(4))) < 1e-13
+
+
+def test_ellipkkp_kat():
+    K = O.ellipkkp(0.5)
+    assert abs(K[0] - 1.588191701877384) < 1e-14
+    assert abs(K[1] - 2.978718299395645) < 1e-14
+
+
+def test_ellipkkp_matches_scipy():
+    from scipy.special import ellipk, ellipkm1
+    for L in (0.05, 0.3, 1.0, 3.0, 9.5):
+        m = np.exp(-2 * np.pi * L)
+        K = O.ellipkkp(L)
+        assert abs(K[0] - ellipk(m)) < 1e-13 * ellipk(m)
+        # K(1-m) for tiny m: ellipkm1 takes m itself (no cancellation in 1-m)
+        assert abs(K[1] - ellipkm1(m)) < 1e-12 * ellipkm1(m)
+    # L > 10 asymptotic branch
+    K = O.ellipkkp(12.0)
+    assert K[0] == np.pi / 2 and abs(K[1] - (np.pi * 12 + np.log(4))) < 1e-14
+
+
+def test_ellipj_real_kat():
+    sn, cn, dn = O.ellipj(1.0, 0.5)
+    assert abs(sn - 0.838274911024583) < 1e-14
+    assert abs(cn - 0.545247809300255) < 1e-14
+    assert abs(dn - 0.984699634947678) < 1e-14
+
+
+def test_ellipj_complex_kat():
+    sn, cn, dn = O.ellipj(0.5j, 1.0)
+    assert abs(sn - 0.521141424070547j) < 1e-14
+    assert abs(cn - 1.127647278133671) < 1e-14
+    assert abs(dn - 1.000253555731494) < 1e-14
+
+
+def test_ellipj_real_matches_scipy():
+    from scipy.special import ellipj
+    for L in (0.1, 0.5, 2.0):
+        m = np.exp(-2 * np.pi * L)
+        for u in (0.1, 0.7, 1.3):
+            sn, cn, dn = O.ellipj(u, L)
+            s2, c2, d2, _ = ellipj(u, m)
+            assert abs(sn - s2) < 1e-13 and abs(cn - c2) < 1e-13 and abs(dn - d2) < 1e-13
+
+
+def test_ellipj_complex_identities():
+    # sn^2 + cn^2 = 1 and dn^2 + m sn^2 = 1 hold for complex arguments too
+    for L in (0.2, 0.8):
+        m = np.exp(-2 * np.pi * L)
+        for u in (0.3 + 0.2j, -0.9 + 0.5j, 1.1 + 0.05j):
+            sn, cn, dn = O.ellipj(u, L)
+            assert abs(sn * sn + cn * cn - 1) < 1e-12
+            assert abs(dn * dn + m * sn * sn - 1) < 1e-12
+
+
+# ---- small f(T) step -------------------------------------------------------
+def test_exp_tridiag_against_scipy_fixtures():
+    g = load_golden("expm_small.npz")
+    for k in (1, 2, 5, 17, 40, 64, 65, 130):
+        for s in (1.0, -0.5):
+            d, e, y = g[f"tri_{k}_{s}_d"], g[f"tri_{k}_{s}_e"], g[f"tri_{k}_{s}_y"]
+            e1 = np.zeros(k)
+            e1[0] = 1
+            got = O.exp_tridiag(d, e, s, e1)
+            assert np.linalg.norm(got - y) <= 1e-12 * np.linalg.norm(y)
+            # the trailing component drives the stopping test: the eigen-decomposition
+            # gets it to an absolute (not relative) accuracy of a few ulps of ||y||
+            assert abs(got[-1] - y[-1]) <= 1e-14 * np.linalg.norm(y), (k, s, got[-1], y[-1])
+
+
+def test_exp_tridiag_2x2_closed_form():
+    a, b, c = -1.0, 0.3, -2.0
+    lam = np.linalg.eigvalsh(np.array([[a, b], [b, c]]))
+    d, e = np.array([a, c]), np.array([b])
+    y = O.exp_tridiag(d, e, 1.0, np.array([1.0, 0.0]))
+    # exp(T) e1 via the spectral formula for 2x2 symmetric matrices
+    l1, l2 = lam
+    E = (np.exp(l1) * (np.array([[a, b], [b, c]]) - l2 * np.eye(2)) -
+         np.exp(l2) * (np.array([[a, b], [b, c]]) - l1 * np.eye(2))) / (l1 - l2)
+    assert np.allclose(y, E[:, 0], rtol=1e-14, atol=1e-15)
+
+
+def test_dense_expm_against_scipy():
+    g = load_golden("expm_small.npz")
+    for k in (3, 12, 30):
+        E = O.dense_expm(g[f"hess_{k}_H"])
+        assert np.linalg.norm(E - g[f"hess_{k}_E"]) <= 1e-12 * np.linalg.norm(g[f"hess_{k}_E"])
+    # nilpotent: exp(N) = I + N + N^2/2
+    N = np.diag([1.0, 2.0], 1)
+    assert np.allclose(O.dense_expm(N), np.eye(3) + N + N @ N / 2, rtol=0, atol=1e-15)
+
+
+# ---- Lanczos / Arnoldi ------------------------------------------------------
+def _dirichlet_case(dims, seed, t=1.0):
+    A = matgen.laplacian(dims, scale=-t)   # the reference ignores fun%scaling: pass -tA
+    x = matgen.uniform_pm1(seed, int(np.prod(dims)))
+    return A, x, matgen.dst_expm_laplacian(dims, x, t)
+
+
+@pytest.mark.parametrize("dims", [(100, 100), (12, 10, 9)])
+def test_lanczos_dst_closed_form(dims):
+    A, x, yex = _dirichlet_case(dims, 1)
+    r = O.lanczos(A, x, 1e-10, 30 if len(dims) == 2 else 50, 1)
+    assert r["info"] == 0
+    rel = np.linalg.norm(r["y"] - yex) / np.linalg.norm(yex)
+    assert rel < 1e-10
+    # res(1) = ||x||, res(itx+1) = err < eps, err below eps on convergence
+    assert abs(r["res"][0] - np.linalg.norm(x)) < 1e-12 * np.linalg.norm(x)
+    assert len(r["res"]) == r["iter"] + 1 and r["res"][-1] == r["err"] < 1e-10
+
+
+def test_lanczos_reference_example_laplacian50():
+    g = load_golden("laplacian50.npz")
+    A = (g["rowptr"] - 1, g["colind"] - 1, g["val"])
+    r = O.lanczos(A, np.ones(50), 1e-6, 100, 1)   # lanczostest.F90:213-216 overrides
+    rel = np.linalg.norm(r["y"] - g["y_exp"]) / np.linalg.norm(g["y_exp"])
+    assert rel < 1e-6 and r["err"] < 1e-6
+
+
+def test_lanczos_matches_dense_expm_with_fscale():
+    A = matgen.laplacian((7, 6))
+    Ad = matgen.to_scipy(A).toarray()
+    x = matgen.uniform_pm1(3, 42)
+    r = O.lanczos(A, x, 1e-12, 40, 1, fscale=-0.3)
+    y = sla.expm(-0.3 * Ad) @ x
+    assert np.linalg.norm(r["y"] - y) < 1e-11 * np.linalg.norm(y)
+
+
+def test_lanczos_maxit_drops_to_nl_minus_2():
+    # istop = 2: no estimate, the result uses T(1:nl-2) and nl-2 basis vectors
+    A, x, _ = _dirichlet_case((20, 20), 5)
+    nl = 8
+    r = O.lanczos(A, x, 1e-10, nl, 2)
+    assert r["iter"] == nl - 1 and r["err"] == 0.0
+    # brute force: build the basis by dense Lanczos and assemble with k = nl-2
+    Ad = matgen.to_scipy(A).toarray()
+    V = np.zeros((400, nl))
+    V[:, 0] = x / np.linalg.norm(x)
+    al, be = [], []
+    w = Ad @ V[:, 0]
+    al.append(w @ V[:, 0])
+    w = w - al[0] * V[:, 0]
+    for i in range(1, nl - 1):
+        be.append(np.linalg.norm(w))
+        V[:, i] = w / be[-1]
+        w = Ad @ V[:, i]
+        al.append(w @ V[:, i])
+        w = w - al[-1] * V[:, i] - be[-1] * V[:, i - 1]
+    k = nl - 2
+    T = np.diag(al[:k]) + np.diag(be[:k - 1], 1) + np.diag(be[:k - 1], -1)
+    yk = sla.expm(T)[:, 0]
+    y = np.linalg.norm(x) * V[:, :k] @ yk
+    assert np.linalg.norm(r["y"] - y) < 1e-12 * np.linalg.norm(y)
+
+
+def test_lanczos_breakdown_on_eigenvector():
+    # x = eigenvector of the 1-D Laplacian -> beta_1 = 0 at the first check
+    m = 30
+    A = matgen.laplacian((m,), scale=-1.0)
+    k = 3
+    x = np.sin(np.arange(1, m + 1) * k * np.pi / (m + 1))
+    lam = -(2 - 2 * np.cos(k * np.pi / (m + 1)))
+    r = O.lanczos(A, x, 1e-10, 20, 1)
+    assert r["iter"] == 1
+    assert np.linalg.norm(r["y"] - np.exp(lam) * x) < 1e-12 * np.linalg.norm(x)
+
+
+def test_lanczos_invalid_istop():
+    A = matgen.laplacian((5,))
+    assert O.lanczos(A, np.ones(5), 1e-8, 10, 3)["info"] == 2
+
+
+def test_arnoldi_dst_closed_form():
+    A, x, yex = _dirichlet_case((30, 30), 2)
+    r = O.arnoldi(A, x, 1e-10, 40, 1)
+    assert np.linalg.norm(r["y"] - yex) / np.linalg.norm(yex) < 1e-10
+
+
+def test_arnoldi_uses_scaling_in_spmv():
+    A = matgen.laplacian((30, 30))
+    x = matgen.uniform_pm1(2, 900)
+    yex = matgen.dst_expm_laplacian((30, 30), x, 0.5)
+    r = O.arnoldi(A, x, 1e-10, 40, 1, scaling=-0.5)
+    assert np.linalg.norm(r["y"] - yex) / np.linalg.norm(yex) < 1e-10
+
+
+# ---- shifted CG -----------------------------------------------------------
+def test_cg_shifted_reference_example():
+    g = load_golden("laplacian50.npz")
+    A = (g["rowptr"] - 1, g["colind"] - 1, g["val"])
+    # shifted.inp: eta=-1, zeta=1, DIAG, istop 2, itmax 500, eps 1e-7
+    r = O.cg_shifted(A, np.ones(50), -1.0, 1.0, eps=1e-7, itmax=500, istop=2, prec=1)
+    assert np.linalg.norm(r["x"] - g["x_shift"]) < 1e-6 * np.linalg.norm(g["x_shift"])
+    assert r["err"] <= 1e-7
+
+
+@pytest.mark.parametrize("istop", [1, 2])
+def test_cg_shifted_dense_solve(istop):
+    A = matgen.laplacian((9, 8))
+    Ad = matgen.to_scipy(A).toarray()
+    b = matgen.uniform_pm1(4, 72)
+    eta, zeta = 0.7, 0.3
+    r = O.cg_shifted(A, b, eta, zeta, eps=1e-12, itmax=500, istop=istop, want_cond=True)
+    x = np.linalg.solve(eta * Ad + zeta * np.eye(72), b)
+    assert np.linalg.norm(r["x"] - x) < 1e-9 * np.linalg.norm(x)
+    ev = np.linalg.eigvalsh(eta * Ad + zeta * np.eye(72))
+    assert abs(r["cond"] - ev[-1] / ev[0]) < 1e-6 * ev[-1] / ev[0]
+
+
+def test_cg_itmax_stops():
+    A = matgen.laplacian((20, 20))
+    r = O.cg_shifted(A, np.ones(400), 1.0, 0.0, eps=1e-30, itmax=7, istop=2)
+    assert r["iter"] == 7
+
+
+# ---- multi-shift CG, quadrature --------------------------------------------
+def test_mscg_dense_solve():
+    A = matgen.laplacian((8, 8))
+    Ad = matgen.to_scipy(A).toarray()
+    b = matgen.uniform_pm1(9, 64)
+    shifts = np.array([0.0, 0.5, 2.0 + 1.0j, -0.1 + 3.0j, 5.0 - 2.0j])
+    r = O.mscg(A, b, shifts, eps=1e-12, itmax=500)
+    for s, xs in zip(shifts, r["X"]):
+        ex = np.linalg.solve(Ad + s * np.eye(64), b)
+        assert np.linalg.norm(xs - ex) < 1e-9 * np.linalg.norm(ex)
+
+
+def test_talbot_scalar_accuracy():
+    s, w = O.talbot(16, 1.0)
+    xs = np.linspace(0, 12, 400)
+    approx = np.array([np.real(np.sum(w / (s + x))) for x in xs])
+    assert np.max(np.abs(approx - np.exp(-xs))) < 2e-9
+    s, w = O.talbot(24, 1.0)
+    approx = np.array([np.real(np.sum(w / (s + x))) for x in xs])
+    assert np.max(np.abs(approx - np.exp(-xs))) < 1e-12
+
+
+def test_talbot_quad_apply_expm():
+    g = load_golden("quad_small.npz")
+    A = (g["rowptr"], g["colind"], g["val"])
+    s, w = O.talbot(24, 1.0)
+    r = O.quad_apply(A, s, w, 1.0, 0, g["x"], eps=1e-13, itmax=500)
+    assert np.linalg.norm(r["y"] - g["y_expm"]) < 1e-10 * np.linalg.norm(g["y_expm"])
+
+
+def test_hht_method1_sqrt():
+    g = load_golden("quad_small.npz")
+    A = (g["rowptr"], g["colind"], g["val"])
+    ev = np.linalg.eigvalsh(matgen.to_scipy(A).toarray())
+    xi, c, eta, sign = O.hhtmethod1(np.sqrt, 24, [ev[0], ev[-1]])
+    assert sign == -1.0
+    r = O.quad_apply(A, sign * xi, c, eta * sign, 1, g["x"], eps=1e-14, itmax=800)
+    assert np.linalg.norm(r["y"] - g["y_sqrt"]) < 1e-9 * np.linalg.norm(g["y_sqrt"])
+
+
+def test_hht_quadraturetest_poles():
+    # examples/quadrature/quadraturetest.F90:60-67: N = 10, [0.01, 4], f = sqrt(1+z)
+    xi, c, eta, sign = O.hhtmethod1(lambda z: np.sqrt(1 + z), 10, [0.01, 4.0])
+    # method 1 places the N poles on the upper half of the contour (the
+    # imaginary part in the assembly accounts for the conjugate half)
+    assert np.all(xi.imag > 0) and eta < 0 and sign == -1.0
+    assert abs(eta - (-4 * O.ellipkkp(-np.log((np.sqrt(400) - 1) / (np.sqrt(400) + 1)) / np.pi)[0] * 0.2 /
+                      (((np.sqrt(400) - 1) / (np.sqrt(400) + 1)) * np.pi * 10))) < 1e-14
+    # the rule integrates sqrt(1+z) at scalar points of the interval (N = 10: ~1e-4)
+    for lam in (0.02, 0.5, 3.9):
+        v = eta * lam * np.imag(np.sum(c / (xi - lam)))
+        assert abs(v - np.sqrt(1 + lam)) < 1e-4
+    # and converges as N grows
+    xi, c, eta, sign = O.hhtmethod1(lambda z: np.sqrt(1 + z), 40, [0.01, 4.0])
+    for lam in (0.02, 0.5, 3.9):
+        assert abs(eta * lam * np.imag(np.sum(c / (xi - lam))) - np.sqrt(1 + lam)) < 1e-12
