@@ -69,9 +69,11 @@ const char *psfun_strerror(int status);
  * stream, a pinned host staging area and a grow-only device workspace. */
 int psfun_ctx_create(int device, psfun_ctx_t *ctx);
 int psfun_ctx_destroy(psfun_ctx_t ctx);
-/* Use an external hipStream_t (e.g. the caller's current stream); NULL restores
- * the context's own stream. */
+/* Order all work on an external hipStream_t (e.g. the caller's current
+ * stream); NULL selects the HIP null stream. */
 int psfun_ctx_set_stream(psfun_ctx_t ctx, void *hip_stream);
+/* Go back to the context's own stream. */
+int psfun_ctx_use_own_stream(psfun_ctx_t ctx);
 int psfun_ctx_synchronize(psfun_ctx_t ctx);
 /* Number of compute units of the context's device (for launch-shape tuning). */
 int psfun_ctx_num_cus(psfun_ctx_t ctx, int *num_cus);
@@ -150,6 +152,57 @@ int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scal
 int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter,
                         double *err, double *res, int *nres);
+
+/* ---- shifted solves (psb_krylov extension) -------------------------------- */
+enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1 };
+/* (eta*A + zeta*I) x = b by preconditioned CG with the semantics of
+ * psfun_dcg_vect (krylov/shifted/psfun_dcg.F90:129-373): x holds the initial
+ * guess on entry; prec NONE or DIAG (diagonal of the unshifted A); istop 1
+ * (||r||_inf <= eps(||A||_inf ||x||_inf + ||b||_inf)) or 2 (||r||_2 <= eps ||b||_2);
+ * breakdowns restart as in the reference.  cond (may be NULL) receives the
+ * extreme-eigenvalue ratio of the CG tridiagonal (lines 314-354). */
+int psfun_cg_shifted(psfun_ctx_t ctx, psfun_csr_t A, int prec, const double *b, double eta, double zeta,
+                     double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond);
+/* Multi-shift CG: (A + sigma_s I) X(:,s) = b for nshift <= 64 complex shifts
+ * (host array of (re,im) pairs), A symmetric positive definite, one shared
+ * real Krylov space.  X is device complex (interleaved), column s at
+ * X + 2*s*ldx.  Shift s stops when its residual is <= eps*||b||; iters/resid
+ * (host, may be NULL) receive each shift's iteration count and final relative
+ * residual.  No reference counterpart (psfun_dcg.F90 solves one real shift). */
+int psfun_mscg(psfun_ctx_t ctx, psfun_csr_t A, int nshift, const double *shifts, const double *b, double *X,
+               int64_t ldx, double eps, int itmax, int *iters, double *resid);
+
+/* ---- quadrature rules and special functions (host, no device needed) ------ */
+/* f evaluated at the complex point z[0] + i z[1]; result to fz[0..1]. */
+typedef void (*psfun_zfun_t)(const double *z, double *fz, void *user);
+/* sum_{i<nc} c[i] x^i  (horner, utils/psfun_d_utils_mod.f90:119-138) */
+double psfun_horner(int nc, const double *c, double x);
+/* K(m), K(1-m) at m = exp(-2 pi L)  (ellipkkp, utils/psfun_d_utils_mod.f90:37-99) */
+int psfun_ellipkkp(double L, double *K);
+/* Jacobi sn, cn, dn at complex u, m = exp(-2 pi L); out = (sn, cn, dn) as
+ * (re, im) pairs  (ellipj, utils/psfun_z_utils_mod.f90:39-107) */
+int psfun_ellipj(double u_re, double u_im, double L, double *out);
+/* Hale-Higham-Trefethen method 1 poles xi(N), weights c(N) (complex pairs),
+ * eta, sign for f on the spectrum interval rparams[0..1]
+ * (hhtmethod1, quadratures/psfun_z_quadrules_mod.f90:40-85). */
+int psfun_hhtmethod1(psfun_zfun_t zfun, void *user, int N, const double *rparams, double *xi, double *c,
+                     double *eta, double *sign);
+/* Modified Talbot rule for exp(-t z): exp(-tA)x ~ Re sum_j c_j (xi_j I + A)^{-1} x
+ * (eta = 1, sign = +1 in the psfun_z_quadrature convention).  No reference
+ * counterpart: the reference ships only hhtmethod1. */
+int psfun_talbot(int N, double t, double *xi, double *c, double *eta, double *sign);
+
+/* ---- rational quadrature apply (psfun_z_quadrature) ----------------------- */
+/* v_s = (A + sigma_s I)^{-1} x by psfun_mscg, then
+ *   form PSFUN_QUAD_RE_SUM:   y = gamma * Re(sum_s w_s v_s)
+ *   form PSFUN_QUAD_A_IM_SUM: y = gamma * A * Im(sum_s w_s v_s)   (HHT method 1)
+ * The reference stores poles/weights only (quadratures/psfun_z_quadrature_mod.F90:
+ * 42-55, 114-142); this is the apply its documentation describes
+ * (docs/pages/algorithms.md:106-125). */
+enum { PSFUN_QUAD_RE_SUM = 0, PSFUN_QUAD_A_IM_SUM = 1 };
+int psfun_quad_apply(psfun_ctx_t ctx, psfun_csr_t A, int nshift, const double *shifts, const double *weights,
+                     double gamma, int form, const double *x, double *y, double eps, int itmax, int *iters,
+                     double *resid);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,10 @@ PSFUN_FN_EXP = 0
 VARIANTS = {"CHBSYM": 0, "CHBHES": 1, "CHBGEN": 2, "GENPADE": 3, "TAYLOR": 4, "SASQ": 5}
 PSFUN_MEM_HOST = 0
 PSFUN_MEM_DEVICE = 1
+PSFUN_PREC_NONE = 0
+PSFUN_PREC_DIAG = 1
+PSFUN_QUAD_RE_SUM = 0
+PSFUN_QUAD_A_IM_SUM = 1
 
 PROF_IDS = {"lz_spmv": 0, "lz_update": 1, "lz_assemble": 2, "spmv": 3, "ms_spmv": 4, "ms_update": 5,
             "cg": 6, "batch": 7, "sddmm": 8}
@@ -44,6 +48,7 @@ SIGNATURES = {
     "psfun_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "psfun_ctx_destroy": (_i, [_vp]),
     "psfun_ctx_set_stream": (_i, [_vp, _vp]),
+    "psfun_ctx_use_own_stream": (_i, [_vp]),
     "psfun_ctx_synchronize": (_i, [_vp]),
     "psfun_ctx_num_cus": (_i, [_vp, _pi]),
     "psfun_prof_enable": (_i, [_vp, _i]),
@@ -67,7 +72,17 @@ SIGNATURES = {
     "psfun_scal": (_i, [_vp, _i64, _d, _vp]),
     "psfun_serial_apply_host": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _vp]),
     "psfun_lanczos_apply": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
+    "psfun_cg_shifted": (_i, [_vp, _vp, _i, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
+    "psfun_mscg": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
+    "psfun_horner": (_d, [_i, _vp, _d]),
+    "psfun_ellipkkp": (_i, [_d, _vp]),
+    "psfun_ellipj": (_i, [_d, _d, _d, _vp]),
+    "psfun_hhtmethod1": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _pd, _pd]),
+    "psfun_talbot": (_i, [_i, _d, _vp, _vp, _pd, _pd]),
+    "psfun_quad_apply": (_i, [_vp, _vp, _i, _vp, _vp, _d, _i, _vp, _vp, _d, _i, _vp, _vp]),
 }
+
+ZFUN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
 
 _lib = None
 

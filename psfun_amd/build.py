@@ -1,7 +1,8 @@
 """Build the gfx950 shared library libpsfun_hip.so in-tree with hipcc.
 
-Every .hip file under psfun_amd/csrc is compiled for gfx950 in parallel and
-linked into psfun_amd/libpsfun_hip.so.  Usage: python -m psfun_amd.build
+Every .hip file (device + host code) and .cpp file (host code) under
+psfun_amd/csrc is compiled with hipcc for gfx950 in parallel and linked into
+psfun_amd/libpsfun_hip.so.  Usage: python -m psfun_amd.build
 """
 import concurrent.futures as cf
 import os
@@ -25,7 +26,7 @@ FLAGS = [
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):

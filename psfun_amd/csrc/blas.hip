@@ -134,11 +134,14 @@ extern "C" int psfun_spmm(psfun_ctx_t c, psfun_csr_t A, int k, double alpha, con
 // ---------------------------------------------------------------------------
 // BLAS-1
 // ---------------------------------------------------------------------------
-// op: 0 dot(x,y), 1 sum of squares -> sqrt, 2 max |x|
+// op: 0 dot(x,y), 1 sum of squares -> sqrt, 2 max |x|.  Two launches: per-block
+// partials, then one block folds them in a fixed order (deterministic, and no
+// grid-wide atomics or fences).
 template <int OP>
-__global__ __launch_bounds__(PSFUN_BS) void k_reduce(int64_t n, const double *__restrict__ x,
-                                                     const double *__restrict__ y, double *partials,
-                                                     unsigned *counter, double *out) {
+__global__ __launch_bounds__(PSFUN_BS) void k_reduce_partials(int64_t n, const double *__restrict__ x,
+                                                              const double *__restrict__ y,
+                                                              double *__restrict__ partials) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
     const double a = x[i];
@@ -146,20 +149,31 @@ __global__ __launch_bounds__(PSFUN_BS) void k_reduce(int64_t n, const double *__
     else if (OP == 1) acc = fma(a, a, acc);
     else acc = fmax(acc, fabs(a));
   }
-  double mine[1] = {acc}, tot[1];
-  if (grid_reduce_last<PSFUN_BS, 1, OP == 2>(mine, partials, counter, tot)) {
-    if (threadIdx.x == 0) *out = (OP == 1) ? sqrt(tot[0]) : tot[0];
-  }
+  acc = (OP == 2) ? block_max<PSFUN_BS>(acc, lds) : block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+template <int OP>
+__global__ __launch_bounds__(PSFUN_BS) void k_reduce_final(const double *__restrict__ partials, int G,
+                                                           double *__restrict__ out) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < G; b += PSFUN_BS) acc = (OP == 2) ? fmax(acc, partials[b]) : acc + partials[b];
+  acc = (OP == 2) ? block_max<PSFUN_BS>(acc, lds) : block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) *out = (OP == 1) ? sqrt(acc) : acc;
 }
 
 int launch_reduce(psfun_ctx *c, int op, int64_t n, const double *x, const double *y, double *d_out) {
   const int64_t chunks = (n + PSFUN_BS - 1) / PSFUN_BS;
-  const int G = psfun_grid(c, chunks > 0 ? chunks : 1, 8);
-  double *part = c->d_partials;
-  unsigned *cnt = c->d_counters + CNT_REDUCE;
-  if (op == 0) hipLaunchKernelGGL(k_reduce<0>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part, cnt, d_out);
-  else if (op == 1) hipLaunchKernelGGL(k_reduce<1>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part, cnt, d_out);
-  else hipLaunchKernelGGL(k_reduce<2>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part, cnt, d_out);
+  const int G = psfun_grid(c, chunks > 0 ? chunks : 1, 4);
+  double *part = c->d_partials + 7 * PSFUN_MAX_GRID;
+#define RED_LAUNCH(OPV)                                                                                  \
+  hipLaunchKernelGGL(k_reduce_partials<OPV>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part);     \
+  hipLaunchKernelGGL(k_reduce_final<OPV>, dim3(1), dim3(PSFUN_BS), 0, c->stream, (const double *)part, G, d_out)
+  if (op == 0) { RED_LAUNCH(0); }
+  else if (op == 1) { RED_LAUNCH(1); }
+  else { RED_LAUNCH(2); }
+#undef RED_LAUNCH
   PSFUN_HIP_TRY(hipGetLastError());
   return PSFUN_SUCCESS;
 }

@@ -1,5 +1,10 @@
 // Shared definitions for the gfx950 f(A)x library: context, workspace,
-// status handling, wave/block/grid reductions and the XCD-aware work split.
+// status handling, wave/block reductions and the XCD-aware work split.
+//
+// Grid-wide reductions never use atomics or fences: a kernel leaves one
+// partial per block, and every consumer (the next kernel in stream order)
+// folds the partials itself in a fixed order, so results are deterministic
+// and cross-XCD visibility comes from the kernel boundary alone.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,7 +18,6 @@
 
 #define PSFUN_BS 256           // threads per block for the streaming kernels
 #define PSFUN_MAX_GRID 4096    // upper bound on reduction grids (partials slots)
-#define PSFUN_NCOUNTERS 64     // grid-reduction completion counters
 #define PSFUN_PINNED_BYTES (1 << 20)
 #define PSFUN_PROF_NK 16       // kernel ids for psfun_prof_*
 #define PSFUN_PROF_MAXPAIRS 8192
@@ -35,29 +39,12 @@
 
 void psfun_record_hip_error(hipError_t e, const char *what, const char *file, int line);
 
-// Counter slots: each kernel family owns one so that back-to-back kernels on
-// one stream never share a live counter.  The last block of a grid resets its
-// counter to zero, so no memset is ever needed between launches.
-enum CounterSlot {
-  CNT_REDUCE = 0,
-  CNT_LZ_SPMV = 1,
-  CNT_LZ_UPD = 2,
-  CNT_CG_A = 3,
-  CNT_CG_B = 4,
-  CNT_MS_A = 5,
-  CNT_MS_B = 6,
-  CNT_AR = 7,
-  CNT_AR2 = 8,
-  CNT_GEN = 9,
-};
-
 struct psfun_ctx {
   int device = 0;
   int num_cu = 256;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
-  double *d_partials = nullptr;   // PSFUN_MAX_GRID * 8 doubles
-  unsigned *d_counters = nullptr; // PSFUN_NCOUNTERS
+  double *d_partials = nullptr;   // 8 slots of PSFUN_MAX_GRID doubles
   double *d_scalar = nullptr;     // 64 doubles of scalar results
   void *h_pinned = nullptr;       // pinned host staging
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -159,46 +146,33 @@ __device__ __forceinline__ double block_max(double v, double *lds) {
   return t;
 }
 
-// Deterministic grid reduction, "last block finishes" form.  Each block
-// publishes NV partial values; the block that arrives last sums the partials
-// in block order and returns true (every thread of that block sees the totals
-// in out[0..NV-1]).  The counter is reset for the next launch.
-template <int BS, int NV, bool MAX = false>
-__device__ __forceinline__ bool grid_reduce_last(const double (&mine)[NV], double *partials,
-                                                 unsigned *counter, double (&out)[NV]) {
-  __shared__ double red_lds[BS / 64 + 1];
-  __shared__ int is_last;
-  double blk[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) blk[q] = MAX ? block_max<BS>(mine[q], red_lds) : block_sum<BS>(mine[q], red_lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) partials[(size_t)q * gridDim.x + blockIdx.x] = blk[q];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double acc = MAX ? 0.0 : 0.0;
-    const volatile double *p = partials + (size_t)q * gridDim.x;
-    // fixed order: thread t sums blocks t, t+BS, ... then a fixed tree
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) acc = MAX ? fmax(acc, p[b]) : acc + p[b];
-    out[q] = MAX ? block_max<BS>(acc, red_lds) : block_sum<BS>(acc, red_lds);
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
 // XCD-aware split of `nchunks` chunks over the grid: the 8 XCDs receive
 // contiguous eighths of the chunk range (blocks are dispatched round-robin
 // over XCDs, block b -> XCD b % 8), so neighbouring rows share an L2.
 struct XcdRange {
   int64_t begin, end, stride;
 };
+// Sum of G partials in a fixed order (thread t takes t, t+BS, ..., then a
+// fixed tree): every block that calls it gets the bit-identical total.
+template <int BS>
+__device__ __forceinline__ double block_sum_partials(const double *__restrict__ P, int G, double *lds) {
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < G; b += BS) acc += P[b];
+  return block_sum<BS>(acc, lds);
+}
+
+// Streaming blocks 0..G-1 of a grid that may carry extra control blocks.
+__device__ __forceinline__ XcdRange xcd_range_g(int64_t nchunks, int G) {
+  const int xcd = blockIdx.x & 7;
+  const int lb = blockIdx.x >> 3;
+  const int gp = G >> 3;
+  XcdRange r;
+  r.begin = nchunks * xcd / 8 + lb;
+  r.end = nchunks * (xcd + 1) / 8;
+  r.stride = gp;
+  return r;
+}
+
 __device__ __forceinline__ XcdRange xcd_range(int64_t nchunks) {
   const int G = gridDim.x;          // multiple of 8
   const int xcd = blockIdx.x & 7;

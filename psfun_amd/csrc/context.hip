@@ -43,8 +43,6 @@ extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
     c->num_cu = cus;
   hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_partials, sizeof(double) * PSFUN_MAX_GRID * 8);
-  if (e == hipSuccess) e = hipMalloc(&c->d_counters, sizeof(unsigned) * PSFUN_NCOUNTERS);
-  if (e == hipSuccess) e = hipMemset(c->d_counters, 0, sizeof(unsigned) * PSFUN_NCOUNTERS);
   if (e == hipSuccess) e = hipMalloc(&c->d_scalar, sizeof(double) * 64);
   if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, PSFUN_PINNED_BYTES, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
@@ -67,7 +65,6 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
   if (c->own_stream) hipStreamSynchronize(c->own_stream);
   if (c->ws) hipFree(c->ws);
   if (c->d_partials) hipFree(c->d_partials);
-  if (c->d_counters) hipFree(c->d_counters);
   if (c->d_scalar) hipFree(c->d_scalar);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   for (auto &e : c->ev)
@@ -80,7 +77,13 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
 
 extern "C" int psfun_ctx_set_stream(psfun_ctx_t c, void *s) {
   if (!c) return PSFUN_ERR_INVALID_ARG;
-  c->stream = s ? (hipStream_t)s : c->own_stream;
+  c->stream = (hipStream_t)s;  // NULL selects the HIP null stream
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_ctx_use_own_stream(psfun_ctx_t c) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  c->stream = c->own_stream;
   return PSFUN_SUCCESS;
 }
 
@@ -119,10 +122,12 @@ int psfun_workspace(psfun_ctx *c, size_t bytes, void **ptr) {
   return PSFUN_SUCCESS;
 }
 
+// Lanes per row: the smallest power of two >= half the mean row length
+// (each lane then handles about two nonzeros), between 2 and 64.
 static int choose_lanes(int64_t n, int64_t nnz) {
   double avg = n > 0 ? (double)nnz / (double)n : 1.0;
   int l = 2;
-  while (l < avg && l < 64) l <<= 1;
+  while (l < 0.5 * avg && l < 64) l <<= 1;
   return l;
 }
 

@@ -1,25 +1,36 @@
 // The small f(T) step on the device: y = exp(s*T) v for a symmetric
-// tridiagonal T of order k <= 64*R, evaluated by ONE wavefront.
+// tridiagonal T of order k <= PSFUN_FT_KMAX, evaluated by ONE wavefront.
 //
 // Method: Chebyshev expansion of exp on the Gershgorin interval [lo, hi] of
-// s*T (Clenshaw-Curtis/Chebyshev series with modified-Bessel coefficients,
-// exp(c + r x) = e^{c+r} [b_0 + 2 sum_m b_m T_m(x)], b_m = e^{-r} I_m(r);
-// the b_m come from Miller's backward recurrence normalised by
-// e^{-r}(I_0 + 2 sum I_m) = 1).  The three-term recurrence t_{m+1} = 2 S t_m -
-// t_{m-1}, S = (sT - cI)/r, keeps |t_m| <= 1, and because T is tridiagonal the
-// j-th component of T_m(S) e1 is exactly zero for m < j-1.  The trailing
-// components that drive the Lanczos stopping estimate are therefore obtained
-// with an error proportional to their own size instead of to ||y||.
+// s*T, exp(c + r x) = e^{c+r} [b_0 + 2 sum_m b_m T_m(x)] with
+// b_m = e^{-r} I_m(r) (modified Bessel functions).  The b_m come from Miller's
+// backward recurrence I_{m-1} = (2m/r) I_m + I_{m+1}, normalised by the
+// identity e^{-r}(I_0 + 2 sum_m I_m) = 1.  The three-term recurrence
+// t_{m+1} = 2 S t_m - t_{m-1}, S = (sT - cI)/r, keeps |t_m| <= 1, and because
+// T is tridiagonal the j-th component of T_m(S) e1 is exactly zero for
+// m < j-1, so the trailing components that drive the Lanczos stopping
+// estimate come out with an error proportional to their own size.  Widths
+// above 2*PSFUN_CHEB_RMAX are split into equal sub-steps
+// exp(sT) = (exp(sT/p))^p.
 //
-// Lane l owns rows l*R .. l*R+R-1; neighbour rows are exchanged with one
-// shuffle up and one shuffle down per term.  The coefficient table lives in
-// LDS (`coef`, capacity `cap` doubles) and is written by lane 0 only; the
-// wave-scope fence orders it for the other lanes of the same wavefront.
+// All vectors live in LDS (one wavefront, rows i = lane, lane+64, ...), so the
+// routine costs few registers and can sit in a control block of a streaming
+// kernel without lowering that kernel's occupancy.  The wave-scope fences
+// order LDS traffic between the lanes of the wavefront.
 #pragma once
 #include "common.hpp"
 
-#define PSFUN_CHEB_CAP 2048     // coefficient table capacity (doubles)
-#define PSFUN_CHEB_RMAX 600.0   // largest half-width per sub-step
+#define PSFUN_CHEB_CAP 1024     // coefficient table capacity (doubles)
+#define PSFUN_CHEB_RMAX 400.0   // largest half-width per sub-step
+#define PSFUN_FT_KMAX 512       // largest order of T
+
+struct ChebLds {
+  double coef[PSFUN_CHEB_CAP];
+  double y[PSFUN_FT_KMAX];
+  double t0[PSFUN_FT_KMAX];
+  double t1[PSFUN_FT_KMAX];
+  int M;
+};
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -61,127 +72,88 @@ __device__ inline int cheb_exp_coefficients(double r, double *coef, int cap) {
   return M;
 }
 
-// y <- exp(s*T) y in place (y distributed as described above).  d, e point to
-// the diagonal (k) and off-diagonal (k-1) of T in global or LDS memory.
-template <int R>
-__device__ void wave_exp_tridiag(int k, const double *d, const double *e, double s, double (&y)[R],
-                                 double *coef) {
+// y <- exp(s*T) y, y = W->y[0..k-1].  d (k) and e (k-1) are the diagonal and
+// off-diagonal of T in global memory.  Called by all 64 lanes of one wave.
+__device__ void wave_exp_tridiag(int k, const double *d, const double *e, double s, ChebLds *W) {
   const int lane = threadIdx.x & 63;
-  double a[R], lo_c[R], hi_c[R];
   double glo = 1e308, ghi = -1e308;
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = lane * R + q;
-    double di = 0.0, el = 0.0, eh = 0.0;
-    if (i < k) {
-      di = s * d[i];
-      if (i > 0) el = s * e[i - 1];
-      if (i < k - 1) eh = s * e[i];
-      glo = fmin(glo, di - fabs(el) - fabs(eh));
-      ghi = fmax(ghi, di + fabs(el) + fabs(eh));
-    }
-    a[q] = di;
-    lo_c[q] = el;
-    hi_c[q] = eh;
+  for (int i = lane; i < k; i += 64) {
+    const double di = s * d[i];
+    const double el = (i > 0) ? fabs(s * e[i - 1]) : 0.0;
+    const double eh = (i < k - 1) ? fabs(s * e[i]) : 0.0;
+    glo = fmin(glo, di - el - eh);
+    ghi = fmax(ghi, di + el + eh);
   }
   glo = wave_min(glo);
   ghi = wave_max(ghi);
   const double width = ghi - glo;
   if (!(width > 1e-300)) {  // s*T is a multiple of the identity
     const double f = exp(ghi);
-#pragma unroll
-    for (int q = 0; q < R; ++q) y[q] *= f;
+    for (int i = lane; i < k; i += 64) W->y[i] *= f;
+    wave_lds_sync();
     return;
   }
   int nsub = (int)ceil(0.5 * width / PSFUN_CHEB_RMAX);
   if (nsub < 1) nsub = 1;
   const double ssub = 1.0 / nsub;
   const double c = 0.5 * (glo + ghi) * ssub, r = 0.5 * width * ssub, hs = ghi * ssub;
-  __shared__ int M_sh;
-  if (lane == 0) M_sh = cheb_exp_coefficients(r, coef, PSFUN_CHEB_CAP);
+  if (lane == 0) W->M = cheb_exp_coefficients(r, W->coef, PSFUN_CHEB_CAP);
   wave_lds_sync();
-  const int M = M_sh;
+  const int M = W->M;
   const double rinv = 1.0 / r;
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    a[q] = (a[q] * ssub - c) * rinv;
-    lo_c[q] *= ssub * rinv;
-    hi_c[q] *= ssub * rinv;
-  }
+  const double sd = s * ssub * rinv, cd = c * rinv;  // S_ii = sd*d_i - cd, S_i,i+1 = sd*e_i
   const double escale = exp(hs);
   for (int sub = 0; sub < nsub; ++sub) {
-    double t0[R], t1[R], acc[R];
-    const double b0 = coef[0];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      t0[q] = y[q];
-      acc[q] = b0 * y[q];
+    double *t0 = W->t0, *t1 = W->t1;
+    const double b0 = W->coef[0];
+    for (int i = lane; i < k; i += 64) {
+      const double v = W->y[i];
+      t0[i] = v;
+      W->y[i] = b0 * v;
     }
-    // t1 = S t0
-    {
-      const double up = __shfl_up(t0[R - 1], 1, 64);
-      const double dn = __shfl_down(t0[0], 1, 64);
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const double tm = (q == 0) ? up : t0[q - 1];
-        const double tp = (q == R - 1) ? dn : t0[q + 1];
-        t1[q] = fma(a[q], t0[q], fma(lo_c[q], tm, hi_c[q] * tp));
-      }
+    wave_lds_sync();
+    for (int i = lane; i < k; i += 64) {  // t1 = S t0
+      double v = (sd * d[i] - cd) * t0[i];
+      if (i > 0) v = fma(sd * e[i - 1], t0[i - 1], v);
+      if (i < k - 1) v = fma(sd * e[i], t0[i + 1], v);
+      t1[i] = v;
+      if (M >= 1) W->y[i] = fma(W->coef[1], v, W->y[i]);
     }
-    if (M >= 1) {
-      const double b1 = coef[1];
-#pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] = fma(b1, t1[q], acc[q]);
-    }
+    wave_lds_sync();
     for (int m = 2; m <= M; ++m) {
-      const double bm = coef[m];
-      const double up = __shfl_up(t1[R - 1], 1, 64);
-      const double dn = __shfl_down(t1[0], 1, 64);
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const double tm = (q == 0) ? up : t1[q - 1];
-        const double tp = (q == R - 1) ? dn : t1[q + 1];
-        const double st = fma(a[q], t1[q], fma(lo_c[q], tm, hi_c[q] * tp));
-        const double t2 = fma(2.0, st, -t0[q]);
-        t0[q] = t1[q];
-        t1[q] = t2;
-        acc[q] = fma(bm, t2, acc[q]);
+      const double bm = W->coef[m];
+      for (int i = lane; i < k; i += 64) {  // t0 <- 2 S t1 - t0 (row i only touches t0[i])
+        double v = (sd * d[i] - cd) * t1[i];
+        if (i > 0) v = fma(sd * e[i - 1], t1[i - 1], v);
+        if (i < k - 1) v = fma(sd * e[i], t1[i + 1], v);
+        const double t2 = fma(2.0, v, -t0[i]);
+        t0[i] = t2;
+        W->y[i] = fma(bm, t2, W->y[i]);
       }
+      wave_lds_sync();
+      double *tmp = t0;
+      t0 = t1;
+      t1 = tmp;
     }
-#pragma unroll
-    for (int q = 0; q < R; ++q) y[q] = acc[q] * escale;
+    for (int i = lane; i < k; i += 64) W->y[i] *= escale;
+    wave_lds_sync();
   }
 }
 
-// exp(s*T) e1 for k <= 64*R; returns ||y||_2 and y[k-1] in every lane and, if
-// yout != nullptr, stores y[0..k-1].
-template <int R>
-__device__ void wave_exp_tridiag_e1(int k, const double *d, const double *e, double s, double *coef,
+// exp(s*T) e1; returns ||y||_2 and y[k-1] in every lane and, if yout is not
+// null, stores y[0..k-1] to global memory.
+__device__ void wave_exp_tridiag_e1(int k, const double *d, const double *e, double s, ChebLds *W,
                                     double *yout, double &nrm, double &last) {
   const int lane = threadIdx.x & 63;
-  double y[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) y[q] = (lane == 0 && q == 0) ? 1.0 : 0.0;
-  wave_exp_tridiag<R>(k, d, e, s, y, coef);
-  double ss = 0.0, lv = 0.0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = lane * R + q;
-    if (i < k) {
-      ss = fma(y[q], y[q], ss);
-      if (i == k - 1) lv = y[q];
-      if (yout) yout[i] = y[q];
-    }
+  for (int i = lane; i < k; i += 64) W->y[i] = (i == 0) ? 1.0 : 0.0;
+  wave_lds_sync();
+  wave_exp_tridiag(k, d, e, s, W);
+  double ss = 0.0;
+  for (int i = lane; i < k; i += 64) {
+    const double v = W->y[i];
+    ss = fma(v, v, ss);
+    if (yout) yout[i] = v;
   }
   nrm = sqrt(wave_sum(ss));
-  last = wave_sum(lv);
-}
-
-// Dispatch on k (k <= 512).
-__device__ inline void wave_exp_tridiag_e1_any(int k, const double *d, const double *e, double s, double *coef,
-                                               double *yout, double &nrm, double &last) {
-  if (k <= 64) wave_exp_tridiag_e1<1>(k, d, e, s, coef, yout, nrm, last);
-  else if (k <= 128) wave_exp_tridiag_e1<2>(k, d, e, s, coef, yout, nrm, last);
-  else if (k <= 256) wave_exp_tridiag_e1<4>(k, d, e, s, coef, yout, nrm, last);
-  else wave_exp_tridiag_e1<8>(k, d, e, s, coef, yout, nrm, last);
+  last = W->y[k - 1];
 }

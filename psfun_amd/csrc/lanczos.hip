@@ -7,39 +7,91 @@
 // rs(i) = rs(1) beta |yk(i-1)| / ||yk||, exit if rs(i) < eps.
 // Assembly: y = rs(1) (yk(1) v_1 + sum_{l=2}^{itx-1} yk(l) v_l).
 //
-// Device schedule per Lanczos index j (j = i-1):
-//   k_lz_update(j):  w = u - alpha_j v_j - beta_{j-1} v_{j-1},  beta_j = ||w||,
-//                    then (last block) itx = j, breakdown test, and the
-//                    a-posteriori estimate rs(j+1) from T_j in one wavefront.
-//   k_lz_spmv(j+1):  v_{j+1} = (1/beta_j) w (written once), u = opscale*A v_{j+1}
-//                    (x gathered from w and scaled on the fly),
-//                    alpha_{j+1} = u.v_{j+1}.
-// The reference's SpMV at the exit index feeds nothing it returns, so it is
-// skipped.  Every kernel returns at once when ctl->done is set; the host only
-// polls that flag every few iterations, one chunk behind the GPU.
+// Device schedule, per Krylov index j = 1, 2, ...:
+//   k_lz_spmv(j):   every streaming block sums the partials of ||w||^2 left by
+//                   the previous kernel (fixed order: identical in all blocks),
+//                   beta = sqrt(.), writes v_j = (1/beta) w once, computes
+//                   u = opscale*A v_j gathering w and scaling on the fly, and
+//                   leaves per-block partials of alpha_j = u.v_j.  One extra
+//                   control block runs the reference's loop control for
+//                   index i = j (breakdown test and the a-posteriori estimate
+//                   from T_{j-1}) concurrently with the streaming blocks.
+//   k_lz_update(j): every block sums the alpha partials, w = u - alpha_j v_j -
+//                   beta_{j-1} v_{j-1} in place, partials of ||w||^2.
+// Scalars cross kernels only through kernel boundaries, so no grid-wide
+// atomics or fences are needed.  When the control block ends the loop at
+// index j, the SpMV/update it overlapped with feed nothing that is returned
+// (the reference's own SpMV at the exit index is unused too).  Every kernel
+// returns at once when ctl->done is set; the host polls that flag every few
+// iterations, one chunk behind the GPU.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "fexp.hpp"
 #include "lanczos.hpp"
 
+// Reference loop control at index i = j (j >= 2): beta = T(j-1, j).
+__device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, double eps, double fscale,
+                           ChebLds *W) {
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    ctl->beta[j - 1] = beta;
+    ctl->itx = j - 1;
+  }
+  if (fabs(beta) < 1e-16) {  // psfun_d_lanczos.f90:225-230
+    if (lane == 0) ctl->done = LZ_DONE_BREAKDOWN;
+    return;
+  }
+  if (istop == 1) {
+    // yk = f(T(1:j-1,1:j-1)) e1; rs(j) = rs1 * beta * |yk(j-1)| / ||yk||  (lines 276-315)
+    double nrm, last;
+    wave_exp_tridiag_e1(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
+    const double errnum = ctl->rs1 * beta * fabs(last);
+    const double rsj = errnum / nrm;
+    if (lane == 0) {
+      ctl->errnum = errnum;
+      ctl->errden = nrm;
+      ctl->rs[j] = rsj;
+      if (rsj < eps) {
+        ctl->m = j - 1;  // yk holds f(T_{j-1}) e1
+        ctl->done = LZ_DONE_CONVERGED;
+      }
+    }
+    if (rsj < eps) return;
+  }
+  if (j >= nl && lane == 0) ctl->done = LZ_DONE_MAXIT;
+}
+
 template <int L>
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restrict__ rowptr,
                                                       const int *__restrict__ col,
                                                       const double *__restrict__ val, double opscale,
-                                                      const double *__restrict__ src, const double *sigma,
+                                                      const double *__restrict__ src, int j, int G,
+                                                      const double *__restrict__ p_in, int g_in,
                                                       double *__restrict__ v_out, double *__restrict__ u_out,
-                                                      double *partials, unsigned *counter, double *alpha_out,
-                                                      LzCtl *ctl) {
+                                                      double *__restrict__ p_out, LzCtl *ctl, int nl, int istop,
+                                                      double eps, double fscale) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
-  const double sg = *sigma;
-  if (sg == 0.0) {  // zero start vector: y = 0
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = LZ_DONE_ZERO;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (blockIdx.x == G) {  // control block
+    if (threadIdx.x >= 64) return;
+    if (j == 1) {
+      if (threadIdx.x == 0) {
+        ctl->rs1 = sigma;
+        ctl->rs[1] = sigma;
+        if (sigma == 0.0) ctl->done = LZ_DONE_ZERO;
+      }
+      return;
+    }
+    __shared__ ChebLds W;
+    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W);
     return;
   }
-  const double inv = 1.0 / sg;
+  if (sigma == 0.0 && j == 1) return;
+  const double inv = 1.0 / sigma;
   constexpr int RPB = PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
-  const XcdRange xr = xcd_range(nchunks);
+  const XcdRange xr = xcd_range_g(nchunks, G);
   const int sub = threadIdx.x / L, lane = threadIdx.x % L;
   double dot = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
@@ -47,7 +99,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
     double sum = 0.0;
     if (row < n) {
       const int s = rowptr[row], e = rowptr[row + 1];
-      for (int j = s + lane; j < e; j += L) sum = fma(val[j], src[col[j]] * inv, sum);
+      for (int k = s + lane; k < e; k += L) sum = fma(val[k], src[col[k]] * inv, sum);
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
@@ -59,19 +111,30 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
       dot = fma(u, v, dot);
     }
   }
-  double mine[1] = {dot}, tot[1];
-  if (grid_reduce_last<PSFUN_BS, 1>(mine, partials, counter, tot)) {
-    if (threadIdx.x == 0) *alpha_out = tot[0];
-  }
+  dot = block_sum<PSFUN_BS>(dot, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = dot;
+}
+
+// Loop control at the last index i = nl (no SpMV needed there).
+__global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in, int g_in, int j, LzCtl *ctl,
+                                                int nl, int istop, double eps, double fscale) {
+  __shared__ ChebLds W;
+  if (ctl->done) return;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < g_in; b += 64) acc += p_in[b];
+  const double beta = sqrt(wave_sum(acc));
+  lz_control(ctl, j, beta, nl, istop, eps, fscale, &W);
 }
 
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restrict__ u,
                                                         const double *__restrict__ vj,
-                                                        const double *__restrict__ vjm1, int j, int nl,
-                                                        int istop, double eps, double fscale,
-                                                        double *partials, unsigned *counter, LzCtl *ctl) {
+                                                        const double *__restrict__ vjm1, int j,
+                                                        const double *__restrict__ p_in, int g_in,
+                                                        double *__restrict__ p_out, LzCtl *ctl) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
-  const double al = ctl->alpha[j];
+  const double al = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha[j] = al;
   const double be = (j > 1) ? ctl->beta[j - 1] : 0.0;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
@@ -80,52 +143,31 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restric
     u[i] = w;
     acc = fma(w, w, acc);
   }
-  double mine[1] = {acc}, tot[1];
-  if (!grid_reduce_last<PSFUN_BS, 1>(mine, partials, counter, tot)) return;
-  if (threadIdx.x >= 64) return;
-  // ---- one wavefront: loop control of the reference at index i = j+1 ----
-  __shared__ double coef[PSFUN_CHEB_CAP];
-  const double beta = sqrt(tot[0]);
-  const int lane = threadIdx.x;
-  if (lane == 0) {
-    ctl->beta[j] = beta;
-    ctl->itx = j;
-  }
-  if (fabs(beta) < 1e-16) {  // psfun_d_lanczos.f90:225-230
-    if (lane == 0) ctl->done = LZ_DONE_BREAKDOWN;
-    return;
-  }
-  if (istop == 1) {
-    // yk = f(T(1:j,1:j)) e1; rs(j+1) = rs1 * beta * |yk(j)| / ||yk||
-    double nrm, last;
-    wave_exp_tridiag_e1_any(j, ctl->alpha + 1, ctl->beta + 1, fscale, coef, ctl->yk, nrm, last);
-    const double errnum = ctl->rs1 * beta * fabs(last);
-    const double rsj = errnum / nrm;
-    if (lane == 0) {
-      ctl->errnum = errnum;
-      ctl->errden = nrm;
-      ctl->rs[j + 1] = rsj;
-      if (rsj < eps) {
-        ctl->done = LZ_DONE_CONVERGED;
-        ctl->m = j;  // yk holds f(T_j) e1
-      }
-    }
-    if (rsj < eps) return;
-  }
-  if (j >= nl - 1 && lane == 0) ctl->done = LZ_DONE_MAXIT;
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_sumsq_partials(int64_t n, const double *__restrict__ x,
+                                                             double *__restrict__ p_out) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
+    acc = fma(x[i], x[i], acc);
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
 }
 
 // Coefficients for the assembly when the loop did not end by convergence:
 // yk = f(T(1:itx-1,1:itx-1)) e1 (psfun_d_lanczos.f90:330-362).  The reference
 // leaves itx-1 = 0 undefined; here the 1x1 block T(1,1) is used.
 __global__ __launch_bounds__(64) void k_lz_final(LzCtl *ctl, double fscale) {
-  __shared__ double coef[PSFUN_CHEB_CAP];
+  __shared__ ChebLds W;
   const int done = ctl->done;
   if (done == LZ_DONE_CONVERGED || done == LZ_DONE_ZERO) return;
   int k = ctl->itx - 1;
   if (k < 1) k = 1;
   double nrm, last;
-  wave_exp_tridiag_e1_any(k, ctl->alpha + 1, ctl->beta + 1, fscale, coef, ctl->yk, nrm, last);
+  wave_exp_tridiag_e1(k, ctl->alpha + 1, ctl->beta + 1, fscale, &W, ctl->yk, nrm, last);
   if (threadIdx.x == 0) ctl->m = k;
 }
 
@@ -165,19 +207,16 @@ __global__ void k_lz_init(LzCtl *ctl) {
 }
 
 // ---------------------------------------------------------------------------
-int lanczos_enqueue_step_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, const double *src,
-                              const double *sigma, double *v_out, double *u_out, double *alpha_out, LzCtl *ctl) {
+static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, const double *src, int j, int G,
+                          const double *p_in, int g_in, double *v_out, double *u_out, double *p_out, LzCtl *ctl,
+                          int nl, int istop, double eps, double fscale) {
   const int L = A->lanes;
-  const int64_t chunks = ((int64_t)A->n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  const int G = psfun_grid(c, chunks);
-  double *part = c->d_partials;
-  unsigned *cnt = c->d_counters + CNT_LZ_SPMV;
   // algorithmic bytes: val+colind, rowptr, src rows, v and u written
   const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes));
-#define LZ_LAUNCH(LL)                                                                                  \
-  hipLaunchKernelGGL((k_lz_spmv<LL>), dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr, A->colind, \
-                     A->val, opscale, src, sigma, v_out, u_out, part, cnt, alpha_out, ctl)
+#define LZ_LAUNCH(LL)                                                                                         \
+  hipLaunchKernelGGL((k_lz_spmv<LL>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr, A->colind, \
+                     A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl, istop, eps, fscale)
   switch (L) {
     case 1: LZ_LAUNCH(1); break;
     case 2: LZ_LAUNCH(2); break;
@@ -218,16 +257,23 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   LzCtl *ctl = (LzCtl *)ws;
   double *V = (double *)((char *)ws + ctl_bytes);
   double *U[2] = {V + (size_t)ldv * nl, V + (size_t)ldv * (nl + 1)};
+  double *Pw = c->d_partials;                      // ||w||^2 (and ||x||^2) partials
+  double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
+
+  const int L = A->lanes;
+  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
+  const int Gs = psfun_grid(c, chunks, 4);                          // SpMV streaming blocks
+  const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);  // update blocks
 
   hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(256), 0, c->stream, ctl);
   PSFUN_HIP_TRY(hipGetLastError());
-  // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; T(1,1) = u.v_1
-  PSFUN_TRY(launch_reduce(c, 1, n, x, nullptr, &ctl->rs1));
-  PSFUN_TRY(lanczos_enqueue_step_spmv(c, A, opscale, x, &ctl->rs1, V, U[0], &ctl->alpha[1], ctl));
+  // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; partials of T(1,1) = u.v_1
+  hipLaunchKernelGGL(k_sumsq_partials, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, Pw);
+  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, U[0], Pa, ctl, nl, istop, eps, fscale));
 
-  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
   volatile int *hflag = (volatile int *)c->h_pinned;
-  const int poll = 4;
+  const int poll = 2;
   int pending = -1;  // slot with an outstanding flag copy
   for (int j = 1; j <= nl - 1; ++j) {
     double *u = U[(j - 1) & 1];
@@ -235,13 +281,15 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     const double *vjm1 = (j > 1) ? V + (size_t)ldv * (j - 2) : nullptr;
     // algorithmic bytes: u read + w written, v_j (and v_{j-1}) read
     PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n));
-    hipLaunchKernelGGL(k_lz_update, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, nl, istop, eps,
-                       fscale, c->d_partials, c->d_counters + CNT_LZ_UPD, ctl);
+    hipLaunchKernelGGL(k_lz_update, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, Pa, Gs, Pw, ctl);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
-    if (j < nl - 1) {
-      PSFUN_TRY(lanczos_enqueue_step_spmv(c, A, opscale, u, &ctl->beta[j], V + (size_t)ldv * j, U[j & 1],
-                                          &ctl->alpha[j + 1], ctl));
+    if (j + 1 <= nl - 1) {
+      PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V + (size_t)ldv * j, U[j & 1], Pa, ctl, nl,
+                               istop, eps, fscale));
+    } else {
+      hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl, istop, eps, fscale);
+      PSFUN_HIP_TRY(hipGetLastError());
     }
     if (j % poll == 0 && j < nl - 1) {
       const int slot = (j / poll) & 1;
@@ -260,8 +308,9 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   }
   hipLaunchKernelGGL(k_lz_final, dim3(1), dim3(64), 0, c->stream, ctl, fscale);
   PSFUN_HIP_TRY(hipGetLastError());
+  const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_ASSEMBLE, 0.0));
-  hipLaunchKernelGGL(k_lz_assemble, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
+  hipLaunchKernelGGL(k_lz_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
   PSFUN_HIP_TRY(hipGetLastError());
   PSFUN_TRY(psfun_prof_end(c));
 
@@ -274,7 +323,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   if (iter) *iter = itx;
   if (err) *err = e;
   if (res) {
-    for (int i = 1; i <= itx; ++i) res[i - 1] = (i == 1) ? h->rs1 : h->rs[i];
+    for (int i = 1; i <= itx; ++i) res[i - 1] = h->rs[i];
     res[itx] = e;
   }
   if (nres) *nres = itx + 1;

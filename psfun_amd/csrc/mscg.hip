@@ -1,0 +1,431 @@
+// Multi-shift CG for (A + sigma_s I) x_s = b, s = 1..S, complex shifts,
+// real SPD A, all shifts in one Krylov space and one launch sequence, and the
+// quadrature apply built on it.  No reference counterpart: the reference's
+// krylov/shifted/psfun_dcg.F90 solves one real shift, and the quadrature
+// module (quadratures/psfun_z_quadrature_mod.F90) only stores poles/weights.
+//
+// Seed: real CG on A with x0 = 0 (r_0 = p_0 = b).  Shifted iterates follow
+// the collinear-residual recurrences r_k^s = zeta_k^s r_k (Jegerlehner 1996;
+// Frommer 2003):
+//   zeta_{k+1} = zeta_k zeta_{k-1} a_{k-1} /
+//                (a_k b_{k-1} (zeta_{k-1} - zeta_k) + zeta_{k-1} a_{k-1} (1 + s a_k))
+//   alpha_k^s = a_k zeta_{k+1}/zeta_k,  beta_k^s = b_k (zeta_{k+1}/zeta_k)^2
+//   x^s += alpha_k^s p^s,  p^s = zeta_{k+1} r_{k+1} + beta_k^s p^s
+// with a_{-1} = 1, b_{-1} = 0, zeta_{-1} = zeta_0 = 1.  Shift s stops when
+// |zeta_{k+1}^s| ||r_{k+1}|| <= eps ||b||; the loop ends when every shift has
+// stopped or after itmax seed iterations.
+//
+// Per seed iteration k (all scalars are folded from per-block partials by
+// every consumer block in the same order, so they are bit-identical
+// everywhere; per-shift state is double-buffered by the parity of k):
+//   k_ms_spmv(k):  b_{k-1} = rr_k/rr_{k-1}; p_k = r_k + b_{k-1} p_{k-1} (own rows
+//                  written, neighbours formed on the fly); q = A p_k; p.q partials.
+//   k_ms_seed(k):  a_k = rr_k / (p.q); r_{k+1} = r_k - a_k q; rr_{k+1} partials.
+//   k_ms_shift(k): every shift's zeta/alpha/beta; x^s += alpha^s p^s;
+//                  p^s = zeta r + beta^s p^s for shifts still running.  The
+//                  control block commits the new shift state and the stop flag.
+// Complex vectors are stored shift-major, n (re, im) pairs per shift.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#define MS_MAX 64
+
+enum { MS_RUN = 0, MS_DONE = 1 };
+
+struct MsState {
+  double zc[MS_MAX][2];  // zeta_k
+  double zp[MS_MAX][2];  // zeta_{k-1}
+  int active[MS_MAX];
+};
+
+struct MsCtl {
+  int done;
+  int k;
+  int nshift;
+  int pad;
+  double bnorm;
+  double rr[2];   // rr_k at [k & 1]
+  double a[2];    // a_k at [k & 1]
+  double b[2];    // b_k at [k & 1]
+  double sig[MS_MAX][2];
+  MsState st[2];  // state at the start of iteration k: st[k & 1]
+  int iters[MS_MAX];
+  double resid[MS_MAX];
+};
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_init(int64_t n, const double *__restrict__ b,
+                                                      double *__restrict__ r, double *__restrict__ p_rr) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const double v = b[i];
+    r[i] = v;
+    acc = fma(v, v, acc);
+  }
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_rr[blockIdx.x] = acc;
+}
+
+__global__ void k_ms_ctl_init(MsCtl *ctl, int nshift, const double *__restrict__ shifts) {
+  const int s = threadIdx.x;
+  if (s == 0) {
+    ctl->done = MS_RUN;
+    ctl->k = 0;
+    ctl->nshift = nshift;
+    ctl->a[1] = 1.0;  // a_{-1}
+    ctl->b[1] = 0.0;  // b_{-1}
+  }
+  if (s < MS_MAX) {
+    const bool on = s < nshift;
+    ctl->st[0].zc[s][0] = 1.0;
+    ctl->st[0].zc[s][1] = 0.0;
+    ctl->st[0].zp[s][0] = 1.0;
+    ctl->st[0].zp[s][1] = 0.0;
+    ctl->st[0].active[s] = on ? 1 : 0;
+    ctl->iters[s] = 0;
+    ctl->resid[s] = on ? 1.0 : 0.0;
+    ctl->sig[s][0] = on ? shifts[2 * s] : 0.0;
+    ctl->sig[s][1] = on ? shifts[2 * s + 1] : 0.0;
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                      const double *__restrict__ val, const double *__restrict__ r,
+                                                      const double *__restrict__ p_old, double *__restrict__ p_new,
+                                                      double *__restrict__ q, int k, int G,
+                                                      const double *__restrict__ p_rr, int g_rr,
+                                                      double *__restrict__ p_pq, MsCtl *ctl) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double rr = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
+  if (blockIdx.x == G) {
+    if (threadIdx.x == 0) {
+      ctl->rr[k & 1] = rr;
+      ctl->k = k;
+      if (k == 0) ctl->bnorm = sqrt(rr);
+    }
+    return;
+  }
+  const double bk = (k > 0) ? rr / ctl->rr[(k - 1) & 1] : 0.0;
+  constexpr int RPB = PSFUN_BS / L;
+  const int64_t nchunks = (n + RPB - 1) / RPB;
+  const XcdRange xr = xcd_range_g(nchunks, G);
+  const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+  double pq = 0.0;
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * RPB + sub);
+    double sum = 0.0;
+    if (row < n) {
+      const int s = rowptr[row], e = rowptr[row + 1];
+      for (int j = s + lane; j < e; j += L) {
+        const int c = col[j];
+        const double pc = (k > 0) ? fma(bk, p_old[c], r[c]) : r[c];
+        sum = fma(val[j], pc, sum);
+      }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    if (row < n && lane == 0) {
+      const double pr = (k > 0) ? fma(bk, p_old[row], r[row]) : r[row];
+      p_new[row] = pr;
+      q[row] = sum;
+      pq = fma(pr, sum, pq);
+    }
+  }
+  pq = block_sum<PSFUN_BS>(pq, lds);
+  if (threadIdx.x == 0) p_pq[blockIdx.x] = pq;
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_seed(int n, double *__restrict__ r, const double *__restrict__ q,
+                                                      int k, const double *__restrict__ p_pq, int g_pq,
+                                                      double *__restrict__ p_rr, MsCtl *ctl) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double pq = block_sum_partials<PSFUN_BS>(p_pq, g_pq, lds);
+  const double a = ctl->rr[k & 1] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->a[k & 1] = a;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const double v = fma(-a, q[i], r[i]);
+    r[i] = v;
+    acc = fma(v, v, acc);
+  }
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_rr[blockIdx.x] = acc;
+}
+
+// x^s += alpha^s p^s;  p^s = zeta_{k+1} r_{k+1} + beta^s p^s.  At k = 0 the
+// starting values x^s = 0, p^s = b are used without being read.
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__restrict__ r,
+                                                       const double *__restrict__ b, double2 *__restrict__ X,
+                                                       int64_t ldx, double2 *__restrict__ P, int64_t ldp, int k, int G,
+                                                       const double *__restrict__ p_rr, int g_rr, MsCtl *ctl,
+                                                       double eps, int itmax) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2];
+  __shared__ int s_upd_x[MS_MAX], s_upd_p[MS_MAX];
+  if (ctl->done) return;
+  const double rr1 = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
+  const int nshift = ctl->nshift;
+  const int par = k & 1;
+  const MsState *st = &ctl->st[par];
+  const double rrk = ctl->rr[par];
+  const double ak = ctl->a[par], akm1 = ctl->a[par ^ 1], bkm1 = ctl->b[par ^ 1];
+  const double bk = rr1 / rrk;
+  const double rnorm = sqrt(rr1);
+  const int t = threadIdx.x;
+  if (t < MS_MAX) {
+    int ux = 0, up = 0;
+    cplx zn = {0.0, 0.0}, al = {0.0, 0.0}, be = {0.0, 0.0};
+    double res = 0.0;
+    if (t < nshift && st->active[t]) {
+      const cplx zc = {st->zc[t][0], st->zc[t][1]}, zp = {st->zp[t][0], st->zp[t][1]};
+      const cplx sg = {ctl->sig[t][0], ctl->sig[t][1]};
+      // denominator: a_k b_{k-1} (zp - zc) + zp a_{k-1} (1 + s a_k)
+      const cplx d1 = {ak * bkm1 * (zp.re - zc.re), ak * bkm1 * (zp.im - zc.im)};
+      const cplx one_sa = {1.0 + sg.re * ak, sg.im * ak};
+      const cplx d2 = cmul(zp, one_sa);
+      const cplx den = {d1.re + akm1 * d2.re, d1.im + akm1 * d2.im};
+      const cplx num0 = cmul(zc, zp);
+      const cplx num = {num0.re * akm1, num0.im * akm1};
+      zn = cdiv(num, den);
+      const cplx ratio = cdiv(zn, zc);
+      al = {ak * ratio.re, ak * ratio.im};
+      const cplx r2 = cmul(ratio, ratio);
+      be = {bk * r2.re, bk * r2.im};
+      res = hypot(zn.re, zn.im) * rnorm;
+      ux = 1;
+      up = (res <= eps * ctl->bnorm || k + 1 >= itmax) ? 0 : 1;
+    }
+    s_alpha[t][0] = al.re;
+    s_alpha[t][1] = al.im;
+    s_beta[t][0] = be.re;
+    s_beta[t][1] = be.im;
+    s_zn[t][0] = zn.re;
+    s_zn[t][1] = zn.im;
+    s_upd_x[t] = ux;
+    s_upd_p[t] = up;
+    if (blockIdx.x == G) {  // control block: commit the state for iteration k+1
+      MsState *nx = &ctl->st[par ^ 1];
+      if (ux) {
+        nx->zp[t][0] = st->zc[t][0];
+        nx->zp[t][1] = st->zc[t][1];
+        nx->zc[t][0] = zn.re;
+        nx->zc[t][1] = zn.im;
+        ctl->iters[t] = k + 1;
+        ctl->resid[t] = (ctl->bnorm > 0.0) ? res / ctl->bnorm : 0.0;
+      } else {
+        nx->zp[t][0] = st->zp[t][0];
+        nx->zp[t][1] = st->zp[t][1];
+        nx->zc[t][0] = st->zc[t][0];
+        nx->zc[t][1] = st->zc[t][1];
+      }
+      nx->active[t] = up;
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == G) {
+    if (t == 0) {
+      ctl->b[par] = bk;
+      int any = 0;
+      for (int s = 0; s < nshift; ++s) any |= s_upd_p[s];
+      if (!any || rr1 == 0.0) ctl->done = MS_DONE;
+    }
+    return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + t; i < n; i += (int64_t)G * PSFUN_BS) {
+    const double ri = r[i];
+    const double bi = (k == 0) ? b[i] : 0.0;
+    for (int s = 0; s < nshift; ++s) {
+      if (!s_upd_x[s]) continue;
+      double2 *xp = X + (int64_t)s * ldx + i;
+      double2 *pp = P + (int64_t)s * ldp + i;
+      const double2 pv = (k == 0) ? make_double2(bi, 0.0) : *pp;
+      const double2 xv = (k == 0) ? make_double2(0.0, 0.0) : *xp;
+      const double ar = s_alpha[s][0], ai = s_alpha[s][1];
+      *xp = make_double2(xv.x + ar * pv.x - ai * pv.y, xv.y + ar * pv.y + ai * pv.x);
+      if (s_upd_p[s]) {
+        const double br = s_beta[s][0], bim = s_beta[s][1];
+        *pp = make_double2(fma(s_zn[s][0], ri, br * pv.x - bim * pv.y), fma(s_zn[s][1], ri, br * pv.y + bim * pv.x));
+      }
+    }
+  }
+}
+
+// y = gamma * Re(sum_s w_s x^s)  (form 0)  or  t = gamma * Im(sum_s w_s x^s) (form 1).
+__global__ __launch_bounds__(PSFUN_BS) void k_quad_combine(int64_t n, int nshift, const double2 *__restrict__ X,
+                                                           int64_t ld, const double *__restrict__ w, double gamma,
+                                                           int form, double *__restrict__ y) {
+  __shared__ double sw[MS_MAX][2];
+  if (threadIdx.x < nshift) {
+    sw[threadIdx.x][0] = w[2 * threadIdx.x];
+    sw[threadIdx.x][1] = w[2 * threadIdx.x + 1];
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    double acc = 0.0;
+    for (int s = 0; s < nshift; ++s) {
+      const double2 v = X[(int64_t)s * ld + i];
+      acc += (form == 0) ? (sw[s][0] * v.x - sw[s][1] * v.y) : (sw[s][0] * v.y + sw[s][1] * v.x);
+    }
+    y[i] = gamma * acc;
+  }
+}
+
+struct MsWork {
+  MsCtl *ctl;
+  double *r, *pa, *pb, *q, *dshift, *dw, *tmp;
+  double2 *P, *X;
+  int64_t ld;
+};
+
+// One workspace holds the control block, the seed vectors, the shifted
+// directions and (optionally) the shifted solutions.
+static int ms_workspace(psfun_ctx *c, int n, int nshift, bool with_x, MsWork *w) {
+  const size_t ctl_bytes = align_up(sizeof(MsCtl), 256);
+  const int64_t ld = (int64_t)align_up((size_t)n, 64);
+  const size_t nsh = (size_t)std::max(nshift, 1);
+  const size_t bytes = ctl_bytes + sizeof(double) * (5 * (size_t)ld + 4 * MS_MAX) +
+                       sizeof(double2) * (size_t)ld * nsh * (with_x ? 2 : 1);
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace(c, bytes, &ws));
+  w->ctl = (MsCtl *)ws;
+  w->ld = ld;
+  w->r = (double *)((char *)ws + ctl_bytes);
+  w->pa = w->r + ld;
+  w->pb = w->pa + ld;
+  w->q = w->pb + ld;
+  w->tmp = w->q + ld;
+  w->dshift = w->tmp + ld;
+  w->dw = w->dshift + 2 * MS_MAX;
+  w->P = (double2 *)(w->dw + 2 * MS_MAX);
+  w->X = with_x ? w->P + (size_t)ld * nsh : nullptr;
+  return PSFUN_SUCCESS;
+}
+
+static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *shifts_host, const double *b,
+                  double2 *X, int64_t ldx, const MsWork &w, double eps, int itmax, int *iters, double *resid) {
+  const int n = A->n;
+  MsCtl *ctl = w.ctl;
+  double *r = w.r, *q = w.q;
+  double2 *P = w.P;
+  PSFUN_HIP_TRY(hipMemcpyAsync(w.dshift, shifts_host, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice,
+                               c->stream));
+  double *Ppq = c->d_partials, *Prr = c->d_partials + PSFUN_MAX_GRID;
+  const int L = A->lanes;
+  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
+  const int Gs = psfun_grid(c, chunks, 4);
+  const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
+
+  hipLaunchKernelGGL(k_ms_ctl_init, dim3(1), dim3(MS_MAX), 0, c->stream, ctl, nshift, (const double *)w.dshift);
+  hipLaunchKernelGGL(k_ms_init, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, r, Prr);
+  PSFUN_HIP_TRY(hipGetLastError());
+  volatile int *hflag = (volatile int *)c->h_pinned;
+  const int poll = 8;
+  int pending = -1;
+  double *pold = w.pa, *pnew = w.pb;
+  for (int k = 0; k < itmax; ++k) {
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_SPMV, 12.0 * A->nnz + 4.0 * (n + 1) + (k ? 32.0 : 24.0) * n));
+#define MSA_LAUNCH(LL)                                                                                           \
+  hipLaunchKernelGGL((k_ms_spmv<LL>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, \
+                     r, pold, pnew, q, k, Gs, Prr, Gu, Ppq, ctl)
+    switch (L) {
+      case 1: MSA_LAUNCH(1); break;
+      case 2: MSA_LAUNCH(2); break;
+      case 4: MSA_LAUNCH(4); break;
+      case 8: MSA_LAUNCH(8); break;
+      case 16: MSA_LAUNCH(16); break;
+      case 32: MSA_LAUNCH(32); break;
+      default: MSA_LAUNCH(64); break;
+    }
+#undef MSA_LAUNCH
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    hipLaunchKernelGGL(k_ms_seed, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, q, k, Ppq, Gs, Prr, ctl);
+    PSFUN_HIP_TRY(hipGetLastError());
+    // algorithmic bytes of the shift kernel: r read, x^s and p^s read+written per running shift
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE, 8.0 * n + 64.0 * n * nshift));
+    hipLaunchKernelGGL(k_ms_shift, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, P, w.ld, k, Gu, Prr,
+                       Gu, ctl, eps, itmax);
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    std::swap(pold, pnew);
+    if ((k + 1) % poll == 0) {
+      const int slot = ((k + 1) / poll) & 1;
+      PSFUN_HIP_TRY(hipMemcpyAsync((void *)(hflag + slot), &ctl->done, sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+      PSFUN_HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
+      if (pending >= 0) {
+        PSFUN_HIP_TRY(hipEventSynchronize(c->ev[pending]));
+        if (hflag[pending] != 0) break;
+      }
+      pending = slot;
+    }
+  }
+  MsCtl *h = (MsCtl *)((char *)c->h_pinned + 256);
+  PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < nshift; ++s) {
+    if (iters) iters[s] = h->iters[s];
+    if (resid) resid[s] = h->resid[s];
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_mscg(psfun_ctx_t c, psfun_csr_t A, int nshift, const double *shifts, const double *b,
+                          double *X, int64_t ldx, double eps, int itmax, int *iters, double *resid) {
+  if (!c || !A || nshift < 0 || nshift > MS_MAX || (nshift && !shifts)) return PSFUN_ERR_INVALID_ARG;
+  if (!b || !X) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (itmax < 0 || ldx < A->n) return PSFUN_ERR_INVALID_ARG;
+  if (A->n == 0 || nshift == 0) return PSFUN_SUCCESS;
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  if (itmax == 0) {
+    PSFUN_HIP_TRY(hipMemsetAsync(X, 0, sizeof(double2) * (size_t)ldx * nshift, c->stream));
+    return psfun_ctx_synchronize(c);
+  }
+  MsWork w;
+  PSFUN_TRY(ms_workspace(c, A->n, nshift, false, &w));
+  return ms_run(c, A, nshift, shifts, b, (double2 *)X, ldx, w, eps, itmax, iters, resid);
+}
+
+extern "C" int psfun_quad_apply(psfun_ctx_t c, psfun_csr_t A, int nshift, const double *shifts,
+                                const double *weights, double gamma, int form, const double *x, double *y,
+                                double eps, int itmax, int *iters, double *resid) {
+  if (!c || !A || nshift < 0 || nshift > MS_MAX || (nshift && (!shifts || !weights))) return PSFUN_ERR_INVALID_ARG;
+  if (form != PSFUN_QUAD_RE_SUM && form != PSFUN_QUAD_A_IM_SUM) return PSFUN_ERR_INVALID_ARG;
+  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (itmax < 1) return PSFUN_ERR_INVALID_ARG;
+  const int n = A->n;
+  if (n == 0) return PSFUN_SUCCESS;
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  if (nshift == 0) {
+    PSFUN_HIP_TRY(hipMemsetAsync(y, 0, sizeof(double) * n, c->stream));
+    return psfun_ctx_synchronize(c);
+  }
+  MsWork w;
+  PSFUN_TRY(ms_workspace(c, n, nshift, true, &w));
+  PSFUN_TRY(ms_run(c, A, nshift, shifts, x, w.X, w.ld, w, eps, itmax, iters, resid));
+  PSFUN_HIP_TRY(hipMemcpyAsync(w.dw, weights, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice, c->stream));
+  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
+  hipLaunchKernelGGL(k_quad_combine, dim3(G), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, nshift,
+                     (const double2 *)w.X, w.ld, (const double *)w.dw, gamma, form,
+                     form == PSFUN_QUAD_RE_SUM ? y : w.tmp);
+  PSFUN_HIP_TRY(hipGetLastError());
+  if (form == PSFUN_QUAD_A_IM_SUM) PSFUN_TRY(launch_spmv(c, A, 1.0, w.tmp, 0.0, y));
+  return psfun_ctx_synchronize(c);
+}

@@ -5,31 +5,18 @@
 #include "kernels.hpp"
 
 // y = exp(s*T) x for symmetric tridiagonal T (d: k, e: k-1), one wavefront.
-template <int R>
 __global__ __launch_bounds__(64) void k_serial_tridiag(int k, const double *d, const double *e, double s,
                                                        const double *x, double *y) {
-  __shared__ double coef[PSFUN_CHEB_CAP];
-  const int lane = threadIdx.x;
-  double v[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = lane * R + q;
-    v[q] = (i < k) ? x[i] : 0.0;
-  }
-  wave_exp_tridiag<R>(k, d, e, s, v, coef);
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = lane * R + q;
-    if (i < k) y[i] = v[q];
-  }
+  __shared__ ChebLds W;
+  for (int i = threadIdx.x; i < k; i += 64) W.y[i] = x[i];
+  wave_lds_sync();
+  wave_exp_tridiag(k, d, e, s, &W);
+  for (int i = threadIdx.x; i < k; i += 64) y[i] = W.y[i];
 }
 
 int launch_serial_tridiag(psfun_ctx *c, int k, const double *d, const double *e, double s, const double *x,
                           double *y) {
-  if (k <= 64) hipLaunchKernelGGL(k_serial_tridiag<1>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
-  else if (k <= 128) hipLaunchKernelGGL(k_serial_tridiag<2>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
-  else if (k <= 256) hipLaunchKernelGGL(k_serial_tridiag<4>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
-  else hipLaunchKernelGGL(k_serial_tridiag<8>, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
+  hipLaunchKernelGGL(k_serial_tridiag, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
   PSFUN_HIP_TRY(hipGetLastError());
   return PSFUN_SUCCESS;
 }
@@ -39,7 +26,7 @@ extern "C" int psfun_serial_apply_host(psfun_ctx_t c, int fname, int variant, do
   if (!c || k < 0 || (k && (!H || !x || !y))) return PSFUN_ERR_INVALID_ARG;
   if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
   if (k == 0) return PSFUN_SUCCESS;
-  if (k > 512) return PSFUN_ERR_INVALID_ARG;
+  if (k > PSFUN_FT_KMAX) return PSFUN_ERR_INVALID_ARG;
   if (variant != PSFUN_VAR_CHBSYM) return PSFUN_ERR_UNSUPPORTED;
   // CHBSYM: H must be symmetric tridiagonal (the Lanczos T_k).
   std::vector<double> buf(3 * (size_t)k, 0.0);
