@@ -153,6 +153,28 @@ int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale
                         const double *x, double *y, double eps, int itmax, int istop, int *iter,
                         double *err, double *res, int *nres);
 
+/* ---- batched Lanczos f(A)x ------------------------------------------------ */
+/* A batch is a set of ngraph independent square CSR matrices stored
+ * block-diagonally: graph g owns rows row_offsets[g]..row_offsets[g+1]-1 of one
+ * 0-based CSR (rowptr over all rows, colind GLOBAL and inside the graph's own
+ * row range).  Host arrays; copied to the device.  No reference counterpart:
+ * it runs ngraph*K independent psfun_d_lanczos instances
+ * (krylov/polynomial/psfun_d_lanczos.f90:37-400) in one launch sequence. */
+typedef struct psfun_batch *psfun_batch_t;
+int psfun_batch_create(psfun_ctx_t ctx, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
+                       const int32_t *colind, const double *val, psfun_batch_t *B);
+int psfun_batch_destroy(psfun_batch_t B);
+/* Tuning knobs (set by the autotuner): rows per block-chunk, and whether the
+ * whole launch sequence is captured once in a hipGraph and replayed. */
+int psfun_batch_set_tuning(psfun_batch_t B, int rows_per_chunk, int use_graph);
+int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, int64_t *nnz, int *rows_per_chunk, int *nchunks);
+/* Y = exp(fscale * opscale*A_g) X for every graph g and each of the K columns of
+ * the row-major (nrows x K, K <= 64) device blocks X, Y, with the reference's
+ * Lanczos stopping rule per instance.  iters/errs (host, ngraph*K, instance
+ * g*K+k; may be NULL) receive iter and err of every instance. */
+int psfun_batch_lanczos(psfun_ctx_t ctx, psfun_batch_t B, int K, double fscale, double opscale, const double *X,
+                        double *Y, double eps, int itmax, int istop, int *iters, double *errs);
+
 /* ---- shifted solves (psb_krylov extension) -------------------------------- */
 enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1 };
 /* (eta*A + zeta*I) x = b by preconditioned CG with the semantics of

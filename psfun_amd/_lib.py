@@ -72,6 +72,11 @@ SIGNATURES = {
     "psfun_scal": (_i, [_vp, _i64, _d, _vp]),
     "psfun_serial_apply_host": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _vp]),
     "psfun_lanczos_apply": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
+    "psfun_batch_create": (_i, [_vp, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "psfun_batch_destroy": (_i, [_vp]),
+    "psfun_batch_set_tuning": (_i, [_vp, _i, _i]),
+    "psfun_batch_info": (_i, [_vp, _pi, _pi64, _pi64, _pi, _pi]),
+    "psfun_batch_lanczos": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _vp, _vp]),
     "psfun_cg_shifted": (_i, [_vp, _vp, _i, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
     "psfun_mscg": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
     "psfun_horner": (_d, [_i, _vp, _d]),

@@ -1,0 +1,603 @@
+// Batched Lanczos f(A)x: many independent square CSR matrices of uneven
+// order ("graphs", concatenated block-diagonally) times a block of K vectors
+// each, i.e. ngraph*K independent runs of the reference's Lanczos
+// (krylov/polynomial/psfun_d_lanczos.f90:183-377), advanced together.
+//
+// Layout: vectors are row-major (total_rows x K) so that the K lanes that
+// own one row gather K contiguous doubles per nonzero.  Rows are cut into
+// chunks that never cross a graph; one block owns one chunk.  Every scalar of
+// instance (g, k) is folded from the per-chunk partials of graph g in a fixed
+// order by every block that needs it (deterministic, no atomics on data).
+//
+// Per Krylov index j (one launch sequence for all instances):
+//   k_bt_spmm(j):   v_j = (1/sigma) src, u = opscale*A v_j, alpha partials
+//   k_bt_update(j): alpha folded; w = u - alpha v_j - beta v_{j-1}; ||w||^2 partials
+//   k_bt_ctrl(j+1): one wavefront per instance: beta = ||w||, breakdown test,
+//                   a-posteriori estimate from T_j, stop decision.
+// Instances stop independently; a stopped instance's rows are skipped.  The
+// whole sequence (itmax steps + assembly) can be captured once in a hipGraph
+// and replayed: kernels of finished steps return at once.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "fexp.hpp"
+
+#include <map>
+#include <tuple>
+
+#define BT_KMAX 64
+
+enum { BT_RUN = 0, BT_CONV = 1, BT_BRK = 2, BT_MAXIT = 3, BT_ZERO = 4 };
+
+struct BtCtl {
+  int remaining;  // instances still running
+  int pad[63];
+};
+
+struct psfun_batch {
+  psfun_ctx *ctx = nullptr;
+  int ngraph = 0;
+  int64_t nrows = 0, nnz = 0;
+  int *rowptr = nullptr, *colind = nullptr;
+  double *val = nullptr;
+  std::vector<int64_t> row_off;
+  int rows_per_chunk = 512;
+  int nchunks = 0;
+  int *d_chunk = nullptr;    // [3*nchunks]: row0, row1, graph
+  int *d_gchunk = nullptr;   // [2*ngraph]: first chunk, end chunk
+  double *d_part = nullptr;  // partials: 2 * nchunks * BT_KMAX
+  size_t part_cap = 0;
+  int use_graph = 0;
+  std::map<std::tuple<const void *, void *, const void *, int, int, int, double, double, double>, hipGraphExec_t>
+      graphs;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+};
+
+struct BtInst {
+  double *alpha, *beta, *rs, *yk;  // [ninst][ld]
+  double *rs1, *errnum, *errden;   // [ninst]
+  int *done, *itx, *m;             // [ninst]
+  int ld;
+};
+
+// ---------------------------------------------------------------------------
+template <int KP>
+__device__ __forceinline__ double fold_graph(const double *__restrict__ P, int c0, int c1, int k) {
+  double acc = 0.0;
+  for (int c = c0; c < c1; ++c) acc += P[(int64_t)c * KP + k];
+  return acc;
+}
+
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_norm(const int *__restrict__ chunk, int K,
+                                                      const double *__restrict__ X, double *__restrict__ P) {
+  constexpr int RP = PSFUN_BS / KP;
+  __shared__ double red[PSFUN_BS];
+  const int c = blockIdx.x;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1];
+  const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  double acc = 0.0;
+  if (k < K)
+    for (int r = r0 + sub; r < r1; r += RP) {
+      const double v = X[(int64_t)r * K + k];
+      acc = fma(v, v, acc);
+    }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    double s = 0.0;
+    for (int q = 0; q < RP; ++q) s += red[q * KP + threadIdx.x];
+    P[(int64_t)c * KP + threadIdx.x] = s;
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                      const double *__restrict__ val, double opscale,
+                                                      const int *__restrict__ chunk, const int *__restrict__ gchunk,
+                                                      int K, int j, const double *__restrict__ src,
+                                                      double *__restrict__ v_out, double *__restrict__ u_out,
+                                                      const double *__restrict__ p_in, double *__restrict__ p_out,
+                                                      BtInst in, BtCtl *bctl) {
+  constexpr int RP = PSFUN_BS / KP;
+  __shared__ double s_inv[KP];
+  __shared__ int s_act[KP];
+  __shared__ int s_any;
+  __shared__ double red[PSFUN_BS];
+  if (bctl->remaining == 0) return;
+  const int c = blockIdx.x;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
+  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    const int k = threadIdx.x;
+    int act = 0;
+    double inv = 0.0;
+    if (k < K) {
+      const int I = g * K + k;
+      if (j == 1) {
+        const double sg = sqrt(fold_graph<KP>(p_in, c0, c1, k));
+        if (c == c0) {
+          in.rs1[I] = sg;
+          in.rs[(int64_t)I * in.ld + 1] = sg;
+          if (sg == 0.0) {  // zero start vector: y = 0
+            in.done[I] = BT_ZERO;
+            atomicSub(&bctl->remaining, 1);
+          }
+        }
+        act = (sg != 0.0);
+        inv = act ? 1.0 / sg : 0.0;
+      } else {
+        act = (in.done[I] == BT_RUN);
+        if (act) inv = 1.0 / in.beta[(int64_t)I * in.ld + j - 1];
+      }
+    }
+    s_inv[k] = inv;
+    s_act[k] = act;
+    if (act) s_any = 1;
+  }
+  __syncthreads();
+  const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  double dot = 0.0;
+  if (s_any && k < K && s_act[k]) {
+    const double inv = s_inv[k];
+    for (int r = r0 + sub; r < r1; r += RP) {
+      const int s = rowptr[r], e = rowptr[r + 1];
+      double sum = 0.0;
+      for (int q = s; q < e; ++q) sum = fma(val[q], src[(int64_t)col[q] * K + k] * inv, sum);
+      const double v = src[(int64_t)r * K + k] * inv;
+      const double u = opscale * sum;
+      v_out[(int64_t)r * K + k] = v;
+      u_out[(int64_t)r * K + k] = u;
+      dot = fma(u, v, dot);
+    }
+  }
+  red[threadIdx.x] = dot;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    double s = 0.0;
+    for (int q = 0; q < RP; ++q) s += red[q * KP + threadIdx.x];
+    p_out[(int64_t)c * KP + threadIdx.x] = s;
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ chunk, const int *__restrict__ gchunk,
+                                                        int K, int j, double *__restrict__ u,
+                                                        const double *__restrict__ vj, const double *__restrict__ vjm1,
+                                                        const double *__restrict__ p_in, double *__restrict__ p_out,
+                                                        BtInst in, const BtCtl *bctl) {
+  constexpr int RP = PSFUN_BS / KP;
+  __shared__ double s_al[KP], s_be[KP];
+  __shared__ int s_act[KP];
+  __shared__ double red[PSFUN_BS];
+  if (bctl->remaining == 0) return;
+  const int c = blockIdx.x;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
+  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
+  if (threadIdx.x < KP) {
+    const int k = threadIdx.x;
+    int act = 0;
+    double al = 0.0, be = 0.0;
+    if (k < K) {
+      const int I = g * K + k;
+      act = (in.done[I] == BT_RUN);
+      if (act) {
+        al = fold_graph<KP>(p_in, c0, c1, k);
+        be = (j > 1) ? in.beta[(int64_t)I * in.ld + j - 1] : 0.0;
+        if (c == c0) in.alpha[(int64_t)I * in.ld + j] = al;
+      }
+    }
+    s_al[k] = al;
+    s_be[k] = be;
+    s_act[k] = act;
+  }
+  __syncthreads();
+  const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  double acc = 0.0;
+  if (k < K && s_act[k]) {
+    const double al = s_al[k], be = s_be[k];
+    for (int r = r0 + sub; r < r1; r += RP) {
+      const int64_t i = (int64_t)r * K + k;
+      double w = fma(-al, vj[i], u[i]);
+      if (j > 1) w = fma(-be, vjm1[i], w);
+      u[i] = w;
+      acc = fma(w, w, acc);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    double s = 0.0;
+    for (int q = 0; q < RP; ++q) s += red[q * KP + threadIdx.x];
+    p_out[(int64_t)c * KP + threadIdx.x] = s;
+  }
+}
+
+// Loop control of instance blockIdx.x at the reference's index i = j
+// (beta = T(j-1, j) from the ||w||^2 partials).
+template <int KP>
+__global__ __launch_bounds__(64) void k_bt_ctrl(const int *__restrict__ gchunk, int K, int j,
+                                                const double *__restrict__ p_in, BtInst in, BtCtl *bctl, int nl,
+                                                int istop, double eps, double fscale) {
+  __shared__ ChebLds W;
+  const int I = blockIdx.x;
+  if (in.done[I] != BT_RUN) return;
+  const int g = I / K, k = I % K;
+  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
+  double acc = 0.0;
+  for (int cc = c0 + threadIdx.x; cc < c1; cc += 64) acc += p_in[(int64_t)cc * KP + k];
+  const double beta = sqrt(wave_sum(acc));
+  const int64_t base = (int64_t)I * in.ld;
+  const int lane = threadIdx.x;
+  int fin = BT_RUN;
+  if (lane == 0) {
+    in.beta[base + j - 1] = beta;
+    in.itx[I] = j - 1;
+  }
+  if (fabs(beta) < 1e-16) {
+    fin = BT_BRK;
+  } else {
+    if (istop == 1) {
+      double nrm, last;
+      wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+      const double errnum = in.rs1[I] * beta * fabs(last);
+      const double rsj = errnum / nrm;
+      if (lane == 0) {
+        in.errnum[I] = errnum;
+        in.errden[I] = nrm;
+        in.rs[base + j] = rsj;
+      }
+      if (rsj < eps) fin = BT_CONV;
+    }
+    if (fin == BT_RUN && j >= nl) fin = BT_MAXIT;
+  }
+  if (fin != BT_RUN && lane == 0) {
+    if (fin == BT_CONV) in.m[I] = j - 1;
+    in.done[I] = fin;
+    atomicSub(&bctl->remaining, 1);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_bt_final(BtInst in, double fscale) {
+  __shared__ ChebLds W;
+  const int I = blockIdx.x;
+  const int done = in.done[I];
+  if (done == BT_CONV || done == BT_ZERO) return;
+  int kk = in.itx[I] - 1;
+  if (kk < 1) kk = 1;
+  const int64_t base = (int64_t)I * in.ld;
+  double nrm, last;
+  wave_exp_tridiag_e1(kk, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+  if (threadIdx.x == 0) in.m[I] = kk;
+}
+
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict__ chunk, int K,
+                                                          const double *__restrict__ V, int64_t vstride, BtInst in,
+                                                          double *__restrict__ Y) {
+  constexpr int RP = PSFUN_BS / KP;
+  __shared__ int s_m[KP];
+  __shared__ double s_rs1[KP];
+  const int c = blockIdx.x;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
+  if (threadIdx.x < KP) {
+    const int k = threadIdx.x;
+    int m = 0;
+    double rs1 = 0.0;
+    if (k < K) {
+      const int I = g * K + k;
+      const int done = in.done[I];
+      m = in.m[I];
+      if (done == BT_CONV) m = (m - 1 < 1) ? 1 : m - 1;
+      if (done == BT_ZERO || done == BT_RUN) m = 0;
+      rs1 = in.rs1[I];
+    }
+    s_m[k] = m;
+    s_rs1[k] = rs1;
+  }
+  __syncthreads();
+  const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  if (k >= K) return;
+  const int m = s_m[k];
+  const double *yk = in.yk + (int64_t)(g * K + k) * in.ld;
+  for (int r = r0 + sub; r < r1; r += RP) {
+    const int64_t i = (int64_t)r * K + k;
+    double acc = 0.0;
+    if (m > 0) {
+      acc = yk[0] * V[i];
+      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * vstride + i], acc);
+    }
+    Y[i] = s_rs1[k] * acc;
+  }
+}
+
+__global__ void k_bt_init(BtInst in, int ninst, BtCtl *bctl) {
+  const int64_t tot = (int64_t)ninst * in.ld;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x)
+    in.rs[i] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+    in.done[i] = BT_RUN;
+    in.itx[i] = 0;
+    in.m[i] = 0;
+    in.errnum[i] = 0.0;
+    in.errden[i] = 1.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) bctl->remaining = ninst;
+}
+
+// ---------------------------------------------------------------------------
+static int bt_build_chunks(psfun_batch *B) {
+  std::vector<int> ch, gc(2 * B->ngraph);
+  for (int g = 0; g < B->ngraph; ++g) {
+    gc[2 * g] = (int)(ch.size() / 3);
+    for (int64_t r = B->row_off[g]; r < B->row_off[g + 1]; r += B->rows_per_chunk) {
+      ch.push_back((int)r);
+      ch.push_back((int)std::min<int64_t>(r + B->rows_per_chunk, B->row_off[g + 1]));
+      ch.push_back(g);
+    }
+    gc[2 * g + 1] = (int)(ch.size() / 3);
+  }
+  psfun_ctx *c = B->ctx;
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (B->d_chunk) PSFUN_HIP_TRY(hipFree(B->d_chunk));
+  B->d_chunk = nullptr;
+  B->nchunks = (int)(ch.size() / 3);
+  PSFUN_HIP_TRY(hipMalloc(&B->d_chunk, sizeof(int) * (ch.size() + gc.size() + 3)));
+  B->d_gchunk = B->d_chunk + ch.size();
+  if (!ch.empty()) PSFUN_HIP_TRY(hipMemcpy(B->d_chunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice));
+  PSFUN_HIP_TRY(hipMemcpy(B->d_gchunk, gc.data(), sizeof(int) * gc.size(), hipMemcpyHostToDevice));
+  const size_t need = 2 * (size_t)std::max(B->nchunks, 1) * BT_KMAX;
+  if (need > B->part_cap) {
+    if (B->d_part) PSFUN_HIP_TRY(hipFree(B->d_part));
+    PSFUN_HIP_TRY(hipMalloc(&B->d_part, sizeof(double) * need));
+    B->part_cap = need;
+  }
+  for (auto &kv : B->graphs) hipGraphExecDestroy(kv.second);
+  B->graphs.clear();
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
+                                  const int32_t *colind, const double *val, psfun_batch_t *out) {
+  if (!c || !out || ngraph < 0 || (ngraph && (!row_offsets || !rowptr))) return PSFUN_ERR_INVALID_ARG;
+  *out = nullptr;
+  const int64_t nrows = ngraph ? row_offsets[ngraph] : 0;
+  if (nrows > INT32_MAX - 1 || (ngraph && row_offsets[0] != 0)) return PSFUN_ERR_INVALID_ARG;
+  const int64_t nnz = nrows ? rowptr[nrows] : 0;
+  if (rowptr && rowptr[0] != 0) return PSFUN_ERR_INVALID_ARG;
+  if (nnz && (!colind || !val)) return PSFUN_ERR_INVALID_ARG;
+  for (int g = 0; g < ngraph; ++g) {
+    if (row_offsets[g + 1] < row_offsets[g]) return PSFUN_ERR_INVALID_ARG;
+    for (int64_t r = row_offsets[g]; r < row_offsets[g + 1]; ++r) {
+      if (rowptr[r + 1] < rowptr[r]) return PSFUN_ERR_INVALID_ARG;
+      for (int q = rowptr[r]; q < rowptr[r + 1]; ++q)
+        if (colind[q] < row_offsets[g] || colind[q] >= row_offsets[g + 1]) return PSFUN_ERR_INVALID_ARG;
+    }
+  }
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  psfun_batch *B = new psfun_batch();
+  B->ctx = c;
+  B->ngraph = ngraph;
+  B->nrows = nrows;
+  B->nnz = nnz;
+  B->row_off.assign(row_offsets, row_offsets + ngraph + 1);
+  hipError_t e = hipMalloc(&B->rowptr, sizeof(int) * (nrows + 1));
+  if (e == hipSuccess) e = hipMalloc(&B->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMalloc(&B->val, sizeof(double) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMemcpy(B->rowptr, rowptr, sizeof(int) * (nrows + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(B->colind, colind, sizeof(int) * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(B->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    psfun_record_hip_error(e, "psfun_batch_create", __FILE__, __LINE__);
+    psfun_batch_destroy(B);
+    return PSFUN_ERR_HIP;
+  }
+  const int st = bt_build_chunks(B);
+  if (st != PSFUN_SUCCESS) {
+    psfun_batch_destroy(B);
+    return st;
+  }
+  *out = B;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_destroy(psfun_batch_t B) {
+  if (!B) return PSFUN_SUCCESS;
+  hipSetDevice(B->ctx->device);
+  hipStreamSynchronize(B->ctx->stream);
+  for (auto &kv : B->graphs) hipGraphExecDestroy(kv.second);
+  if (B->ev_in) hipEventDestroy(B->ev_in);
+  if (B->ev_out) hipEventDestroy(B->ev_out);
+  if (B->rowptr) hipFree(B->rowptr);
+  if (B->colind) hipFree(B->colind);
+  if (B->val) hipFree(B->val);
+  if (B->d_chunk) hipFree(B->d_chunk);
+  if (B->d_part) hipFree(B->d_part);
+  delete B;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_set_tuning(psfun_batch_t B, int rows_per_chunk, int use_graph) {
+  if (!B || rows_per_chunk < 16 || rows_per_chunk > (1 << 20)) return PSFUN_ERR_INVALID_ARG;
+  B->use_graph = use_graph ? 1 : 0;
+  if (rows_per_chunk != B->rows_per_chunk) {
+    B->rows_per_chunk = rows_per_chunk;
+    return bt_build_chunks(B);
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, int64_t *nnz, int *rows_per_chunk,
+                                int *nchunks) {
+  if (!B) return PSFUN_ERR_INVALID_ARG;
+  if (ngraph) *ngraph = B->ngraph;
+  if (nrows) *nrows = B->nrows;
+  if (nnz) *nnz = B->nnz;
+  if (rows_per_chunk) *rows_per_chunk = B->rows_per_chunk;
+  if (nchunks) *nchunks = B->nchunks;
+  return PSFUN_SUCCESS;
+}
+
+template <int KP>
+static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double opscale, const double *X, double *Y,
+                      double eps, int itmax, int istop, const BtInst &in, BtCtl *bctl, double *V, double *U0,
+                      double *U1, bool poll_host) {
+  const int NC = B->nchunks;
+  const int ninst = B->ngraph * K;
+  const int64_t vs = B->nrows * K;
+  double *P0 = B->d_part, *P1 = B->d_part + (size_t)NC * BT_KMAX;
+  hipLaunchKernelGGL(k_bt_init, dim3(std::min(1024, (ninst * in.ld + 255) / 256 + 1)), dim3(256), 0, c->stream, in,
+                     ninst, bctl);
+  hipLaunchKernelGGL((k_bt_norm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, X, P0);
+  const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 24.0 * B->nrows * K;
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
+  hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val, opscale,
+                     B->d_chunk, B->d_gchunk, K, 1, X, V, U0, P0, P1, in, bctl);
+  PSFUN_TRY(psfun_prof_end(c));
+  PSFUN_HIP_TRY(hipGetLastError());
+  volatile int *hflag = (volatile int *)c->h_pinned;
+  int pending = -1;
+  for (int j = 1; j <= itmax - 1; ++j) {
+    double *u = (j & 1) ? U0 : U1;
+    const double *vj = V + (size_t)vs * (j - 1);
+    const double *vjm1 = (j > 1) ? V + (size_t)vs * (j - 2) : nullptr;
+    hipLaunchKernelGGL((k_bt_update<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, B->d_gchunk, K, j, u,
+                       vj, vjm1, P1, P0, in, bctl);
+    hipLaunchKernelGGL((k_bt_ctrl<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
+                       itmax, istop, eps, fscale);
+    if (j + 1 <= itmax - 1) {
+      PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
+      hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val,
+                         opscale, B->d_chunk, B->d_gchunk, K, j + 1, u, V + (size_t)vs * j, (j & 1) ? U1 : U0, P0, P1,
+                         in, bctl);
+      PSFUN_TRY(psfun_prof_end(c));
+    }
+    PSFUN_HIP_TRY(hipGetLastError());
+    if (poll_host && j % 2 == 0) {
+      const int slot = (j / 2) & 1;
+      PSFUN_HIP_TRY(hipMemcpyAsync((void *)(hflag + slot), &bctl->remaining, sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+      PSFUN_HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
+      if (pending >= 0) {
+        PSFUN_HIP_TRY(hipEventSynchronize(c->ev[pending]));
+        if (hflag[pending] == 0) break;
+      }
+      pending = slot;
+    }
+  }
+  hipLaunchKernelGGL(k_bt_final, dim3(ninst), dim3(64), 0, c->stream, in, fscale);
+  hipLaunchKernelGGL((k_bt_assemble<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, V, vs, in, Y);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, double opscale,
+                                   const double *X, double *Y, double eps, int itmax, int istop, int *iters,
+                                   double *errs) {
+  if (!c || !B || K < 1 || K > BT_KMAX) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!X || !Y) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (itmax < 2 || itmax > PSFUN_FT_KMAX) return PSFUN_ERR_INVALID_ARG;
+  if (B->nrows == 0 || B->ngraph == 0) return PSFUN_SUCCESS;
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  const int ninst = B->ngraph * K;
+  const int ld = itmax + 2;
+  const int64_t vs = B->nrows * K;
+  // workspace: control, per-instance arrays, basis v_1..v_{itmax-1}, two u buffers
+  const size_t inst_bytes = align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256) +
+                            align_up(sizeof(int) * (size_t)ninst * 3, 256);
+  const size_t bytes = 256 + inst_bytes + sizeof(double) * (size_t)vs * (itmax + 1);
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace(c, bytes, &ws));
+  BtCtl *bctl = (BtCtl *)ws;
+  BtInst in;
+  double *d = (double *)((char *)ws + 256);
+  in.ld = ld;
+  in.alpha = d;
+  in.beta = d + (size_t)ninst * ld;
+  in.rs = d + 2 * (size_t)ninst * ld;
+  in.yk = d + 3 * (size_t)ninst * ld;
+  in.rs1 = d + 4 * (size_t)ninst * ld;
+  in.errnum = in.rs1 + ninst;
+  in.errden = in.errnum + ninst;
+  int *ip = (int *)((char *)d + align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256));
+  in.done = ip;
+  in.itx = ip + ninst;
+  in.m = ip + 2 * ninst;
+  double *V = (double *)((char *)ws + 256 + inst_bytes);
+  double *U0 = V + (size_t)vs * (itmax - 1), *U1 = U0 + vs;
+
+  int KP = 1;
+  while (KP < K) KP <<= 1;
+  // hipGraph path: captured and replayed on the context's own stream, joined
+  // to the caller's stream by events (the null stream cannot be captured).
+  const bool graph = B->use_graph && !c->prof_on;
+  hipStream_t caller = c->stream;
+  if (graph) {
+    if (!B->ev_in) PSFUN_HIP_TRY(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+    if (!B->ev_out) PSFUN_HIP_TRY(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+    PSFUN_HIP_TRY(hipEventRecord(B->ev_in, caller));
+    PSFUN_HIP_TRY(hipStreamWaitEvent(c->own_stream, B->ev_in, 0));
+    c->stream = c->own_stream;
+  }
+  hipGraphExec_t exec = nullptr;
+  auto key = std::make_tuple((const void *)X, (void *)Y, (const void *)ws, K, itmax, istop, eps, fscale, opscale);
+  if (graph) {
+    auto it = B->graphs.find(key);
+    if (it != B->graphs.end()) exec = it->second;
+  }
+  int st = PSFUN_SUCCESS;
+  if (!exec) {
+    if (graph && hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) st = PSFUN_ERR_HIP;
+    if (st == PSFUN_SUCCESS) {
+#define BT_CASE(KPV) \
+  case KPV: st = bt_enqueue<KPV>(c, B, K, fscale, opscale, X, Y, eps, itmax, istop, in, bctl, V, U0, U1, !graph); break;
+      switch (KP) {
+        BT_CASE(1) BT_CASE(2) BT_CASE(4) BT_CASE(8) BT_CASE(16) BT_CASE(32) BT_CASE(64)
+        default: st = PSFUN_ERR_INVALID_ARG;
+      }
+#undef BT_CASE
+    }
+    if (graph) {
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (st == PSFUN_SUCCESS && e != hipSuccess) {
+        psfun_record_hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        st = PSFUN_ERR_HIP;
+      }
+      if (st == PSFUN_SUCCESS) {
+        const hipError_t e2 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+          psfun_record_hip_error(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+          st = PSFUN_ERR_HIP;
+          exec = nullptr;
+        } else {
+          B->graphs[key] = exec;
+        }
+      }
+      if (g) hipGraphDestroy(g);
+    }
+  }
+  if (st == PSFUN_SUCCESS && exec && hipGraphLaunch(exec, c->stream) != hipSuccess) st = PSFUN_ERR_HIP;
+  if (graph) {
+    c->stream = caller;
+    if (hipEventRecord(B->ev_out, c->own_stream) != hipSuccess || hipStreamWaitEvent(caller, B->ev_out, 0) != hipSuccess)
+      st = (st == PSFUN_SUCCESS) ? PSFUN_ERR_HIP : st;
+  }
+  if (st != PSFUN_SUCCESS) return st;
+  // per-instance results
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (iters || errs) {
+    std::vector<int> itv(ninst);
+    std::vector<double> en(ninst), ed(ninst);
+    PSFUN_HIP_TRY(hipMemcpy(itv.data(), in.itx, sizeof(int) * ninst, hipMemcpyDeviceToHost));
+    PSFUN_HIP_TRY(hipMemcpy(en.data(), in.errnum, sizeof(double) * ninst, hipMemcpyDeviceToHost));
+    PSFUN_HIP_TRY(hipMemcpy(ed.data(), in.errden, sizeof(double) * ninst, hipMemcpyDeviceToHost));
+    for (int i = 0; i < ninst; ++i) {
+      if (iters) iters[i] = itv[i];
+      if (errs) errs[i] = ed[i] == 0.0 ? en[i] : en[i] / ed[i];
+    }
+  }
+  return PSFUN_SUCCESS;
+}
