@@ -175,6 +175,13 @@ int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, int64_t *nnz,
 int psfun_batch_lanczos(psfun_ctx_t ctx, psfun_batch_t B, int K, double fscale, double opscale, const double *X,
                         double *Y, double eps, int itmax, int istop, int *iters, double *errs);
 
+/* Y = alpha*A_blockdiag*X + beta*Y on row-major (nrows x K) device blocks. */
+int psfun_batch_spmm(psfun_ctx_t ctx, psfun_batch_t B, int K, double alpha, const double *X, double beta, double *Y);
+/* Replace the nonzero values (same pattern) from a device array of nnz doubles. */
+int psfun_batch_set_values(psfun_batch_t B, const double *val);
+/* Device pointers of the batch's 0-based block-diagonal CSR arrays. */
+int psfun_batch_csr(psfun_batch_t B, const int32_t **rowptr, const int32_t **colind, const double **val);
+
 /* ---- shifted solves (psb_krylov extension) -------------------------------- */
 enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1 };
 /* (eta*A + zeta*I) x = b by preconditioned CG with the semantics of
@@ -222,6 +229,14 @@ int psfun_talbot(int N, double t, double *xi, double *c, double *eta, double *si
  * 42-55, 114-142); this is the apply its documentation describes
  * (docs/pages/algorithms.md:106-125). */
 enum { PSFUN_QUAD_RE_SUM = 0, PSFUN_QUAD_A_IM_SUM = 1 };
+/* Sampled derivative for the adjoint of a resolvent sum: for every nonzero q
+ * = (i, k) of the device CSR pattern (rowptr, colind; n rows),
+ *   grad[q] += scale * sum_s Re(w_s U_s[i] V_s[k]),
+ * with U_s, V_s device complex vectors (interleaved) at U + 2*s*ldu, V + 2*s*ldv
+ * and host weights w (nshift <= 64).  No reference counterpart. */
+int psfun_sddmm_shifted(psfun_ctx_t ctx, int n, const int32_t *rowptr, const int32_t *colind, int nshift,
+                        const double *weights, const double *U, int64_t ldu, const double *V, int64_t ldv,
+                        double scale, double *grad);
 int psfun_quad_apply(psfun_ctx_t ctx, psfun_csr_t A, int nshift, const double *shifts, const double *weights,
                      double gamma, int form, const double *x, double *y, double eps, int itmax, int *iters,
                      double *resid);

@@ -601,3 +601,61 @@ extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double
   }
   return PSFUN_SUCCESS;
 }
+
+// ---------------------------------------------------------------------------
+// Row-major SpMM on the batch and value replacement (used by the adjoint).
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm_plain(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                            const double *__restrict__ val,
+                                                            const int *__restrict__ chunk, int K, double alpha,
+                                                            const double *__restrict__ X, double beta,
+                                                            double *__restrict__ Y) {
+  constexpr int RP = PSFUN_BS / KP;
+  const int c = blockIdx.x;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1];
+  const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  if (k >= K) return;
+  for (int r = r0 + sub; r < r1; r += RP) {
+    double sum = 0.0;
+    for (int q = rowptr[r]; q < rowptr[r + 1]; ++q) sum = fma(val[q], X[(int64_t)col[q] * K + k], sum);
+    const int64_t i = (int64_t)r * K + k;
+    double out = alpha * sum;
+    if (beta != 0.0) out = fma(beta, Y[i], out);
+    Y[i] = out;
+  }
+}
+
+extern "C" int psfun_batch_spmm(psfun_ctx_t c, psfun_batch_t B, int K, double alpha, const double *X, double beta,
+                                double *Y) {
+  if (!c || !B || K < 1 || K > BT_KMAX || (B->nrows && (!X || !Y))) return PSFUN_ERR_INVALID_ARG;
+  if (B->nrows == 0 || B->nchunks == 0) return PSFUN_SUCCESS;
+  int KP = 1;
+  while (KP < K) KP <<= 1;
+#define SP_CASE(KPV)                                                                                       \
+  case KPV:                                                                                                \
+    hipLaunchKernelGGL((k_bt_spmm_plain<KPV>), dim3(B->nchunks), dim3(PSFUN_BS), 0, c->stream, B->rowptr, \
+                       B->colind, B->val, B->d_chunk, K, alpha, X, beta, Y);                              \
+    break;
+  switch (KP) {
+    SP_CASE(1) SP_CASE(2) SP_CASE(4) SP_CASE(8) SP_CASE(16) SP_CASE(32) SP_CASE(64)
+    default: return PSFUN_ERR_INVALID_ARG;
+  }
+#undef SP_CASE
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_set_values(psfun_batch_t B, const double *val) {
+  if (!B || (B->nnz && !val)) return PSFUN_ERR_INVALID_ARG;
+  if (B->nnz)
+    PSFUN_HIP_TRY(hipMemcpyAsync(B->val, val, sizeof(double) * B->nnz, hipMemcpyDeviceToDevice, B->ctx->stream));
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_batch_csr(psfun_batch_t B, const int32_t **rowptr, const int32_t **colind, const double **val) {
+  if (!B) return PSFUN_ERR_INVALID_ARG;
+  if (rowptr) *rowptr = B->rowptr;
+  if (colind) *colind = B->colind;
+  if (val) *val = B->val;
+  return PSFUN_SUCCESS;
+}

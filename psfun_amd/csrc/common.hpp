@@ -45,7 +45,7 @@ struct psfun_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   double *d_partials = nullptr;   // 8 slots of PSFUN_MAX_GRID doubles
-  double *d_scalar = nullptr;     // 64 doubles of scalar results
+  double *d_scalar = nullptr;     // 256 doubles of scalar results / small parameter tables
   void *h_pinned = nullptr;       // pinned host staging
   hipEvent_t ev[2] = {nullptr, nullptr};
   // grow-only workspace

@@ -43,7 +43,7 @@ extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
     c->num_cu = cus;
   hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_partials, sizeof(double) * PSFUN_MAX_GRID * 8);
-  if (e == hipSuccess) e = hipMalloc(&c->d_scalar, sizeof(double) * 64);
+  if (e == hipSuccess) e = hipMalloc(&c->d_scalar, sizeof(double) * 256);
   if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, PSFUN_PINNED_BYTES, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);

@@ -429,3 +429,50 @@ extern "C" int psfun_quad_apply(psfun_ctx_t c, psfun_csr_t A, int nshift, const 
   if (form == PSFUN_QUAD_A_IM_SUM) PSFUN_TRY(launch_spmv(c, A, 1.0, w.tmp, 0.0, y));
   return psfun_ctx_synchronize(c);
 }
+
+// grad[q] += scale * sum_s Re(w_s U_s[i] V_s[k]) for every nonzero q = (i, k)
+// of A: the sampled (SDDMM) form of the derivative of
+// Re sum_s w_s g^T (A + sigma_s I)^{-1} x with respect to the entries of A.
+__global__ __launch_bounds__(PSFUN_BS) void k_sddmm_shifted(int n, const int *__restrict__ rowptr,
+                                                            const int *__restrict__ col, int nshift,
+                                                            const double *__restrict__ w, const double2 *__restrict__ U,
+                                                            int64_t ldu, const double2 *__restrict__ V, int64_t ldv,
+                                                            double scale, double *__restrict__ grad) {
+  __shared__ double sw[MS_MAX][2];
+  if (threadIdx.x < nshift) {
+    sw[threadIdx.x][0] = w[2 * threadIdx.x];
+    sw[threadIdx.x][1] = w[2 * threadIdx.x + 1];
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const int s0 = rowptr[i], s1 = rowptr[i + 1];
+    for (int q = s0; q < s1; ++q) {
+      const int k = col[q];
+      double acc = 0.0;
+      for (int s = 0; s < nshift; ++s) {
+        const double2 a = U[(int64_t)s * ldu + i], b = V[(int64_t)s * ldv + k];
+        // w * a * b, real part
+        const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+        acc += sw[s][0] * pr - sw[s][1] * pi;
+      }
+      grad[q] = fma(scale, acc, grad[q]);
+    }
+  }
+}
+
+extern "C" int psfun_sddmm_shifted(psfun_ctx_t c, int n, const int32_t *rowptr, const int32_t *colind, int nshift,
+                                   const double *weights, const double *U, int64_t ldu, const double *V, int64_t ldv,
+                                   double scale, double *grad) {
+  if (!c || n < 0 || nshift < 0 || nshift > MS_MAX || (nshift && !weights)) return PSFUN_ERR_INVALID_ARG;
+  if (n == 0 || nshift == 0) return PSFUN_SUCCESS;
+  if (!rowptr || !colind || !U || !V || !grad || ldu < n || ldv < n) return PSFUN_ERR_INVALID_ARG;
+  // the weight table (2*nshift <= 128 doubles) goes to the context's small parameter area
+  PSFUN_HIP_TRY(hipMemcpyAsync(c->d_scalar, weights, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice, c->stream));
+  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_SDDMM, 0.0));
+  hipLaunchKernelGGL(k_sddmm_shifted, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, rowptr, colind, nshift,
+                     (const double *)c->d_scalar, (const double2 *)U, ldu, (const double2 *)V, ldv, scale, grad);
+  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_TRY(psfun_prof_end(c));
+  return psfun_ctx_synchronize(c);
+}

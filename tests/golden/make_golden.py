@@ -99,21 +99,24 @@ def quad_small():
 
 
 def frechet_small():
-    m, t = 8, 0.1
+    """32x32-grid anisotropic operator (n = 1024), K = 2 columns, t = 0.1."""
+    m, t, K = 32, 0.1, 2
     A = matgen.anisotropic_2d(m, seed=11)
     Ad = matgen.to_scipy(A).toarray()
     n = m * m
-    x = matgen.normal(12, n)
-    g = matgen.normal(13, n)
-    # L = <g, exp(-tA) x>;  dL/dA = -t * L_exp(-t A^T, g x^T)  (Frechet adjoint),
-    # sampled on the sparsity pattern; dL/dx = exp(-t A^T) g; dL/dt = -<g, A exp(-tA) x>
-    _, Lf = sla.expm_frechet(-t * Ad.T, np.outer(g, x))
+    X = matgen.normal(12, n * K).reshape(n, K)
+    G = matgen.normal(13, n * K).reshape(n, K)
+    # L = <G, exp(-tA) X>;  dL/dA = -t * L_exp(-t A^T, G X^T)  (adjoint of the
+    # Frechet derivative), sampled on the sparsity pattern;
+    # dL/dX = exp(-t A^T) G;  dL/dt = -<G, A exp(-tA) X>
+    _, Lf = sla.expm_frechet(-t * Ad.T, G @ X.T)
     gA = -t * Lf
     rp, ci, _ = A
     rows = np.repeat(np.arange(n), np.diff(rp))
-    y = sla.expm(-t * Ad) @ x
-    np.savez(os.path.join(HERE, "frechet_small.npz"), rowptr=A[0], colind=A[1], val=A[2], x=x, g=g, t=t,
-             y=y, grad_val=gA[rows, ci], grad_x=sla.expm(-t * Ad.T) @ g, grad_t=-g @ (Ad @ y))
+    E = sla.expm(-t * Ad)
+    Y = E @ X
+    np.savez(os.path.join(HERE, "frechet_small.npz"), rowptr=A[0], colind=A[1], val=A[2], X=X, G=G, t=t,
+             Y=Y, grad_val=gA[rows, ci], grad_X=E.T @ G, grad_t=-np.sum(G * (Ad @ Y)))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,75 @@
+"""GPU checks of the differentiable exp(-tA)X: against the dense FP64
+Frechet-derivative oracle on a 32x32-grid anisotropic operator (fixture) and
+against central finite differences (step 1e-6).  A central difference with
+h = 1e-6 resolves a derivative only to about eps_forward*|L|/h (forward
+tolerance 1e-12, |L| ~ 10-100) plus h^2 terms, hence its 5e-5 bound; the
+oracle comparison carries the tight bounds."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_golden
+from psfun_amd.autograd import ExpmvOperator, expmv
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup():
+    g = load_golden("frechet_small.npz")
+    op = ExpmvOperator(g["rowptr"], g["colind"], eps=1e-12, itmax=200, quad_poles=24, quad_eps=1e-14)
+    dev = "cuda:0"
+    vals = torch.tensor(g["val"], device=dev, requires_grad=True)
+    X = torch.tensor(g["X"], device=dev, requires_grad=True)
+    t = torch.tensor(float(g["t"]), dtype=torch.float64, device=dev, requires_grad=True)
+    G = torch.tensor(g["G"], device=dev)
+    return g, op, vals, X, t, G
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_forward_and_gradients_vs_frechet_oracle(gpu_ctx):
+    g, op, vals, X, t, G = _setup()
+    Y = expmv(vals, X, t, op)
+    assert _rel(Y.detach().cpu().numpy(), g["Y"]) <= 1e-10
+    loss = (G * Y).sum()
+    loss.backward()
+    assert _rel(X.grad.cpu().numpy(), g["grad_X"]) <= 1e-10
+    assert abs(t.grad.item() - float(g["grad_t"])) <= 1e-10 * abs(float(g["grad_t"]))
+    assert _rel(vals.grad.cpu().numpy(), g["grad_val"]) <= 1e-9
+
+
+def test_gradients_vs_central_differences(gpu_ctx):
+    g, op, vals, X, t, G = _setup()
+    Y = expmv(vals, X, t, op)
+    (G * Y).sum().backward()
+
+    def loss(v, x, tt):
+        with torch.no_grad():
+            return float((G * expmv(v, x, tt, op)).sum().item())
+
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    with torch.no_grad():
+        # symmetric direction in the nonzeros of A (the operator is symmetric;
+        # a one-sided perturbation would leave the Lanczos forward's domain)
+        import scipy.sparse as sp
+        n = len(g["rowptr"]) - 1
+        D = sp.csr_matrix((rng.normal(size=vals.numel()), g["colind"], g["rowptr"]), shape=(n, n))
+        D = ((D + D.T) * 0.5).tocsr()
+        D.sort_indices()
+        d = torch.tensor(D.data, device=vals.device)
+        fd = (loss(vals + h * d, X, t) - loss(vals - h * d, X, t)) / (2 * h)
+        an = float((vals.grad * d).sum().item())
+        assert abs(fd - an) <= 5e-5 * max(abs(an), 1.0)
+        # direction in X
+        dX = torch.tensor(rng.normal(size=tuple(X.shape)), device=X.device)
+        fd = (loss(vals, X + h * dX, t) - loss(vals, X - h * dX, t)) / (2 * h)
+        an = float((X.grad * dX).sum().item())
+        assert abs(fd - an) <= 5e-5 * max(abs(an), 1.0)
+        # t
+        fd = (loss(vals, X, t + h) - loss(vals, X, t - h)) / (2 * h)
+        assert abs(fd - t.grad.item()) <= 5e-5 * max(abs(t.grad.item()), 1.0)
