@@ -182,6 +182,33 @@ int psfun_batch_set_values(psfun_batch_t B, const double *val);
 /* Device pointers of the batch's 0-based block-diagonal CSR arrays. */
 int psfun_batch_csr(psfun_batch_t B, const int32_t **rowptr, const int32_t **colind, const double **val);
 
+/* As psfun_batch_lanczos, but the Krylov basis goes to the caller's device
+ * buffer V ((itmax-1)*nrows*K doubles; basis vector l of every instance at
+ * V + l*nrows*K, row-major like X) and each instance's Lanczos data to caller
+ * device arrays: T_alpha/T_beta (ninst*(itmax+2); instance I at I*(itmax+2),
+ * 1-based as T(i,i) and T(i,i+1)), nbasis (ninst ints: valid basis vectors)
+ * and rs1 (ninst: ||x||).  Used to reuse the basis in an adjoint pass. */
+int psfun_batch_lanczos_basis(psfun_ctx_t ctx, psfun_batch_t B, int K, double fscale, double opscale, const double *X,
+                              double *Y, double eps, int itmax, int istop, double *V, double *T_alpha, double *T_beta,
+                              int *nbasis, double *rs1);
+
+/* ---- adjoint helpers (differentiable exp(-tA)X; no reference counterpart) - */
+/* Galerkin resolvents from a stored Krylov basis (one matrix, K columns):
+ * out_s[r, col] = ||x_col|| (V_m (xi_s I + T_m)^{-1} e1)[r] for nshift <= 32
+ * host complex shifts xi_s; out is device complex, entry (s, r, col) at
+ * out + 2*((s*nrows + r)*K + col).  V, T_alpha, T_beta, nbasis, rs1 as written
+ * by psfun_batch_lanczos_basis (ldt = itmax+2, mmax >= max nbasis); the stored
+ * tridiagonal belongs to the Lanczos operator opscale*A and tscale = 1/opscale
+ * maps it to A. */
+int psfun_krylov_resolvents(psfun_ctx_t ctx, int64_t nrows, int K, int mmax, const double *V, const double *T_alpha,
+                            const double *T_beta, int ldt, double tscale, const int *nbasis, const double *rs1,
+                            int nshift, const double *shifts, double *out);
+/* grad[q] += scale * sum_s Re(w_s sum_col U_s[i,col] V_s[k,col]) for every
+ * nonzero q = (i, k) of the device CSR pattern; U, V laid out as the output
+ * of psfun_krylov_resolvents; host weights w (nshift <= 32). */
+int psfun_sddmm_cols(psfun_ctx_t ctx, int n, const int32_t *rowptr, const int32_t *colind, int K, int nshift,
+                     const double *weights, const double *U, const double *V, double scale, double *grad);
+
 /* ---- shifted solves (psb_krylov extension) -------------------------------- */
 enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1 };
 /* (eta*A + zeta*I) x = b by preconditioned CG with the semantics of

@@ -81,6 +81,9 @@ SIGNATURES = {
     "psfun_batch_set_values": (_i, [_vp, _vp]),
     "psfun_batch_csr": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "psfun_sddmm_shifted": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _i64, _vp, _i64, _d, _vp]),
+    "psfun_batch_lanczos_basis": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "psfun_krylov_resolvents": (_i, [_vp, _i64, _i, _i, _vp, _vp, _vp, _i, _d, _vp, _vp, _i, _vp, _vp]),
+    "psfun_sddmm_cols": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _d, _vp]),
     "psfun_cg_shifted": (_i, [_vp, _vp, _i, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
     "psfun_mscg": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
     "psfun_horner": (_d, [_i, _vp, _d]),

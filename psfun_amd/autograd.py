@@ -2,16 +2,21 @@
 hand-written adjoint kernels (no reference counterpart; the reference is not
 differentiable).
 
-Forward:   Y = exp(-tA) X by the batched device Lanczos (one matrix, K columns).
+Forward:  Y = exp(-tA) X by the batched device Lanczos (one matrix, K columns),
+          keeping the Krylov basis V of every column (or not: `store_basis`).
 Backward (A symmetric, loss L with dL/dY = G):
-  dL/dX    = exp(-tA^T) G = exp(-tA) G          -- a second batched Lanczos
-  dL/dt    = -<G, A Y>                          -- one SpMM + a dot
-  dL/da_ik = -Re sum_j c_j sum_col u_j[i] v_j[k] -- sampled (SDDMM) over A's pattern
-             with exp(-tA) ~ Re sum_j c_j (xi_j I + A)^{-1} (modified Talbot rule),
-             v_j = (xi_j I + A)^{-1} x_col and u_j = (xi_j I + A)^{-T} g_col from the
-             multi-shift CG (all poles of one column in one Krylov space).
-The Krylov basis of the forward is not kept across the call: the backward
-recomputes what it needs (a Lanczos run on G), so memory stays at one basis.
+  dL/dX    = exp(-tA^T) G = exp(-tA) G        -- a second batched Lanczos, on G,
+                                                 which also yields its basis W
+  dL/dt    = -<G, A Y>                        -- one SpMM + a dot
+  dL/da_ik = -Re sum_s w_s sum_col u_s[i] v_s[k] -- SDDMM over A's pattern, with the
+             resolvent form exp(-tA) ~ Re sum_s w_s (xi_s I + A)^{-1} (modified Talbot
+             rule), v_s = (xi_s I + A)^{-1} x_col and u_s = (xi_s I + A)^{-T} g_col taken
+             from the bases: v_s = ||x|| V_m (xi_s I + T_m)^{-1} e1 (the multi-shift CG
+             iterate of step m), likewise u_s from W.
+With store_basis=False the forward keeps nothing and the backward rebuilds V
+by re-running the forward Lanczos (memory for one basis at a time).
+grad_mode="mscg" takes v_s, u_s from full multi-shift CG solves instead (one
+column at a time; slower, used to cross-check).
 """
 import ctypes as C
 
@@ -27,8 +32,8 @@ from .batched import psfun_batch
 class ExpmvOperator:
     """exp(-tA) on a fixed symmetric sparsity pattern (0-based CSR)."""
 
-    def __init__(self, rowptr, colind, eps=1e-10, itmax=250, quad_poles=24, quad_eps=1e-13, quad_itmax=20000,
-                 ctxt=None):
+    def __init__(self, rowptr, colind, eps=1e-10, itmax=150, quad_poles=24, quad_eps=1e-13, quad_itmax=20000,
+                 store_basis=True, grad_mode="krylov", ctxt=None):
         self.ctxt = ctxt if ctxt is not None else psb.default_context()
         self.rowptr = np.ascontiguousarray(rowptr, np.int32)
         self.colind = np.ascontiguousarray(colind, np.int32)
@@ -36,21 +41,27 @@ class ExpmvOperator:
         self.nnz = len(self.colind)
         ones = np.ones(self.nnz)
         self.batch = psfun_batch([(self.rowptr, self.colind, ones)], self.ctxt)
-        self.mat = psb.psb_dspmat_type(self.rowptr, self.colind, ones, 0, self.ctxt)
+        self.mat = None
+        if grad_mode == "mscg":
+            self.mat = psb.psb_dspmat_type(self.rowptr, self.colind, ones, 0, self.ctxt)
         self.eps, self.itmax = eps, itmax
         self.quad_poles, self.quad_eps, self.quad_itmax = quad_poles, quad_eps, quad_itmax
+        self.store_basis = store_basis
+        self.grad_mode = grad_mode
         rp = C.c_void_p()
         ci = C.c_void_p()
         _lib.check(_lib.lib().psfun_batch_csr(self.batch.handle, C.byref(rp), C.byref(ci), None))
         self._d_rowptr, self._d_colind = rp, ci
         self.last_iters = None
 
+    # ---- primitives ------------------------------------------------------
     def set_values(self, vals):
         v = vals.detach().contiguous()
         assert v.is_cuda and v.dtype == torch.float64 and v.numel() == self.nnz
         self.ctxt.bind_stream()
         _lib.check(_lib.lib().psfun_batch_set_values(self.batch.handle, C.c_void_p(v.data_ptr())))
-        _lib.check(_lib.lib().psfun_csr_set_values(self.mat.handle, C.c_void_p(v.data_ptr())))
+        if self.mat is not None:
+            _lib.check(_lib.lib().psfun_csr_set_values(self.mat.handle, C.c_void_p(v.data_ptr())))
         torch.cuda.current_stream().synchronize()
 
     def expm(self, X, t):
@@ -59,6 +70,27 @@ class ExpmvOperator:
         iters, _ = self.batch.lanczos(X, Y, fscale=1.0, opscale=-float(t), eps=self.eps, itmax=self.itmax)
         self.last_iters = iters
         return Y
+
+    def expm_basis(self, X, t):
+        """Y = exp(-tA) X and the Krylov data (V, T_alpha, T_beta, nbasis, rs1)."""
+        X = X.contiguous()
+        K = 1 if X.dim() == 1 else int(X.shape[1])
+        dev = X.device
+        Y = torch.empty_like(X)
+        V = torch.empty(((self.itmax - 1) * self.n * K,), dtype=torch.float64, device=dev)
+        ld = self.itmax + 2
+        Ta = torch.empty((K * ld,), dtype=torch.float64, device=dev)
+        Tb = torch.empty_like(Ta)
+        nb = torch.empty((K,), dtype=torch.int32, device=dev)
+        rs1 = torch.empty((K,), dtype=torch.float64, device=dev)
+        self.ctxt.bind_stream()
+        st = _lib.lib().psfun_batch_lanczos_basis(self.ctxt.handle, self.batch.handle, K, 1.0, -float(t),
+                                                  C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), float(self.eps),
+                                                  int(self.itmax), 1, C.c_void_p(V.data_ptr()),
+                                                  C.c_void_p(Ta.data_ptr()), C.c_void_p(Tb.data_ptr()),
+                                                  C.c_void_p(nb.data_ptr()), C.c_void_p(rs1.data_ptr()))
+        _lib.check(st, "psfun_batch_lanczos_basis")
+        return Y, (V, Ta, Tb, nb, rs1, K)
 
     def spmm(self, X):
         X = X.contiguous()
@@ -69,17 +101,45 @@ class ExpmvOperator:
                                                0.0, C.c_void_p(Y.data_ptr())))
         return Y
 
-    def grad_values(self, X, G, t):
-        """dL/da for every stored nonzero, L = <G, exp(-tA) X>."""
+    def _rule(self, t):
         xi, c, eta, sign, info = Q.talbot(None, self.quad_poles, rparams=[float(t)])
         shifts, weights = Q.fold_conjugate_pairs(sign * xi, c)
-        S = len(shifts)
         sh = np.ascontiguousarray(np.stack([shifts.real, shifts.imag], 1).reshape(-1))
         ww = np.ascontiguousarray(np.stack([weights.real, weights.imag], 1).reshape(-1))
+        return len(shifts), sh, ww
+
+    def resolvents(self, krylov, t):
+        """Galerkin resolvents of every folded pole from stored Krylov data
+        (the runs used the operator -t*A; their tridiagonals scale by -1/t)."""
+        V, Ta, Tb, nb, rs1, K = krylov
+        S, sh, _ = self._rule(t)
+        mmax = int(nb.max().item())
+        out = torch.empty((S, self.n, K), dtype=torch.complex128, device=V.device)
+        self.ctxt.bind_stream()
+        st = _lib.lib().psfun_krylov_resolvents(self.ctxt.handle, self.n, K, max(mmax, 1), C.c_void_p(V.data_ptr()),
+                                                C.c_void_p(Ta.data_ptr()), C.c_void_p(Tb.data_ptr()), self.itmax + 2,
+                                                -1.0 / float(t), C.c_void_p(nb.data_ptr()),
+                                                C.c_void_p(rs1.data_ptr()), S,
+                                                sh.ctypes.data_as(C.c_void_p), C.c_void_p(out.data_ptr()))
+        _lib.check(st, "psfun_krylov_resolvents")
+        return out
+
+    def sddmm(self, U, Vr, t, grad, scale=-1.0):
+        S, _, ww = self._rule(t)
+        K = int(U.shape[2])
+        self.ctxt.bind_stream()
+        st = _lib.lib().psfun_sddmm_cols(self.ctxt.handle, self.n, self._d_rowptr, self._d_colind, K, S,
+                                         ww.ctypes.data_as(C.c_void_p), C.c_void_p(U.data_ptr()),
+                                         C.c_void_p(Vr.data_ptr()), float(scale), C.c_void_p(grad.data_ptr()))
+        _lib.check(st, "psfun_sddmm_cols")
+        return grad
+
+    # ---- exact-solve variant (cross-check) --------------------------------
+    def grad_values_mscg(self, X, G, t):
+        S, sh, ww = self._rule(t)
         grad = torch.zeros(self.nnz, dtype=torch.float64, device=X.device)
         K = 1 if X.dim() == 1 else int(X.shape[1])
-        Xc = X.reshape(self.n, K)
-        Gc = G.reshape(self.n, K)
+        Xc, Gc = X.reshape(self.n, K), G.reshape(self.n, K)
         V = torch.empty((S, self.n), dtype=torch.complex128, device=X.device)
         U = torch.empty_like(V)
         L = _lib.lib()
@@ -104,7 +164,14 @@ class ExpmvFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vals, X, t, op):
         op.set_values(vals)
-        Y = op.expm(X, float(t))
+        tf = float(t)
+        need_a = ctx.needs_input_grad[0] and op.grad_mode == "krylov"
+        if need_a and op.store_basis:
+            Y, kry = op.expm_basis(X, tf)
+            ctx.krylov = kry
+        else:
+            Y = op.expm(X, tf)
+            ctx.krylov = None
         ctx.op = op
         ctx.save_for_backward(vals, X, t, Y)
         return Y
@@ -117,12 +184,29 @@ class ExpmvFunction(torch.autograd.Function):
         op.set_values(vals)
         tf = float(t)
         gvals = gX = gt = None
-        if ctx.needs_input_grad[1]:
-            gX = op.expm(G, tf)
+        need_a = ctx.needs_input_grad[0]
+        if need_a and op.grad_mode == "krylov":
+            gX, kry_g = op.expm_basis(G, tf)          # dL/dX and the adjoint basis W
+            U = op.resolvents(kry_g, tf)
+            del kry_g
+            kry_x = ctx.krylov
+            if kry_x is None:                          # rebuild the forward basis
+                _, kry_x = op.expm_basis(X, tf)
+            Vr = op.resolvents(kry_x, tf)
+            del kry_x
+            ctx.krylov = None
+            gvals = torch.zeros(op.nnz, dtype=torch.float64, device=X.device)
+            op.sddmm(U, Vr, tf, gvals, -1.0)
+            del U, Vr
+        else:
+            if ctx.needs_input_grad[1]:
+                gX = op.expm(G, tf)
+            if need_a:
+                gvals = op.grad_values_mscg(X, G, tf)
+        if not ctx.needs_input_grad[1]:
+            gX = None
         if ctx.needs_input_grad[2]:
             gt = -(G * op.spmm(Y)).sum().reshape(t.shape)
-        if ctx.needs_input_grad[0]:
-            gvals = op.grad_values(X, G, tf)
         return gvals, gX, gt, None
 
 

@@ -492,9 +492,40 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
   return PSFUN_SUCCESS;
 }
 
+__global__ void k_bt_export(BtInst in, int ninst, double *T_alpha, double *T_beta, int *nbasis, double *rs1) {
+  const int64_t tot = (int64_t)ninst * in.ld;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+    T_alpha[i] = in.alpha[i];
+    T_beta[i] = in.beta[i];
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+    nbasis[i] = (in.done[i] == BT_ZERO) ? 0 : in.itx[i];
+    rs1[i] = in.rs1[i];
+  }
+}
+
+static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, double opscale, const double *X, double *Y,
+                      double eps, int itmax, int istop, int *iters, double *errs, double *V_ext, double *T_alpha,
+                      double *T_beta, int *nbasis, double *rs1_out);
+
 extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, double opscale,
                                    const double *X, double *Y, double eps, int itmax, int istop, int *iters,
                                    double *errs) {
+  return bt_lanczos(c, B, K, fscale, opscale, X, Y, eps, itmax, istop, iters, errs, nullptr, nullptr, nullptr,
+                    nullptr, nullptr);
+}
+
+extern "C" int psfun_batch_lanczos_basis(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, double opscale,
+                                         const double *X, double *Y, double eps, int itmax, int istop, double *V,
+                                         double *T_alpha, double *T_beta, int *nbasis, double *rs1) {
+  if (!V || !T_alpha || !T_beta || !nbasis || !rs1) return PSFUN_ERR_INVALID_ARG;
+  return bt_lanczos(c, B, K, fscale, opscale, X, Y, eps, itmax, istop, nullptr, nullptr, V, T_alpha, T_beta, nbasis,
+                    rs1);
+}
+
+static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, double opscale, const double *X, double *Y,
+                      double eps, int itmax, int istop, int *iters, double *errs, double *V_ext, double *T_alpha,
+                      double *T_beta, int *nbasis, double *rs1_out) {
   if (!c || !B || K < 1 || K > BT_KMAX) return PSFUN_ERR_INVALID_ARG;
   if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
   if (!X || !Y) return PSFUN_ERR_INVALID_VECT_STATE;
@@ -507,7 +538,8 @@ extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double
   // workspace: control, per-instance arrays, basis v_1..v_{itmax-1}, two u buffers
   const size_t inst_bytes = align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256) +
                             align_up(sizeof(int) * (size_t)ninst * 3, 256);
-  const size_t bytes = 256 + inst_bytes + sizeof(double) * (size_t)vs * (itmax + 1);
+  // the basis lives in the caller's buffer when one is given
+  const size_t bytes = 256 + inst_bytes + sizeof(double) * (size_t)vs * (V_ext ? 2 : itmax + 1);
   void *ws = nullptr;
   PSFUN_TRY(psfun_workspace(c, bytes, &ws));
   BtCtl *bctl = (BtCtl *)ws;
@@ -525,14 +557,21 @@ extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double
   in.done = ip;
   in.itx = ip + ninst;
   in.m = ip + 2 * ninst;
-  double *V = (double *)((char *)ws + 256 + inst_bytes);
-  double *U0 = V + (size_t)vs * (itmax - 1), *U1 = U0 + vs;
+  double *V, *U0;
+  if (V_ext) {
+    V = V_ext;
+    U0 = (double *)((char *)ws + 256 + inst_bytes);
+  } else {
+    V = (double *)((char *)ws + 256 + inst_bytes);
+    U0 = V + (size_t)vs * (itmax - 1);
+  }
+  double *U1 = U0 + vs;
 
   int KP = 1;
   while (KP < K) KP <<= 1;
   // hipGraph path: captured and replayed on the context's own stream, joined
   // to the caller's stream by events (the null stream cannot be captured).
-  const bool graph = B->use_graph && !c->prof_on;
+  const bool graph = B->use_graph && !c->prof_on && !V_ext;
   hipStream_t caller = c->stream;
   if (graph) {
     if (!B->ev_in) PSFUN_HIP_TRY(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
@@ -586,6 +625,11 @@ extern "C" int psfun_batch_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double
       st = (st == PSFUN_SUCCESS) ? PSFUN_ERR_HIP : st;
   }
   if (st != PSFUN_SUCCESS) return st;
+  if (V_ext) {
+    hipLaunchKernelGGL(k_bt_export, dim3(std::min(1024, (ninst * ld + 255) / 256 + 1)), dim3(256), 0, c->stream, in,
+                       ninst, T_alpha, T_beta, nbasis, rs1_out);
+    PSFUN_HIP_TRY(hipGetLastError());
+  }
   // per-instance results
   PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
   if (iters || errs) {

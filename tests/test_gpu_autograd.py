@@ -14,9 +14,10 @@ from psfun_amd.autograd import ExpmvOperator, expmv
 pytestmark = pytest.mark.gpu
 
 
-def _setup():
+def _setup(mode="krylov", store=True):
     g = load_golden("frechet_small.npz")
-    op = ExpmvOperator(g["rowptr"], g["colind"], eps=1e-12, itmax=200, quad_poles=24, quad_eps=1e-14)
+    op = ExpmvOperator(g["rowptr"], g["colind"], eps=1e-12, itmax=200, quad_poles=24, quad_eps=1e-14,
+                       store_basis=store, grad_mode=mode)
     dev = "cuda:0"
     vals = torch.tensor(g["val"], device=dev, requires_grad=True)
     X = torch.tensor(g["X"], device=dev, requires_grad=True)
@@ -31,8 +32,9 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_forward_and_gradients_vs_frechet_oracle(gpu_ctx):
-    g, op, vals, X, t, G = _setup()
+@pytest.mark.parametrize("mode,store", [("krylov", True), ("krylov", False), ("mscg", True)])
+def test_forward_and_gradients_vs_frechet_oracle(gpu_ctx, mode, store):
+    g, op, vals, X, t, G = _setup(mode, store)
     Y = expmv(vals, X, t, op)
     assert _rel(Y.detach().cpu().numpy(), g["Y"]) <= 1e-10
     loss = (G * Y).sum()
