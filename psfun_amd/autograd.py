@@ -90,6 +90,7 @@ class ExpmvOperator:
                                                   C.c_void_p(Ta.data_ptr()), C.c_void_p(Tb.data_ptr()),
                                                   C.c_void_p(nb.data_ptr()), C.c_void_p(rs1.data_ptr()))
         _lib.check(st, "psfun_batch_lanczos_basis")
+        self.last_iters = nb.cpu().numpy()
         return Y, (V, Ta, Tb, nb, rs1, K)
 
     def spmm(self, X):

@@ -55,6 +55,7 @@ struct psfun_ctx {
   int prof_on = 0;
   std::vector<hipEvent_t> prof_ev;       // event pool, 2 per timed launch
   std::vector<int> prof_kid;             // kernel id of each pending pair
+  std::vector<int> prof_tag;             // caller tag (e.g. Krylov index) of each pending pair
   std::vector<double> prof_bytes_pending;
   int prof_used = 0;
   double prof_ms[PSFUN_PROF_NK] = {};
@@ -65,7 +66,10 @@ struct psfun_ctx {
 // Timing hooks: bracket one launch of kernel `kid` (with `bytes` algorithmic
 // bytes) by two events when profiling is on.  psfun_prof_collect() folds the
 // finished pairs into the per-kernel totals (call after a synchronize).
-int psfun_prof_begin(psfun_ctx *c, int kid, double bytes);
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag = 0);
+// Drop pending pairs of kernel `kid` whose tag exceeds `tag_max` (launches
+// that found their loop already finished and did no work).
+void psfun_prof_discard_after(psfun_ctx *c, int kid, int tag_max);
 int psfun_prof_end(psfun_ctx *c);
 int psfun_prof_collect(psfun_ctx *c);
 

@@ -279,7 +279,7 @@ extern "C" int psfun_vec_download(psfun_ctx_t c, double *h, const double *d, int
 // ---------------------------------------------------------------------------
 // Per-kernel timing
 // ---------------------------------------------------------------------------
-int psfun_prof_begin(psfun_ctx *c, int kid, double bytes) {
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag) {
   if (!c->prof_on) return PSFUN_SUCCESS;
   if (c->prof_used + 1 >= PSFUN_PROF_MAXPAIRS) {  // pool full: fold finished pairs first
     PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -293,12 +293,19 @@ int psfun_prof_begin(psfun_ctx *c, int kid, double bytes) {
   }
   if (c->prof_kid.size() < (size_t)c->prof_used + 1) {
     c->prof_kid.resize(c->prof_used + 1);
+    c->prof_tag.resize(c->prof_used + 1);
     c->prof_bytes_pending.resize(c->prof_used + 1);
   }
   c->prof_kid[c->prof_used] = kid;
+  c->prof_tag[c->prof_used] = tag;
   c->prof_bytes_pending[c->prof_used] = bytes;
   PSFUN_HIP_TRY(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
   return PSFUN_SUCCESS;
+}
+
+void psfun_prof_discard_after(psfun_ctx *c, int kid, int tag_max) {
+  for (int i = 0; i < c->prof_used; ++i)
+    if (c->prof_kid[i] == kid && c->prof_tag[i] > tag_max) c->prof_kid[i] = -1;
 }
 
 int psfun_prof_end(psfun_ctx *c) {

@@ -213,7 +213,7 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   const int L = A->lanes;
   // algorithmic bytes: val+colind, rowptr, src rows, v and u written
   const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes));
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j));
 #define LZ_LAUNCH(LL)                                                                                         \
   hipLaunchKernelGGL((k_lz_spmv<LL>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr, A->colind, \
                      A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl, istop, eps, fscale)
@@ -280,7 +280,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     const double *vj = V + (size_t)ldv * (j - 1);
     const double *vjm1 = (j > 1) ? V + (size_t)ldv * (j - 2) : nullptr;
     // algorithmic bytes: u read + w written, v_j (and v_{j-1}) read
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n));
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n, j));
     hipLaunchKernelGGL(k_lz_update, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, Pa, Gs, Pw, ctl);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
@@ -319,6 +319,10 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(LzCtl), hipMemcpyDeviceToHost, c->stream));
   PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
   const int itx = h->itx;
+  // timing pairs of launches that came after the stop did no work: SpMV steps
+  // 1..itx+1 and updates 1..itx ran (the last SpMV overlapped the stop test)
+  psfun_prof_discard_after(c, PSFUN_PROF_LZ_SPMV, itx + 1);
+  psfun_prof_discard_after(c, PSFUN_PROF_LZ_UPDATE, itx);
   const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
   if (iter) *iter = itx;
   if (err) *err = e;
