@@ -63,6 +63,9 @@ typedef struct psfun_csr *psfun_csr_t;
 /* Memory space of pointers handed to psfun_csr_create. */
 enum { PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1 };
 
+/* Storage the SpMV kernels read (psfun_csr_set_format). */
+enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2 };
+
 /* ---- context / errors --------------------------------------------------- */
 const char *psfun_strerror(int status);
 /* Create a context on HIP device `device`.  The context owns a default-priority
@@ -82,7 +85,7 @@ int psfun_ctx_num_cus(psfun_ctx_t ctx, int *num_cus);
 /* Kernel ids for psfun_prof_read. */
 enum {
   PSFUN_PROF_LZ_SPMV = 0,    /* Lanczos SpMV + alpha dot (fused)          */
-  PSFUN_PROF_LZ_UPDATE = 1,  /* Lanczos 3-term update + beta nrm2 + f(T)  */
+  PSFUN_PROF_LZ_UPDATE = 1,  /* Lanczos 3-term update + ||w||^2 partials */
   PSFUN_PROF_LZ_ASSEMBLE = 2,/* y = ||x|| V yk                           */
   PSFUN_PROF_SPMV = 3,       /* plain CSR SpMV (psfun_spmv)              */
   PSFUN_PROF_MS_SPMV = 4,    /* multi-shift CG seed SpMV + dots          */
@@ -91,7 +94,11 @@ enum {
   PSFUN_PROF_BATCH = 7,      /* batched Lanczos step kernels             */
   PSFUN_PROF_SDDMM = 8       /* gradient SDDMM                           */
 };
-/* Turn timing on/off (off by default); reset clears the totals. */
+/* Turn timing on/off (off by default); reset clears the totals.  on = 1
+ * brackets each timed launch by two events; on = R in 2..16 also launches the
+ * fused Lanczos SpMV step R times back to back inside its event pair (the
+ * step is idempotent, results are unchanged), which amortises the events'
+ * own cost in the per-launch average. */
 int psfun_prof_enable(psfun_ctx_t ctx, int on);
 int psfun_prof_reset(psfun_ctx_t ctx);
 /* Totals since the last reset: milliseconds, launches, algorithmic bytes. */
@@ -111,6 +118,13 @@ int psfun_csr_set_values(psfun_csr_t A, const double *val);
 /* Force the SpMV lanes-per-row (0 = automatic; 1..64 power of two). */
 int psfun_csr_set_lanes(psfun_csr_t A, int lanes_per_row);
 int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
+/* Choose the storage the SpMV kernels read: PSFUN_FORMAT_CSR (vector CSR,
+ * lanes_per_row lanes per row), PSFUN_FORMAT_SELL (sliced ELL, 64-row slices,
+ * one thread per row; built at creation) or PSFUN_FORMAT_AUTO (SELL when its
+ * padding stays within 25% of nnz).  Results differ only in rounding order. */
+int psfun_csr_set_format(psfun_csr_t A, int format);
+/* Format in use and the padded sliced-ELL entry count (0 when not built). */
+int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_entries);
 
 /* ---- device vectors ------------------------------------------------------ */
 int psfun_vec_alloc(psfun_ctx_t ctx, int64_t n, double **dptr);

@@ -26,6 +26,9 @@ PSFUN_FN_EXP = 0
 VARIANTS = {"CHBSYM": 0, "CHBHES": 1, "CHBGEN": 2, "GENPADE": 3, "TAYLOR": 4, "SASQ": 5}
 PSFUN_MEM_HOST = 0
 PSFUN_MEM_DEVICE = 1
+PSFUN_FORMAT_AUTO = 0
+PSFUN_FORMAT_CSR = 1
+PSFUN_FORMAT_SELL = 2
 PSFUN_PREC_NONE = 0
 PSFUN_PREC_DIAG = 1
 PSFUN_QUAD_RE_SUM = 0
@@ -59,6 +62,8 @@ SIGNATURES = {
     "psfun_csr_set_values": (_i, [_vp, _vp]),
     "psfun_csr_set_lanes": (_i, [_vp, _i]),
     "psfun_csr_info": (_i, [_vp, _pi64, _pi64, _pi]),
+    "psfun_csr_set_format": (_i, [_vp, _i]),
+    "psfun_csr_get_format": (_i, [_vp, _pi, _pi64]),
     "psfun_vec_alloc": (_i, [_vp, _i64, C.POINTER(_vp)]),
     "psfun_vec_free": (_i, [_vp, _vp]),
     "psfun_vec_upload": (_i, [_vp, _vp, _vp, _i64]),

@@ -39,6 +39,25 @@ __global__ __launch_bounds__(PSFUN_BS) void k_csr_spmv(int n, const int *__restr
   }
 }
 
+// Sliced-ELL SpMV: one thread per row (chunks of PSFUN_BS rows = 4 slices),
+// each entry step reads one full cache line of colind and two of val per wave.
+__global__ __launch_bounds__(PSFUN_BS) void k_sell_spmv(int n, const int *__restrict__ sp,
+                                                        const int *__restrict__ col,
+                                                        const double *__restrict__ val, double alpha,
+                                                        const double *__restrict__ x, double beta,
+                                                        double *__restrict__ y) {
+  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
+  const XcdRange xr = xcd_range(nchunks);
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * PSFUN_BS + threadIdx.x);
+    if (row < n) {
+      double r = alpha * sell_row_dot(row, sp, col, val, x, 1.0);
+      if (beta != 0.0) r = fma(beta, y[row], r);
+      y[row] = r;
+    }
+  }
+}
+
 // Column-major multi-RHS: Y(:,c) = alpha*A*X(:,c) + beta*Y(:,c), KB columns per pass.
 template <int L, int KB>
 __global__ __launch_bounds__(PSFUN_BS) void k_csr_spmm(int n, const int *__restrict__ rowptr,
@@ -98,11 +117,19 @@ __global__ __launch_bounds__(PSFUN_BS) void k_csr_spmm(int n, const int *__restr
 
 int launch_spmv(psfun_ctx *c, const psfun_csr *A, double alpha, const double *x, double beta, double *y) {
   if (A->n == 0) return PSFUN_SUCCESS;
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_SPMV,
+                             12.0 * A->nnz + 4.0 * (A->n + 1) + (beta != 0.0 ? 24.0 : 16.0) * A->n));
+  if (A->format == PSFUN_FORMAT_SELL) {
+    const int G = psfun_grid(c, ((int64_t)A->n + PSFUN_BS - 1) / PSFUN_BS);
+    hipLaunchKernelGGL(k_sell_spmv, dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr, A->sell_col,
+                       A->sell_val, alpha, x, beta, y);
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    return PSFUN_SUCCESS;
+  }
   const int L = A->lanes;
   const int64_t chunks = ((int64_t)A->n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
   const int G = psfun_grid(c, chunks);
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_SPMV,
-                             12.0 * A->nnz + 4.0 * (A->n + 1) + (beta != 0.0 ? 24.0 : 16.0) * A->n));
   PSFUN_DISPATCH_LANES(L, hipLaunchKernelGGL((k_csr_spmv<LL>), dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n,
                                              A->rowptr, A->colind, A->val, alpha, x, beta, y));
   PSFUN_HIP_TRY(hipGetLastError());

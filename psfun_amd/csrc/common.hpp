@@ -47,16 +47,23 @@ struct psfun_ctx {
   double *d_partials = nullptr;   // 8 slots of PSFUN_MAX_GRID doubles
   double *d_scalar = nullptr;     // 256 doubles of scalar results / small parameter tables
   void *h_pinned = nullptr;       // pinned host staging
+  // Host-visible loop flags in fine-grained (coherent) pinned memory that the
+  // control blocks of the iterative kernels store to directly: [0] progress
+  // (last loop index started), [1] stop state (0 = running).  The host paces
+  // its launches on them with no copies or events.
+  volatile int *h_flags = nullptr;
+  int *d_flags = nullptr;         // device address of h_flags
   hipEvent_t ev[2] = {nullptr, nullptr};
   // grow-only workspace
   void *ws = nullptr;
   size_t ws_bytes = 0;
   // optional per-launch timing of the hot kernels (psfun_prof_*)
-  int prof_on = 0;
+  int prof_on = 0;                       // 0 off; R >= 1: timing on, idempotent steps run R times
   std::vector<hipEvent_t> prof_ev;       // event pool, 2 per timed launch
   std::vector<int> prof_kid;             // kernel id of each pending pair
   std::vector<int> prof_tag;             // caller tag (e.g. Krylov index) of each pending pair
   std::vector<double> prof_bytes_pending;
+  std::vector<int> prof_reps;            // launches inside each pending pair
   int prof_used = 0;
   double prof_ms[PSFUN_PROF_NK] = {};
   double prof_bytes[PSFUN_PROF_NK] = {};
@@ -66,7 +73,10 @@ struct psfun_ctx {
 // Timing hooks: bracket one launch of kernel `kid` (with `bytes` algorithmic
 // bytes) by two events when profiling is on.  psfun_prof_collect() folds the
 // finished pairs into the per-kernel totals (call after a synchronize).
-int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag = 0);
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag = 0, int reps = 1);
+// Launches per timing pair for a kernel that may run repeatedly with the same
+// result (1 unless profiling asked for repeats).
+static inline int psfun_prof_reps(const psfun_ctx *c) { return c->prof_on > 1 ? c->prof_on : 1; }
 // Drop pending pairs of kernel `kid` whose tag exceeds `tag_max` (launches
 // that found their loop already finished and did no work).
 void psfun_prof_discard_after(psfun_ctx *c, int kid, int tag_max);
@@ -83,7 +93,31 @@ struct psfun_csr {
   bool owns = false;
   int lanes = 8;   // lanes per row for the vector-CSR kernels
   int max_row = 0;
+  // Sliced ELL copy (slices of PSFUN_SELL_C consecutive rows, no row
+  // reordering): slice s holds its width w_s = (sell_ptr[s+1]-sell_ptr[s])/C
+  // entries per row, stored entry-major (entry k of row 64s+l at
+  // sell_ptr[s] + k*C + l), each row's entries in CSR order, padding with
+  // value 0 and the row's own column index.  One thread per row then reads
+  // whole cache lines of colind/val with no intra-row reduction.
+  int nslice = 0;
+  int64_t sell_nnz = 0;  // padded entry count (0: not built)
+  int *sell_ptr = nullptr;
+  int *sell_col = nullptr;
+  double *sell_val = nullptr;
+  int format = PSFUN_FORMAT_CSR;  // format the kernels use
 };
+#define PSFUN_SELL_C 64
+
+// (Re)build the sliced-ELL copy from the device CSR arrays; `rp_host` is the
+// 0-based row pointer on the host.  Leaves sell_nnz = 0 when the padded size
+// does not fit int32 offsets.
+int psfun_sell_build(psfun_csr *A, const int *rp_host);
+// Refresh the sliced-ELL values from A->val (same pattern).
+int psfun_sell_refill(psfun_csr *A);
+
+// Host side of the flag pacing: spin until h_flags[0] >= target, a stop is
+// flagged, or the stream has drained (so the wait always ends).
+int psfun_host_wait(psfun_ctx *c, int target);
 
 // Device workspace: returns a pointer to at least `bytes` bytes (256-byte
 // aligned).  Grows by reallocation; previous contents are not preserved.
@@ -188,4 +222,40 @@ __device__ __forceinline__ XcdRange xcd_range(int64_t nchunks) {
   r.end = hi;
   r.stride = gp;
   return r;
+}
+
+// Grid of 1 + G blocks whose block 0 is a loop-control block (dispatched
+// first, so it starts at once however many streaming blocks are resident)
+// and blocks 1..G stream (G a multiple of 8).  Block k runs on XCD k % 8; the
+// blocks of one XCD get that XCD's contiguous eighth of the chunks.
+__device__ __forceinline__ XcdRange xcd_range_c0(int64_t nchunks, int G) {
+  const int k = blockIdx.x;  // 1..G
+  const int xcd = k & 7;
+  XcdRange r;
+  r.begin = nchunks * xcd / 8 + ((k - 1) >> 3);
+  r.end = nchunks * (xcd + 1) / 8;
+  r.stride = G >> 3;
+  return r;
+}
+
+// Sum over the sliced-ELL entries of `row`, gathering x[col] * xs.
+__device__ __forceinline__ double sell_row_dot(int row, const int *__restrict__ sp, const int *__restrict__ col,
+                                               const double *__restrict__ val, const double *__restrict__ x,
+                                               double xs) {
+  const int s = row >> 6, l = row & 63;
+  const int b = sp[s];
+  const int w = (sp[s + 1] - b) >> 6;
+  const int *cp = col + b + l;
+  const double *vp = val + b + l;
+  double sum = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], x[cp[k * 64]] * xs, sum);
+  return sum;
+}
+
+// Store to a host-visible flag word (coherent pinned memory): a relaxed
+// system-scope atomic store, so the value leaves the GPU caches at once and
+// the host sees it while the kernel still runs.
+__device__ __forceinline__ void host_flag_store(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }

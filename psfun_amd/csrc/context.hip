@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
 
 static thread_local char g_last_hip_error[512];
 
@@ -45,6 +46,9 @@ extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
   if (e == hipSuccess) e = hipMalloc(&c->d_partials, sizeof(double) * PSFUN_MAX_GRID * 8);
   if (e == hipSuccess) e = hipMalloc(&c->d_scalar, sizeof(double) * 256);
   if (e == hipSuccess) e = hipHostMalloc(&c->h_pinned, PSFUN_PINNED_BYTES, hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&c->h_flags, 256, hipHostMallocCoherent | hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&c->d_flags, (void *)c->h_flags, 0);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -67,6 +71,7 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_scalar) hipFree(c->d_scalar);
   if (c->h_pinned) hipHostFree(c->h_pinned);
+  if (c->h_flags) hipHostFree((void *)c->h_flags);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   for (auto &e : c->prof_ev) hipEventDestroy(e);
@@ -96,6 +101,25 @@ extern "C" int psfun_ctx_synchronize(psfun_ctx_t c) {
 extern "C" int psfun_ctx_num_cus(psfun_ctx_t c, int *n) {
   if (!c || !n) return PSFUN_ERR_INVALID_ARG;
   *n = c->num_cu;
+  return PSFUN_SUCCESS;
+}
+
+int psfun_host_wait(psfun_ctx *c, int target) {
+  volatile int *hf = c->h_flags;
+  // Read the flags in a tight loop; ask the runtime whether the stream has
+  // drained (after which every flag store is visible) only every 50 us, since
+  // a stream query costs far more than a flag read.
+  auto next = std::chrono::steady_clock::now() + std::chrono::microseconds(50);
+  while (hf[0] < target && hf[1] == 0) {
+    if (std::chrono::steady_clock::now() < next) continue;
+    const hipError_t q = hipStreamQuery(c->stream);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) {
+      psfun_record_hip_error(q, "hipStreamQuery", __FILE__, __LINE__);
+      return PSFUN_ERR_HIP;
+    }
+    next = std::chrono::steady_clock::now() + std::chrono::microseconds(50);
+  }
   return PSFUN_SUCCESS;
 }
 
@@ -131,6 +155,93 @@ static int choose_lanes(int64_t n, int64_t nnz) {
   return l;
 }
 
+// Sliced-ELL fill: thread r covers row r of slice r/64 (rows >= n pad the
+// last slice).  Writes colind only when `cols` is set (value refresh skips it).
+__global__ __launch_bounds__(PSFUN_BS) void k_sell_fill(int n, int nslice, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const double *__restrict__ va,
+                                                        const int *__restrict__ sp, int *__restrict__ scol,
+                                                        double *__restrict__ sval, int cols) {
+  const int64_t r = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  if (r >= (int64_t)nslice * PSFUN_SELL_C) return;
+  const int s = (int)(r / PSFUN_SELL_C), l = (int)(r % PSFUN_SELL_C);
+  const int b = sp[s];
+  const int w = (sp[s + 1] - b) / PSFUN_SELL_C;
+  const int start = r < n ? rp[r] : 0;
+  const int len = r < n ? rp[r + 1] - start : 0;
+  const int self = r < n ? (int)r : 0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t idx = (int64_t)b + (int64_t)k * PSFUN_SELL_C + l;
+    if (k < len) {
+      if (cols) scol[idx] = ci[start + k];
+      sval[idx] = va[start + k];
+    } else {
+      if (cols) scol[idx] = self;
+      sval[idx] = 0.0;
+    }
+  }
+}
+
+static void sell_free(psfun_csr *A) {
+  if (A->sell_ptr) hipFree(A->sell_ptr);
+  if (A->sell_col) hipFree(A->sell_col);
+  if (A->sell_val) hipFree(A->sell_val);
+  A->sell_ptr = A->sell_col = nullptr;
+  A->sell_val = nullptr;
+  A->sell_nnz = 0;
+  A->nslice = 0;
+}
+
+static int sell_launch_fill(psfun_csr *A, int cols) {
+  if (!A->sell_nnz || A->n == 0) return PSFUN_SUCCESS;
+  const int64_t threads = (int64_t)A->nslice * PSFUN_SELL_C;
+  hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + PSFUN_BS - 1) / PSFUN_BS)), dim3(PSFUN_BS), 0,
+                     A->ctx->stream, A->n, A->nslice, A->rowptr, A->colind, A->val, A->sell_ptr, A->sell_col,
+                     A->sell_val, cols);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+int psfun_sell_build(psfun_csr *A, const int *rp) {
+  sell_free(A);
+  const int n = A->n;
+  if (n == 0) return PSFUN_SUCCESS;
+  const int ns = (n + PSFUN_SELL_C - 1) / PSFUN_SELL_C;
+  std::vector<int> sp((size_t)ns + 1);
+  int64_t tot = 0;
+  for (int s = 0; s < ns; ++s) {
+    int w = 0;
+    const int r1 = std::min(n, (s + 1) * PSFUN_SELL_C);
+    for (int r = s * PSFUN_SELL_C; r < r1; ++r) w = std::max(w, rp[r + 1] - rp[r]);
+    if (tot > INT32_MAX) return PSFUN_SUCCESS;  // offsets would not fit: CSR only
+    sp[s] = (int)tot;
+    tot += (int64_t)w * PSFUN_SELL_C;
+  }
+  if (tot > INT32_MAX) return PSFUN_SUCCESS;
+  sp[ns] = (int)tot;
+  hipError_t e = hipMalloc(&A->sell_ptr, sizeof(int) * ((size_t)ns + 1));
+  if (e == hipSuccess) e = hipMalloc(&A->sell_col, sizeof(int) * (size_t)std::max<int64_t>(tot, 1));
+  if (e == hipSuccess) e = hipMalloc(&A->sell_val, sizeof(double) * (size_t)std::max<int64_t>(tot, 1));
+  if (e == hipSuccess) e = hipMemcpy(A->sell_ptr, sp.data(), sizeof(int) * ((size_t)ns + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    sell_free(A);
+    return PSFUN_SUCCESS;  // no room for the copy: CSR only
+  }
+  A->nslice = ns;
+  A->sell_nnz = tot;
+  PSFUN_TRY(sell_launch_fill(A, 1));
+  PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+  return PSFUN_SUCCESS;
+}
+
+int psfun_sell_refill(psfun_csr *A) { return sell_launch_fill(A, 0); }
+
+static int auto_format(const psfun_csr *A) {
+  if (A->sell_nnz > 0 && (double)A->sell_nnz <= 1.25 * (double)std::max<int64_t>(A->nnz, 1))
+    return PSFUN_FORMAT_SELL;
+  return PSFUN_FORMAT_CSR;
+}
+
 extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int32_t *rowptr,
                                 const int32_t *colind, const double *val, int base, int mem,
                                 psfun_csr_t *out) {
@@ -143,6 +254,7 @@ extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int
   A->ctx = c;
   A->n = (int)n;
   A->nnz = nnz;
+  std::vector<int> host_rp;
   if (mem == PSFUN_MEM_DEVICE) {
     if (base != 0) {
       delete A;
@@ -184,6 +296,7 @@ extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int
     if (e == hipSuccess && nnz) e = hipMemcpy(A->colind, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
     if (e == hipSuccess && nnz) e = hipMemcpy(A->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice);
     A->owns = true;
+    host_rp.swap(rp);
     if (e != hipSuccess) {
       psfun_record_hip_error(e, "psfun_csr_create", __FILE__, __LINE__);
       psfun_csr_destroy(A);
@@ -194,6 +307,23 @@ extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int
     return PSFUN_ERR_INVALID_ARG;
   }
   A->lanes = choose_lanes(n, nnz);
+  {
+    std::vector<int> rph;
+    const int *rpp = nullptr;
+    if (mem == PSFUN_MEM_DEVICE) {
+      rph.resize((size_t)n + 1);
+      PSFUN_HIP_TRY(hipMemcpy(rph.data(), A->rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+      rpp = rph.data();
+    } else {
+      rpp = host_rp.data();
+    }
+    const int st = psfun_sell_build(A, rpp);
+    if (st != PSFUN_SUCCESS) {
+      psfun_csr_destroy(A);
+      return st;
+    }
+  }
+  A->format = auto_format(A);
   *out = A;
   return PSFUN_SUCCESS;
 }
@@ -206,6 +336,11 @@ extern "C" int psfun_csr_destroy(psfun_csr_t A) {
     if (A->rowptr) hipFree(A->rowptr);
     if (A->colind) hipFree(A->colind);
     if (A->val) hipFree(A->val);
+  }
+  if (A->sell_nnz || A->sell_ptr) {
+    hipSetDevice(A->ctx->device);
+    hipStreamSynchronize(A->ctx->stream);
+    sell_free(A);
   }
   delete A;
   return PSFUN_SUCCESS;
@@ -220,7 +355,7 @@ extern "C" int psfun_csr_set_values(psfun_csr_t A, const double *val) {
   } else {
     A->val = const_cast<double *>(val);
   }
-  return PSFUN_SUCCESS;
+  return psfun_sell_refill(A);
 }
 
 extern "C" int psfun_csr_set_lanes(psfun_csr_t A, int lanes) {
@@ -239,6 +374,28 @@ extern "C" int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lane
   if (n) *n = A->n;
   if (nnz) *nnz = A->nnz;
   if (lanes) *lanes = A->lanes;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_set_format(psfun_csr_t A, int format) {
+  if (!A) return PSFUN_ERR_INVALID_ARG;
+  if (format == PSFUN_FORMAT_AUTO) {
+    A->format = auto_format(A);
+  } else if (format == PSFUN_FORMAT_CSR) {
+    A->format = PSFUN_FORMAT_CSR;
+  } else if (format == PSFUN_FORMAT_SELL) {
+    if (A->sell_nnz == 0 && A->n > 0) return PSFUN_ERR_UNSUPPORTED;
+    A->format = PSFUN_FORMAT_SELL;
+  } else {
+    return PSFUN_ERR_INVALID_ARG;
+  }
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_entries) {
+  if (!A) return PSFUN_ERR_INVALID_ARG;
+  if (format) *format = A->format;
+  if (sell_entries) *sell_entries = A->sell_nnz;
   return PSFUN_SUCCESS;
 }
 
@@ -279,7 +436,7 @@ extern "C" int psfun_vec_download(psfun_ctx_t c, double *h, const double *d, int
 // ---------------------------------------------------------------------------
 // Per-kernel timing
 // ---------------------------------------------------------------------------
-int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag) {
+int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag, int reps) {
   if (!c->prof_on) return PSFUN_SUCCESS;
   if (c->prof_used + 1 >= PSFUN_PROF_MAXPAIRS) {  // pool full: fold finished pairs first
     PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -295,10 +452,12 @@ int psfun_prof_begin(psfun_ctx *c, int kid, double bytes, int tag) {
     c->prof_kid.resize(c->prof_used + 1);
     c->prof_tag.resize(c->prof_used + 1);
     c->prof_bytes_pending.resize(c->prof_used + 1);
+    c->prof_reps.resize(c->prof_used + 1);
   }
   c->prof_kid[c->prof_used] = kid;
   c->prof_tag[c->prof_used] = tag;
-  c->prof_bytes_pending[c->prof_used] = bytes;
+  c->prof_bytes_pending[c->prof_used] = bytes * reps;
+  c->prof_reps[c->prof_used] = reps;
   PSFUN_HIP_TRY(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
   return PSFUN_SUCCESS;
 }
@@ -324,7 +483,7 @@ int psfun_prof_collect(psfun_ctx *c) {
     if (k >= 0 && k < PSFUN_PROF_NK) {
       c->prof_ms[k] += ms;
       c->prof_bytes[k] += c->prof_bytes_pending[i];
-      c->prof_count[k] += 1;
+      c->prof_count[k] += c->prof_reps[i];
     }
   }
   c->prof_used = 0;
@@ -337,7 +496,7 @@ extern "C" int psfun_prof_enable(psfun_ctx_t c, int on) {
     PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
     PSFUN_TRY(psfun_prof_collect(c));
   }
-  c->prof_on = on ? 1 : 0;
+  c->prof_on = on <= 0 ? 0 : (on > 16 ? 16 : on);
   return PSFUN_SUCCESS;
 }
 

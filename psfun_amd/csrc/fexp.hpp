@@ -20,7 +20,7 @@
 #pragma once
 #include "common.hpp"
 
-#define PSFUN_CHEB_CAP 1024     // coefficient table capacity (doubles)
+#define PSFUN_CHEB_CAP 704      // coefficient table capacity (doubles; r + 12 sqrt(r) + 42 at r = RMAX)
 #define PSFUN_CHEB_RMAX 400.0   // largest half-width per sub-step
 #define PSFUN_FT_KMAX 512       // largest order of T
 

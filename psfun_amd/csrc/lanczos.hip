@@ -22,23 +22,31 @@
 // atomics or fences are needed.  When the control block ends the loop at
 // index j, the SpMV/update it overlapped with feed nothing that is returned
 // (the reference's own SpMV at the exit index is unused too).  Every kernel
-// returns at once when ctl->done is set; the host polls that flag every few
-// iterations, one chunk behind the GPU.
+// returns at once when ctl->done is set.  The control block also stores its
+// loop index and stop state to coherent pinned host memory, on which the host
+// paces its launches (no flag copies or events in the loop).
 #include "common.hpp"
 #include "kernels.hpp"
 #include "fexp.hpp"
 #include "lanczos.hpp"
 
 // Reference loop control at index i = j (j >= 2): beta = T(j-1, j).
+// Stop the loop: device state for the kernels, host flag for the launcher.
+__device__ __forceinline__ void lz_stop(LzCtl *ctl, int *hf, int state) {
+  ctl->done = state;
+  host_flag_store(hf + 1, state);
+}
+
 __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, double eps, double fscale,
-                           ChebLds *W) {
+                           ChebLds *W, int *hf) {
   const int lane = threadIdx.x;
   if (lane == 0) {
+    host_flag_store(hf, j);
     ctl->beta[j - 1] = beta;
     ctl->itx = j - 1;
   }
   if (fabs(beta) < 1e-16) {  // psfun_d_lanczos.f90:225-230
-    if (lane == 0) ctl->done = LZ_DONE_BREAKDOWN;
+    if (lane == 0) lz_stop(ctl, hf, LZ_DONE_BREAKDOWN);
     return;
   }
   if (istop == 1) {
@@ -53,12 +61,12 @@ __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, do
       ctl->rs[j] = rsj;
       if (rsj < eps) {
         ctl->m = j - 1;  // yk holds f(T_{j-1}) e1
-        ctl->done = LZ_DONE_CONVERGED;
+        lz_stop(ctl, hf, LZ_DONE_CONVERGED);
       }
     }
     if (rsj < eps) return;
   }
-  if (j >= nl && lane == 0) ctl->done = LZ_DONE_MAXIT;
+  if (j >= nl && lane == 0) lz_stop(ctl, hf, LZ_DONE_MAXIT);
 }
 
 template <int L>
@@ -69,7 +77,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
                                                       const double *__restrict__ p_in, int g_in,
                                                       double *__restrict__ v_out, double *__restrict__ u_out,
                                                       double *__restrict__ p_out, LzCtl *ctl, int nl, int istop,
-                                                      double eps, double fscale) {
+                                                      double eps, double fscale, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
   const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
@@ -77,14 +85,15 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
     if (threadIdx.x >= 64) return;
     if (j == 1) {
       if (threadIdx.x == 0) {
+        host_flag_store(hf, 1);
         ctl->rs1 = sigma;
         ctl->rs[1] = sigma;
-        if (sigma == 0.0) ctl->done = LZ_DONE_ZERO;
+        if (sigma == 0.0) lz_stop(ctl, hf, LZ_DONE_ZERO);
       }
       return;
     }
     __shared__ ChebLds W;
-    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W);
+    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
   if (sigma == 0.0 && j == 1) return;
@@ -115,15 +124,65 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
   if (threadIdx.x == 0) p_out[blockIdx.x] = dot;
 }
 
+// Same step on the sliced-ELL copy: block 0 is the control block (dispatched
+// first), blocks 1..G stream one row per thread.
+// 8 waves per SIMD: at most 64 VGPRs, so 8 blocks of 256 fit a CU.
+__global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *__restrict__ sp,
+                                                           const int *__restrict__ col,
+                                                           const double *__restrict__ val, double opscale,
+                                                           const double *__restrict__ src, int j, int G,
+                                                           const double *__restrict__ p_in, int g_in,
+                                                           double *__restrict__ v_out, double *__restrict__ u_out,
+                                                           double *__restrict__ p_out, LzCtl *ctl, int nl,
+                                                           int istop, double eps, double fscale,
+                                                           int *hf) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (blockIdx.x == 0) {  // control block
+    if (threadIdx.x >= 64) return;
+    if (j == 1) {
+      if (threadIdx.x == 0) {
+        host_flag_store(hf, 1);
+        ctl->rs1 = sigma;
+        ctl->rs[1] = sigma;
+        if (sigma == 0.0) lz_stop(ctl, hf, LZ_DONE_ZERO);
+      }
+      return;
+    }
+    __shared__ ChebLds W;
+    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
+    return;
+  }
+  if (sigma == 0.0 && j == 1) return;
+  const double inv = 1.0 / sigma;
+  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
+  const XcdRange xr = xcd_range_c0(nchunks, G);
+  double dot = 0.0;
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * PSFUN_BS + threadIdx.x);
+    if (row < n) {
+      const double sum = sell_row_dot(row, sp, col, val, src, inv);
+      const double v = src[row] * inv;
+      const double u = opscale * sum;
+      v_out[row] = v;
+      u_out[row] = u;
+      dot = fma(u, v, dot);
+    }
+  }
+  dot = block_sum<PSFUN_BS>(dot, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
+}
+
 // Loop control at the last index i = nl (no SpMV needed there).
 __global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in, int g_in, int j, LzCtl *ctl,
-                                                int nl, int istop, double eps, double fscale) {
+                                                int nl, int istop, double eps, double fscale, int *hf) {
   __shared__ ChebLds W;
   if (ctl->done) return;
   double acc = 0.0;
   for (int b = threadIdx.x; b < g_in; b += 64) acc += p_in[b];
   const double beta = sqrt(wave_sum(acc));
-  lz_control(ctl, j, beta, nl, istop, eps, fscale, &W);
+  lz_control(ctl, j, beta, nl, istop, eps, fscale, &W, hf);
 }
 
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restrict__ u,
@@ -213,11 +272,25 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   const int L = A->lanes;
   // algorithmic bytes: val+colind, rowptr, src rows, v and u written
   const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j));
+  // The step is idempotent (it reads src and the partials and rewrites v_j, u
+  // and its own partials and control fields with the same values), so timing
+  // may repeat it back to back inside one event pair to amortise the events.
+  const int reps = psfun_prof_reps(c);
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
+  if (A->format == PSFUN_FORMAT_SELL) {
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_lz_spmv_sell, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
+                       A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
+                       istop, eps, fscale, c->d_flags);
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    return PSFUN_SUCCESS;
+  }
 #define LZ_LAUNCH(LL)                                                                                         \
   hipLaunchKernelGGL((k_lz_spmv<LL>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr, A->colind, \
-                     A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl, istop, eps, fscale)
-  switch (L) {
+                     A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl, istop, eps, fscale, \
+                     c->d_flags)
+  for (int r = 0; r < reps; ++r) switch (L) {
     case 1: LZ_LAUNCH(1); break;
     case 2: LZ_LAUNCH(2); break;
     case 4: LZ_LAUNCH(4); break;
@@ -260,11 +333,17 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   double *Pw = c->d_partials;                      // ||w||^2 (and ||x||^2) partials
   double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
 
-  const int L = A->lanes;
+  const bool sell = A->format == PSFUN_FORMAT_SELL;
+  const int L = sell ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  const int Gs = psfun_grid(c, chunks, 4);                          // SpMV streaming blocks
+  // SpMV streaming blocks; with sliced ELL 8 blocks per CU are resident, and
+  // the grid (control block + Gs) is kept within one residency round
+  int Gs = psfun_grid(c, chunks, sell ? 8 : 4);
+  if (sell && Gs + 1 > 8 * c->num_cu) Gs = (8 * c->num_cu - 1) / 8 * 8;
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);  // update blocks
 
+  c->h_flags[0] = 0;
+  c->h_flags[1] = 0;
   hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(256), 0, c->stream, ctl);
   PSFUN_HIP_TRY(hipGetLastError());
   // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; partials of T(1,1) = u.v_1
@@ -272,9 +351,10 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   PSFUN_HIP_TRY(hipGetLastError());
   PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, U[0], Pa, ctl, nl, istop, eps, fscale));
 
-  volatile int *hflag = (volatile int *)c->h_pinned;
-  const int poll = 2;
-  int pending = -1;  // slot with an outstanding flag copy
+  // Pacing: before queueing step j+2, wait until step j-1 has reported
+  // (h_flags[0] >= j-1).  The stores from a running kernel may only reach the
+  // host when that kernel ends, so this keeps two steps queued ahead of the
+  // GPU; at most two steps run as no-ops after a stop.
   for (int j = 1; j <= nl - 1; ++j) {
     double *u = U[(j - 1) & 1];
     const double *vj = V + (size_t)ldv * (j - 1);
@@ -288,22 +368,13 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
       PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V + (size_t)ldv * j, U[j & 1], Pa, ctl, nl,
                                istop, eps, fscale));
     } else {
-      hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl, istop, eps, fscale);
+      hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl, istop, eps, fscale,
+                         c->d_flags);
       PSFUN_HIP_TRY(hipGetLastError());
     }
-    if (j % poll == 0 && j < nl - 1) {
-      const int slot = (j / poll) & 1;
-      PSFUN_HIP_TRY(hipMemcpyAsync((void *)(hflag + slot), &ctl->done, sizeof(int), hipMemcpyDeviceToHost,
-                                   c->stream));
-      PSFUN_HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
-      if (pending >= 0) {
-        PSFUN_HIP_TRY(hipEventSynchronize(c->ev[pending]));
-        if (hflag[pending] != 0) {
-          pending = slot;
-          break;
-        }
-      }
-      pending = slot;
+    if (j < nl - 1 && j >= 2) {
+      PSFUN_TRY(psfun_host_wait(c, j - 1));
+      if (c->h_flags[1] != 0) break;
     }
   }
   hipLaunchKernelGGL(k_lz_final, dim3(1), dim3(64), 0, c->stream, ctl, fscale);
@@ -321,7 +392,8 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   const int itx = h->itx;
   // timing pairs of launches that came after the stop did no work: SpMV steps
   // 1..itx+1 and updates 1..itx ran (the last SpMV overlapped the stop test)
-  psfun_prof_discard_after(c, PSFUN_PROF_LZ_SPMV, itx + 1);
+  // (with repeats, the repeats of the stopping step itself return at once)
+  psfun_prof_discard_after(c, PSFUN_PROF_LZ_SPMV, psfun_prof_reps(c) > 1 ? itx : itx + 1);
   psfun_prof_discard_after(c, PSFUN_PROF_LZ_UPDATE, itx);
   const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
   if (iter) *iter = itx;

@@ -153,6 +153,29 @@ class psb_dspmat_type:
         _lib.check(_lib.lib().psfun_csr_info(self.handle, None, None, C.byref(v)))
         return v.value
 
+    def set_values(self, vals):
+        """Replace the nonzero values (same pattern) from a float64 device tensor."""
+        v = vals.detach().contiguous()
+        if not (v.is_cuda and v.dtype == torch.float64 and v.numel() == self.nnz):
+            raise ValueError("set_values needs a float64 device tensor with nnz entries")
+        self.ctxt.bind_stream()
+        _lib.check(_lib.lib().psfun_csr_set_values(self.handle, _dptr(v)))
+        torch.cuda.current_stream().synchronize()
+
+    _FORMATS = {"AUTO": _lib.PSFUN_FORMAT_AUTO, "CSR": _lib.PSFUN_FORMAT_CSR, "SELL": _lib.PSFUN_FORMAT_SELL}
+
+    def set_format(self, fmt):
+        """'CSR' (vector CSR), 'SELL' (sliced ELL, 64-row slices) or 'AUTO'."""
+        _lib.check(_lib.lib().psfun_csr_set_format(self.handle, self._FORMATS[str(fmt).upper()]))
+
+    def get_format(self):
+        """(format name, padded sliced-ELL entry count)."""
+        f = C.c_int(0)
+        e = C.c_int64(0)
+        _lib.check(_lib.lib().psfun_csr_get_format(self.handle, C.byref(f), C.byref(e)))
+        name = {v: k for k, v in self._FORMATS.items()}[f.value]
+        return name, e.value
+
     def free(self):
         if getattr(self, "handle", None):
             _lib.lib().psfun_csr_destroy(self.handle)

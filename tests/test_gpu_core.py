@@ -57,6 +57,43 @@ def test_spmv_ragged_all_lane_widths(gpu_ctx, lanes):
         assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 3001])
+def test_spmv_sliced_ell_matches_oracle(gpu_ctx, n):
+    """Sliced ELL (64-row slices): partial last slice, empty rows, one long
+    row (heavy padding), and a value refresh through set_values."""
+    A = _ragged(n, 11)
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format("SELL")
+    fmt, entries = M.get_format()
+    nslice = (n + 63) // 64
+    lens = np.diff(A[0])
+    widths = [lens[64 * s:64 * s + 64].max() for s in range(nslice)]
+    assert fmt == "SELL" and entries == 64 * sum(widths)
+    x = matgen.uniform_pm1(12, n)
+    y0 = matgen.uniform_pm1(13, n)
+    desc = psb.psb_desc_type(n)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        y = _vec(y0)
+        assert psb.psb_spmm(alpha, M, _vec(x), beta, y, desc) == 0
+        ref = O.spmv(A, x, alpha, beta, y0)
+        assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
+    v2 = matgen.normal(14, len(A[2]))
+    M.set_values(_dev(v2))
+    y = _vec(np.zeros(n))
+    psb.psb_spmm(1.0, M, _vec(x), 0.0, y, desc)
+    ref = O.spmv((A[0], A[1], v2), x)
+    assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
+
+
+def test_format_auto_choice(gpu_ctx):
+    M = psb.psb_dspmat_type.from_csr(matgen.laplacian((40, 40, 3)))
+    assert M.get_format()[0] == "SELL"        # uniform rows: padding ~0
+    M = psb.psb_dspmat_type.from_csr(_ragged(3001, 1))
+    assert M.get_format()[0] == "CSR"         # one 150-entry row pads its slice
+    M.set_format("SELL")
+    assert M.get_format()[0] == "SELL"
+
+
 def test_spmv_one_based_input(gpu_ctx):
     g = load_golden("laplacian50.npz")
     M = psb.psb_dspmat_type(g["rowptr"], g["colind"], g["val"], index_base=1)
@@ -116,9 +153,10 @@ def test_serial_rejects_default_variant(gpu_ctx):
     assert fun.apply(np.eye(3), out, np.ones(3)) == psb.psb_err_from_subroutine_
 
 
-def _lanczos_gpu(A, x, eps, itmax, istop=1):
+def _lanczos_gpu(A, x, eps, itmax, istop=1, fmt="AUTO"):
     n = len(x)
     M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format(fmt)
     desc = psb.psb_desc_type(n)
     fun = psfun_d_serial()
     fun.set("VARIANT", "CHBSYM")
@@ -129,11 +167,12 @@ def _lanczos_gpu(A, x, eps, itmax, istop=1):
     return info, it, err, res, y.get_vect()
 
 
+@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
 @pytest.mark.parametrize("dims,itmax", [((100, 100), 30), ((12, 10, 9), 50), ((64, 64, 64), 50)])
-def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax):
+def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     A = matgen.laplacian(dims, scale=-1.0)
     x = matgen.uniform_pm1(1, int(np.prod(dims)))
-    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, itmax)
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, itmax, fmt=fmt)
     assert info == 0
     ref = O.lanczos(A, x, 1e-10, itmax, 1)
     rel = np.linalg.norm(y - ref["y"]) / np.linalg.norm(ref["y"])
