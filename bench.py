@@ -52,10 +52,25 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--prof-steps", type=int, default=3)
+    p.add_argument("--copy-probe", action="store_true", help="also run the 1 GiB copy that scales PMC counters")
     p.add_argument("--prof-reps", type=int, default=4, help="back-to-back launches per timed SpMV step")
     p.add_argument("--no-extra", action="store_true", help="headline only (no batched/adjoint/quadrature)")
     p.add_argument("--format", default="", help="SELL, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     return p.parse_args()
+
+
+def pmc_traffic(kernel, workload_file):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy of known size), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", workload_file)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch " + kernel + " "):
+                    return float(line.split()[-1])
+    except OSError:
+        pass
+    return None
 
 
 def read_prof(L, _lib, ctx, name):
@@ -151,11 +166,13 @@ def headline(args):
     # counted by the library)
     per_launch_bytes = k["bytes"] / max(k["launches"], 1)
     avg_us = 1e3 * k["ms"] / max(k["launches"], 1)
-    achieved = per_launch_bytes / (avg_us * 1e-6) / 1e9
+    achieved = per_launch_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     upd_us = 1e3 * ku["ms"] / max(ku["launches"], 1)
     upd_bytes = ku["bytes"] / max(ku["launches"], 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": (pmc_traffic("k_lz_spmv_sell", "lz_spmv_sell_128cube_pmc.txt")
+                            if fmt == "SELL" and m == 128 else None),
                 "kernel": ("k_lz_spmv_sell" if fmt == "SELL" else f"k_lz_spmv<{lanes}>")
                 + " (SpMV 12*nnz+4(n+1) algorithmic matrix bytes + w read, v and u written: 24n)",
                 "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_us, 3),
@@ -361,13 +378,33 @@ def quadrature_section(args):
             "seed_spmv_GBs": round(ks["bytes"] / max(ks["ms"], 1e-9) / 1e6, 1)}
 
 
+def copy_probe():
+    """One y = 1.0*x over 2^27 doubles (1 GiB read + 1 GiB written) through
+    psfun_axpby: the known-size kernel that scales the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters into bytes."""
+    import ctypes as C
+    import torch
+    from psfun_amd import _lib, psb
+    ctx = psb.default_context(0)
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ctx.bind_stream()
+    _lib.check(_lib.lib().psfun_axpby(ctx.handle, n, 1.0, C.c_void_p(x.data_ptr()), 0.0, C.c_void_p(y.data_ptr())))
+    torch.cuda.synchronize()
+    log(f"copy probe: {n} doubles, y == x: {bool(torch.equal(x, y))}")
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the f(A)x path on a single MI355X "
-              "(multi-GPU row sharding is a design note, not part of this build).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the f(A)x path here is single-GPU "
+              "(multi-GPU row sharding is not implemented).", file=sys.stderr)
         sys.exit(2)
     value, elapsed, it, rel_err, roofline, cpu, cfg = headline(args)
+    if args.copy_probe:
+        copy_probe()
     extra = {}
     if not args.no_extra:
         for name, fn in (("batched", batched_section), ("adjoint", adjoint_section),

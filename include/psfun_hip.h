@@ -92,7 +92,9 @@ enum {
   PSFUN_PROF_MS_UPDATE = 5,  /* multi-shift CG per-shift updates         */
   PSFUN_PROF_CG = 6,         /* shifted CG fused kernels                 */
   PSFUN_PROF_BATCH = 7,      /* batched Lanczos step kernels             */
-  PSFUN_PROF_SDDMM = 8       /* gradient SDDMM                           */
+  PSFUN_PROF_SDDMM = 8,      /* gradient SDDMM                           */
+  PSFUN_PROF_AR_SPMV = 9,    /* Arnoldi SpMV step (+ control block)      */
+  PSFUN_PROF_AR_MGS = 10     /* Arnoldi modified Gram-Schmidt step       */
 };
 /* Turn timing on/off (off by default); reset clears the totals.  on = 1
  * brackets each timed launch by two events; on = R in 2..16 also launches the
@@ -150,9 +152,12 @@ int psfun_axpby(psfun_ctx_t ctx, int64_t n, double alpha, const double *x, doubl
 int psfun_scal(psfun_ctx_t ctx, int64_t n, double alpha, double *x);
 
 /* ---- f(H) step on a small dense matrix (psfun_d_serial%apply) ------------ */
-/* y = f(scaling*H) x for a k-by-k column-major host matrix H (k <= 256),
- * evaluated on the device.  fname = PSFUN_FN_EXP.  Symmetric variants
- * (CHBSYM) require a symmetric tridiagonal H; the others accept any H. */
+/* y = f(scaling*H) x for a k-by-k column-major host matrix H (k <= 512),
+ * evaluated on the device (psfun_d_serial_apply_array,
+ * serial/psfun_d_serial_mod.F90:172-523).  fname = PSFUN_FN_EXP.  The variant
+ * fixes how H is read, as the reference's EXPOKIT routines read it: CHBSYM as
+ * symmetric from its upper triangle (DSCHBV), CHBHES as upper Hessenberg
+ * (DNCHBV), CHBGEN/GENPADE/TAYLOR/SASQ as a full matrix. */
 int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scaling, int k,
                             const double *H, const double *x, double *y);
 
@@ -166,6 +171,17 @@ int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scal
 int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter,
                         double *err, double *res, int *nres);
+
+/* y = ||x|| V_k f(fscale*H_k) e1 by Arnoldi (modified Gram-Schmidt) with the
+ * reference's stopping rule and assembly (psfun_d_arnoldi.f90:183-329): the
+ * Krylov operator is scaling*A (the reference multiplies its SpMV by
+ * fun%scaling and evaluates f with scaling 1, lines 172-173, 202), f(H) is
+ * read as `variant` reads a matrix (see psfun_serial_apply_host).  No
+ * breakdown test, as in the reference; x = 0 gives y = 0 and iter = 0.
+ * itmax in 2..512, istop in {1,2}; outputs as for psfun_lanczos_apply. */
+int psfun_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, double fscale, double scaling,
+                        const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+                        double *res, int *nres);
 
 /* ---- batched Lanczos f(A)x ------------------------------------------------ */
 /* A batch is a set of ngraph independent square CSR matrices stored

@@ -35,7 +35,7 @@ PSFUN_QUAD_RE_SUM = 0
 PSFUN_QUAD_A_IM_SUM = 1
 
 PROF_IDS = {"lz_spmv": 0, "lz_update": 1, "lz_assemble": 2, "spmv": 3, "ms_spmv": 4, "ms_update": 5,
-            "cg": 6, "batch": 7, "sddmm": 8}
+            "cg": 6, "batch": 7, "sddmm": 8, "ar_spmv": 9, "ar_mgs": 10}
 
 _vp = C.c_void_p
 _i = C.c_int
@@ -77,6 +77,7 @@ SIGNATURES = {
     "psfun_scal": (_i, [_vp, _i64, _d, _vp]),
     "psfun_serial_apply_host": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _vp]),
     "psfun_lanczos_apply": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
+    "psfun_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
     "psfun_batch_create": (_i, [_vp, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "psfun_batch_destroy": (_i, [_vp]),
     "psfun_batch_set_tuning": (_i, [_vp, _i, _i]),

@@ -1,0 +1,44 @@
+"""psfun_d_arnoldi (krylov/polynomial/psfun_d_arnoldi.f90:37-352) on the GPU.
+
+The whole loop runs on the device: SpMV, modified Gram-Schmidt, the f(H) step
+with the reference's a-posteriori estimate, and the assembly.  As in the
+reference, the SpMV is multiplied by fun%scaling while the f(H) step uses
+scaling 1 (lines 172-173, 202); f(H) reads H as fun%variant reads a matrix.
+
+arnoldi_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None)
+    -> (info, iter, err, res)    with y overwritten in place.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import psb
+
+
+def arnoldi_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None):
+    istop_ = 1 if istop is None else int(istop)            # arnoldi:99-110
+    if istop_ < 1 or istop_ > 2:
+        return psb.psb_err_invalid_istop_, 0, float(psb.psb_err_invalid_istop_), None
+    nl = 200 if itmax is None else int(itmax)              # arnoldi:113-118
+    if x is None or x.v is None or y is None or y.v is None:
+        return psb.psb_err_invalid_vect_state_, 0, 0.0, None
+    n = desc_a.get_global_rows() if desc_a is not None else a.n
+    if x.get_nrows() != n or y.get_nrows() != n or a.n != n:
+        return psb.psb_err_from_subroutine_, 0, 0.0, None
+    st, var = fun.device_variant()
+    if st != _lib.PSFUN_SUCCESS:
+        return psb.psb_err_from_subroutine_non_, 0, 0.0, None
+    ctx = a.ctxt
+    ctx.bind_stream()
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    nres = C.c_int(0)
+    res = np.zeros(max(nl, 2) + 2)
+    st = _lib.lib().psfun_arnoldi_apply(ctx.handle, a.handle, _lib.PSFUN_FN_EXP, var, 1.0, float(fun.scaling),
+                                        C.c_void_p(x.v.data_ptr()), C.c_void_p(y.v.data_ptr()), float(eps), nl,
+                                        istop_, C.byref(it), C.byref(err), res.ctypes.data_as(C.POINTER(C.c_double)),
+                                        C.byref(nres))
+    if st != _lib.PSFUN_SUCCESS:
+        return psb.status_to_info(st), 0, 0.0, None
+    return psb.psb_success_, it.value, err.value, res[: nres.value].copy()

@@ -1,0 +1,358 @@
+// Device-resident Arnoldi f(A)x with the reference's semantics
+// (krylov/polynomial/psfun_d_arnoldi.f90:183-329).
+//
+// Reference loop (i = 2..nl):  itx = i-1; w = scaling*A v_{i-1};
+// for k = 1..i-1: h(k,i-1) = <v_k, w>; w -= h(k,i-1) v_k   (modified Gram-Schmidt);
+// h(i,i-1) = ||w||; v_i = (1/h(i,i-1)) w; if istop == 1: yk = f(H(1:i-1,1:i-1)) e1,
+// rs(i) = rs(1) h(i,i-1) |yk(i-1)| / ||yk||, exit if rs(i) < eps.  No breakdown
+// test (an exact breakdown gives rs(i) = 0 and ends the loop by convergence).
+// Assembly: y = rs(1) (yk(1) v_1 + sum_{l=2}^{itx-1} yk(l) v_l)  (lines 318-324).
+//
+// Device schedule per column j = 1, 2, ...:
+//   k_ar_spmv(j):   all blocks fold the ||w||^2 partials into h(j,j-1) (||x||
+//                   for j = 1), write v_j = (1/h) w once and u = scaling*A v_j
+//                   (gathering w*(1/h) on the fly), and leave partials of
+//                   <v_1, u>.  Block 0 is the control block: it runs the
+//                   reference's control for loop index i = j (the a-posteriori
+//                   estimate from f(H_{j-1}) e1, by the dense block exp of
+//                   fdense.hpp) while the streaming blocks work.
+//   k_ar_mgs(k), k = 1..j: fold <v_k, w> into h(k,j), w -= h(k,j) v_k in place,
+//                   leave partials of <v_{k+1}, w> (k < j) or ||w||^2 (k = j).
+// Scalars cross kernels through kernel boundaries only; the control block
+// stores its loop index and stop state to host-visible flags on which the
+// host paces its launches (as in the Lanczos path).
+#include "common.hpp"
+#include "kernels.hpp"
+#include "fdense.hpp"
+#include "lanczos.hpp"
+
+// Reference control at loop index i = j >= 2, with h(j,j-1) = sigma.  All
+// threads of the control block.
+__device__ void ar_control(LzCtl *ctl, double *H, int ldh, int model, int j, double sigma, int nl, int istop,
+                           double eps, double fscale, DenseLds *W, int *hf) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    host_flag_store(hf, j);
+    H[(int64_t)(j - 2) * ldh + (j - 1)] = sigma;  // h(j, j-1)
+    ctl->itx = j - 1;
+  }
+  if (istop == 1) {
+    const int k = j - 1;
+    for (int i = tid; i < k; i += PSFUN_BS) W->y[i] = (i == 0) ? 1.0 : 0.0;
+    __syncthreads();
+    block_expmv(k, H, ldh, model, fscale, W);
+    double ss = 0.0;
+    for (int i = tid; i < k; i += PSFUN_BS) {
+      const double v = W->y[i];
+      ss = fma(v, v, ss);
+      ctl->yk[i] = v;
+    }
+    const double nrm = sqrt(dense_block_sum(ss, W));
+    const double errnum = ctl->rs1 * sigma * fabs(W->y[k - 1]);
+    const double rsj = errnum / nrm;
+    if (tid == 0) {
+      ctl->errnum = errnum;
+      ctl->errden = nrm;
+      ctl->rs[j] = rsj;
+      if (rsj < eps) {
+        ctl->m = k;
+        ctl->done = LZ_DONE_CONVERGED;
+        host_flag_store(hf + 1, LZ_DONE_CONVERGED);
+      }
+    }
+    if (rsj < eps) return;
+  }
+  if (j >= nl && tid == 0) {
+    ctl->done = LZ_DONE_MAXIT;
+    host_flag_store(hf + 1, LZ_DONE_MAXIT);
+  }
+}
+
+template <int L, bool SELL>
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                                      const double *__restrict__ val, double scaling,
+                                                      const double *__restrict__ src, int j, int G,
+                                                      const double *__restrict__ p_in, int g_in,
+                                                      double *__restrict__ V, int64_t ldv, double *__restrict__ u_out,
+                                                      double *__restrict__ p_out, LzCtl *ctl, double *H, int ldh,
+                                                      int model, int nl, int istop, double eps, double fscale,
+                                                      int *hf) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (blockIdx.x == 0) {  // control block
+    if (j == 1) {
+      if (threadIdx.x == 0) {
+        host_flag_store(hf, 1);
+        ctl->rs1 = sigma;
+        ctl->rs[1] = sigma;
+        if (sigma == 0.0) {  // x = 0: y = 0 (the reference would divide by zero)
+          ctl->done = LZ_DONE_ZERO;
+          host_flag_store(hf + 1, LZ_DONE_ZERO);
+        }
+      }
+      return;
+    }
+    __shared__ DenseLds W;
+    ar_control(ctl, H, ldh, model, j, sigma, nl, istop, eps, fscale, &W, hf);
+    return;
+  }
+  if (sigma == 0.0 && j == 1) return;
+  const double inv = 1.0 / sigma;
+  double *vj = V + (int64_t)(j - 1) * ldv;
+  constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
+  const int64_t nchunks = (n + RPB - 1) / RPB;
+  const XcdRange xr = xcd_range_c0(nchunks, G);
+  const int sub = SELL ? threadIdx.x : threadIdx.x / L, lane = SELL ? 0 : threadIdx.x % L;
+  double dot = 0.0;
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * RPB + sub);
+    double sum = 0.0;
+    if (SELL) {
+      if (row < n) sum = sell_row_dot(row, rp, col, val, src, inv);
+    } else {
+      if (row < n) {
+        const int s = rp[row], e = rp[row + 1];
+        for (int q = s + lane; q < e; q += L) sum = fma(val[q], src[col[q]] * inv, sum);
+      }
+#pragma unroll
+      for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    }
+    if (row < n && lane == 0) {
+      const double v = src[row] * inv;
+      const double u = scaling * sum;
+      vj[row] = v;
+      u_out[row] = u;
+      const double v1 = (j == 1) ? v : V[row];
+      dot = fma(v1, u, dot);
+    }
+  }
+  dot = block_sum<PSFUN_BS>(dot, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
+}
+
+// One modified Gram-Schmidt step k of column j (1-based).
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_mgs(int n, int k, int j, const double *__restrict__ V, int64_t ldv,
+                                                     double *__restrict__ w, const double *__restrict__ p_in, int g_in,
+                                                     double *__restrict__ p_out, LzCtl *ctl, double *H, int ldh) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double h = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) H[(int64_t)(j - 1) * ldh + (k - 1)] = h;  // h(k, j)
+  const double *vk = V + (int64_t)(k - 1) * ldv;
+  const double *vn = V + (int64_t)k * ldv;  // v_{k+1} (read only when k < j)
+  const bool last = (k == j);
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const double x = fma(-h, vk[i], w[i]);
+    w[i] = x;
+    acc = last ? fma(x, x, acc) : fma(vn[i], x, acc);
+  }
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
+}
+
+// Control at the last loop index i = nl (no SpMV needed there).
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_ctrl(const double *__restrict__ p_in, int g_in, int j, LzCtl *ctl,
+                                                      double *H, int ldh, int model, int nl, int istop, double eps,
+                                                      double fscale, int *hf) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  __shared__ DenseLds W;
+  if (ctl->done) return;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  ar_control(ctl, H, ldh, model, j, sigma, nl, istop, eps, fscale, &W, hf);
+}
+
+// Coefficients when the loop did not end by convergence:
+// yk = f(H(1:itx-1,1:itx-1)) e1 (psfun_d_arnoldi.f90:259-296); the 1x1 block
+// stands in for itx-1 = 0.
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_final(LzCtl *ctl, const double *H, int ldh, int model,
+                                                       double fscale) {
+  __shared__ DenseLds W;
+  const int done = ctl->done;
+  if (done == LZ_DONE_CONVERGED || done == LZ_DONE_ZERO) return;
+  int k = ctl->itx - 1;
+  if (k < 1) k = 1;
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) W.y[i] = (i == 0) ? 1.0 : 0.0;
+  __syncthreads();
+  block_expmv(k, H, ldh, model, fscale, &W);
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) ctl->yk[i] = W.y[i];
+  if (threadIdx.x == 0) ctl->m = k;
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_zero(double *H, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < count; i += (int64_t)gridDim.x * PSFUN_BS)
+    H[i] = 0.0;
+}
+
+// y = rs1 * (yk(1) v_1 + sum_{l=2}^{m'} yk(l) v_l); m' = max(1, m-1) after
+// convergence (the last coefficient is dropped), m otherwise.
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_assemble(int n, const double *__restrict__ V, int64_t ldv,
+                                                          const LzCtl *ctl, double *__restrict__ y) {
+  const int done = ctl->done;
+  int m = ctl->m;
+  if (done == LZ_DONE_CONVERGED) m = (m - 1 < 1) ? 1 : m - 1;
+  if (done == LZ_DONE_ZERO) m = 0;
+  const double rs1 = ctl->rs1;
+  __shared__ double yk[LZ_NL_MAX];
+  for (int l = threadIdx.x; l < m; l += PSFUN_BS) yk[l] = ctl->yk[l];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+    double acc = 0.0;
+    if (m > 0) {
+      acc = yk[0] * V[i];
+      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * ldv + i], acc);
+    }
+    y[i] = rs1 * acc;
+  }
+}
+
+__global__ void k_ar_init(LzCtl *ctl) {
+  if (threadIdx.x == 0) {
+    ctl->done = 0;
+    ctl->itx = 0;
+    ctl->m = 0;
+    ctl->errnum = 0.0;
+    ctl->errden = 1.0;
+  }
+  for (int i = threadIdx.x; i < LZ_NL_MAX + 2; i += blockDim.x) ctl->rs[i] = 0.0;
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_sumsq(int64_t n, const double *__restrict__ x,
+                                                       double *__restrict__ p_out) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
+    acc = fma(x[i], x[i], acc);
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
+}
+
+// ---------------------------------------------------------------------------
+static int dense_model(int variant) {
+  if (variant == PSFUN_VAR_CHBSYM) return DENSE_SYM_UPPER;
+  if (variant == PSFUN_VAR_CHBHES) return DENSE_HESSENBERG;
+  return DENSE_FULL;
+}
+
+extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int variant, double fscale,
+                                   double scaling, const double *x, double *y, double eps, int itmax, int istop,
+                                   int *iter, double *err, double *res, int *nres) {
+  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+  if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  if (itmax < 2 || itmax > LZ_NL_MAX || itmax > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
+  const int n = A->n;
+  const int nl = itmax;
+  if (n == 0) {
+    if (iter) *iter = 0;
+    if (err) *err = 0.0;
+    if (nres) *nres = 0;
+    return PSFUN_SUCCESS;
+  }
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  const int model = dense_model(variant);
+  const int ldh = nl + 1;
+  const int64_t ldv = (int64_t)align_up((size_t)n, 64);
+  const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
+  const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
+  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)(nl + 2);  // v_1..v_{nl-1}, u0, u1
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace(c, ctl_bytes + h_bytes + vbytes, &ws));
+  LzCtl *ctl = (LzCtl *)ws;
+  double *H = (double *)((char *)ws + ctl_bytes);
+  double *V = (double *)((char *)ws + ctl_bytes + h_bytes);
+  double *U[2] = {V + (size_t)ldv * nl, V + (size_t)ldv * (nl + 1)};
+  double *P[2] = {c->d_partials, c->d_partials + PSFUN_MAX_GRID};
+
+  const bool sell = A->format == PSFUN_FORMAT_SELL;
+  const int L = sell ? 1 : A->lanes;
+  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
+  int Gs = psfun_grid(c, chunks, sell ? 8 : 4);
+  if (Gs + 1 > 8 * c->num_cu) Gs = (8 * c->num_cu - 1) / 8 * 8;
+  const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
+
+  c->h_flags[0] = 0;
+  c->h_flags[1] = 0;
+  hipLaunchKernelGGL(k_ar_init, dim3(1), dim3(256), 0, c->stream, ctl);
+  hipLaunchKernelGGL(k_ar_zero, dim3(psfun_grid(c, ((int64_t)ldh * ldh + PSFUN_BS - 1) / PSFUN_BS, 1)),
+                     dim3(PSFUN_BS), 0, c->stream, H, (int64_t)ldh * ldh);
+  hipLaunchKernelGGL(k_ar_sumsq, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, P[0]);
+  PSFUN_HIP_TRY(hipGetLastError());
+  int cur = 0, gcur = Gu;  // slot holding the newest partials, and their count
+  const double spmv_bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
+  bool ran_all = true;
+  for (int j = 1; j <= nl - 1; ++j) {
+    const double *src = (j == 1) ? x : U[j & 1];
+    double *u = U[(j + 1) & 1];
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_SPMV, spmv_bytes + (j > 1 ? 8.0 * n : 0.0), j));
+    if (sell) {
+      hipLaunchKernelGGL((k_ar_spmv<1, true>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->sell_ptr,
+                         A->sell_col, A->sell_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H,
+                         ldh, model, nl, istop, eps, fscale, c->d_flags);
+    } else {
+#define AR_LAUNCH(LL)                                                                                               \
+  hipLaunchKernelGGL((k_ar_spmv<LL, false>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, \
+                     A->val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H, ldh, model, nl,     \
+                     istop, eps, fscale, c->d_flags)
+      switch (L) {
+        case 1: AR_LAUNCH(1); break;
+        case 2: AR_LAUNCH(2); break;
+        case 4: AR_LAUNCH(4); break;
+        case 8: AR_LAUNCH(8); break;
+        case 16: AR_LAUNCH(16); break;
+        case 32: AR_LAUNCH(32); break;
+        default: AR_LAUNCH(64); break;
+      }
+#undef AR_LAUNCH
+    }
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    cur = 1 - cur;
+    gcur = Gs;
+    for (int k = 1; k <= j; ++k) {
+      PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_MGS, (k < j ? 32.0 : 24.0) * n, j));
+      hipLaunchKernelGGL(k_ar_mgs, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, k, j, V, ldv, u, P[cur], gcur,
+                         P[1 - cur], ctl, H, ldh);
+      PSFUN_HIP_TRY(hipGetLastError());
+      PSFUN_TRY(psfun_prof_end(c));
+      cur = 1 - cur;
+      gcur = Gu;
+    }
+    if (j >= 2 && j < nl - 1) {
+      PSFUN_TRY(psfun_host_wait(c, j - 1));
+      if (c->h_flags[1] != 0) {
+        ran_all = false;
+        break;
+      }
+    }
+  }
+  if (ran_all) {
+    hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model, nl,
+                       istop, eps, fscale, c->d_flags);
+    PSFUN_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_ar_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, fscale);
+  PSFUN_HIP_TRY(hipGetLastError());
+  const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
+  hipLaunchKernelGGL(k_ar_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
+  PSFUN_HIP_TRY(hipGetLastError());
+
+  LzCtl *h = (LzCtl *)((char *)c->h_pinned + 256);
+  PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(LzCtl), hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  const int itx = h->itx;
+  psfun_prof_discard_after(c, PSFUN_PROF_AR_SPMV, itx + 1);
+  psfun_prof_discard_after(c, PSFUN_PROF_AR_MGS, itx);
+  const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
+  if (iter) *iter = itx;
+  if (err) *err = e;
+  if (res) {
+    for (int i = 1; i <= itx; ++i) res[i - 1] = h->rs[i];
+    res[itx] = e;
+  }
+  if (nres) *nres = itx + 1;
+  return PSFUN_SUCCESS;
+}

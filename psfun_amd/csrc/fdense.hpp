@@ -1,0 +1,129 @@
+// The small f(H) step for a DENSE (general, Hessenberg or symmetric) H on the
+// device: y = exp(s*H) x, evaluated by one block of PSFUN_BS threads.
+//
+// Method (the Taylor scheme of Al-Mohy & Higham, "Computing the action of the
+// matrix exponential", SIAM J. Sci. Comput. 33 (2011)): shift by
+// mu = s*trace(H)/k, split into p substeps with ||B||_1 / p <= 1 for
+// B = s*H - mu*I, and on each substep sum the Taylor series of exp(B/p)
+// applied to the current vector until two successive terms fall below
+// 2^-53 of the partial sum (max norms); then multiply by exp(mu/p).  With
+// substep norm <= 1 the terms never exceed the result by more than about e^2,
+// so the cancellation error stays near the unit roundoff.
+//
+// H is read from memory (column-major, leading dimension ldh) through the
+// caches; how it is read is the variant's matrix model:
+//   DENSE_FULL       every entry                        (GENPADE, TAYLOR, SASQ, CHBGEN)
+//   DENSE_HESSENBERG entries with i <= j+1 only          (CHBHES: DNCHBV zeroes the rest)
+//   DENSE_SYM_UPPER  H(i,j) read as H(min(i,j),max(i,j)) (CHBSYM: DSCHBV solves with ZSYSV 'U')
+// The vectors live in LDS (struct DenseLds).  The matrix-vector product splits
+// each row over nseg threads, whose partial sums meet in LDS in a fixed order.
+#pragma once
+#include "common.hpp"
+
+#define PSFUN_DENSE_KMAX 512
+
+enum { DENSE_FULL = 0, DENSE_HESSENBERG = 1, DENSE_SYM_UPPER = 2 };
+
+struct DenseLds {
+  double y[PSFUN_DENSE_KMAX];
+  double t0[PSFUN_DENSE_KMAX];
+  double t1[PSFUN_DENSE_KMAX];
+  double part[PSFUN_BS];
+  double red[PSFUN_BS / 64 + 1];
+};
+
+__device__ __forceinline__ double dense_h(const double *H, int ldh, int model, int i, int j) {
+  if (model == DENSE_SYM_UPPER && i > j) {
+    const int t = i;
+    i = j;
+    j = t;
+  }
+  if (model == DENSE_HESSENBERG && i > j + 1) return 0.0;
+  return H[(int64_t)j * ldh + i];
+}
+
+// out = (s*H - mu*I) * in / div, with in/out in LDS.  All threads of the block.
+__device__ inline void dense_matvec(int k, const double *H, int ldh, int model, double s, double mu, double div,
+                                    const double *in, double *out, DenseLds *W) {
+  const int tid = threadIdx.x;
+  int nseg = PSFUN_BS / (k > 0 ? k : 1);
+  if (nseg < 1) nseg = 1;
+  if (nseg > 16) nseg = 16;
+  if (nseg == 1) {
+    for (int r = tid; r < k; r += PSFUN_BS) {
+      double acc = 0.0;
+#pragma unroll 2
+      for (int c = 0; c < k; ++c) acc = fma(dense_h(H, ldh, model, r, c), in[c], acc);
+      out[r] = fma(s, acc, -mu * in[r]) / div;
+    }
+    __syncthreads();
+    return;
+  }
+  const int r = tid / nseg, sg = tid % nseg;
+  if (r < k) {
+    double acc = 0.0;
+#pragma unroll 2
+    for (int c = sg; c < k; c += nseg) acc = fma(dense_h(H, ldh, model, r, c), in[c], acc);
+    W->part[tid] = acc;
+  }
+  __syncthreads();
+  if (tid < k) {
+    double acc = 0.0;
+    for (int q = 0; q < nseg; ++q) acc += W->part[tid * nseg + q];
+    out[tid] = fma(s, acc, -mu * in[tid]) / div;
+  }
+  __syncthreads();
+}
+
+__device__ inline double dense_block_max(double v, DenseLds *W) { return block_max<PSFUN_BS>(v, W->red); }
+__device__ inline double dense_block_sum(double v, DenseLds *W) { return block_sum<PSFUN_BS>(v, W->red); }
+
+// W->y <- exp(s*H) W->y.  Called by all PSFUN_BS threads of one block.
+__device__ void block_expmv(int k, const double *H, int ldh, int model, double s, DenseLds *W) {
+  const int tid = threadIdx.x;
+  // shift mu = s*trace(H)/k and the 1-norm of B = s*H - mu*I
+  double tr = 0.0;
+  for (int i = tid; i < k; i += PSFUN_BS) tr += dense_h(H, ldh, model, i, i);
+  const double mu = s * dense_block_sum(tr, W) / k;
+  double cmax = 0.0;
+  for (int c = tid; c < k; c += PSFUN_BS) {
+    double cs = 0.0;
+#pragma unroll 2
+    for (int r = 0; r < k; ++r) {
+      double b = s * dense_h(H, ldh, model, r, c);
+      if (r == c) b -= mu;
+      cs += fabs(b);
+    }
+    cmax = fmax(cmax, cs);
+  }
+  const double nrm = dense_block_max(cmax, W);
+  int p = 1;
+  if (nrm > 1.0) p = (nrm < 1e6) ? (int)ceil(nrm) : 1000000;
+  const double emu = exp(mu / p);
+  const double tiny = 0x1p-53;
+  for (int step = 0; step < p; ++step) {
+    for (int i = tid; i < k; i += PSFUN_BS) W->t0[i] = W->y[i];
+    __syncthreads();
+    double *t = W->t0, *tn = W->t1;
+    double prev = INFINITY;
+    for (int m = 1; m <= 64; ++m) {
+      dense_matvec(k, H, ldh, model, s, mu, (double)p * m, t, tn, W);
+      double tm = 0.0, ym = 0.0;
+      for (int i = tid; i < k; i += PSFUN_BS) {
+        const double v = W->y[i] + tn[i];
+        W->y[i] = v;
+        tm = fmax(tm, fabs(tn[i]));
+        ym = fmax(ym, fabs(v));
+      }
+      tm = dense_block_max(tm, W);
+      ym = dense_block_max(ym, W);
+      double *sw = t;
+      t = tn;
+      tn = sw;
+      if (prev + tm <= tiny * ym) break;
+      prev = tm;
+    }
+    for (int i = tid; i < k; i += PSFUN_BS) W->y[i] *= emu;
+    __syncthreads();
+  }
+}
