@@ -52,3 +52,21 @@ def test_ctx_create_fails_cleanly_without_device():
     h = ctypes.c_void_p()
     st = _lib.lib().psfun_ctx_create(0, ctypes.byref(h))
     assert st == _lib.PSFUN_ERR_NO_DEVICE
+
+
+def test_fortran_module_binds_declared_symbols():
+    """Every bind(C, name=...) of the Fortran interface module names a symbol
+    of the C header, and the constants it defines match the header's."""
+    src = open(os.path.join(REPO, "include", "psfun_hip_mod.f90")).read()
+    binds = re.findall(r"bind\(C,\s*name='(\w+)'\)", src)
+    assert len(binds) >= 20
+    assert set(binds) <= set(declared_symbols())
+    from psfun_amd import _lib
+    consts = dict((k, int(v)) for k, v in re.findall(r"\b(PSFUN_[A-Z_0-9]+)\s*=\s*(\d+)", src))
+    for k, v in consts.items():
+        if hasattr(_lib, k):
+            assert getattr(_lib, k) == v, k
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(REPO, "include", "psfun_hip.h")).read(), flags=re.S)
+    hconst = dict((k, int(v)) for k, v in re.findall(r"\b(PSFUN_[A-Z_0-9]+)\s*=\s*(\d+)", hdr))
+    for k, v in consts.items():
+        assert hconst.get(k) == v, k

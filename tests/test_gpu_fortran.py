@@ -1,0 +1,50 @@
+"""The iso_c_binding boundary from Fortran: tests/fortran/psfun_driver (built
+by tests/fortran/build.py against include/psfun_hip_mod.f90) runs the
+reference example decks lanczos.inp / arnoldi.inp / shifted.inp on the 50x50
+example matrix; its results are checked against the oracle and the fixture."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO, load_golden
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(REPO, "tests", "fortran", "_build", "psfun_driver")
+
+
+def _parse(text):
+    out, cur = {}, None
+    for line in text.strip().splitlines():
+        parts = line.split()
+        if parts[0].isalpha():
+            cur = parts[0]
+            out[cur] = dict(iter=int(parts[1]), err=float(parts[2]), y=[])
+        else:
+            out[cur]["y"].append(float(parts[0]))
+    return out
+
+
+def test_fortran_driver_matches_oracle(gpu_ctx):
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (flang unavailable when building)")
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _parse(r.stdout)
+    g = load_golden("laplacian50.npz")
+    A = (g["rowptr"] - 1, g["colind"] - 1, g["val"])
+    ones = np.ones(50)
+    ref = O.lanczos(A, ones, 1e-6, 100, 1)
+    y = np.array(got["LANCZOS"]["y"])
+    assert got["LANCZOS"]["iter"] == ref["iter"]
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    ref = O.arnoldi(A, ones, 1e-6, 100, 1)
+    y = np.array(got["ARNOLDI"]["y"])
+    assert got["ARNOLDI"]["iter"] == ref["iter"]
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    x = np.array(got["CG"]["y"])
+    assert np.linalg.norm(x - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
+    assert got["CG"]["err"] <= 1e-7
