@@ -42,7 +42,7 @@ __device__ __forceinline__ double prec_apply(const double *__restrict__ dinv, in
   return dinv ? r * dinv[i] : r;
 }
 
-template <int L>
+template <int L, bool SELL>
 __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
                                                    const double *__restrict__ val, double eta, double zeta,
                                                    const double *__restrict__ r, const double *__restrict__ dinv,
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
                                                    const double *__restrict__ p_rho, int g_rho,
                                                    const double *__restrict__ p_conv, int g_in,
                                                    double *__restrict__ p_sig, CgCtl *ctl, int istop, double eps,
-                                                   int itmax) {
+                                                   int itmax, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
   const double rho = block_sum_partials<PSFUN_BS>(p_rho, g_rho, lds);
@@ -86,32 +86,27 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
         else if (rho_old == 0.0) ctl->done = CG_BRK;  // psfun_dcg.F90:290-295
       }
       if (!ctl->done) ctl->itx += 1;
+      host_flag_store(hf, it);
+      if (ctl->done) host_flag_store(hf + 1, ctl->done);
     }
     return;
   }
   if (it > 1 && rho_old == 0.0) return;
   const double beta = (it > 1) ? rho / rho_old : 0.0;
-  constexpr int RPB = PSFUN_BS / L;
+  // direction p = M^{-1} r + beta p_old, formed on the fly at every gathered column
+  auto pk = [&](int c) -> double {
+    return (it > 1) ? fma(beta, p_old[c], prec_apply(dinv, c, r[c])) : prec_apply(dinv, c, r[c]);
+  };
+  constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
   const XcdRange xr = xcd_range_g(nchunks, G);
-  const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+  const int sub = SELL ? threadIdx.x : threadIdx.x / L, lane = SELL ? 0 : threadIdx.x % L;
   double sig = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
     const int row = (int)(ch * RPB + sub);
-    double sum = 0.0;
-    if (row < n) {
-      const int s = rowptr[row], e = rowptr[row + 1];
-      for (int k = s + lane; k < e; k += L) {
-        const int c = col[k];
-        const double pc = (it > 1) ? fma(beta, p_old[c], prec_apply(dinv, c, r[c])) : prec_apply(dinv, c, r[c]);
-        sum = fma(val[k], pc, sum);
-      }
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    const double sum = row_dot<L, SELL>(row, n, rowptr, col, val, lane, pk);
     if (row < n && lane == 0) {
-      const double zr = prec_apply(dinv, row, r[row]);
-      const double pr = (it > 1) ? fma(beta, p_old[row], zr) : zr;
+      const double pr = pk(row);
       const double qr = fma(zeta, pr, eta * sum);
       p_new[row] = pr;
       q[row] = qr;
@@ -173,27 +168,21 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
 }
 
 // r = b - eta*A x - zeta*x; partials of rho = r.M^{-1}r (restart entry, lines 259-262).
-template <int L>
+template <int L, bool SELL>
 __global__ __launch_bounds__(PSFUN_BS) void k_cg_resid(int n, const int *__restrict__ rowptr,
                                                        const int *__restrict__ col, const double *__restrict__ val,
                                                        double eta, double zeta, const double *__restrict__ b,
                                                        const double *__restrict__ x, double *__restrict__ r,
                                                        const double *__restrict__ dinv, double *__restrict__ p_rho) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  constexpr int RPB = PSFUN_BS / L;
+  constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
   const XcdRange xr = xcd_range(nchunks);
-  const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+  const int sub = SELL ? threadIdx.x : threadIdx.x / L, lane = SELL ? 0 : threadIdx.x % L;
   double acc = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
     const int row = (int)(ch * RPB + sub);
-    double sum = 0.0;
-    if (row < n) {
-      const int s = rowptr[row], e = rowptr[row + 1];
-      for (int k = s + lane; k < e; k += L) sum = fma(val[k], x[col[k]], sum);
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    const double sum = row_dot<L, SELL>(row, n, rowptr, col, val, lane, [&](int c) { return x[c]; });
     if (row < n && lane == 0) {
       const double rr = fma(-zeta, x[row], fma(-eta, sum, b[row]));
       r[row] = rr;
@@ -333,9 +322,7 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
 
   double *Psig = c->d_partials, *Prho = c->d_partials + PSFUN_MAX_GRID,
          *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID;
-  const int L = A->lanes;
-  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  const int Gs = psfun_grid(c, chunks, 4);
+  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), A->format == PSFUN_FORMAT_SELL ? 8 : 4);
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
   hipLaunchKernelGGL(k_cg_rowstats, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val,
@@ -344,67 +331,40 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
   hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, Pani, Gu, ctl, 1);
   PSFUN_HIP_TRY(hipGetLastError());
 
-  volatile int *hflag = (volatile int *)c->h_pinned;
   CgCtl *hctl = (CgCtl *)((char *)c->h_pinned + 256);
   int itx_host = 0;
   int done = CG_RUN;
   for (int restart = 0;; ++restart) {
     if (itx_host >= itmax) break;  // psfun_dcg.F90:256
     // r = b - (eta A + zeta I) x, rho partials
-#define RES_LAUNCH(LL)                                                                                          \
-  hipLaunchKernelGGL((k_cg_resid<LL>), dim3(Gs), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, \
-                     eta, zeta, b, x, r, dinv, Prho)
-    switch (L) {
-      case 1: RES_LAUNCH(1); break;
-      case 2: RES_LAUNCH(2); break;
-      case 4: RES_LAUNCH(4); break;
-      case 8: RES_LAUNCH(8); break;
-      case 16: RES_LAUNCH(16); break;
-      case 32: RES_LAUNCH(32); break;
-      default: RES_LAUNCH(64); break;
-    }
-#undef RES_LAUNCH
+    PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_cg_resid<LL, SS>), dim3(Gs), dim3(PSFUN_BS), 0, c->stream, n,
+                                                 MRP, MCOL, MVAL, eta, zeta, b, x, r, dinv, Prho));
     PSFUN_HIP_TRY(hipGetLastError());
     if (restart > 0) {
       hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, Pani, Gu, ctl, 0);
       PSFUN_HIP_TRY(hipGetLastError());
     }
-    const int poll = 8;
-    int pending = -1;
+    c->h_flags[0] = 0;
+    c->h_flags[1] = 0;
     int it;
     double *pold = pA, *pnew = pB;
     for (it = 1; it <= itmax - itx_host + 1; ++it) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_CG, 12.0 * A->nnz + 4.0 * (n + 1) + (it > 1 ? 40.0 : 32.0) * n));
-#define CGA_LAUNCH(LL)                                                                                          \
-  hipLaunchKernelGGL((k_cg_a<LL>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, \
-                     eta, zeta, r, dinv, pold, pnew, q, it, Gs, Prho, it == 1 ? Gs : Gu, Pconv, Gu, Psig, ctl,  \
-                     istop, eps, itmax)
-      switch (L) {
-        case 1: CGA_LAUNCH(1); break;
-        case 2: CGA_LAUNCH(2); break;
-        case 4: CGA_LAUNCH(4); break;
-        case 8: CGA_LAUNCH(8); break;
-        case 16: CGA_LAUNCH(16); break;
-        case 32: CGA_LAUNCH(32); break;
-        default: CGA_LAUNCH(64); break;
-      }
-#undef CGA_LAUNCH
+      PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_cg_a<LL, SS>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n,
+                                                   MRP, MCOL, MVAL, eta, zeta, r, dinv, pold, pnew, q, it, Gs, Prho,
+                                                   it == 1 ? Gs : Gu, Pconv, Gu, Psig, ctl, istop, eps, itmax,
+                                                   c->d_flags));
       PSFUN_HIP_TRY(hipGetLastError());
       PSFUN_TRY(psfun_prof_end(c));
       hipLaunchKernelGGL(k_cg_b, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, x, r, pnew, q, dinv, it, Psig, Gs, Prho,
                          Pconv, ctl, istop);
       PSFUN_HIP_TRY(hipGetLastError());
       std::swap(pold, pnew);
-      if (it % poll == 0) {
-        const int slot = (it / poll) & 1;
-        PSFUN_HIP_TRY(hipMemcpyAsync((void *)(hflag + slot), &ctl->done, sizeof(int), hipMemcpyDeviceToHost,
-                                     c->stream));
-        PSFUN_HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
-        if (pending >= 0) {
-          PSFUN_HIP_TRY(hipEventSynchronize(c->ev[pending]));
-          if (hflag[pending] != 0) break;
-        }
-        pending = slot;
+      // pacing on the host flags: about two iterations queued ahead of the
+      // one whose control block has reported
+      if (it >= 3) {
+        PSFUN_TRY(psfun_host_wait(c, it - 2));
+        if (c->h_flags[1] != 0) break;
       }
     }
     hipLaunchKernelGGL(k_cg_lastcheck, dim3(1), dim3(PSFUN_BS), 0, c->stream, Pconv, Gu, ctl, istop, eps, itmax);

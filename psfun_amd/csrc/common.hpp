@@ -259,3 +259,64 @@ __device__ __forceinline__ double sell_row_dot(int row, const int *__restrict__ 
 __device__ __forceinline__ void host_flag_store(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+
+// Row sum  sum_k val_k * g(col_k)  of `row` for the storage chosen at compile
+// time: SELL (sliced ELL, one thread per row; rp = sell_ptr, entries in CSR
+// order) or vector CSR with L lanes per row (every lane of the group calls it;
+// the sum is valid in every lane).  g(c) yields the gathered operand entry.
+template <int L, bool SELL, typename Gather>
+__device__ __forceinline__ double row_dot(int row, int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                          const double *__restrict__ val, int lane, Gather g) {
+  double sum = 0.0;
+  if (SELL) {
+    if (row < n) {
+      const int s = row >> 6, l = row & 63;
+      const int b = rp[s];
+      const int w = (rp[s + 1] - b) >> 6;
+      const int *cp = col + b + l;
+      const double *vp = val + b + l;
+#pragma unroll 4
+      for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], g(cp[k * 64]), sum);
+    }
+  } else {
+    if (row < n) {
+      const int e = rp[row + 1];
+      for (int q = rp[row] + lane; q < e; q += L) sum = fma(val[q], g(col[q]), sum);
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+  }
+  return sum;
+}
+
+// Launch-side dispatch over the matrix storage: defines LL (lanes) and SS
+// (sliced ELL) as constants and the matrix arrays MRP/MCOL/MVAL for __VA_ARGS__.
+#define PSFUN_DISPATCH_STORAGE(A, ...)                                                           \
+  do {                                                                                           \
+    if ((A)->format == PSFUN_FORMAT_SELL) {                                                      \
+      constexpr int LL = 1;                                                                      \
+      constexpr bool SS = true;                                                                  \
+      const int *MRP = (A)->sell_ptr, *MCOL = (A)->sell_col;                                     \
+      const double *MVAL = (A)->sell_val;                                                        \
+      __VA_ARGS__;                                                                               \
+    } else {                                                                                     \
+      const int *MRP = (A)->rowptr, *MCOL = (A)->colind;                                         \
+      const double *MVAL = (A)->val;                                                             \
+      constexpr bool SS = false;                                                                 \
+      switch ((A)->lanes) {                                                                      \
+        case 1: { constexpr int LL = 1; __VA_ARGS__; } break;                                    \
+        case 2: { constexpr int LL = 2; __VA_ARGS__; } break;                                    \
+        case 4: { constexpr int LL = 4; __VA_ARGS__; } break;                                    \
+        case 8: { constexpr int LL = 8; __VA_ARGS__; } break;                                    \
+        case 16: { constexpr int LL = 16; __VA_ARGS__; } break;                                  \
+        case 32: { constexpr int LL = 32; __VA_ARGS__; } break;                                  \
+        default: { constexpr int LL = 64; __VA_ARGS__; } break;                                  \
+      }                                                                                          \
+    }                                                                                            \
+  } while (0)
+
+// Rows per chunk of a streaming SpMV kernel for the storage.
+static inline int64_t psfun_spmv_chunks(const psfun_csr *A) {
+  const int rpb = (A->format == PSFUN_FORMAT_SELL) ? PSFUN_BS : PSFUN_BS / A->lanes;
+  return ((int64_t)A->n + rpb - 1) / rpb;
+}

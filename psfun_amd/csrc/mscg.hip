@@ -103,7 +103,7 @@ __global__ void k_ms_ctl_init(MsCtl *ctl, int nshift, const double *__restrict__
   }
 }
 
-template <int L>
+template <int L, bool SELL>
 __global__ __launch_bounds__(PSFUN_BS) void k_ms_spmv(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
                                                       const double *__restrict__ val, const double *__restrict__ r,
                                                       const double *__restrict__ p_old, double *__restrict__ p_new,
@@ -122,26 +122,18 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_spmv(int n, const int *__restri
     return;
   }
   const double bk = (k > 0) ? rr / ctl->rr[(k - 1) & 1] : 0.0;
-  constexpr int RPB = PSFUN_BS / L;
+  // seed direction p = r + bk p_old, formed on the fly at every gathered column
+  auto pk = [&](int c) -> double { return (k > 0) ? fma(bk, p_old[c], r[c]) : r[c]; };
+  constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
   const XcdRange xr = xcd_range_g(nchunks, G);
-  const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+  const int sub = SELL ? threadIdx.x : threadIdx.x / L, lane = SELL ? 0 : threadIdx.x % L;
   double pq = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
     const int row = (int)(ch * RPB + sub);
-    double sum = 0.0;
-    if (row < n) {
-      const int s = rowptr[row], e = rowptr[row + 1];
-      for (int j = s + lane; j < e; j += L) {
-        const int c = col[j];
-        const double pc = (k > 0) ? fma(bk, p_old[c], r[c]) : r[c];
-        sum = fma(val[j], pc, sum);
-      }
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    const double sum = row_dot<L, SELL>(row, n, rowptr, col, val, lane, pk);
     if (row < n && lane == 0) {
-      const double pr = (k > 0) ? fma(bk, p_old[row], r[row]) : r[row];
+      const double pr = pk(row);
       p_new[row] = pr;
       q[row] = sum;
       pq = fma(pr, sum, pq);
@@ -175,7 +167,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
                                                        const double *__restrict__ b, double2 *__restrict__ X,
                                                        int64_t ldx, double2 *__restrict__ P, int64_t ldp, int k, int G,
                                                        const double *__restrict__ p_rr, int g_rr, MsCtl *ctl,
-                                                       double eps, int itmax) {
+                                                       double eps, int itmax, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2];
   __shared__ int s_upd_x[MS_MAX], s_upd_p[MS_MAX];
@@ -244,7 +236,11 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
       ctl->b[par] = bk;
       int any = 0;
       for (int s = 0; s < nshift; ++s) any |= s_upd_p[s];
-      if (!any || rr1 == 0.0) ctl->done = MS_DONE;
+      host_flag_store(hf, k + 1);
+      if (!any || rr1 == 0.0) {
+        ctl->done = MS_DONE;
+        host_flag_store(hf + 1, MS_DONE);
+      }
     }
     return;
   }
@@ -327,59 +323,43 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
   PSFUN_HIP_TRY(hipMemcpyAsync(w.dshift, shifts_host, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice,
                                c->stream));
   double *Ppq = c->d_partials, *Prr = c->d_partials + PSFUN_MAX_GRID;
-  const int L = A->lanes;
-  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  const int Gs = psfun_grid(c, chunks, 4);
+  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), A->format == PSFUN_FORMAT_SELL ? 8 : 4);
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
+  c->h_flags[0] = 0;
+  c->h_flags[1] = 0;
   hipLaunchKernelGGL(k_ms_ctl_init, dim3(1), dim3(MS_MAX), 0, c->stream, ctl, nshift, (const double *)w.dshift);
   hipLaunchKernelGGL(k_ms_init, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, r, Prr);
   PSFUN_HIP_TRY(hipGetLastError());
-  volatile int *hflag = (volatile int *)c->h_pinned;
-  const int poll = 8;
-  int pending = -1;
   double *pold = w.pa, *pnew = w.pb;
   for (int k = 0; k < itmax; ++k) {
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_SPMV, 12.0 * A->nnz + 4.0 * (n + 1) + (k ? 32.0 : 24.0) * n));
-#define MSA_LAUNCH(LL)                                                                                           \
-  hipLaunchKernelGGL((k_ms_spmv<LL>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, \
-                     r, pold, pnew, q, k, Gs, Prr, Gu, Ppq, ctl)
-    switch (L) {
-      case 1: MSA_LAUNCH(1); break;
-      case 2: MSA_LAUNCH(2); break;
-      case 4: MSA_LAUNCH(4); break;
-      case 8: MSA_LAUNCH(8); break;
-      case 16: MSA_LAUNCH(16); break;
-      case 32: MSA_LAUNCH(32); break;
-      default: MSA_LAUNCH(64); break;
-    }
-#undef MSA_LAUNCH
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_SPMV, 12.0 * A->nnz + 4.0 * (n + 1) + (k ? 32.0 : 24.0) * n, k));
+    PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_ms_spmv<LL, SS>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n,
+                                                 MRP, MCOL, MVAL, r, pold, pnew, q, k, Gs, Prr, Gu, Ppq, ctl));
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     hipLaunchKernelGGL(k_ms_seed, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, q, k, Ppq, Gs, Prr, ctl);
     PSFUN_HIP_TRY(hipGetLastError());
     // algorithmic bytes of the shift kernel: r read, x^s and p^s read+written per running shift
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE, 8.0 * n + 64.0 * n * nshift));
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE, 8.0 * n + 64.0 * n * nshift, k));
     hipLaunchKernelGGL(k_ms_shift, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, P, w.ld, k, Gu, Prr,
-                       Gu, ctl, eps, itmax);
+                       Gu, ctl, eps, itmax, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     std::swap(pold, pnew);
-    if ((k + 1) % poll == 0) {
-      const int slot = ((k + 1) / poll) & 1;
-      PSFUN_HIP_TRY(hipMemcpyAsync((void *)(hflag + slot), &ctl->done, sizeof(int), hipMemcpyDeviceToHost,
-                                   c->stream));
-      PSFUN_HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
-      if (pending >= 0) {
-        PSFUN_HIP_TRY(hipEventSynchronize(c->ev[pending]));
-        if (hflag[pending] != 0) break;
-      }
-      pending = slot;
+    // pacing on the host flags: keep about two iterations queued ahead of
+    // the one whose shift kernel has reported
+    if (k >= 2 && k + 1 < itmax) {
+      PSFUN_TRY(psfun_host_wait(c, k - 1));
+      if (c->h_flags[1] != 0) break;
     }
   }
   MsCtl *h = (MsCtl *)((char *)c->h_pinned + 256);
   PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, c->stream));
   PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  // iterations after the one that stopped the loop (h->k) did no work
+  psfun_prof_discard_after(c, PSFUN_PROF_MS_SPMV, h->k);
+  psfun_prof_discard_after(c, PSFUN_PROF_MS_UPDATE, h->k);
   for (int s = 0; s < nshift; ++s) {
     if (iters) iters[s] = h->iters[s];
     if (resid) resid[s] = h->resid[s];
