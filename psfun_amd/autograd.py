@@ -14,7 +14,9 @@ Backward (A symmetric, loss L with dL/dY = G):
              from the bases: v_s = ||x|| V_m (xi_s I + T_m)^{-1} e1 (the multi-shift CG
              iterate of step m), likewise u_s from W.
 With store_basis=False the forward keeps nothing and the backward rebuilds V
-by re-running the forward Lanczos (memory for one basis at a time).
+by re-running the forward Lanczos (memory for one basis at a time);
+store_basis="auto" (the default) keeps the basis only when it fits in half
+of the device memory that is free at forward time.
 grad_mode="mscg" takes v_s, u_s from full multi-shift CG solves instead (one
 column at a time; slower, used to cross-check).
 """
@@ -33,7 +35,7 @@ class ExpmvOperator:
     """exp(-tA) on a fixed symmetric sparsity pattern (0-based CSR)."""
 
     def __init__(self, rowptr, colind, eps=1e-10, itmax=150, quad_poles=24, quad_eps=1e-13, quad_itmax=20000,
-                 store_basis=True, grad_mode="krylov", ctxt=None):
+                 store_basis="auto", grad_mode="krylov", ctxt=None):
         self.ctxt = ctxt if ctxt is not None else psb.default_context()
         self.rowptr = np.ascontiguousarray(rowptr, np.int32)
         self.colind = np.ascontiguousarray(colind, np.int32)
@@ -63,6 +65,18 @@ class ExpmvOperator:
         if self.mat is not None:
             _lib.check(_lib.lib().psfun_csr_set_values(self.mat.handle, C.c_void_p(v.data_ptr())))
         torch.cuda.current_stream().synchronize()
+
+    def basis_bytes(self, K):
+        """Device bytes of one stored Krylov basis for K columns."""
+        return 8 * (self.itmax - 1) * self.n * K + 16 * K * (self.itmax + 2)
+
+    def keep_basis(self, K, device):
+        """Whether the forward keeps its basis for the backward."""
+        if self.store_basis != "auto":
+            return bool(self.store_basis)
+        free, _total = torch.cuda.mem_get_info(device)
+        # the backward holds the adjoint basis too, so two bases must fit
+        return 2 * self.basis_bytes(K) <= 0.5 * free
 
     def expm(self, X, t):
         X = X.contiguous()
@@ -167,7 +181,8 @@ class ExpmvFunction(torch.autograd.Function):
         op.set_values(vals)
         tf = float(t)
         need_a = ctx.needs_input_grad[0] and op.grad_mode == "krylov"
-        if need_a and op.store_basis:
+        K = 1 if X.dim() == 1 else int(X.shape[1])
+        if need_a and op.keep_basis(K, X.device):
             Y, kry = op.expm_basis(X, tf)
             ctx.krylov = kry
         else:

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
                                                       int model, int nl, int istop, double eps, double fscale,
                                                       int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
+  // block-uniform: the control block of this same grid may set the flag
+  if (__syncthreads_or(ctl->done)) return;
   const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
   if (blockIdx.x == 0) {  // control block
     if (j == 1) {

@@ -53,7 +53,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
                                                    double *__restrict__ p_sig, CgCtl *ctl, int istop, double eps,
                                                    int itmax, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
+  // block-uniform: the control block of this same grid may set the flag
+  if (__syncthreads_or(ctl->done)) return;
   const double rho = block_sum_partials<PSFUN_BS>(p_rho, g_rho, lds);
   const double rho_old = (it > 1) ? ctl->rho[(it - 1) & 1] : 0.0;
   if (blockIdx.x == G) {  // control block

@@ -195,6 +195,7 @@ struct XcdRange {
 template <int BS>
 __device__ __forceinline__ double block_sum_partials(const double *__restrict__ P, int G, double *lds) {
   double acc = 0.0;
+#pragma unroll 4
   for (int b = threadIdx.x; b < G; b += BS) acc += P[b];
   return block_sum<BS>(acc, lds);
 }

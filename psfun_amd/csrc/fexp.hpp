@@ -46,24 +46,28 @@ __device__ inline int cheb_exp_coefficients(double r, double *coef, int cap) {
   const double twoinv = 2.0 / r;
   double ip1 = 0.0, i0 = 1e-280;
   coef[ms] = i0;
+#pragma unroll 1
   for (int m = ms; m >= 1; --m) {
     const double im1 = fma((double)m * twoinv, i0, ip1);
     coef[m - 1] = im1;
     ip1 = i0;
     i0 = im1;
     if (im1 > 1e250) {
+#pragma unroll 1
       for (int q = m - 1; q <= ms; ++q) coef[q] *= 1e-250;
       ip1 *= 1e-250;
       i0 *= 1e-250;
     }
   }
   double s = 0.0;
+#pragma unroll 1
   for (int m = ms; m >= 1; --m) s += coef[m];
   s = coef[0] + 2.0 * s;
   const double inv = 1.0 / s;
   const double b0 = coef[0] * inv;
   coef[0] = b0;
   int M = 0;
+#pragma unroll 1
   for (int m = 1; m <= ms; ++m) {
     const double b = coef[m] * inv;
     coef[m] = 2.0 * b;
@@ -140,10 +144,76 @@ __device__ void wave_exp_tridiag(int k, const double *d, const double *e, double
   }
 }
 
-// exp(s*T) e1; returns ||y||_2 and y[k-1] in every lane and, if yout is not
-// null, stores y[0..k-1] to global memory.
-__device__ void wave_exp_tridiag_e1(int k, const double *d, const double *e, double s, ChebLds *W,
-                                    double *yout, double &nrm, double &last) {
+// exp(s*T) e1 for k <= 64: row i = lane lives in registers (its entries of
+// the scaled S and of the Chebyshev vectors); the neighbours t[i-1], t[i+1]
+// come from cross-lane shuffles, so the recurrence needs no LDS traffic or
+// fences.  Same formulas, in the same order, as wave_exp_tridiag.  Returns
+// y[lane] (0 for lane >= k).
+__device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const double *e, double s,
+                                                       ChebLds *W) {
+  const int lane = threadIdx.x & 63;
+  const bool in = lane < k;
+  const double dl = in ? d[lane] : 0.0;
+  const double elo = (in && lane > 0) ? e[lane - 1] : 0.0;
+  const double ehi = (lane < k - 1) ? e[lane] : 0.0;
+  const double di = s * dl;
+  const double al = fabs(s * elo), ah = fabs(s * ehi);
+  const double glo = wave_min(in ? di - al - ah : 1e308);
+  const double ghi = wave_max(in ? di + al + ah : -1e308);
+  double y = (lane == 0) ? 1.0 : 0.0;
+  const double width = ghi - glo;
+  if (!(width > 1e-300)) return y * exp(ghi);  // s*T is a multiple of the identity
+  int nsub = (int)ceil(0.5 * width / PSFUN_CHEB_RMAX);
+  if (nsub < 1) nsub = 1;
+  const double ssub = 1.0 / nsub;
+  const double c = 0.5 * (glo + ghi) * ssub, r = 0.5 * width * ssub, hs = ghi * ssub;
+  if (lane == 0) W->M = cheb_exp_coefficients(r, W->coef, PSFUN_CHEB_CAP);
+  wave_lds_sync();
+  const int M = W->M;
+  const double rinv = 1.0 / r;
+  const double sd = s * ssub * rinv, cd = c * rinv;
+  const double sdi = in ? sd * dl - cd : 0.0;   // S_ii
+  const double slo = in ? sd * elo : 0.0;       // S_i,i-1
+  const double shi = in ? sd * ehi : 0.0;       // S_i,i+1
+  const double escale = exp(hs);
+  const double b0 = W->coef[0], b1 = (M >= 1) ? W->coef[1] : 0.0;
+  for (int sub = 0; sub < nsub; ++sub) {
+    double t0 = y;
+    y = b0 * t0;
+    double t1 = sdi * t0;
+    t1 = fma(slo, __shfl_up(t0, 1, 64), t1);
+    t1 = fma(shi, __shfl_down(t0, 1, 64), t1);
+    if (M >= 1) y = fma(b1, t1, y);
+#pragma unroll 2
+    for (int m = 2; m <= M; ++m) {
+      const double bm = W->coef[m];
+      double v = sdi * t1;
+      v = fma(slo, __shfl_up(t1, 1, 64), v);
+      v = fma(shi, __shfl_down(t1, 1, 64), v);
+      const double t2 = fma(2.0, v, -t0);
+      y = fma(bm, t2, y);
+      t0 = t1;
+      t1 = t2;
+    }
+    y *= escale;
+  }
+  return in ? y : 0.0;
+}
+
+// exp(s*T) e1 for k <= 64 through the register routine; returns ||y||_2 and
+// y[k-1] in every lane and, if yout is not null, stores y[0..k-1].
+__device__ inline void wave_exp_tridiag_e1_small(int k, const double *d, const double *e, double s, ChebLds *W,
+                                                 double *yout, double &nrm, double &last) {
+  const int lane = threadIdx.x & 63;
+  const double y = wave_exp_tridiag_e1_reg(k, d, e, s, W);
+  if (yout && lane < k) yout[lane] = y;
+  nrm = sqrt(wave_sum(y * y));
+  last = __shfl(y, k - 1, 64);
+}
+
+// Any k <= PSFUN_FT_KMAX through the LDS routine.
+__device__ void wave_exp_tridiag_e1_lds(int k, const double *d, const double *e, double s, ChebLds *W,
+                                        double *yout, double &nrm, double &last) {
   const int lane = threadIdx.x & 63;
   for (int i = lane; i < k; i += 64) W->y[i] = (i == 0) ? 1.0 : 0.0;
   wave_lds_sync();
@@ -157,3 +227,13 @@ __device__ void wave_exp_tridiag_e1(int k, const double *d, const double *e, dou
   nrm = sqrt(wave_sum(ss));
   last = W->y[k - 1];
 }
+
+// exp(s*T) e1 for any k <= PSFUN_FT_KMAX (registers up to 64, LDS beyond).
+__device__ inline void wave_exp_tridiag_e1(int k, const double *d, const double *e, double s, ChebLds *W,
+                                           double *yout, double &nrm, double &last) {
+  if (k <= 64)
+    wave_exp_tridiag_e1_small(k, d, e, s, W, yout, nrm, last);
+  else
+    wave_exp_tridiag_e1_lds(k, d, e, s, W, yout, nrm, last);
+}
+

@@ -37,6 +37,10 @@ __device__ __forceinline__ void lz_stop(LzCtl *ctl, int *hf, int state) {
   host_flag_store(hf + 1, state);
 }
 
+// SMALLT: every T this loop can meet has order <= 64 (nl <= 65), so only the
+// register f(T) routine is compiled in, which keeps the streaming kernel
+// that hosts this control block within 64 VGPRs.
+template <bool SMALLT>
 __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, double eps, double fscale,
                            ChebLds *W, int *hf) {
   const int lane = threadIdx.x;
@@ -52,7 +56,10 @@ __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, do
   if (istop == 1) {
     // yk = f(T(1:j-1,1:j-1)) e1; rs(j) = rs1 * beta * |yk(j-1)| / ||yk||  (lines 276-315)
     double nrm, last;
-    wave_exp_tridiag_e1(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
+    if (SMALLT)
+      wave_exp_tridiag_e1_small(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
+    else
+      wave_exp_tridiag_e1_lds(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
     const double errnum = ctl->rs1 * beta * fabs(last);
     const double rsj = errnum / nrm;
     if (lane == 0) {
@@ -69,7 +76,7 @@ __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, do
   if (j >= nl && lane == 0) lz_stop(ctl, hf, LZ_DONE_MAXIT);
 }
 
-template <int L>
+template <int L, bool SMALLT>
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restrict__ rowptr,
                                                       const int *__restrict__ col,
                                                       const double *__restrict__ val, double opscale,
@@ -79,7 +86,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
                                                       double *__restrict__ p_out, LzCtl *ctl, int nl, int istop,
                                                       double eps, double fscale, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
+  // block-uniform: the control block of this same grid may set the flag
+  if (__syncthreads_or(ctl->done)) return;
   const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
   if (blockIdx.x == G) {  // control block
     if (threadIdx.x >= 64) return;
@@ -93,7 +101,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
       return;
     }
     __shared__ ChebLds W;
-    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
+    lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
   if (sigma == 0.0 && j == 1) return;
@@ -125,8 +133,12 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
 }
 
 // Same step on the sliced-ELL copy: block 0 is the control block (dispatched
-// first), blocks 1..G stream one row per thread.
-// 8 waves per SIMD: at most 64 VGPRs, so 8 blocks of 256 fit a CU.
+// first), blocks 1..G stream one row per thread.  The stop flag, the partials
+// and the first chunk's row sums are all loaded before anything waits on them,
+// so the fold's latency hides behind the first chunk's loads; each row sum is
+// taken unscaled and multiplied by 1/beta afterwards (u = opscale (1/beta)
+// sum_k a_k w_k, equal to opscale sum_k a_k v_k up to rounding).
+template <bool SMALLT>
 __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *__restrict__ sp,
                                                            const int *__restrict__ col,
                                                            const double *__restrict__ val, double opscale,
@@ -137,8 +149,23 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
                                                            int istop, double eps, double fscale,
                                                            int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
-  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  const int stopped = ctl->done;
+  double pacc = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < g_in; b += PSFUN_BS) pacc += p_in[b];
+  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
+  XcdRange xr = {0, 0, 1};
+  if (blockIdx.x > 0) xr = xcd_range_c0(nchunks, G);
+  double acc0 = 0.0, src0 = 0.0;
+  if (xr.begin < xr.end) {
+    const int row0 = (int)(xr.begin * PSFUN_BS + threadIdx.x);
+    if (row0 < n) {
+      acc0 = sell_row_dot(row0, sp, col, val, src, 1.0);
+      src0 = src[row0];
+    }
+  }
+  if (__syncthreads_or(stopped)) return;  // block-uniform (the control block may set it meanwhile)
+  const double sigma = sqrt(block_sum<PSFUN_BS>(pacc, lds));
   if (blockIdx.x == 0) {  // control block
     if (threadIdx.x >= 64) return;
     if (j == 1) {
@@ -151,20 +178,20 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
       return;
     }
     __shared__ ChebLds W;
-    lz_control(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
+    lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
   if (sigma == 0.0 && j == 1) return;
   const double inv = 1.0 / sigma;
-  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
-  const XcdRange xr = xcd_range_c0(nchunks, G);
+  const double uscale = opscale * inv;
   double dot = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
     const int row = (int)(ch * PSFUN_BS + threadIdx.x);
     if (row < n) {
-      const double sum = sell_row_dot(row, sp, col, val, src, inv);
-      const double v = src[row] * inv;
-      const double u = opscale * sum;
+      const bool first = (ch == xr.begin);
+      const double sum = first ? acc0 : sell_row_dot(row, sp, col, val, src, 1.0);
+      const double v = (first ? src0 : src[row]) * inv;
+      const double u = uscale * sum;
       v_out[row] = v;
       u_out[row] = u;
       dot = fma(u, v, dot);
@@ -180,9 +207,10 @@ __global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in,
   __shared__ ChebLds W;
   if (ctl->done) return;
   double acc = 0.0;
+#pragma unroll 4
   for (int b = threadIdx.x; b < g_in; b += 64) acc += p_in[b];
   const double beta = sqrt(wave_sum(acc));
-  lz_control(ctl, j, beta, nl, istop, eps, fscale, &W, hf);
+  lz_control<false>(ctl, j, beta, nl, istop, eps, fscale, &W, hf);
 }
 
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restrict__ u,
@@ -278,18 +306,31 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   const int reps = psfun_prof_reps(c);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
   if (A->format == PSFUN_FORMAT_SELL) {
-    for (int r = 0; r < reps; ++r)
-      hipLaunchKernelGGL(k_lz_spmv_sell, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
-                       A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
-                       istop, eps, fscale, c->d_flags);
+    for (int r = 0; r < reps; ++r) {
+      if (nl <= 65)
+        hipLaunchKernelGGL(k_lz_spmv_sell<true>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
+                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
+                           istop, eps, fscale, c->d_flags);
+      else
+        hipLaunchKernelGGL(k_lz_spmv_sell<false>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
+                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
+                           istop, eps, fscale, c->d_flags);
+    }
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;
   }
-#define LZ_LAUNCH(LL)                                                                                         \
-  hipLaunchKernelGGL((k_lz_spmv<LL>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr, A->colind, \
-                     A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl, istop, eps, fscale, \
-                     c->d_flags)
+#define LZ_LAUNCH(LL)                                                                                          \
+  do {                                                                                                         \
+    if (nl <= 65)                                                                                              \
+      hipLaunchKernelGGL((k_lz_spmv<LL, true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr,   \
+                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,     \
+                         istop, eps, fscale, c->d_flags);                                                      \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_lz_spmv<LL, false>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr,  \
+                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,     \
+                         istop, eps, fscale, c->d_flags);                                                      \
+  } while (0)
   for (int r = 0; r < reps; ++r) switch (L) {
     case 1: LZ_LAUNCH(1); break;
     case 2: LZ_LAUNCH(2); break;

@@ -171,7 +171,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
   __shared__ double lds[PSFUN_BS / 64 + 1];
   __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2];
   __shared__ int s_upd_x[MS_MAX], s_upd_p[MS_MAX];
-  if (ctl->done) return;
+  // block-uniform: the control block of this same grid may set the flag
+  if (__syncthreads_or(ctl->done)) return;
   const double rr1 = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
   const int nshift = ctl->nshift;
   const int par = k & 1;

@@ -32,7 +32,7 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("mode,store", [("krylov", True), ("krylov", False), ("mscg", True)])
+@pytest.mark.parametrize("mode,store", [("krylov", True), ("krylov", False), ("krylov", "auto"), ("mscg", True)])
 def test_forward_and_gradients_vs_frechet_oracle(gpu_ctx, mode, store):
     g, op, vals, X, t, G = _setup(mode, store)
     Y = expmv(vals, X, t, op)
@@ -75,3 +75,16 @@ def test_gradients_vs_central_differences(gpu_ctx):
         # t
         fd = (loss(vals, X, t + h) - loss(vals, X, t - h)) / (2 * h)
         assert abs(fd - t.grad.item()) <= 5e-5 * max(abs(t.grad.item()), 1.0)
+
+
+def test_basis_memory_decision(gpu_ctx):
+    """store_basis="auto": keep the basis when two bases fit in half the free
+    device memory, recompute otherwise (checked with a pattern whose basis
+    cannot fit and one whose basis does)."""
+    g = load_golden("frechet_small.npz")
+    op = ExpmvOperator(g["rowptr"], g["colind"], itmax=200)
+    assert op.keep_basis(2, "cuda:0")
+    free, _ = torch.cuda.mem_get_info("cuda:0")
+    K_huge = int(free // op.basis_bytes(1)) + 1
+    assert not op.keep_basis(K_huge, "cuda:0")
+
