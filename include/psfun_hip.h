@@ -14,6 +14,7 @@
  *          krylov/polynomial/psfun_d_arnoldi.f90:183-329, krylov/shifted/psfun_dcg.F90:259-326
  *   psfun_d_lanczos  -- krylov/polynomial/psfun_d_lanczos.f90:37-400
  *   psfun_d_arnoldi  -- krylov/polynomial/psfun_d_arnoldi.f90:37-352
+ *   psfun_d_saiarnoldi -- krylov/rational/psfun_d_sai.f90:37-419
  *   psfun_dcg_vect   -- krylov/shifted/psfun_dcg.F90:129-373
  *   psfun_d_serial%apply (f(H) step, EXP family) -- serial/psfun_d_serial_mod.F90:172-523
  *   hhtmethod1       -- quadratures/psfun_z_quadrules_mod.f90:40-85
@@ -182,6 +183,23 @@ int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale
 int psfun_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, double fscale, double scaling,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
                         double *res, int *nres);
+
+/* Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90:37-419): the
+ * Krylov space of (A + tau I)^{-1}, each inner solve (A + tau I) w = v by
+ * psfun_cg_shifted (prec NONE/DIAG, tolerance eps, inner_itmax, inner_istop;
+ * A + tau I must be definite), modified Gram-Schmidt, and
+ * y = ||x|| V_k f(fscale (H_k^{-1} - tau I)) e1 with the reference's
+ * a-posteriori estimate from that same projection.  Differences from the
+ * reference, which the reference's own comments describe as the intent:
+ * the shift is applied to every diagonal entry (its loop `do j=1,i-1,i`
+ * reaches only (1,1)), the inner solve starts from zero, and a run that does
+ * not converge also assembles from f(H^{-1} - tau I) (the reference uses
+ * f(H) there).  For tau = 0 and a converged run the results coincide.  f is
+ * read as `variant` reads a matrix; GENPADE/CHBGEN/TAYLOR/SASQ treat the full
+ * H^{-1}.  Returns PSFUN_ERR_BREAKDOWN if H_k is singular. */
+int psfun_sai_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, double fscale, double tau,
+                            int prec, int inner_itmax, int inner_istop, const double *x, double *y, double eps,
+                            int itmax, int istop, int *iter, double *err, double *res, int *nres);
 
 /* ---- batched Lanczos f(A)x ------------------------------------------------ */
 /* A batch is a set of ngraph independent square CSR matrices stored

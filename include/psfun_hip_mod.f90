@@ -159,6 +159,18 @@ module psfun_hip_mod
       integer(c_int) :: st
     end function
 
+    function psfun_sai_arnoldi_apply(ctx, a, fname, variant, fscale, tau, prec, inner_itmax, inner_istop, x, y, &
+         & eps, itmax, istop, iter, err, res, nres) bind(C, name='psfun_sai_arnoldi_apply') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, a, x, y
+      integer(c_int), value :: fname, variant, prec, inner_itmax, inner_istop, itmax, istop
+      real(c_double), value :: fscale, tau, eps
+      integer(c_int), intent(out) :: iter, nres
+      real(c_double), intent(out) :: err
+      real(c_double), intent(out) :: res(*)
+      integer(c_int) :: st
+    end function
+
     ! ---- shifted solves (psb_krylov) ---------------------------------------
     function psfun_cg_shifted(ctx, a, prec, b, eta, zeta, x, eps, itmax, istop, iter, err, cond) &
          & bind(C, name='psfun_cg_shifted') result(st)

@@ -48,6 +48,12 @@ def lib():
         L.or_lanczos.restype = C.c_int
         L.or_arnoldi.argtypes = kry
         L.or_arnoldi.restype = C.c_int
+        L.or_sai_arnoldi.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                     _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), _dp, C.POINTER(C.c_int)]
+        L.or_sai_arnoldi.restype = C.c_int
+        L.or_dense_inverse.argtypes = [C.c_int, _dp, _dp]
+        L.or_dense_inverse.restype = C.c_int
         L.or_cg_shifted.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, C.c_double,
                                     C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
@@ -145,6 +151,32 @@ def lanczos(A, x, eps, itmax, istop=1, fscale=1.0, opscale=1.0):
 def arnoldi(A, x, eps, itmax, istop=1, fscale=1.0, scaling=1.0):
     """Reference-semantics Arnoldi (krylov/polynomial/psfun_d_arnoldi.f90)."""
     return _krylov(lib().or_arnoldi, A, x, eps, itmax, istop, fscale, scaling)
+
+
+def sai_arnoldi(A, x, eps, itmax, tau=0.0, istop=1, prec=0, in_itmax=1000, in_istop=2, fscale=1.0):
+    """Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90), with the
+    shift applied to every diagonal entry of H^{-1}."""
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    y = np.zeros(n)
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    nres = C.c_int(0)
+    res = np.zeros(max(itmax, 2) + 2)
+    rc = lib().or_sai_arnoldi(n, rp, ci, va, fscale, tau, prec, in_itmax, in_istop,
+                              np.ascontiguousarray(x, np.float64), y, eps, itmax, istop, C.byref(it), C.byref(err),
+                              res, C.byref(nres))
+    return dict(info=rc, y=y, iter=it.value, err=err.value, res=res[:nres.value].copy())
+
+
+def dense_inverse(H):
+    H = np.asfortranarray(H, dtype=np.float64)
+    k = H.shape[0]
+    out = np.zeros(k * k)
+    rc = lib().or_dense_inverse(k, np.ascontiguousarray(H.reshape(-1, order="F")), out)
+    if rc:
+        raise RuntimeError("singular matrix")
+    return out.reshape(k, k, order="F")
 
 
 def cg_shifted(A, b, eta, zeta, x0=None, eps=1e-8, itmax=1000, istop=2, prec=0, want_cond=False):

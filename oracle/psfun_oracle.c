@@ -17,6 +17,7 @@
  *   or_lanczos                         -- DST closed form exp(-tA) for Dirichlet Laplacians,
  *                                         scipy dense expm on small matrices, breakdown cases
  *   or_arnoldi                         -- same oracles as or_lanczos
+ *   or_sai_arnoldi / or_dense_inverse  -- DST closed form, dense expm, A*inverse = I
  *   or_cg_shifted                      -- dense solve, residual check, PSBLAS stop rules
  *   or_mscg                            -- dense complex solve per shift
  *   or_ellipkkp / or_ellipj / horner  -- the control values of examples/utils/utiltest.f90,
@@ -468,6 +469,146 @@ int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fsc
     nyk = k;
     or_dense_expm_apply(k, Hk, fscale, e1, yk);
     free(Hk); free(e1);
+  }
+  const double derr = (errden == 0.0) ? errnum : errnum / errden;
+  if (iter) *iter = itx;
+  if (err) *err = derr;
+  if (res) {
+    for (int q = 0; q < itx; ++q) res[q] = rs[q];
+    res[itx] = derr;
+  }
+  if (nres) *nres = itx + 1;
+  or_axpby(n, yk[0], VV(1), 0.0, y);
+  for (int q = 2; q <= itx - 1 && q <= nyk; ++q) or_axpby(n, yk[q - 1], VV(q), 1.0, y);
+  or_axpby(n, rs[0], y, 0.0, y);
+#undef HH
+#undef VV
+  free(V); free(w); free(Hm); free(rs); free(yk);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90:183-390)  */
+/* ------------------------------------------------------------------ */
+/* Inverse of a k x k column-major matrix by Gauss-Jordan elimination with
+ * partial pivoting (replaces the reference's dgetrf/dgetri, sai:269-289).
+ * Returns 0 on success, 1 on a zero pivot. */
+int or_dense_inverse(int k, const double *H, double *Hinv) {
+  double *W = (double *)malloc(sizeof(double) * (size_t)k * k);
+  memcpy(W, H, sizeof(double) * (size_t)k * k);
+  for (int i = 0; i < k * k; ++i) Hinv[i] = 0.0;
+  for (int i = 0; i < k; ++i) Hinv[i + (size_t)i * k] = 1.0;
+#define WW(i, j) W[(i) + (size_t)(j) * k]
+#define II(i, j) Hinv[(i) + (size_t)(j) * k]
+  for (int c = 0; c < k; ++c) {
+    int p = c;
+    for (int r = c + 1; r < k; ++r)
+      if (fabs(WW(r, c)) > fabs(WW(p, c))) p = r;
+    if (WW(p, c) == 0.0) {
+      free(W);
+      return 1;
+    }
+    if (p != c)
+      for (int j = 0; j < k; ++j) {
+        double t = WW(c, j); WW(c, j) = WW(p, j); WW(p, j) = t;
+        t = II(c, j); II(c, j) = II(p, j); II(p, j) = t;
+      }
+    const double inv = 1.0 / WW(c, c);
+    for (int j = 0; j < k; ++j) {
+      WW(c, j) *= inv;
+      II(c, j) *= inv;
+    }
+    for (int r = 0; r < k; ++r) {
+      if (r == c) continue;
+      const double f = WW(r, c);
+      if (f == 0.0) continue;
+      for (int j = 0; j < k; ++j) {
+        WW(r, j) -= f * WW(c, j);
+        II(r, j) -= f * II(c, j);
+      }
+    }
+  }
+#undef WW
+#undef II
+  free(W);
+  return 0;
+}
+
+/* y = f(H_k^{-1} - tau I) e1 (the SAI projection of A). */
+static int sai_fproj(int k, const double *Hm, int ldh, double tau, double fscale, double *yk) {
+  double *Hk = (double *)malloc(sizeof(double) * k * k), *Hi = (double *)malloc(sizeof(double) * k * k);
+  double *e1 = (double *)calloc((size_t)k, sizeof(double));
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) Hk[r + (size_t)c * k] = Hm[r + (size_t)c * ldh];
+  int rc = or_dense_inverse(k, Hk, Hi);
+  for (int i = 0; i < k; ++i) Hi[i + (size_t)i * k] -= tau;
+  e1[0] = 1.0;
+  if (!rc) rc = or_dense_expm_apply(k, Hi, fscale, e1, yk);
+  free(Hk); free(Hi); free(e1);
+  return rc;
+}
+
+int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int prec, const double *b, double eta,
+                  double zeta, double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond);
+
+/* Krylov space of (A + tau I)^{-1}, each inner solve by PCG (prec 0/1) with
+ * tolerance eps, inner itmax/istop, zero initial guess; MGS as Arnoldi; the
+ * estimate and the fallback both from f(H_k^{-1} - tau I) e1 with the shift
+ * on every diagonal entry; assembly as Arnoldi. */
+int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double tau, int prec,
+                   int in_itmax, int in_istop, const double *x, double *y, double eps, int itmax, int istop,
+                   int *iter, double *err, double *res, int *nres) {
+  if (istop < 1 || istop > 2) return 2;
+  if (itmax < 2) return 1;
+  const int nl = itmax;
+  double *V = (double *)malloc(sizeof(double) * (size_t)n * (nl + 1));
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *Hm = (double *)calloc((size_t)(nl + 1) * (nl + 1), sizeof(double));
+  double *rs = (double *)calloc((size_t)nl + 1, sizeof(double));
+  double *yk = NULL;
+  int nyk = 0;
+#define HH(i, j) Hm[((i)-1) + (size_t)((j)-1) * (nl + 1)]
+#define VV(i) (V + (size_t)((i)-1) * n)
+  double errnum = 0.0, errden = 1.0;
+  or_axpby(n, 1.0, x, 0.0, VV(1));
+  rs[0] = or_nrm2(n, VV(1));
+  or_axpby(n, 1.0 / rs[0], VV(1), 0.0, VV(1));
+  int itx = 0;
+  for (int i = 2; i <= nl; ++i) {
+    itx++;
+    int it_in = 0;
+    double err_in = 0.0;
+    memset(w, 0, sizeof(double) * (size_t)n);
+    or_cg_shifted(n, rp, ci, va, prec, VV(i - 1), 1.0, tau, w, eps, in_itmax, in_istop, &it_in, &err_in, NULL);
+    for (int k = 1; k <= i - 1; ++k) {
+      HH(k, i - 1) = or_dot(n, VV(k), w);
+      or_axpby(n, -HH(k, i - 1), VV(k), 1.0, w);
+    }
+    HH(i, i - 1) = or_nrm2(n, w);
+    or_axpby(n, 1.0 / HH(i, i - 1), w, 0.0, VV(i));
+    if (istop == 1) {
+      const int k = i - 1;
+      free(yk);
+      yk = (double *)malloc(sizeof(double) * k);
+      nyk = k;
+      sai_fproj(k, Hm, nl + 1, tau, fscale, yk);
+      errnum = rs[0] * HH(i, i - 1) * fabs(yk[k - 1]);
+      double s2 = 0.0;
+      for (int q = 0; q < k; ++q) s2 += yk[q] * yk[q];
+      errden = sqrt(s2);
+      rs[i - 1] = errnum / errden;
+      if (rs[i - 1] < eps) break;
+      free(yk);
+      yk = NULL;
+      nyk = 0;
+    }
+  }
+  if (!yk) {
+    int k = itx - 1;
+    if (k < 1) k = 1;
+    yk = (double *)malloc(sizeof(double) * k);
+    nyk = k;
+    sai_fproj(k, Hm, nl + 1, tau, fscale, yk);
   }
   const double derr = (errden == 0.0) ? errnum : errnum / errden;
   if (iter) *iter = itx;

@@ -78,6 +78,8 @@ SIGNATURES = {
     "psfun_serial_apply_host": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _vp]),
     "psfun_lanczos_apply": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
     "psfun_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
+    "psfun_sai_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _i, _i, _i, _vp, _vp, _d, _i, _i, _pi, _pd, _pd,
+                                     _pi]),
     "psfun_batch_create": (_i, [_vp, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "psfun_batch_destroy": (_i, [_vp]),
     "psfun_batch_set_tuning": (_i, [_vp, _i, _i]),

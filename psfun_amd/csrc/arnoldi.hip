@@ -357,3 +357,209 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   if (nres) *nres = itx + 1;
   return PSFUN_SUCCESS;
 }
+
+// ===========================================================================
+// Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90:183-390): the
+// Krylov space of (A + tau I)^{-1}, every inner solve by the device CG of
+// psfun_cg_shifted; MGS as in Arnoldi; the estimate from
+// f(H_k^{-1} - tau I) e1, H_k^{-1} by an in-block Gauss-Jordan inverse.
+// ===========================================================================
+
+// v_out = w / ||w|| from the partials of ||w||^2; with first != 0 this is
+// v_1 = x / ||x|| and also records rs(1) (zero x: stop with y = 0).
+__global__ __launch_bounds__(PSFUN_BS) void k_sai_normalize(int n, const double *__restrict__ w,
+                                                            double *__restrict__ v_out,
+                                                            const double *__restrict__ p_in, int g_in, LzCtl *ctl,
+                                                            int first) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (first && blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->rs1 = sigma;
+    ctl->rs[1] = sigma;
+    if (sigma == 0.0) ctl->done = LZ_DONE_ZERO;
+  }
+  if (sigma == 0.0) return;
+  const double inv = 1.0 / sigma;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
+    v_out[i] = w[i] * inv;
+}
+
+__global__ __launch_bounds__(PSFUN_BS) void k_sai_dot(int64_t n, const double *__restrict__ a,
+                                                      const double *__restrict__ b, double *__restrict__ p_out) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
+    acc = fma(a[i], b[i], acc);
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
+}
+
+// yk = f(H_k^{-1} - tau I) e1 into W->y; false when H_k is singular.
+__device__ bool sai_fproj(int k, const double *H, int ldh, int model, double *M, double *Hi, double tau,
+                          double fscale, DenseLds *W) {
+  if (!block_gj_inverse(k, H, ldh, M, Hi, W)) return false;
+  for (int d = threadIdx.x; d < k; d += PSFUN_BS) Hi[d * k + d] -= tau;
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) W->y[i] = (i == 0) ? 1.0 : 0.0;
+  __syncthreads();
+  block_expmv(k, Hi, k, model, fscale, W);
+  return true;
+}
+
+// Reference control at loop index i (h(i,i-1) = ||w|| from the partials).
+__global__ __launch_bounds__(PSFUN_BS) void k_sai_ctrl(const double *__restrict__ p_in, int g_in, int i, LzCtl *ctl,
+                                                       double *H, int ldh, int model, double *M, double *Hi,
+                                                       double tau, int nl, int istop, double eps, double fscale) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  __shared__ DenseLds W;
+  if (ctl->done) return;
+  const int tid = threadIdx.x;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (tid == 0) {
+    H[(int64_t)(i - 2) * ldh + (i - 1)] = sigma;  // h(i, i-1)
+    ctl->itx = i - 1;
+  }
+  if (istop == 1) {
+    const int k = i - 1;
+    if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, &W)) {
+      if (tid == 0) ctl->done = LZ_DONE_BREAKDOWN;
+      return;
+    }
+    double ss = 0.0;
+    for (int q = tid; q < k; q += PSFUN_BS) {
+      const double v = W.y[q];
+      ss = fma(v, v, ss);
+      ctl->yk[q] = v;
+    }
+    const double nrm = sqrt(dense_block_sum(ss, &W));
+    const double errnum = ctl->rs1 * sigma * fabs(W.y[k - 1]);
+    const double rsi = errnum / nrm;
+    if (tid == 0) {
+      ctl->errnum = errnum;
+      ctl->errden = nrm;
+      ctl->rs[i] = rsi;
+      if (rsi < eps) {
+        ctl->m = k;
+        ctl->done = LZ_DONE_CONVERGED;
+      }
+    }
+    if (rsi < eps) return;
+  }
+  if (i >= nl && tid == 0) ctl->done = LZ_DONE_MAXIT;
+}
+
+// Coefficients when the loop did not end by convergence: the same projection
+// f(H_{itx-1}^{-1} - tau I) e1 (the 1x1 block stands in for itx-1 = 0).
+__global__ __launch_bounds__(PSFUN_BS) void k_sai_final(LzCtl *ctl, const double *H, int ldh, int model, double *M,
+                                                        double *Hi, double tau, double fscale) {
+  __shared__ DenseLds W;
+  const int done = ctl->done;
+  if (done == LZ_DONE_CONVERGED || done == LZ_DONE_ZERO || done == LZ_DONE_BREAKDOWN) return;
+  int k = ctl->itx - 1;
+  if (k < 1) k = 1;
+  if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, &W)) {
+    if (threadIdx.x == 0) ctl->done = LZ_DONE_BREAKDOWN;
+    return;
+  }
+  for (int q = threadIdx.x; q < k; q += PSFUN_BS) ctl->yk[q] = W.y[q];
+  if (threadIdx.x == 0) ctl->m = k;
+}
+
+extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int variant, double fscale,
+                                       double tau, int prec, int inner_itmax, int inner_istop, const double *x,
+                                       double *y, double eps, int itmax, int istop, int *iter, double *err,
+                                       double *res, int *nres) {
+  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+  if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  if (itmax < 2 || itmax > LZ_NL_MAX || itmax > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
+  if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG) return PSFUN_ERR_INVALID_ARG;
+  if (inner_istop < 1 || inner_istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  const int n = A->n;
+  const int nl = itmax;
+  if (n == 0) {
+    if (iter) *iter = 0;
+    if (err) *err = 0.0;
+    if (nres) *nres = 0;
+    return PSFUN_SUCCESS;
+  }
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  const int model = dense_model(variant);
+  const int ldh = nl + 1;
+  const int64_t ldv = (int64_t)align_up((size_t)n, 64);
+  const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
+  const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
+  const size_t m_bytes = align_up(sizeof(double) * (size_t)nl * nl, 256);
+  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)(nl + 1);  // v_1..v_nl, w
+  // the basis lives in the second workspace: the inner CG takes the first
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes, &ws));
+  LzCtl *ctl = (LzCtl *)ws;
+  double *H = (double *)((char *)ws + ctl_bytes);
+  double *M = (double *)((char *)ws + ctl_bytes + h_bytes);
+  double *Hi = (double *)((char *)ws + ctl_bytes + h_bytes + m_bytes);
+  double *V = (double *)((char *)ws + ctl_bytes + h_bytes + 2 * m_bytes);
+  double *w = V + (size_t)ldv * nl;
+  double *P[2] = {c->d_partials + 5 * PSFUN_MAX_GRID, c->d_partials + 6 * PSFUN_MAX_GRID};
+  const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
+  LzCtl *h = (LzCtl *)((char *)c->h_pinned + 256);
+
+  hipLaunchKernelGGL(k_ar_init, dim3(1), dim3(256), 0, c->stream, ctl);
+  hipLaunchKernelGGL(k_ar_zero, dim3(psfun_grid(c, ((int64_t)ldh * ldh + PSFUN_BS - 1) / PSFUN_BS, 1)),
+                     dim3(PSFUN_BS), 0, c->stream, H, (int64_t)ldh * ldh);
+  hipLaunchKernelGGL(k_ar_sumsq, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, P[0]);
+  hipLaunchKernelGGL(k_sai_normalize, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, x, V, P[0], Gu, ctl, 1);
+  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  const int Gz = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
+  if (h->done == 0) {
+    for (int i = 2; i <= nl; ++i) {
+      const int j = i - 1;
+      // w = (A + tau I)^{-1} v_j, zero initial guess (psfun_d_sai.f90:216-218)
+      hipLaunchKernelGGL(k_ar_zero, dim3(Gz), dim3(PSFUN_BS), 0, c->stream, w, (int64_t)n);
+      PSFUN_HIP_TRY(hipGetLastError());
+      int it_in = 0;
+      double err_in = 0.0;
+      const int st = psfun_cg_shifted(c, A, prec, V + (size_t)ldv * (j - 1), 1.0, tau, w, eps, inner_itmax,
+                                      inner_istop, &it_in, &err_in, nullptr);
+      if (st != PSFUN_SUCCESS) return st;
+      // modified Gram-Schmidt against v_1..v_j, then v_{j+1} = w / h(j+1, j)
+      hipLaunchKernelGGL(k_sai_dot, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, V, w, P[0]);
+      int cur = 0;
+      for (int k = 1; k <= j; ++k) {
+        hipLaunchKernelGGL(k_ar_mgs, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, k, j, V, ldv, w, P[cur], Gu,
+                           P[1 - cur], ctl, H, ldh);
+        cur = 1 - cur;
+      }
+      hipLaunchKernelGGL(k_sai_normalize, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, w, V + (size_t)ldv * j,
+                         P[cur], Gu, ctl, 0);
+      hipLaunchKernelGGL(k_sai_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], Gu, i, ctl, H, ldh, model, M,
+                         Hi, tau, nl, istop, eps, fscale);
+      PSFUN_HIP_TRY(hipGetLastError());
+      PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+      if (h->done != 0) break;
+    }
+  }
+  hipLaunchKernelGGL(k_sai_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, M, Hi, tau, fscale);
+  PSFUN_HIP_TRY(hipGetLastError());
+  const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
+  hipLaunchKernelGGL(k_ar_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
+  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(LzCtl), hipMemcpyDeviceToHost, c->stream));
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (h->done == LZ_DONE_BREAKDOWN) return PSFUN_ERR_BREAKDOWN;
+  const int itx = h->itx;
+  const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
+  if (iter) *iter = itx;
+  if (err) *err = e;
+  if (res) {
+    for (int q = 1; q <= itx; ++q) res[q - 1] = h->rs[q];
+    res[itx] = e;
+  }
+  if (nres) *nres = itx + 1;
+  return PSFUN_SUCCESS;
+}

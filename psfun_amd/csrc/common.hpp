@@ -54,9 +54,12 @@ struct psfun_ctx {
   volatile int *h_flags = nullptr;
   int *d_flags = nullptr;         // device address of h_flags
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // grow-only workspace
+  // grow-only workspaces: ws for each solver's own state; ws2 for an outer
+  // method whose state must survive inner solver calls (shift-and-invert Arnoldi)
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  void *ws2 = nullptr;
+  size_t ws2_bytes = 0;
   // optional per-launch timing of the hot kernels (psfun_prof_*)
   int prof_on = 0;                       // 0 off; R >= 1: timing on, idempotent steps run R times
   std::vector<hipEvent_t> prof_ev;       // event pool, 2 per timed launch
@@ -122,6 +125,7 @@ int psfun_host_wait(psfun_ctx *c, int target);
 // Device workspace: returns a pointer to at least `bytes` bytes (256-byte
 // aligned).  Grows by reallocation; previous contents are not preserved.
 int psfun_workspace(psfun_ctx *ctx, size_t bytes, void **ptr);
+int psfun_workspace2(psfun_ctx *ctx, size_t bytes, void **ptr);
 
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 

@@ -68,6 +68,7 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->own_stream) hipStreamSynchronize(c->own_stream);
   if (c->ws) hipFree(c->ws);
+  if (c->ws2) hipFree(c->ws2);
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_scalar) hipFree(c->d_scalar);
   if (c->h_pinned) hipHostFree(c->h_pinned);
@@ -123,27 +124,32 @@ int psfun_host_wait(psfun_ctx *c, int target) {
   return PSFUN_SUCCESS;
 }
 
-int psfun_workspace(psfun_ctx *c, size_t bytes, void **ptr) {
+static int grow_workspace(psfun_ctx *c, void **buf, size_t *have, size_t bytes, void **ptr) {
   bytes = align_up(bytes < 256 ? 256 : bytes, 256);
-  if (bytes > c->ws_bytes) {
+  if (bytes > *have) {
     PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->ws) PSFUN_HIP_TRY(hipFree(c->ws));
-    c->ws = nullptr;
-    c->ws_bytes = 0;
+    if (*buf) PSFUN_HIP_TRY(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
     size_t want = align_up(bytes + bytes / 8, 1 << 20);
-    if (hipMalloc(&c->ws, want) != hipSuccess) {
+    if (hipMalloc(buf, want) != hipSuccess) {
       (void)hipGetLastError();
-      if (hipMalloc(&c->ws, bytes) != hipSuccess) {
+      if (hipMalloc(buf, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        c->ws = nullptr;
+        *buf = nullptr;
         return PSFUN_ERR_ALLOC;
       }
       want = bytes;
     }
-    c->ws_bytes = want;
+    *have = want;
   }
-  *ptr = c->ws;
+  *ptr = *buf;
   return PSFUN_SUCCESS;
+}
+
+int psfun_workspace(psfun_ctx *c, size_t bytes, void **ptr) { return grow_workspace(c, &c->ws, &c->ws_bytes, bytes, ptr); }
+int psfun_workspace2(psfun_ctx *c, size_t bytes, void **ptr) {
+  return grow_workspace(c, &c->ws2, &c->ws2_bytes, bytes, ptr);
 }
 
 // Lanes per row: the smallest power of two >= half the mean row length

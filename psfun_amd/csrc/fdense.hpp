@@ -127,3 +127,92 @@ __device__ void block_expmv(int k, const double *H, int ldh, int model, double s
     __syncthreads();
   }
 }
+
+// Hi = H_k^{-1} (k x k, leading dimension k) by Gauss-Jordan elimination with
+// partial pivoting, one block of PSFUN_BS threads (replaces the LAPACK
+// dgetrf/dgetri pair of the shift-and-invert step).  H is read with leading
+// dimension ldh; M (k*k) is the work copy.  Both M and Hi are global memory
+// private to this block.  Returns false (in every thread) on a zero pivot.
+__device__ bool block_gj_inverse(int k, const double *H, int ldh, double *M, double *Hi, DenseLds *W) {
+  const int tid = threadIdx.x;
+  const int kk = k * k;
+  for (int idx = tid; idx < kk; idx += PSFUN_BS) {
+    const int r = idx % k, c = idx / k;
+    M[idx] = H[(int64_t)c * ldh + r];
+    Hi[idx] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    // pivot row: largest |M(r,c)|, r >= c (ties: the smaller row)
+    double best = -1.0;
+    int bi = k;
+    for (int r = c + tid; r < k; r += PSFUN_BS) {
+      const double v = fabs(M[c * k + r]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if ((tid & 63) == 0) {
+      W->part[tid >> 6] = best;
+      W->part[8 + (tid >> 6)] = (double)bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b = W->part[0];
+      int p = (int)W->part[8];
+      for (int w = 1; w < PSFUN_BS / 64; ++w) {
+        const double ob = W->part[w];
+        const int oi = (int)W->part[8 + w];
+        if (ob > b || (ob == b && oi < p)) {
+          b = ob;
+          p = oi;
+        }
+      }
+      W->part[16] = b;
+      W->part[17] = (double)p;
+    }
+    __syncthreads();
+    if (!(W->part[16] > 0.0)) return false;
+    const int p = (int)W->part[17];
+    if (p != c) {
+      for (int j = tid; j < k; j += PSFUN_BS) {
+        double t = M[j * k + c];
+        M[j * k + c] = M[j * k + p];
+        M[j * k + p] = t;
+        t = Hi[j * k + c];
+        Hi[j * k + c] = Hi[j * k + p];
+        Hi[j * k + p] = t;
+      }
+    }
+    __syncthreads();
+    const double inv = 1.0 / M[c * k + c];
+    __syncthreads();
+    for (int j = tid; j < k; j += PSFUN_BS) {
+      M[j * k + c] *= inv;
+      Hi[j * k + c] *= inv;
+    }
+    __syncthreads();
+    for (int r = tid; r < k; r += PSFUN_BS) W->t0[r] = (r == c) ? 0.0 : M[c * k + r];
+    __syncthreads();
+    for (int idx = tid; idx < kk; idx += PSFUN_BS) {
+      const int r = idx % k, j = idx / k;
+      const double f = W->t0[r];
+      if (f != 0.0) {
+        M[idx] = fma(-f, M[j * k + c], M[idx]);
+        Hi[idx] = fma(-f, Hi[j * k + c], Hi[idx]);
+      }
+    }
+    __syncthreads();
+  }
+  return true;
+}

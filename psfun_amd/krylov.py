@@ -2,7 +2,9 @@
 
 `set` mirrors psfun_d_setstring/psfun_d_setinteger (lines 179-223);
 `apply` mirrors psfun_d_parallel_apply (lines 225-269), dispatching on
-psb_toupper(kname) to the device LANCZOS / ARNOLDI implementations.  The
+psb_toupper(kname) to the device LANCZOS / ARNOLDI / SAIARNOLDI
+implementations (SAIARNOLDI takes kmethd, prec, tau and the inner
+itmax/itrace/istopc from the object, lines 262-264).  The
 whole Krylov loop, the f(T) step and the stopping test run on the GPU.
 
 apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None)
@@ -20,6 +22,7 @@ class psfun_d_krylov:
     def __init__(self):
         self.kname = "ARNOLDI"
         self.kmethd = "CG"
+        self.prec = None          # inner preconditioner (shifted.psb_dprec_type) for SAIARNOLDI
         self.tau = 0.0
         self.itmax = 200
         self.itrace = 0
@@ -52,6 +55,10 @@ class psfun_d_krylov:
         if name == "ARNOLDI":
             from .arnoldi import arnoldi_apply
             return arnoldi_apply(fun, a, desc_a, y, x, eps, itmax, itrace, istop)
+        if name == "SAIARNOLDI":   # inner method, preconditioner, shift and inner limits from the object
+            from .arnoldi import saiarnoldi_apply
+            return saiarnoldi_apply(fun, a, self.kmethd, self.prec, self.tau, desc_a, self.itmax, self.itrace,
+                                    self.istopc, y, x, eps, itmax, itrace, istop)
         return psb.psb_err_invalid_args_combination_, 0, 0.0, None
 
 

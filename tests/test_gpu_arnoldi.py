@@ -134,7 +134,7 @@ def test_krylov_default_kname_is_arnoldi(gpu_ctx):
     info, it, err, res = km.apply(fun, M, psb.psb_desc_type(100), y, _vec(x), 1e-10, itmax=40)
     assert info == 0
     assert _rel(y.get_vect(), O.arnoldi(A, x, 1e-10, 40, 1)["y"]) <= 1e-10
-    km.set("KNAME", "SAIARNOLDI")
+    km.set("KNAME", "NOPE")
     assert km.apply(fun, M, psb.psb_desc_type(100), y, _vec(x), 1e-10)[0] == psb.psb_err_invalid_args_combination_
 
 

@@ -1,0 +1,92 @@
+"""GPU parity of shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90)
+against the oracle restatement and the DST closed form.
+
+Both sides solve every inner system (A + tau I) w = v by CG to the outer
+tolerance, so they agree to about that tolerance (bar 1e-8, the rational /
+quadrature path's bar), not to the polynomial path's 1e-10."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from psfun_amd import matgen, psb
+from psfun_amd.arnoldi import saiarnoldi_apply
+from psfun_amd.krylov import psfun_d_krylov
+from psfun_amd.serial import psfun_d_serial
+from psfun_amd.shifted import psb_dprec_type
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (20, 20)
+SCALE = 20.0
+
+
+def _vec(a):
+    return psb.psb_d_vect_type(torch.as_tensor(np.ascontiguousarray(a, np.float64), device="cuda:0"))
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _fun():
+    f = psfun_d_serial()
+    f.set("VARIANT", "GENPADE")
+    return f
+
+
+@pytest.mark.parametrize("tau", [0.0, -5.0])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+def test_sai_parity_and_closed_form(gpu_ctx, tau, fmt):
+    A = matgen.laplacian(DIMS, scale=-SCALE)          # negative definite: CG applies for tau <= 0
+    n = int(np.prod(DIMS))
+    x = matgen.uniform_pm1(17, n)
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format(fmt)
+    y = _vec(np.zeros(n))
+    info, it, err, res = saiarnoldi_apply(_fun(), M, "CG", None, tau, psb.psb_desc_type(n), 4000, 0, 2, y, _vec(x),
+                                          1e-10, itmax=150, istop=1)
+    assert info == 0
+    ref = O.sai_arnoldi(A, x, 1e-10, 150, tau=tau, istop=1, in_itmax=4000, in_istop=2)
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-8
+    assert abs(it - ref["iter"]) <= 1
+    assert _rel(y.get_vect(), matgen.dst_expm_laplacian(DIMS, x, SCALE)) <= 2e-8
+    assert len(res) == it + 1 and abs(res[0] - np.linalg.norm(x)) <= 1e-12 * np.linalg.norm(x)
+
+
+def test_sai_through_krylov_object(gpu_ctx):
+    """kname 'SAIARNOLDI' dispatches with kmethd, prec, tau and the inner
+    itmax/istopc of the method object (psfun_d_krylov_mod.F90:262-264)."""
+    A = matgen.laplacian(DIMS, scale=-SCALE)
+    n = int(np.prod(DIMS))
+    x = matgen.normal(18, n)
+    Mt = psb.psb_dspmat_type.from_csr(A)
+    km = psfun_d_krylov()
+    km.set("KNAME", "saiarnoldi")
+    km.set("ITMAX", 4000)          # inner iterations
+    km.set("ISTOPC", 2)
+    km.tau = -2.0
+    km.prec = psb_dprec_type()
+    km.prec.init(None, "DIAG")
+    y = _vec(np.zeros(n))
+    info, it, err, res = km.apply(_fun(), Mt, psb.psb_desc_type(n), y, _vec(x), 1e-10, itmax=150, istop=1)
+    assert info == 0 and err < 1e-10
+    ref = O.sai_arnoldi(A, x, 1e-10, 150, tau=-2.0, istop=1, prec=1, in_itmax=4000, in_istop=2)
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-8
+    assert _rel(y.get_vect(), matgen.dst_expm_laplacian(DIMS, x, SCALE)) <= 2e-8
+
+
+def test_sai_istop2_and_zero_vector(gpu_ctx):
+    A = matgen.laplacian(DIMS, scale=-SCALE)
+    n = int(np.prod(DIMS))
+    x = matgen.uniform_pm1(19, n)
+    M = psb.psb_dspmat_type.from_csr(A)
+    y = _vec(np.zeros(n))
+    info, it, err, res = saiarnoldi_apply(_fun(), M, "CG", None, -1.0, psb.psb_desc_type(n), 4000, 0, 2, y, _vec(x),
+                                          1e-10, itmax=12, istop=2)
+    ref = O.sai_arnoldi(A, x, 1e-10, 12, tau=-1.0, istop=2, in_itmax=4000, in_istop=2)
+    assert info == 0 and it == 11 == ref["iter"] and err == 0.0
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-8
+    info, it, err, res = saiarnoldi_apply(_fun(), M, "CG", None, -1.0, psb.psb_desc_type(n), 4000, 0, 2, y,
+                                          _vec(np.zeros(n)), 1e-10, itmax=12)
+    assert info == 0 and it == 0 and np.all(y.get_vect() == 0.0)

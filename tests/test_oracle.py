@@ -288,3 +288,33 @@ def test_hht_quadraturetest_poles():
     xi, c, eta, sign = O.hhtmethod1(lambda z: np.sqrt(1 + z), 40, [0.01, 4.0])
     for lam in (0.02, 0.5, 3.9):
         assert abs(eta * lam * np.imag(np.sum(c / (xi - lam))) - np.sqrt(1 + lam)) < 1e-12
+
+
+def test_dense_inverse_is_inverse():
+    rng = np.random.default_rng(3)
+    for k in (1, 2, 7, 31):
+        H = rng.normal(size=(k, k)) + k * np.eye(k)
+        Hi = O.dense_inverse(H)
+        assert np.allclose(H @ Hi, np.eye(k), rtol=0, atol=1e-12)
+        assert np.allclose(Hi, np.linalg.inv(H), rtol=1e-12, atol=1e-13)
+    # a permutation needs pivoting
+    P = np.eye(4)[[2, 0, 3, 1]]
+    assert np.array_equal(O.dense_inverse(P), P.T)
+
+
+@pytest.mark.parametrize("tau", [0.0, -5.0])
+def test_sai_arnoldi_closed_form(tau):
+    """Shift-and-invert Arnoldi on the stiff A = -20 L (negative definite, so
+    (A + tau I) is definite for tau <= 0 and the inner CG applies): exp(A) x
+    against the DST closed form (inner solves to the outer eps 1e-10 bound
+    the accuracy near 1e-8), in far fewer steps than polynomial Arnoldi."""
+    dims = (20, 20)
+    A = matgen.laplacian(dims, scale=-20.0)
+    x = matgen.uniform_pm1(17, 400)
+    r = O.sai_arnoldi(A, x, 1e-10, 150, tau=tau, istop=1, in_itmax=4000, in_istop=2)
+    assert r["info"] == 0
+    yex = matgen.dst_expm_laplacian(dims, x, 20.0)
+    assert np.linalg.norm(r["y"] - yex) <= 2e-8 * np.linalg.norm(yex)
+    ra = O.arnoldi(A, x, 1e-10, 150, 1)
+    assert r["iter"] < 0.5 * ra["iter"]
+    assert abs(r["res"][0] - np.linalg.norm(x)) <= 1e-12 * np.linalg.norm(x)
