@@ -54,7 +54,9 @@ def parse():
     p.add_argument("--prof-steps", type=int, default=3)
     p.add_argument("--copy-probe", action="store_true", help="also run the 1 GiB copy that scales PMC counters")
     p.add_argument("--prof-reps", type=int, default=4, help="back-to-back launches per timed SpMV step")
-    p.add_argument("--no-extra", action="store_true", help="headline only (no batched/adjoint/quadrature)")
+    p.add_argument("--no-extra", action="store_true", help="headline only (no blas1/batched/adjoint/quadrature)")
+    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature",
+                   help="comma-separated extra sections to run after the headline")
     p.add_argument("--format", default="", help="SELL, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     return p.parse_args()
 
@@ -195,7 +197,19 @@ def headline(args):
             if time.perf_counter() - tc0 > args.cpu_seconds or runs >= 50:
                 break
         tc = time.perf_counter() - tc0
+        # the same restatement on one core, a shorter sample (about 5 s)
+        O.set_threads(1)
+        runs1 = 0
+        t10 = time.perf_counter()
+        while True:
+            O.lanczos(A, x, args.eps, args.itmax, 1)
+            runs1 += 1
+            if time.perf_counter() - t10 > args.cpu_seconds / 3 or runs1 >= 20:
+                break
+        rate1 = runs1 / (time.perf_counter() - t10)
+        O.set_threads(threads)
         cpu = {"value": round(runs / tc, 4), "unit": "f(A)x/s", "cores": threads, "kind": "port",
+               "value_1core": round(rate1, 4), "sample_1core": f"{runs1} full f(A)x on 1 core",
                "sample": f"{runs} full f(A)x of the same {m}^3 workload (oracle/psfun_oracle.c: unfused "
                          f"PSBLAS-style passes, OpenMP over rows, f(T) by tridiagonal QL every iteration)",
                "gpu_vs_cpu_rel_diff": float(np.linalg.norm(y - ref["y"]) / np.linalg.norm(ref["y"])),
@@ -277,7 +291,9 @@ def adjoint_section(args):
     A = matgen.anisotropic_2d(m, seed=5)
     n = m * m
     dev = "cuda:0"
-    op = ExpmvOperator(A[0], A[1], eps=1e-10, itmax=200, quad_poles=16)
+    torch.cuda.empty_cache()
+    # the forward keeps its Krylov basis for the backward (2 x 51 GB here)
+    op = ExpmvOperator(A[0], A[1], eps=1e-10, itmax=200, quad_poles=16, store_basis=True)
     vals = torch.tensor(A[2], device=dev, requires_grad=True)
     X = torch.tensor(matgen.normal(21, n * K).reshape(n, K), device=dev, requires_grad=True)
     t = torch.tensor(tt, dtype=torch.float64, device=dev, requires_grad=True)
@@ -378,6 +394,49 @@ def quadrature_section(args):
             "seed_spmv_GBs": round(ks["bytes"] / max(ks["ms"], 1e-9) / 1e6, 1)}
 
 
+def blas1_section(args):
+    """The PSBLAS BLAS-1 primitives behind the Krylov loops (psb_gedot,
+    psb_genrm2, psb_geamax, psb_geaxpby) on 1 GiB vectors: per-kernel device
+    time from HIP events on the library stream, against 8 TB/s."""
+    import ctypes as C
+    import torch
+    from psfun_amd import _lib, psb
+    ctx = psb.default_context(0)
+    n = 1 << 27
+    x = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    y = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    ctx.bind_stream()
+    r = C.c_double()
+    xp, yp = C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())
+    ops = {
+        "dot": (lambda: L.psfun_dot(ctx.handle, n, xp, yp, C.byref(r)), "reduce"),
+        "nrm2": (lambda: L.psfun_nrm2(ctx.handle, n, xp, C.byref(r)), "reduce"),
+        "amax": (lambda: L.psfun_amax(ctx.handle, n, xp, C.byref(r)), "reduce"),
+        "axpby": (lambda: L.psfun_axpby(ctx.handle, n, 0.5, xp, 0.25, yp), "axpby"),
+    }
+    out = {}
+    for name, (fn, kid) in ops.items():
+        for _ in range(2):
+            _lib.check(fn())
+        torch.cuda.synchronize()
+        L.psfun_prof_reset(ctx.handle)
+        L.psfun_prof_enable(ctx.handle, 1)
+        for _ in range(5):
+            _lib.check(fn())
+        torch.cuda.synchronize()
+        L.psfun_prof_enable(ctx.handle, 0)
+        k = read_prof(L, _lib, ctx, kid)
+        us = 1e3 * k["ms"] / max(k["launches"], 1)
+        gbs = k["bytes"] / max(k["ms"], 1e-9) / 1e6
+        out[name] = {"avg_launch_us": round(us, 2), "bytes_per_launch": k["bytes"] / max(k["launches"], 1),
+                     "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log("blas1: " + ", ".join(f"{k} {v['achieved_GBs']} GB/s" for k, v in out.items()))
+    out["workload"] = "BLAS-1 over 2^27 doubles (1 GiB per vector), partial sums folded by a second launch"
+    return out
+
+
 def copy_probe():
     """One y = 1.0*x over 2^27 doubles (1 GiB read + 1 GiB written) through
     psfun_axpby: the known-size kernel that scales the rocprofv3 FETCH_SIZE /
@@ -407,8 +466,11 @@ def main():
         copy_probe()
     extra = {}
     if not args.no_extra:
-        for name, fn in (("batched", batched_section), ("adjoint", adjoint_section),
+        wanted = set(args.sections.split(","))
+        for name, fn in (("blas1", blas1_section), ("batched", batched_section), ("adjoint", adjoint_section),
                          ("quadrature", quadrature_section)):
+            if name not in wanted:
+                continue
             try:
                 extra[name] = fn(args)
             except Exception as e:  # report, do not hide, a failing section

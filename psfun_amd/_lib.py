@@ -35,7 +35,8 @@ PSFUN_QUAD_RE_SUM = 0
 PSFUN_QUAD_A_IM_SUM = 1
 
 PROF_IDS = {"lz_spmv": 0, "lz_update": 1, "lz_assemble": 2, "spmv": 3, "ms_spmv": 4, "ms_update": 5,
-            "cg": 6, "batch": 7, "sddmm": 8, "ar_spmv": 9, "ar_mgs": 10}
+            "cg": 6, "batch": 7, "sddmm": 8, "ar_spmv": 9, "ar_mgs": 10,
+            "reduce": 11, "axpby": 12}
 
 _vp = C.c_void_p
 _i = C.c_int

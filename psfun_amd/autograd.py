@@ -15,8 +15,9 @@ Backward (A symmetric, loss L with dL/dY = G):
              iterate of step m), likewise u_s from W.
 With store_basis=False the forward keeps nothing and the backward rebuilds V
 by re-running the forward Lanczos (memory for one basis at a time);
-store_basis="auto" (the default) keeps the basis only when it fits in half
-of the device memory that is free at forward time.
+store_basis="auto" (the default) keeps the basis only when the forward and
+adjoint bases fit in half of the device memory that is free at forward time
+(including what torch's caching allocator holds unused).
 grad_mode="mscg" takes v_s, u_s from full multi-shift CG solves instead (one
 column at a time; slower, used to cross-check).
 """
@@ -75,6 +76,8 @@ class ExpmvOperator:
         if self.store_basis != "auto":
             return bool(self.store_basis)
         free, _total = torch.cuda.mem_get_info(device)
+        # memory torch's caching allocator holds but does not use is free for us too
+        free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
         # the backward holds the adjoint basis too, so two bases must fit
         return 2 * self.basis_bytes(K) <= 0.5 * free
 

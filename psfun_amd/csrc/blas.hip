@@ -192,15 +192,18 @@ __global__ __launch_bounds__(PSFUN_BS) void k_reduce_final(const double *__restr
 
 int launch_reduce(psfun_ctx *c, int op, int64_t n, const double *x, const double *y, double *d_out) {
   const int64_t chunks = (n + PSFUN_BS - 1) / PSFUN_BS;
-  const int G = psfun_grid(c, chunks > 0 ? chunks : 1, 4);
+  const int G = psfun_grid(c, chunks > 0 ? chunks : 1, 16);
   double *part = c->d_partials + 7 * PSFUN_MAX_GRID;
-#define RED_LAUNCH(OPV)                                                                                  \
-  hipLaunchKernelGGL(k_reduce_partials<OPV>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part);     \
-  hipLaunchKernelGGL(k_reduce_final<OPV>, dim3(1), dim3(PSFUN_BS), 0, c->stream, (const double *)part, G, d_out)
-  if (op == 0) { RED_LAUNCH(0); }
-  else if (op == 1) { RED_LAUNCH(1); }
-  else { RED_LAUNCH(2); }
-#undef RED_LAUNCH
+  // algorithmic bytes: x (and y for a dot) read once
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_REDUCE, (op == 0 ? 16.0 : 8.0) * n));
+  if (op == 0) hipLaunchKernelGGL(k_reduce_partials<0>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part);
+  else if (op == 1) hipLaunchKernelGGL(k_reduce_partials<1>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part);
+  else hipLaunchKernelGGL(k_reduce_partials<2>, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, x, y, part);
+  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_TRY(psfun_prof_end(c));
+  if (op == 0) hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(PSFUN_BS), 0, c->stream, (const double *)part, G, d_out);
+  else if (op == 1) hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(PSFUN_BS), 0, c->stream, (const double *)part, G, d_out);
+  else hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(PSFUN_BS), 0, c->stream, (const double *)part, G, d_out);
   PSFUN_HIP_TRY(hipGetLastError());
   return PSFUN_SUCCESS;
 }
@@ -240,9 +243,11 @@ __global__ __launch_bounds__(PSFUN_BS) void k_scal(int64_t n, double alpha, doub
 
 int launch_axpby(psfun_ctx *c, int64_t n, double alpha, const double *x, double beta, double *y) {
   if (n == 0) return PSFUN_SUCCESS;
-  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
+  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 16);
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AXPBY, (beta == 0.0 ? 16.0 : 24.0) * n));
   hipLaunchKernelGGL(k_axpby, dim3(G), dim3(PSFUN_BS), 0, c->stream, n, alpha, x, beta, y);
   PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_TRY(psfun_prof_end(c));
   return PSFUN_SUCCESS;
 }
 

@@ -213,18 +213,39 @@ __global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in,
   lz_control<false>(ctl, j, beta, nl, istop, eps, fscale, &W, hf);
 }
 
+// The stop flag, the partials and this thread's first (u, v_j, v_{j-1})
+// triple are loaded before anything waits on them, so the fold's latency hides
+// behind the first loads.  No block of the grid writes the flag, so every
+// thread reads the same value.
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restrict__ u,
                                                         const double *__restrict__ vj,
                                                         const double *__restrict__ vjm1, int j,
                                                         const double *__restrict__ p_in, int g_in,
                                                         double *__restrict__ p_out, LzCtl *ctl) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
-  const double al = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha[j] = al;
+  const int stopped = ctl->done;
+  double pacc = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < g_in; b += PSFUN_BS) pacc += p_in[b];
+  const int64_t i0 = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  double u0 = 0.0, a0 = 0.0, b0 = 0.0;
+  if (i0 < n) {
+    u0 = u[i0];
+    a0 = vj[i0];
+    if (j > 1) b0 = vjm1[i0];
+  }
   const double be = (j > 1) ? ctl->beta[j - 1] : 0.0;
+  if (stopped) return;
+  const double al = block_sum<PSFUN_BS>(pacc, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha[j] = al;
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+  if (i0 < n) {
+    double w = fma(-al, a0, u0);
+    if (j > 1) w = fma(-be, b0, w);
+    u[i0] = w;
+    acc = fma(w, w, acc);
+  }
+  for (int64_t i = i0 + (int64_t)gridDim.x * PSFUN_BS; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
     double w = fma(-al, vj[i], u[i]);
     if (j > 1) w = fma(-be, vjm1[i], w);
     u[i] = w;
