@@ -11,7 +11,8 @@
 //
 // Per Krylov index j (one launch sequence for all instances):
 //   k_bt_spmm(j):   v_j = (1/sigma) src, u = opscale*A v_j, alpha partials
-//   k_bt_update(j): alpha folded; w = u - alpha v_j - beta v_{j-1}; ||w||^2 partials
+//   k_bt_fold(j):   one wavefront per instance folds its alpha partials
+//   k_bt_update(j): w = u - alpha v_j - beta v_{j-1}; ||w||^2 partials
 //   k_bt_ctrl(j+1): one wavefront per instance: beta = ||w||, breakdown test,
 //                   a-posteriori estimate from T_j, stop decision.
 // Instances stop independently; a stopped instance's rows are skipped.  The
@@ -60,11 +61,34 @@ struct BtInst {
 };
 
 // ---------------------------------------------------------------------------
-template <int KP>
-__device__ __forceinline__ double fold_graph(const double *__restrict__ P, int c0, int c1, int k) {
+// Per-instance fold of per-chunk partials, one wavefront per instance (the
+// 64 lanes stride the graph's chunks, then a fixed-order wave sum), so the
+// streaming kernels read one scalar instead of folding a large graph's
+// partials in every block.  MODE 0: alpha_j of running instances;
+// MODE 1: ||x|| -> rs(1) (a zero start vector stops the instance).
+template <int KP, int MODE>
+__global__ __launch_bounds__(64) void k_bt_fold(const int *__restrict__ gchunk, int K, int j,
+                                                const double *__restrict__ p_in, BtInst in, BtCtl *bctl) {
+  const int I = blockIdx.x;
+  if (MODE == 0 && in.done[I] != BT_RUN) return;
+  const int g = I / K, k = I % K;
+  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
   double acc = 0.0;
-  for (int c = c0; c < c1; ++c) acc += P[(int64_t)c * KP + k];
-  return acc;
+  for (int cc = c0 + threadIdx.x; cc < c1; cc += 64) acc += p_in[(int64_t)cc * KP + k];
+  acc = wave_sum(acc);
+  if (threadIdx.x != 0) return;
+  const int64_t base = (int64_t)I * in.ld;
+  if (MODE == 0) {
+    in.alpha[base + j] = acc;
+  } else {
+    const double sg = sqrt(acc);
+    in.rs1[I] = sg;
+    in.rs[base + 1] = sg;
+    if (sg == 0.0) {  // zero start vector: y = 0
+      in.done[I] = BT_ZERO;
+      atomicSub(&bctl->remaining, 1);
+    }
+  }
 }
 
 template <int KP>
@@ -106,7 +130,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
   if (bctl->remaining == 0) return;
   const int c = blockIdx.x;
   const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
-  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
   if (threadIdx.x == 0) s_any = 0;
   __syncthreads();
   if (threadIdx.x < KP) {
@@ -116,15 +139,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
     if (k < K) {
       const int I = g * K + k;
       if (j == 1) {
-        const double sg = sqrt(fold_graph<KP>(p_in, c0, c1, k));
-        if (c == c0) {
-          in.rs1[I] = sg;
-          in.rs[(int64_t)I * in.ld + 1] = sg;
-          if (sg == 0.0) {  // zero start vector: y = 0
-            in.done[I] = BT_ZERO;
-            atomicSub(&bctl->remaining, 1);
-          }
-        }
+        const double sg = in.rs1[I];  // folded by k_bt_fold<KP, 1>
         act = (sg != 0.0);
         inv = act ? 1.0 / sg : 0.0;
       } else {
@@ -174,7 +189,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
   if (bctl->remaining == 0) return;
   const int c = blockIdx.x;
   const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
-  const int c0 = gchunk[2 * g], c1 = gchunk[2 * g + 1];
   if (threadIdx.x < KP) {
     const int k = threadIdx.x;
     int act = 0;
@@ -183,9 +197,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
       const int I = g * K + k;
       act = (in.done[I] == BT_RUN);
       if (act) {
-        al = fold_graph<KP>(p_in, c0, c1, k);
+        al = in.alpha[(int64_t)I * in.ld + j];  // folded by k_bt_fold<KP, 0>
         be = (j > 1) ? in.beta[(int64_t)I * in.ld + j - 1] : 0.0;
-        if (c == c0) in.alpha[(int64_t)I * in.ld + j] = al;
       }
     }
     s_al[k] = al;
@@ -450,6 +463,7 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
   hipLaunchKernelGGL(k_bt_init, dim3(std::min(1024, (ninst * in.ld + 255) / 256 + 1)), dim3(256), 0, c->stream, in,
                      ninst, bctl);
   hipLaunchKernelGGL((k_bt_norm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, X, P0);
+  hipLaunchKernelGGL((k_bt_fold<KP, 1>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, 0, P0, in, bctl);
   const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 24.0 * B->nrows * K;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
   hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val, opscale,
@@ -462,6 +476,7 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     double *u = (j & 1) ? U0 : U1;
     const double *vj = V + (size_t)vs * (j - 1);
     const double *vjm1 = (j > 1) ? V + (size_t)vs * (j - 2) : nullptr;
+    hipLaunchKernelGGL((k_bt_fold<KP, 0>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j, P1, in, bctl);
     hipLaunchKernelGGL((k_bt_update<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, B->d_gchunk, K, j, u,
                        vj, vjm1, P1, P0, in, bctl);
     hipLaunchKernelGGL((k_bt_ctrl<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
