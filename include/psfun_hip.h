@@ -46,17 +46,27 @@ enum {
   PSFUN_ERR_BREAKDOWN = 10            /* solver breakdown reported as an error (CG/COCG only) */
 };
 
-/* f(H) step selection: function names follow psfun_d_serial%fname ('EXP'),
- * variants follow psfun_d_serial%variant (serial/psfun_d_serial_mod.F90:214-390). */
-enum { PSFUN_FN_EXP = 0 };
+/* f(H) step selection: function names follow psfun_d_serial%fname ('EXP',
+ * 'USERF'), variants follow psfun_d_serial%variant
+ * (serial/psfun_d_serial_mod.F90:214-498).  EXP takes the first six variants;
+ * USERF takes SYM only. */
+enum { PSFUN_FN_EXP = 0, PSFUN_FN_USERF = 1 };
 enum {
   PSFUN_VAR_CHBSYM = 0,   /* symmetric H (Lanczos) */
   PSFUN_VAR_CHBHES = 1,   /* Hessenberg H (Arnoldi) */
   PSFUN_VAR_CHBGEN = 2,
   PSFUN_VAR_GENPADE = 3,
   PSFUN_VAR_TAYLOR = 4,
-  PSFUN_VAR_SASQ = 5
+  PSFUN_VAR_SASQ = 5,
+  PSFUN_VAR_SYM = 6       /* USERF: y = Q diag(f(scaling*lambda)) Q^T x of the
+                             symmetric matrix read from H's upper triangle */
 };
+
+/* User function of the USERF f(H) step: f(x), called on the host with the
+ * `user` pointer given to psfun_set_userf (psfun_d_serial%f_ptr,
+ * serial/psfun_d_serial_mod.F90:49, 71-80; the reference's optional
+ * derivative order is never passed by its SYM variant). */
+typedef double (*psfun_userf_t)(double x, void *user);
 
 typedef struct psfun_ctx *psfun_ctx_t;
 typedef struct psfun_csr *psfun_csr_t;
@@ -78,6 +88,11 @@ int psfun_ctx_destroy(psfun_ctx_t ctx);
 int psfun_ctx_set_stream(psfun_ctx_t ctx, void *hip_stream);
 /* Go back to the context's own stream. */
 int psfun_ctx_use_own_stream(psfun_ctx_t ctx);
+/* The user function that fname = PSFUN_FN_USERF evaluates (fun%f_ptr set by
+ * psfun_d_setpointer, serial/psfun_d_serial_mod.F90:153-168).  It is called
+ * from the calling thread inside the apply, once per eigenvalue of each f(H)
+ * step; USERF applies without a function return PSFUN_ERR_INVALID_ARG. */
+int psfun_set_userf(psfun_ctx_t ctx, psfun_userf_t f, void *user);
 int psfun_ctx_synchronize(psfun_ctx_t ctx);
 /* Number of compute units of the context's device (for launch-shape tuning). */
 int psfun_ctx_num_cus(psfun_ctx_t ctx, int *num_cus);
@@ -157,10 +172,14 @@ int psfun_scal(psfun_ctx_t ctx, int64_t n, double alpha, double *x);
 /* ---- f(H) step on a small dense matrix (psfun_d_serial%apply) ------------ */
 /* y = f(scaling*H) x for a k-by-k column-major host matrix H (k <= 512),
  * evaluated on the device (psfun_d_serial_apply_array,
- * serial/psfun_d_serial_mod.F90:172-523).  fname = PSFUN_FN_EXP.  The variant
+ * serial/psfun_d_serial_mod.F90:172-523).  fname = PSFUN_FN_EXP: the variant
  * fixes how H is read, as the reference's EXPOKIT routines read it: CHBSYM as
  * symmetric from its upper triangle (DSCHBV), CHBHES as upper Hessenberg
- * (DNCHBV), CHBGEN/GENPADE/TAYLOR/SASQ as a full matrix. */
+ * (DNCHBV), CHBGEN/GENPADE/TAYLOR/SASQ as a full matrix.  fname =
+ * PSFUN_FN_USERF, variant PSFUN_VAR_SYM (lines 420-485): the symmetric matrix
+ * of H's upper triangle is diagonalised on the device (DSYEV('V','U') there),
+ * f(scaling*lambda_i) is evaluated by the psfun_set_userf function, and
+ * y = Q diag(f) Q^T x. */
 int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scaling, int k,
                             const double *H, const double *x, double *y);
 
@@ -170,7 +189,10 @@ int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scal
  * The Krylov operator is opscale*A (the reference uses opscale = 1, ignoring
  * fun%scaling, psfun_d_lanczos.f90:172-173,198,239).  itmax >= 2, istop in {1,2}.
  * Outputs: *iter, *err; res (host, length >= itmax+1) receives res(1:itx+1)
- * and *nres = itx+1.  x and y are device vectors of length n. */
+ * and *nres = itx+1.  x and y are device vectors of length n.
+ * fname = PSFUN_FN_USERF evaluates f(T) as the USERF/SYM variant does
+ * (psfun_set_userf); with istop = 1 the loop then waits for the host once
+ * per iteration, where the user function runs. */
 int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter,
                         double *err, double *res, int *nres);
@@ -181,7 +203,10 @@ int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale
  * fun%scaling and evaluates f with scaling 1, lines 172-173, 202), f(H) is
  * read as `variant` reads a matrix (see psfun_serial_apply_host).  No
  * breakdown test, as in the reference; x = 0 gives y = 0 and iter = 0.
- * itmax in 2..512, istop in {1,2}; outputs as for psfun_lanczos_apply. */
+ * itmax in 2..512, istop in {1,2}; outputs as for psfun_lanczos_apply.
+ * fname = PSFUN_FN_USERF with variant PSFUN_VAR_SYM: f(H_k) as the USERF/SYM
+ * f(H) step reads H_k (the symmetric matrix of its upper triangle), with one
+ * host wait per iteration when istop = 1. */
 int psfun_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, double fscale, double scaling,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
                         double *res, int *nres);
@@ -198,7 +223,9 @@ int psfun_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, 
  * not converge also assembles from f(H^{-1} - tau I) (the reference uses
  * f(H) there).  For tau = 0 and a converged run the results coincide.  f is
  * read as `variant` reads a matrix; GENPADE/CHBGEN/TAYLOR/SASQ treat the full
- * H^{-1}.  Returns PSFUN_ERR_BREAKDOWN if H_k is singular. */
+ * H^{-1}; fname = PSFUN_FN_USERF with PSFUN_VAR_SYM takes the symmetric
+ * matrix of its upper triangle.  Returns PSFUN_ERR_BREAKDOWN if H_k is
+ * singular. */
 int psfun_sai_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, double fscale, double tau,
                             int prec, int inner_itmax, int inner_istop, const double *x, double *y, double eps,
                             int itmax, int istop, int *iter, double *err, double *res, int *nres);

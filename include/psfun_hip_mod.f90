@@ -15,9 +15,9 @@ module psfun_hip_mod
        & PSFUN_ERR_INVALID_VECT_STATE = 3, PSFUN_ERR_INVALID_ARGS_COMBINATION = 4, PSFUN_ERR_NOT_SQUARE = 5, &
        & PSFUN_ERR_UNSUPPORTED = 6, PSFUN_ERR_ALLOC = 7, PSFUN_ERR_HIP = 8, PSFUN_ERR_NO_DEVICE = 9, &
        & PSFUN_ERR_BREAKDOWN = 10
-  integer(c_int), parameter :: PSFUN_FN_EXP = 0
+  integer(c_int), parameter :: PSFUN_FN_EXP = 0, PSFUN_FN_USERF = 1
   integer(c_int), parameter :: PSFUN_VAR_CHBSYM = 0, PSFUN_VAR_CHBHES = 1, PSFUN_VAR_CHBGEN = 2, &
-       & PSFUN_VAR_GENPADE = 3, PSFUN_VAR_TAYLOR = 4, PSFUN_VAR_SASQ = 5
+       & PSFUN_VAR_GENPADE = 3, PSFUN_VAR_TAYLOR = 4, PSFUN_VAR_SASQ = 5, PSFUN_VAR_SYM = 6
   integer(c_int), parameter :: PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1
   integer(c_int), parameter :: PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2
   integer(c_int), parameter :: PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1
@@ -39,6 +39,15 @@ module psfun_hip_mod
     function psfun_ctx_synchronize(ctx) bind(C, name='psfun_ctx_synchronize') result(st)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+      integer(c_int) :: st
+    end function
+    ! USERF: f is a bind(C) function  real(c_double) f(x, user)  with x and
+    ! user passed by value; a PSFUN wrapper forwards to fun%f_ptr(x).
+    function psfun_set_userf(ctx, f, user) bind(C, name='psfun_set_userf') result(st)
+      import :: c_int, c_ptr, c_funptr
+      type(c_ptr), value :: ctx
+      type(c_funptr), value :: f
+      type(c_ptr), value :: user
       integer(c_int) :: st
     end function
 

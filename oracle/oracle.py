@@ -17,6 +17,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lib = None
 
 ZFUN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+DFUN = C.CFUNCTYPE(C.c_double, C.c_double)
 
 
 def lib():
@@ -48,10 +49,20 @@ def lib():
         L.or_lanczos.restype = C.c_int
         L.or_arnoldi.argtypes = kry
         L.or_arnoldi.restype = C.c_int
+        L.or_lanczos_f.argtypes = kry + [DFUN]
+        L.or_lanczos_f.restype = C.c_int
+        L.or_arnoldi_f.argtypes = kry + [DFUN]
+        L.or_arnoldi_f.restype = C.c_int
+        L.or_sym_eig.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp]
+        L.or_sym_eig.restype = C.c_int
+        L.or_userf_sym_apply.argtypes = [C.c_int, _dp, C.c_int, C.c_double, DFUN, _dp, _dp]
+        L.or_userf_sym_apply.restype = C.c_int
         L.or_sai_arnoldi.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                      _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), _dp, C.POINTER(C.c_int)]
         L.or_sai_arnoldi.restype = C.c_int
+        L.or_sai_arnoldi_f.argtypes = list(L.or_sai_arnoldi.argtypes) + [DFUN]
+        L.or_sai_arnoldi_f.restype = C.c_int
         L.or_dense_inverse.argtypes = [C.c_int, _dp, _dp]
         L.or_dense_inverse.restype = C.c_int
         L.or_cg_shifted.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, C.c_double,
@@ -130,7 +141,7 @@ def dense_expm(H, s=1.0):
     return E.reshape(k, k, order="F")
 
 
-def _krylov(fn, A, x, eps, itmax, istop, fscale, opscale):
+def _krylov(fn, A, x, eps, itmax, istop, fscale, opscale, *extra):
     rp, ci, va = _csr(A)
     n = len(rp) - 1
     y = np.zeros(n)
@@ -139,23 +150,34 @@ def _krylov(fn, A, x, eps, itmax, istop, fscale, opscale):
     nres = C.c_int(0)
     res = np.zeros(max(itmax, 2) + 2)
     rc = fn(n, rp, ci, va, fscale, opscale, np.ascontiguousarray(x, np.float64), y, eps, itmax, istop,
-            C.byref(it), C.byref(err), res, C.byref(nres))
+            C.byref(it), C.byref(err), res, C.byref(nres), *extra)
     return dict(info=rc, y=y, iter=it.value, err=err.value, res=res[:nres.value].copy())
 
 
-def lanczos(A, x, eps, itmax, istop=1, fscale=1.0, opscale=1.0):
-    """Reference-semantics Lanczos (krylov/polynomial/psfun_d_lanczos.f90)."""
-    return _krylov(lib().or_lanczos, A, x, eps, itmax, istop, fscale, opscale)
+def _dfun(f):
+    return DFUN(lambda z: float(f(z)))
 
 
-def arnoldi(A, x, eps, itmax, istop=1, fscale=1.0, scaling=1.0):
-    """Reference-semantics Arnoldi (krylov/polynomial/psfun_d_arnoldi.f90)."""
-    return _krylov(lib().or_arnoldi, A, x, eps, itmax, istop, fscale, scaling)
+def lanczos(A, x, eps, itmax, istop=1, fscale=1.0, opscale=1.0, f=None):
+    """Reference-semantics Lanczos (krylov/polynomial/psfun_d_lanczos.f90);
+    f(T) by EXP, or with f given by the USERF/SYM step with that f."""
+    if f is None:
+        return _krylov(lib().or_lanczos, A, x, eps, itmax, istop, fscale, opscale)
+    return _krylov(lib().or_lanczos_f, A, x, eps, itmax, istop, fscale, opscale, _dfun(f))
 
 
-def sai_arnoldi(A, x, eps, itmax, tau=0.0, istop=1, prec=0, in_itmax=1000, in_istop=2, fscale=1.0):
+def arnoldi(A, x, eps, itmax, istop=1, fscale=1.0, scaling=1.0, f=None):
+    """Reference-semantics Arnoldi (krylov/polynomial/psfun_d_arnoldi.f90);
+    with f given, f(H) by the USERF/SYM step (H's upper triangle)."""
+    if f is None:
+        return _krylov(lib().or_arnoldi, A, x, eps, itmax, istop, fscale, scaling)
+    return _krylov(lib().or_arnoldi_f, A, x, eps, itmax, istop, fscale, scaling, _dfun(f))
+
+
+def sai_arnoldi(A, x, eps, itmax, tau=0.0, istop=1, prec=0, in_itmax=1000, in_istop=2, fscale=1.0, f=None):
     """Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90), with the
-    shift applied to every diagonal entry of H^{-1}."""
+    shift applied to every diagonal entry of H^{-1}; with f given, the
+    projection by the USERF/SYM step."""
     rp, ci, va = _csr(A)
     n = len(rp) - 1
     y = np.zeros(n)
@@ -163,10 +185,38 @@ def sai_arnoldi(A, x, eps, itmax, tau=0.0, istop=1, prec=0, in_itmax=1000, in_is
     err = C.c_double(0.0)
     nres = C.c_int(0)
     res = np.zeros(max(itmax, 2) + 2)
-    rc = lib().or_sai_arnoldi(n, rp, ci, va, fscale, tau, prec, in_itmax, in_istop,
-                              np.ascontiguousarray(x, np.float64), y, eps, itmax, istop, C.byref(it), C.byref(err),
-                              res, C.byref(nres))
+    args = (n, rp, ci, va, fscale, tau, prec, in_itmax, in_istop, np.ascontiguousarray(x, np.float64), y, eps,
+            itmax, istop, C.byref(it), C.byref(err), res, C.byref(nres))
+    if f is None:
+        rc = lib().or_sai_arnoldi(*args)
+    else:
+        rc = lib().or_sai_arnoldi_f(*args, _dfun(f))
     return dict(info=rc, y=y, iter=it.value, err=err.value, res=res[:nres.value].copy())
+
+
+def sym_eig(H):
+    """(w, Z) of the symmetric matrix of H's upper triangle (DSYEV('V','U'))."""
+    H = np.asfortranarray(H, dtype=np.float64)
+    k = H.shape[0]
+    w = np.zeros(k)
+    Z = np.zeros(k * k)
+    rc = lib().or_sym_eig(k, np.ascontiguousarray(H.reshape(-1, order="F")), k, w, Z)
+    if rc:
+        raise RuntimeError("symmetric eigen-decomposition did not converge")
+    return w, Z.reshape(k, k, order="F")
+
+
+def userf_sym_apply(H, s, f, x):
+    """USERF/SYM f(H) step (serial/psfun_d_serial_mod.F90:420-485):
+    y = Z diag(f(s*lambda)) Z^T x for the symmetric matrix of H's upper triangle."""
+    H = np.asfortranarray(H, dtype=np.float64)
+    k = H.shape[0]
+    y = np.zeros(k)
+    rc = lib().or_userf_sym_apply(k, np.ascontiguousarray(H.reshape(-1, order="F")), k, s, _dfun(f),
+                                  np.ascontiguousarray(x, np.float64), y)
+    if rc:
+        raise RuntimeError("symmetric eigen-decomposition did not converge")
+    return y
 
 
 def dense_inverse(H):

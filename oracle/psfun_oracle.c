@@ -14,6 +14,11 @@
  * Pinning (what checks each routine; see tests/):
  *   or_tridiag_eig / or_exp_tridiag   -- scipy.linalg.expm fixtures, closed-form 2x2
  *   or_dense_expm                      -- scipy.linalg.expm fixtures, diagonal/nilpotent cases
+ *   or_sym_eig / or_userf_sym_apply    -- numpy.linalg.eigh (LAPACK) eigenvalues and reconstruction,
+ *                                         scipy expm / sqrtm of the upper-triangle matrix
+ *   or_lanczos_f / or_arnoldi_f /
+ *   or_sai_arnoldi_f                   -- f = exp equals the EXP restatements; f = sqrt against
+ *                                         scipy sqrtm on small SPD matrices
  *   or_lanczos                         -- DST closed form exp(-tA) for Dirichlet Laplacians,
  *                                         scipy dense expm on small matrices, breakdown cases
  *   or_arnoldi                         -- same oracles as or_lanczos
@@ -164,9 +169,19 @@ int or_tridiag_eig(int k, double *d, const double *e_in, double *Z) {
   return 0;
 }
 
+typedef double (*or_f_t)(double);
+
+static int tridiag_fapply(int k, const double *d, const double *e, double s, or_f_t f, const double *x, double *y);
+
 /* y = exp(s*T) x for symmetric tridiagonal T (d: k, e: k-1), via the
  * eigen-decomposition T = Z diag(lambda) Z^T. */
 int or_exp_tridiag(int k, const double *d, const double *e, double s, const double *x, double *y) {
+  return tridiag_fapply(k, d, e, s, NULL, x, y);
+}
+
+/* y = Z diag(f(s*lambda)) Z^T x for symmetric tridiagonal T = Z diag(lambda) Z^T;
+ * f = NULL is exp. */
+static int tridiag_fapply(int k, const double *d, const double *e, double s, or_f_t f, const double *x, double *y) {
   double *lam = (double *)malloc(sizeof(double) * (size_t)k);
   double *Z = (double *)malloc(sizeof(double) * (size_t)k * k);
   double *t = (double *)malloc(sizeof(double) * (size_t)k);
@@ -176,7 +191,7 @@ int or_exp_tridiag(int k, const double *d, const double *e, double s, const doub
     for (int j = 0; j < k; ++j) {
       double a = 0.0;
       for (int i = 0; i < k; ++i) a += Z[i + (size_t)j * k] * x[i];
-      t[j] = a * exp(s * lam[j]);
+      t[j] = a * (f ? f(s * lam[j]) : exp(s * lam[j]));
     }
     for (int i = 0; i < k; ++i) {
       double a = 0.0;
@@ -187,6 +202,94 @@ int or_exp_tridiag(int k, const double *d, const double *e, double s, const doub
   free(lam);
   free(Z);
   free(t);
+  return rc;
+}
+
+/* USERF/SYM f(H) step (serial/psfun_d_serial_mod.F90:420-485, DSYEV('V','U')
+ * then DGEMV / f / DGEMV).  The symmetric matrix S of H's upper triangle is
+ * reduced to tridiagonal form by Householder reflections (Golub & Van Loan,
+ * Matrix Computations, Alg. 8.3.1) accumulated into Q, T's eigenvectors come
+ * from or_tridiag_eig, and Z = Q Z_T.  w receives the eigenvalues, Z (k x k,
+ * column-major) the eigenvectors.  Returns 0 on success. */
+int or_sym_eig(int k, const double *H, int ldh, double *w, double *Z) {
+  double *A = (double *)malloc(sizeof(double) * (size_t)k * k);
+  double *Q = (double *)calloc((size_t)k * k, sizeof(double));
+  double *Zt = (double *)malloc(sizeof(double) * (size_t)k * k);
+  double *v = (double *)malloc(sizeof(double) * (size_t)k), *pv = (double *)malloc(sizeof(double) * (size_t)k);
+  double *e = (double *)calloc((size_t)k + 1, sizeof(double));
+#define AA(i, j) A[(i) + (size_t)(j)*k]
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) AA(i, j) = (i <= j) ? H[i + (size_t)j * ldh] : H[j + (size_t)i * ldh];
+  for (int i = 0; i < k; ++i) Q[i + (size_t)i * k] = 1.0;
+  for (int j = 0; j + 2 < k; ++j) {
+    /* reflector P = I - beta v v^T with P x = alpha e1, x = A(j+1:k, j) */
+    const int m = k - j - 1;
+    double sig = 0.0;
+    for (int i = 1; i < m; ++i) sig += AA(j + 1 + i, j) * AA(j + 1 + i, j);
+    const double x0 = AA(j + 1, j);
+    if (sig == 0.0) continue; /* already tridiagonal in this column */
+    const double mu = sqrt(x0 * x0 + sig);
+    const double alpha = (x0 <= 0.0) ? mu : -mu;
+    v[0] = x0 - alpha;
+    for (int i = 1; i < m; ++i) v[i] = AA(j + 1 + i, j);
+    double vv = 0.0;
+    for (int i = 0; i < m; ++i) vv += v[i] * v[i];
+    const double beta = 2.0 / vv;
+    /* A22 <- P A22 P: p = beta A22 v, q = p - (beta p.v / 2) v, A22 -= v q^T + q v^T */
+    double pvv = 0.0;
+    for (int r = 0; r < m; ++r) {
+      double acc = 0.0;
+      for (int c = 0; c < m; ++c) acc += AA(j + 1 + r, j + 1 + c) * v[c];
+      pv[r] = beta * acc;
+      pvv += pv[r] * v[r];
+    }
+    for (int r = 0; r < m; ++r) pv[r] -= 0.5 * beta * pvv * v[r];
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < m; ++r) AA(j + 1 + r, j + 1 + c) -= v[r] * pv[c] + pv[r] * v[c];
+    AA(j + 1, j) = alpha;
+    AA(j, j + 1) = alpha;
+    for (int i = 1; i < m; ++i) AA(j + 1 + i, j) = AA(j, j + 1 + i) = 0.0;
+    /* Q <- Q P */
+    for (int r = 0; r < k; ++r) {
+      double acc = 0.0;
+      for (int c = 0; c < m; ++c) acc += Q[r + (size_t)(j + 1 + c) * k] * v[c];
+      acc *= beta;
+      for (int c = 0; c < m; ++c) Q[r + (size_t)(j + 1 + c) * k] -= acc * v[c];
+    }
+  }
+  for (int i = 0; i < k; ++i) w[i] = AA(i, i);
+  for (int i = 0; i + 1 < k; ++i) e[i] = AA(i + 1, i);
+#undef AA
+  int rc = or_tridiag_eig(k, w, e, Zt);
+  if (rc == 0)
+    for (int c = 0; c < k; ++c)
+      for (int r = 0; r < k; ++r) {
+        double acc = 0.0;
+        for (int l = 0; l < k; ++l) acc += Q[r + (size_t)l * k] * Zt[l + (size_t)c * k];
+        Z[r + (size_t)c * k] = acc;
+      }
+  free(A); free(Q); free(Zt); free(v); free(pv); free(e);
+  return rc;
+}
+
+/* y = Z diag(f(s*lambda)) Z^T x for the symmetric matrix of H's upper triangle. */
+int or_userf_sym_apply(int k, const double *H, int ldh, double s, or_f_t f, const double *x, double *y) {
+  double *w = (double *)malloc(sizeof(double) * (size_t)k), *Z = (double *)malloc(sizeof(double) * (size_t)k * k);
+  double *t = (double *)malloc(sizeof(double) * (size_t)k);
+  int rc = or_sym_eig(k, H, ldh, w, Z);
+  if (rc == 0) {
+    for (int j = 0; j < k; ++j) { /* t = Z^T x, scaled by f(s lambda_j) */
+      double a = 0.0;
+      for (int i = 0; i < k; ++i) a += Z[i + (size_t)j * k] * x[i];
+      t[j] = f(s * w[j]) * a;
+    }
+    for (int i = 0; i < k; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < k; ++j) a += Z[i + (size_t)j * k] * t[j];
+      y[i] = a;
+    }
+  }
+  free(w); free(Z); free(t);
   return rc;
 }
 
@@ -313,9 +416,10 @@ int or_dense_expm_apply(int k, const double *H, double s, const double *x, doubl
 /* f = exp(fscale * .), Krylov operator opscale*A (the reference passes 1).
  * res must hold itmax+1 doubles.  Returns 0 or an error code:
  * 2 = invalid istop (lines 108-113), 1 = other invalid argument. */
-int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fscale, double opscale,
-               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
-               double *res, int *nres) {
+/* f = NULL: EXP (CHBSYM); otherwise the USERF/SYM step with that f. */
+int or_lanczos_f(int n, const int *rp, const int *ci, const double *va, double fscale, double opscale,
+                 const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+                 double *res, int *nres, or_f_t f) {
   if (istop < 1 || istop > 2) return 2;
   if (itmax < 2) return 1;
   const int nl = itmax;
@@ -357,7 +461,7 @@ int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fsc
       for (int q = 1; q <= k; ++q) d[q - 1] = TT(q, q);
       for (int q = 1; q < k; ++q) e[q - 1] = TT(q, q + 1);
       e1[0] = 1.0;
-      or_exp_tridiag(k, d, e, fscale, e1, yk); /* fun%apply(t(1:i-1,1:i-1),yk,e)  (292) */
+      tridiag_fapply(k, d, e, fscale, f, e1, yk); /* fun%apply(t(1:i-1,1:i-1),yk,e)  (292) */
       free(d); free(e); free(e1);
       errnum = rs[0] * TT(i, i - 1) * fabs(yk[i - 2]);
       double s2 = 0.0;
@@ -381,7 +485,7 @@ int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fsc
     for (int q = 1; q <= k; ++q) d[q - 1] = TT(q, q);
     for (int q = 1; q < k; ++q) e[q - 1] = TT(q, q + 1);
     e1[0] = 1.0;
-    or_exp_tridiag(k, d, e, fscale, e1, yk);
+    tridiag_fapply(k, d, e, fscale, f, e1, yk);
     free(d); free(e); free(e1);
   }
   /* log_end: err = errnum/errden, iter = itx   (365-371) */
@@ -403,14 +507,21 @@ int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fsc
   return 0;
 }
 
+int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fscale, double opscale,
+               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+               double *res, int *nres) {
+  return or_lanczos_f(n, rp, ci, va, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, nres, NULL);
+}
+
 /* ------------------------------------------------------------------ */
 /* Arnoldi f(A)x  (krylov/polynomial/psfun_d_arnoldi.f90:172-329)      */
 /* ------------------------------------------------------------------ */
 /* The Krylov operator is scaling*A (line 202) and f(H) uses scaling 1
  * (line 174); fscale multiplies H inside f for callers that want it. */
-int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double scaling,
-               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
-               double *res, int *nres) {
+/* f = NULL: EXP of the full H; otherwise USERF/SYM (upper triangle of H). */
+int or_arnoldi_f(int n, const int *rp, const int *ci, const double *va, double fscale, double scaling,
+                 const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+                 double *res, int *nres, or_f_t f) {
   if (istop < 1 || istop > 2) return 2;
   if (itmax < 2) return 1;
   const int nl = itmax;
@@ -445,7 +556,8 @@ int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fsc
       free(yk);
       yk = (double *)malloc(sizeof(double) * k);
       nyk = k;
-      or_dense_expm_apply(k, Hk, fscale, e1, yk);
+      if (f) or_userf_sym_apply(k, Hk, k, fscale, f, e1, yk);
+      else or_dense_expm_apply(k, Hk, fscale, e1, yk);
       free(Hk); free(e1);
       errnum = rs[0] * HH(i, i - 1) * fabs(yk[i - 2]);
       double s2 = 0.0;
@@ -467,7 +579,8 @@ int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fsc
     e1[0] = 1.0;
     yk = (double *)malloc(sizeof(double) * k);
     nyk = k;
-    or_dense_expm_apply(k, Hk, fscale, e1, yk);
+    if (f) or_userf_sym_apply(k, Hk, k, fscale, f, e1, yk);
+      else or_dense_expm_apply(k, Hk, fscale, e1, yk);
     free(Hk); free(e1);
   }
   const double derr = (errden == 0.0) ? errnum : errnum / errden;
@@ -485,6 +598,12 @@ int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fsc
 #undef VV
   free(V); free(w); free(Hm); free(rs); free(yk);
   return 0;
+}
+
+int or_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double scaling,
+               const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
+               double *res, int *nres) {
+  return or_arnoldi_f(n, rp, ci, va, fscale, scaling, x, y, eps, itmax, istop, iter, err, res, nres, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -535,7 +654,7 @@ int or_dense_inverse(int k, const double *H, double *Hinv) {
 }
 
 /* y = f(H_k^{-1} - tau I) e1 (the SAI projection of A). */
-static int sai_fproj(int k, const double *Hm, int ldh, double tau, double fscale, double *yk) {
+static int sai_fproj(int k, const double *Hm, int ldh, double tau, double fscale, or_f_t f, double *yk) {
   double *Hk = (double *)malloc(sizeof(double) * k * k), *Hi = (double *)malloc(sizeof(double) * k * k);
   double *e1 = (double *)calloc((size_t)k, sizeof(double));
   for (int c = 0; c < k; ++c)
@@ -543,7 +662,7 @@ static int sai_fproj(int k, const double *Hm, int ldh, double tau, double fscale
   int rc = or_dense_inverse(k, Hk, Hi);
   for (int i = 0; i < k; ++i) Hi[i + (size_t)i * k] -= tau;
   e1[0] = 1.0;
-  if (!rc) rc = or_dense_expm_apply(k, Hi, fscale, e1, yk);
+  if (!rc) rc = f ? or_userf_sym_apply(k, Hi, k, fscale, f, e1, yk) : or_dense_expm_apply(k, Hi, fscale, e1, yk);
   free(Hk); free(Hi); free(e1);
   return rc;
 }
@@ -555,9 +674,9 @@ int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int pre
  * tolerance eps, inner itmax/istop, zero initial guess; MGS as Arnoldi; the
  * estimate and the fallback both from f(H_k^{-1} - tau I) e1 with the shift
  * on every diagonal entry; assembly as Arnoldi. */
-int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double tau, int prec,
-                   int in_itmax, int in_istop, const double *x, double *y, double eps, int itmax, int istop,
-                   int *iter, double *err, double *res, int *nres) {
+int or_sai_arnoldi_f(int n, const int *rp, const int *ci, const double *va, double fscale, double tau, int prec,
+                     int in_itmax, int in_istop, const double *x, double *y, double eps, int itmax, int istop,
+                     int *iter, double *err, double *res, int *nres, or_f_t f) {
   if (istop < 1 || istop > 2) return 2;
   if (itmax < 2) return 1;
   const int nl = itmax;
@@ -591,7 +710,7 @@ int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double
       free(yk);
       yk = (double *)malloc(sizeof(double) * k);
       nyk = k;
-      sai_fproj(k, Hm, nl + 1, tau, fscale, yk);
+      sai_fproj(k, Hm, nl + 1, tau, fscale, f, yk);
       errnum = rs[0] * HH(i, i - 1) * fabs(yk[k - 1]);
       double s2 = 0.0;
       for (int q = 0; q < k; ++q) s2 += yk[q] * yk[q];
@@ -608,7 +727,7 @@ int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double
     if (k < 1) k = 1;
     yk = (double *)malloc(sizeof(double) * k);
     nyk = k;
-    sai_fproj(k, Hm, nl + 1, tau, fscale, yk);
+    sai_fproj(k, Hm, nl + 1, tau, fscale, f, yk);
   }
   const double derr = (errden == 0.0) ? errnum : errnum / errden;
   if (iter) *iter = itx;
@@ -625,6 +744,13 @@ int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double
 #undef VV
   free(V); free(w); free(Hm); free(rs); free(yk);
   return 0;
+}
+
+int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double fscale, double tau, int prec,
+                   int in_itmax, int in_istop, const double *x, double *y, double eps, int itmax, int istop,
+                   int *iter, double *err, double *res, int *nres) {
+  return or_sai_arnoldi_f(n, rp, ci, va, fscale, tau, prec, in_itmax, in_istop, x, y, eps, itmax, istop, iter, err,
+                          res, nres, NULL);
 }
 
 /* ------------------------------------------------------------------ */

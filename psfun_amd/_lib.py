@@ -23,7 +23,9 @@ PSFUN_ERR_NO_DEVICE = 9
 PSFUN_ERR_BREAKDOWN = 10
 
 PSFUN_FN_EXP = 0
+PSFUN_FN_USERF = 1
 VARIANTS = {"CHBSYM": 0, "CHBHES": 1, "CHBGEN": 2, "GENPADE": 3, "TAYLOR": 4, "SASQ": 5}
+PSFUN_VAR_SYM = 6
 PSFUN_MEM_HOST = 0
 PSFUN_MEM_DEVICE = 1
 PSFUN_FORMAT_AUTO = 0
@@ -53,6 +55,7 @@ SIGNATURES = {
     "psfun_ctx_destroy": (_i, [_vp]),
     "psfun_ctx_set_stream": (_i, [_vp, _vp]),
     "psfun_ctx_use_own_stream": (_i, [_vp]),
+    "psfun_set_userf": (_i, [_vp, _vp, _vp]),
     "psfun_ctx_synchronize": (_i, [_vp]),
     "psfun_ctx_num_cus": (_i, [_vp, _pi]),
     "psfun_prof_enable": (_i, [_vp, _i]),
@@ -104,6 +107,7 @@ SIGNATURES = {
 }
 
 ZFUN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+USERF = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)   # psfun_userf_t
 
 _lib = None
 

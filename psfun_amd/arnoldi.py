@@ -26,7 +26,7 @@ def arnoldi_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None
     n = desc_a.get_global_rows() if desc_a is not None else a.n
     if x.get_nrows() != n or y.get_nrows() != n or a.n != n:
         return psb.psb_err_from_subroutine_, 0, 0.0, None
-    st, var = fun.device_variant()
+    st, fn, var = fun.device_fn()
     if st != _lib.PSFUN_SUCCESS:
         return psb.psb_err_from_subroutine_non_, 0, 0.0, None
     ctx = a.ctxt
@@ -35,10 +35,11 @@ def arnoldi_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None
     err = C.c_double(0.0)
     nres = C.c_int(0)
     res = np.zeros(max(nl, 2) + 2)
-    st = _lib.lib().psfun_arnoldi_apply(ctx.handle, a.handle, _lib.PSFUN_FN_EXP, var, 1.0, float(fun.scaling),
-                                        C.c_void_p(x.v.data_ptr()), C.c_void_p(y.v.data_ptr()), float(eps), nl,
-                                        istop_, C.byref(it), C.byref(err), res.ctypes.data_as(C.POINTER(C.c_double)),
-                                        C.byref(nres))
+    with fun.bound(ctx):
+        st = _lib.lib().psfun_arnoldi_apply(ctx.handle, a.handle, fn, var, 1.0, float(fun.scaling),
+                                            C.c_void_p(x.v.data_ptr()), C.c_void_p(y.v.data_ptr()), float(eps), nl,
+                                            istop_, C.byref(it), C.byref(err),
+                                            res.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nres))
     if st != _lib.PSFUN_SUCCESS:
         return psb.status_to_info(st), 0, 0.0, None
     return psb.psb_success_, it.value, err.value, res[: nres.value].copy()
@@ -63,7 +64,7 @@ def saiarnoldi_apply(fun, a, kryl, prec, tau, desc_a, initmax, initrace, inistop
         return psb.psb_err_from_subroutine_, 0, 0.0, None
     if str(kryl).upper() != "CG":
         sys.stderr.write(f"psfun_krylov: Warning: Unknown method  {kryl}, defaulting to CG\n")
-    st, var = fun.device_variant()
+    st, fn, var = fun.device_fn()
     if st != _lib.PSFUN_SUCCESS:
         return psb.psb_err_from_subroutine_non_, 0, 0.0, None
     pc = _lib.PSFUN_PREC_NONE if prec is None else prec.code()
@@ -73,10 +74,12 @@ def saiarnoldi_apply(fun, a, kryl, prec, tau, desc_a, initmax, initrace, inistop
     err = C.c_double(0.0)
     nres = C.c_int(0)
     res = np.zeros(max(nl, 2) + 2)
-    st = _lib.lib().psfun_sai_arnoldi_apply(ctx.handle, a.handle, _lib.PSFUN_FN_EXP, var, 1.0, float(tau), pc,
-                                            int(initmax), int(inistop), C.c_void_p(x.v.data_ptr()),
-                                            C.c_void_p(y.v.data_ptr()), float(eps), nl, istop_, C.byref(it),
-                                            C.byref(err), res.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nres))
+    with fun.bound(ctx):
+        st = _lib.lib().psfun_sai_arnoldi_apply(ctx.handle, a.handle, fn, var, 1.0, float(tau), pc,
+                                                int(initmax), int(inistop), C.c_void_p(x.v.data_ptr()),
+                                                C.c_void_p(y.v.data_ptr()), float(eps), nl, istop_, C.byref(it),
+                                                C.byref(err), res.ctypes.data_as(C.POINTER(C.c_double)),
+                                                C.byref(nres))
     if st != _lib.PSFUN_SUCCESS:
         return psb.status_to_info(st), 0, 0.0, None
     return psb.psb_success_, it.value, err.value, res[: nres.value].copy()

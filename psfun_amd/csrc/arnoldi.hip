@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 #include "fdense.hpp"
 #include "lanczos.hpp"
+#include "userf.hpp"
 
 // Reference control at loop index i = j >= 2, with h(j,j-1) = sigma.  All
 // threads of the control block.
@@ -242,8 +243,13 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   if (!c || !A) return PSFUN_ERR_INVALID_ARG;
   if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
   if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
-  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
-  if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  const bool userf = fname == PSFUN_FN_USERF;
+  if (userf) {
+    PSFUN_TRY(userf_check(c, variant));
+  } else {
+    if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+    if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  }
   if (itmax < 2 || itmax > LZ_NL_MAX || itmax > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
   const int n = A->n;
   const int nl = itmax;
@@ -254,7 +260,19 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     return PSFUN_SUCCESS;
   }
   PSFUN_HIP_TRY(hipSetDevice(c->device));
-  const int model = dense_model(variant);
+  const int model = userf ? DENSE_SYM_UPPER : dense_model(variant);
+  // USERF: the control blocks keep the bookkeeping; with istop = 1 the
+  // estimate and the iteration limit run in a host-synchronous step after
+  // each control index (the user function runs on the host).
+  UserfScratch sc;
+  if (userf) {
+    void *ws2 = nullptr;
+    PSFUN_TRY(psfun_workspace2(c, userf_scratch_bytes(nl), &ws2));
+    sc = userf_scratch_carve(ws2, nl);
+  }
+  const bool fsync = userf && istop == 1;
+  const int istop_k = userf ? 2 : istop;
+  const int nl_k = fsync ? (1 << 30) : nl;
   const int ldh = nl + 1;
   const int64_t ldv = (int64_t)align_up((size_t)n, 64);
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
@@ -292,12 +310,12 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     if (sell) {
       hipLaunchKernelGGL((k_ar_spmv<1, true>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->sell_ptr,
                          A->sell_col, A->sell_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H,
-                         ldh, model, nl, istop, eps, fscale, c->d_flags);
+                         ldh, model, nl_k, istop_k, eps, fscale, c->d_flags);
     } else {
 #define AR_LAUNCH(LL)                                                                                               \
   hipLaunchKernelGGL((k_ar_spmv<LL, false>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, \
-                     A->val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H, ldh, model, nl,     \
-                     istop, eps, fscale, c->d_flags)
+                     A->val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H, ldh, model, nl_k,   \
+                     istop_k, eps, fscale, c->d_flags)
       switch (L) {
         case 1: AR_LAUNCH(1); break;
         case 2: AR_LAUNCH(2); break;
@@ -313,6 +331,16 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     PSFUN_TRY(psfun_prof_end(c));
     cur = 1 - cur;
     gcur = Gs;
+    if (fsync && j >= 2) {
+      // control index j: yk = f(H(1:j-1,1:j-1)) e1, rs(j) from h(j, j-1)
+      bool stopped = false;
+      PSFUN_TRY(userf_eval(c, j - 1, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc, &stopped));
+      if (stopped) {  // stopped at index j-1 (seen after the wait)
+        ran_all = false;
+        break;
+      }
+      PSFUN_TRY(userf_launch_est(c, ctl, j, H + (size_t)(j - 2) * ldh + (j - 1), sc, nl, eps));
+    }
     for (int k = 1; k <= j; ++k) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_MGS, (k < j ? 32.0 : 24.0) * n, j));
       hipLaunchKernelGGL(k_ar_mgs, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, k, j, V, ldv, u, P[cur], gcur,
@@ -322,7 +350,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
       cur = 1 - cur;
       gcur = Gu;
     }
-    if (j >= 2 && j < nl - 1) {
+    if (!fsync && j >= 2 && j < nl - 1) {
       PSFUN_TRY(psfun_host_wait(c, j - 1));
       if (c->h_flags[1] != 0) {
         ran_all = false;
@@ -331,12 +359,29 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     }
   }
   if (ran_all) {
-    hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model, nl,
-                       istop, eps, fscale, c->d_flags);
+    hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model,
+                       nl_k, istop_k, eps, fscale, c->d_flags);
+    PSFUN_HIP_TRY(hipGetLastError());
+    if (fsync) {
+      bool stopped = false;
+      PSFUN_TRY(userf_eval(c, nl - 1, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc, &stopped));
+      if (!stopped) PSFUN_TRY(userf_launch_est(c, ctl, nl, H + (size_t)(nl - 2) * ldh + (nl - 1), sc, nl, eps));
+    }
+  }
+  if (userf) {
+    // yk = f(H(1:itx-1,1:itx-1)) e1 unless the loop converged (or x = 0)
+    LzCtl *hs = (LzCtl *)((char *)c->h_pinned + 256);
+    PSFUN_HIP_TRY(hipMemcpyAsync(hs, ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hs->done != LZ_DONE_CONVERGED && hs->done != LZ_DONE_ZERO) {
+      const int k = hs->itx - 1 < 1 ? 1 : hs->itx - 1;
+      PSFUN_TRY(userf_eval(c, k, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc));
+      PSFUN_TRY(userf_launch_final(c, ctl, k, sc));
+    }
+  } else {
+    hipLaunchKernelGGL(k_ar_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, fscale);
     PSFUN_HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_ar_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, fscale);
-  PSFUN_HIP_TRY(hipGetLastError());
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
   hipLaunchKernelGGL(k_ar_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
   PSFUN_HIP_TRY(hipGetLastError());
@@ -465,6 +510,20 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sai_final(LzCtl *ctl, const double
   if (threadIdx.x == 0) ctl->m = k;
 }
 
+// USERF projection matrix Hi = H_k^{-1} - tau I (k x k, leading dimension k)
+// for the symmetric eigen-step; a singular H_k stops the loop with a
+// breakdown.  Inside the loop (final == 0) a stopped loop skips it.
+__global__ __launch_bounds__(PSFUN_BS) void k_sai_inv(LzCtl *ctl, int k, const double *H, int ldh, double *M,
+                                                      double *Hi, double tau, int final) {
+  __shared__ DenseLds W;
+  if (!final && ctl->done) return;
+  if (!block_gj_inverse(k, H, ldh, M, Hi, &W)) {
+    if (threadIdx.x == 0) ctl->done = LZ_DONE_BREAKDOWN;
+    return;
+  }
+  for (int d = threadIdx.x; d < k; d += PSFUN_BS) Hi[d * k + d] -= tau;
+}
+
 extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int variant, double fscale,
                                        double tau, int prec, int inner_itmax, int inner_istop, const double *x,
                                        double *y, double eps, int itmax, int istop, int *iter, double *err,
@@ -472,8 +531,13 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
   if (!c || !A) return PSFUN_ERR_INVALID_ARG;
   if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
   if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
-  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
-  if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  const bool userf = fname == PSFUN_FN_USERF;
+  if (userf) {
+    PSFUN_TRY(userf_check(c, variant));
+  } else {
+    if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+    if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  }
   if (itmax < 2 || itmax > LZ_NL_MAX || itmax > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
   if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG) return PSFUN_ERR_INVALID_ARG;
   if (inner_istop < 1 || inner_istop > 2) return PSFUN_ERR_INVALID_ISTOP;
@@ -486,16 +550,22 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
     return PSFUN_SUCCESS;
   }
   PSFUN_HIP_TRY(hipSetDevice(c->device));
-  const int model = dense_model(variant);
+  const int model = userf ? DENSE_SYM_UPPER : dense_model(variant);
+  const bool fsync = userf && istop == 1;
+  const int istop_k = userf ? 2 : istop;
+  const int nl_k = fsync ? (1 << 30) : nl;
   const int ldh = nl + 1;
   const int64_t ldv = (int64_t)align_up((size_t)n, 64);
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
   const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
   const size_t m_bytes = align_up(sizeof(double) * (size_t)nl * nl, 256);
-  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)(nl + 1);  // v_1..v_nl, w
+  const size_t vbytes = align_up(sizeof(double) * (size_t)ldv * (size_t)(nl + 1), 256);  // v_1..v_nl, w
+  const size_t u_bytes = userf ? userf_scratch_bytes(nl) : 0;
   // the basis lives in the second workspace: the inner CG takes the first
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes, &ws));
+  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes, &ws));
+  UserfScratch sc;
+  if (userf) sc = userf_scratch_carve((char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes, nl);
   LzCtl *ctl = (LzCtl *)ws;
   double *H = (double *)((char *)ws + ctl_bytes);
   double *M = (double *)((char *)ws + ctl_bytes + h_bytes);
@@ -537,15 +607,41 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
       hipLaunchKernelGGL(k_sai_normalize, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, w, V + (size_t)ldv * j,
                          P[cur], Gu, ctl, 0);
       hipLaunchKernelGGL(k_sai_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], Gu, i, ctl, H, ldh, model, M,
-                         Hi, tau, nl, istop, eps, fscale);
+                         Hi, tau, nl_k, istop_k, eps, fscale);
       PSFUN_HIP_TRY(hipGetLastError());
+      if (fsync) {
+        // yk = f(H_{i-1}^{-1} - tau I) e1 with f on the host, rs(i) from h(i, i-1)
+        hipLaunchKernelGGL(k_sai_inv, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, i - 1, H, ldh, M, Hi, tau, 0);
+        PSFUN_HIP_TRY(hipGetLastError());
+        PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (h->done != 0) break;
+        PSFUN_TRY(userf_eval(c, i - 1, SYM_FROM_UPPER, Hi, nullptr, i - 1, fscale, sc));
+        PSFUN_TRY(userf_launch_est(c, ctl, i, H + (size_t)(i - 2) * ldh + (i - 1), sc, nl, eps));
+      }
       PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
       if (h->done != 0) break;
     }
   }
-  hipLaunchKernelGGL(k_sai_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, M, Hi, tau, fscale);
-  PSFUN_HIP_TRY(hipGetLastError());
+  if (userf) {
+    PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (h->done != LZ_DONE_CONVERGED && h->done != LZ_DONE_ZERO && h->done != LZ_DONE_BREAKDOWN) {
+      const int k = h->itx - 1 < 1 ? 1 : h->itx - 1;
+      hipLaunchKernelGGL(k_sai_inv, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, k, H, ldh, M, Hi, tau, 1);
+      PSFUN_HIP_TRY(hipGetLastError());
+      PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+      if (h->done != LZ_DONE_BREAKDOWN) {
+        PSFUN_TRY(userf_eval(c, k, SYM_FROM_UPPER, Hi, nullptr, k, fscale, sc));
+        PSFUN_TRY(userf_launch_final(c, ctl, k, sc));
+      }
+    }
+  } else {
+    hipLaunchKernelGGL(k_sai_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, M, Hi, tau, fscale);
+    PSFUN_HIP_TRY(hipGetLastError());
+  }
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
   hipLaunchKernelGGL(k_ar_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
   PSFUN_HIP_TRY(hipGetLastError());

@@ -11,6 +11,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <math.h>
+#include <float.h>
 #include <mutex>
 #include <vector>
 
@@ -60,6 +61,9 @@ struct psfun_ctx {
   size_t ws_bytes = 0;
   void *ws2 = nullptr;
   size_t ws2_bytes = 0;
+  // user function of the USERF f(H) step (psfun_set_userf), called on the host
+  psfun_userf_t userf = nullptr;
+  void *userf_data = nullptr;
   // optional per-launch timing of the hot kernels (psfun_prof_*)
   int prof_on = 0;                       // 0 off; R >= 1: timing on, idempotent steps run R times
   std::vector<hipEvent_t> prof_ev;       // event pool, 2 per timed launch

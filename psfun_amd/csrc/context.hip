@@ -87,6 +87,13 @@ extern "C" int psfun_ctx_set_stream(psfun_ctx_t c, void *s) {
   return PSFUN_SUCCESS;
 }
 
+extern "C" int psfun_set_userf(psfun_ctx_t c, psfun_userf_t f, void *user) {
+  if (!c) return PSFUN_ERR_INVALID_ARG;
+  c->userf = f;
+  c->userf_data = user;
+  return PSFUN_SUCCESS;
+}
+
 extern "C" int psfun_ctx_use_own_stream(psfun_ctx_t c) {
   if (!c) return PSFUN_ERR_INVALID_ARG;
   c->stream = c->own_stream;

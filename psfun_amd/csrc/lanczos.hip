@@ -29,6 +29,7 @@
 #include "kernels.hpp"
 #include "fexp.hpp"
 #include "lanczos.hpp"
+#include "userf.hpp"
 
 // Reference loop control at index i = j (j >= 2): beta = T(j-1, j).
 // Stop the loop: device state for the kernels, host flag for the launcher.
@@ -317,7 +318,7 @@ __global__ void k_lz_init(LzCtl *ctl) {
 // ---------------------------------------------------------------------------
 static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, const double *src, int j, int G,
                           const double *p_in, int g_in, double *v_out, double *u_out, double *p_out, LzCtl *ctl,
-                          int nl, int istop, double eps, double fscale) {
+                          int nl, int nl_ctl, int istop, double eps, double fscale) {
   const int L = A->lanes;
   // algorithmic bytes: val+colind, rowptr, src rows, v and u written
   const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
@@ -330,11 +331,11 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
     for (int r = 0; r < reps; ++r) {
       if (nl <= 65)
         hipLaunchKernelGGL(k_lz_spmv_sell<true>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
-                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
+                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
                            istop, eps, fscale, c->d_flags);
       else
         hipLaunchKernelGGL(k_lz_spmv_sell<false>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
-                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,
+                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
                            istop, eps, fscale, c->d_flags);
     }
     PSFUN_HIP_TRY(hipGetLastError());
@@ -345,11 +346,11 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   do {                                                                                                         \
     if (nl <= 65)                                                                                              \
       hipLaunchKernelGGL((k_lz_spmv<LL, true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr,   \
-                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,     \
+                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, \
                          istop, eps, fscale, c->d_flags);                                                      \
     else                                                                                                       \
       hipLaunchKernelGGL((k_lz_spmv<LL, false>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->rowptr,  \
-                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl,     \
+                         A->colind, A->val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, \
                          istop, eps, fscale, c->d_flags);                                                      \
   } while (0)
   for (int r = 0; r < reps; ++r) switch (L) {
@@ -373,7 +374,11 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   if (!c || !A) return PSFUN_ERR_INVALID_ARG;
   if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
   if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
-  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+  const bool userf = fname == PSFUN_FN_USERF;
+  if (userf)
+    PSFUN_TRY(userf_check(c, PSFUN_VAR_SYM));
+  else if (fname != PSFUN_FN_EXP)
+    return PSFUN_ERR_UNSUPPORTED;
   if (itmax < 2 || itmax > LZ_NL_MAX) return PSFUN_ERR_INVALID_ARG;
   const int n = A->n;
   const int nl = itmax;
@@ -392,6 +397,19 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   LzCtl *ctl = (LzCtl *)ws;
   double *V = (double *)((char *)ws + ctl_bytes);
   double *U[2] = {V + (size_t)ldv * nl, V + (size_t)ldv * (nl + 1)};
+  // USERF: the kernels' control blocks keep the loop bookkeeping and the
+  // breakdown test; with istop = 1 the estimate (and with it the iteration
+  // limit) moves to a host-synchronous step after each control index, since
+  // the user function runs on the host.
+  UserfScratch sc;
+  if (userf) {
+    void *ws2 = nullptr;
+    PSFUN_TRY(psfun_workspace2(c, userf_scratch_bytes(nl), &ws2));
+    sc = userf_scratch_carve(ws2, nl);
+  }
+  const bool fsync = userf && istop == 1;
+  const int istop_k = userf ? 2 : istop;
+  const int nl_k = fsync ? (1 << 30) : nl;
   double *Pw = c->d_partials;                      // ||w||^2 (and ||x||^2) partials
   double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
 
@@ -411,7 +429,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; partials of T(1,1) = u.v_1
   hipLaunchKernelGGL(k_sumsq_partials, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, Pw);
   PSFUN_HIP_TRY(hipGetLastError());
-  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, U[0], Pa, ctl, nl, istop, eps, fscale));
+  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, U[0], Pa, ctl, nl, nl_k, istop_k, eps, fscale));
 
   // Pacing: before queueing step j+2, wait until step j-1 has reported
   // (h_flags[0] >= j-1).  The stores from a running kernel may only reach the
@@ -428,19 +446,39 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     PSFUN_TRY(psfun_prof_end(c));
     if (j + 1 <= nl - 1) {
       PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V + (size_t)ldv * j, U[j & 1], Pa, ctl, nl,
-                               istop, eps, fscale));
+                               nl_k, istop_k, eps, fscale));
     } else {
-      hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl, istop, eps, fscale,
+      hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl_k, istop_k, eps, fscale,
                          c->d_flags);
       PSFUN_HIP_TRY(hipGetLastError());
+    }
+    if (fsync) {
+      // control index i = j+1: yk = f(T(1:j,1:j)) e1, rs(j+1) from beta = T(j, j+1)
+      bool stopped = false;
+      PSFUN_TRY(userf_eval(c, j, SYM_FROM_TRIDIAG, ctl->alpha + 1, ctl->beta + 1, 0, fscale, sc, &stopped));
+      if (stopped) break;  // converged or at the limit at index j, or a breakdown at j+1
+      PSFUN_TRY(userf_launch_est(c, ctl, j + 1, ctl->beta + j, sc, nl, eps));
+      continue;
     }
     if (j < nl - 1 && j >= 2) {
       PSFUN_TRY(psfun_host_wait(c, j - 1));
       if (c->h_flags[1] != 0) break;
     }
   }
-  hipLaunchKernelGGL(k_lz_final, dim3(1), dim3(64), 0, c->stream, ctl, fscale);
-  PSFUN_HIP_TRY(hipGetLastError());
+  if (userf) {
+    // yk = f(T(1:itx-1,1:itx-1)) e1 unless the loop converged (or x = 0)
+    LzCtl *hs = (LzCtl *)((char *)c->h_pinned + 256);
+    PSFUN_HIP_TRY(hipMemcpyAsync(hs, ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hs->done != LZ_DONE_CONVERGED && hs->done != LZ_DONE_ZERO) {
+      const int k = hs->itx - 1 < 1 ? 1 : hs->itx - 1;
+      PSFUN_TRY(userf_eval(c, k, SYM_FROM_TRIDIAG, ctl->alpha + 1, ctl->beta + 1, 0, fscale, sc));
+      PSFUN_TRY(userf_launch_final(c, ctl, k, sc));
+    }
+  } else {
+    hipLaunchKernelGGL(k_lz_final, dim3(1), dim3(64), 0, c->stream, ctl, fscale);
+    PSFUN_HIP_TRY(hipGetLastError());
+  }
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_ASSEMBLE, 0.0));
   hipLaunchKernelGGL(k_lz_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);

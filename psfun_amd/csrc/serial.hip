@@ -4,6 +4,7 @@
 #include "fdense.hpp"
 #include "fexp.hpp"
 #include "kernels.hpp"
+#include "userf.hpp"
 
 // y = exp(s*T) x for symmetric tridiagonal T (d: k, e: k-1), one wavefront.
 __global__ __launch_bounds__(64) void k_serial_tridiag(int k, const double *d, const double *e, double s,
@@ -36,11 +37,32 @@ int launch_serial_tridiag(psfun_ctx *c, int k, const double *d, const double *e,
 extern "C" int psfun_serial_apply_host(psfun_ctx_t c, int fname, int variant, double scaling, int k,
                                        const double *H, const double *x, double *y) {
   if (!c || k < 0 || (k && (!H || !x || !y))) return PSFUN_ERR_INVALID_ARG;
-  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
-  if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  if (fname == PSFUN_FN_USERF) {
+    PSFUN_TRY(userf_check(c, variant));
+  } else {
+    if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;
+    if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
+  }
   if (k == 0) return PSFUN_SUCCESS;
   if (k > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
   PSFUN_HIP_TRY(hipSetDevice(c->device));
+  if (fname == PSFUN_FN_USERF) {
+    // DSYEV('V','U') on H, y = Q diag(f(scaling*lambda)) Q^T x  (serial_mod:420-485)
+    const size_t hk = align_up(sizeof(double) * (size_t)k * k, 256);
+    const size_t kv = align_up(sizeof(double) * (size_t)k, 256);
+    void *ws = nullptr;
+    PSFUN_TRY(psfun_workspace2(c, hk + 2 * kv + userf_scratch_bytes(k), &ws));
+    double *dH = (double *)ws;
+    double *dx = (double *)((char *)ws + hk), *dy = (double *)((char *)ws + hk + kv);
+    const UserfScratch sc = userf_scratch_carve((char *)ws + hk + 2 * kv, k);
+    PSFUN_HIP_TRY(hipMemcpyAsync(dH, H, sizeof(double) * k * k, hipMemcpyHostToDevice, c->stream));
+    PSFUN_HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+    PSFUN_TRY(userf_eval(c, k, SYM_FROM_UPPER, dH, nullptr, k, scaling, sc));
+    PSFUN_TRY(userf_launch_apply(c, k, sc, dx, dy));
+    PSFUN_HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PSFUN_SUCCESS;
+  }
   // CHBSYM reads H as symmetric from its upper triangle (DSCHBV factors with
   // ZSYSV 'U'); when that matrix is tridiagonal the Chebyshev LDS routine is
   // used, otherwise the dense block routine, as for every other variant.

@@ -78,7 +78,7 @@ def lanczos_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None
     n = desc_a.get_global_rows() if desc_a is not None else a.n
     if x.get_nrows() != n or y.get_nrows() != n or a.n != n:
         return psb.psb_err_from_subroutine_, 0, 0.0, None
-    st, _var = fun.device_variant()
+    st, fn, _var = fun.device_fn()
     if st != _lib.PSFUN_SUCCESS:
         return psb.psb_err_from_subroutine_non_, 0, 0.0, None
     ctx = a.ctxt
@@ -87,10 +87,11 @@ def lanczos_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None
     err = C.c_double(0.0)
     nres = C.c_int(0)
     res = np.zeros(max(nl, 2) + 2)
-    st = _lib.lib().psfun_lanczos_apply(ctx.handle, a.handle, _lib.PSFUN_FN_EXP, 1.0, float(opscale),
-                                        C.c_void_p(x.v.data_ptr()), C.c_void_p(y.v.data_ptr()), float(eps), nl,
-                                        istop_, C.byref(it), C.byref(err), res.ctypes.data_as(C.POINTER(C.c_double)),
-                                        C.byref(nres))
+    with fun.bound(ctx):
+        st = _lib.lib().psfun_lanczos_apply(ctx.handle, a.handle, fn, 1.0, float(opscale),
+                                            C.c_void_p(x.v.data_ptr()), C.c_void_p(y.v.data_ptr()), float(eps), nl,
+                                            istop_, C.byref(it), C.byref(err),
+                                            res.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nres))
     if st != _lib.PSFUN_SUCCESS:
         return psb.status_to_info(st), 0, 0.0, None
     return psb.psb_success_, it.value, err.value, res[: nres.value].copy()

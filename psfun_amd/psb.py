@@ -10,6 +10,7 @@ torch tensors; every primitive runs a HIP kernel of libpsfun_hip.so.
 Error behaviour follows PSBLAS: routines return an integer `info`
 (psb_success_ == 0) instead of raising.
 """
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -58,6 +59,31 @@ class Context:
     def bind_stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
         _lib.lib().psfun_ctx_set_stream(self.handle, C.c_void_p(s))
+
+    @contextlib.contextmanager
+    def userf(self, f):
+        """Make f the USERF function (psfun_set_userf) for the calls inside
+        the block.  The library calls it on this thread; an exception raised
+        by f makes the remaining calls return NaN and is re-raised here."""
+        errors = []
+
+        def cb(x, _user):
+            if errors:
+                return float("nan")
+            try:
+                return float(f(x))
+            except BaseException as e:  # noqa: BLE001  (re-raised after the call)
+                errors.append(e)
+                return float("nan")
+
+        cfun = _lib.USERF(cb)
+        _lib.check(_lib.lib().psfun_set_userf(self.handle, C.cast(cfun, C.c_void_p), None), "psfun_set_userf")
+        try:
+            yield
+        finally:
+            _lib.lib().psfun_set_userf(self.handle, None, None)
+        if errors:
+            raise errors[0]
 
     def num_cus(self):
         v = C.c_int(0)
@@ -130,6 +156,15 @@ class psb_dspmat_type:
                                           int(index_base), _lib.PSFUN_MEM_HOST, C.byref(h))
         _lib.check(st, "psfun_csr_create")
         self.handle = h
+        self._src = (rp, ci, va, int(index_base))  # the host arrays as given, for host_csr()
+
+    def host_csr(self):
+        """0-based host (rowptr, colind, val) of this matrix, with the values
+        last given (at creation or by set_values)."""
+        rp, ci, va, base = self._src
+        if isinstance(va, torch.Tensor):
+            va = va.detach().cpu().numpy()
+        return rp - base, ci - base, np.asarray(va, dtype=np.float64)
 
     @classmethod
     def from_csr(cls, csr, ctxt=None):
@@ -161,6 +196,7 @@ class psb_dspmat_type:
         self.ctxt.bind_stream()
         _lib.check(_lib.lib().psfun_csr_set_values(self.handle, _dptr(v)))
         torch.cuda.current_stream().synchronize()
+        self._src = self._src[:2] + (v, self._src[3])
 
     _FORMATS = {"AUTO": _lib.PSFUN_FORMAT_AUTO, "CSR": _lib.PSFUN_FORMAT_CSR, "SELL": _lib.PSFUN_FORMAT_SELL}
 

@@ -1,11 +1,22 @@
 """psfun_d_serial: the function object of the f(H) step
 (serial/psfun_d_serial_mod.F90:43-170).
 
-Holds fname ('EXP'), variant, scaling, padedegree, phiorder and an optional
-function pointer; option names are matched case-insensitively, values
+Holds fname ('EXP' or 'USERF'), variant, scaling, padedegree, phiorder and an
+optional function pointer; option names are matched case-insensitively, values
 case-sensitively (serial_mod:97-101, 214-216).  `apply` evaluates
-y = f(scaling*H) x for a small dense H on the GPU.
+y = f(scaling*H) x for a small H (dense array, or a sparse matrix densified
+first as psfun_d_serial_apply_sparse does) on the GPU:
+
+  EXP    TAYLOR, SASQ, GENPADE, CHBHES, CHBGEN, CHBSYM (serial_mod:216-390)
+  USERF  SYM: eigen-decomposition of the symmetric matrix of H's upper
+         triangle on the device, f_ptr(scaling*lambda) on the host, as
+         DSYEV('V','U') + f_ptr + DGEMV there (serial_mod:420-485)
+
+'PHI' needs the external phi-function library, which the reference only
+uses when built WITHPHILIBRARY; without it the reference reports an error,
+and so does this apply.
 """
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -15,6 +26,8 @@ from . import psb
 
 # Variants of fname 'EXP' that the reference accepts (serial_mod:216-390).
 EXP_VARIANTS = ("TAYLOR", "SASQ", "GENPADE", "CHBHES", "CHBGEN", "CHBSYM")
+# Variants of fname 'USERF' (serial_mod:420-498).
+USERF_VARIANTS = ("SYM",)
 
 
 class psfun_d_serial:
@@ -55,21 +68,45 @@ class psfun_d_serial:
             return psb.psb_err_invalid_args_combination_
         return psb.psb_success_
 
-    def device_variant(self):
-        """(status, variant id) of the device f(H) step for this object."""
-        if self.fname != "EXP":
-            return _lib.PSFUN_ERR_UNSUPPORTED, None
-        if self.variant not in EXP_VARIANTS:
-            return _lib.PSFUN_ERR_INVALID_ARG, None
-        return _lib.PSFUN_SUCCESS, _lib.VARIANTS[self.variant]
+    def device_fn(self):
+        """(status, fname id, variant id) of the device f(H) step for this
+        object (the select cases of serial_mod:214-505)."""
+        if self.fname == "EXP":
+            if self.variant not in EXP_VARIANTS:
+                return _lib.PSFUN_ERR_INVALID_ARG, None, None
+            return _lib.PSFUN_SUCCESS, _lib.PSFUN_FN_EXP, _lib.VARIANTS[self.variant]
+        if self.fname == "USERF":
+            if self.variant not in USERF_VARIANTS or self.f_ptr is None:
+                return _lib.PSFUN_ERR_INVALID_ARG, None, None
+            return _lib.PSFUN_SUCCESS, _lib.PSFUN_FN_USERF, _lib.PSFUN_VAR_SYM
+        return _lib.PSFUN_ERR_UNSUPPORTED, None, None
+
+    def bound(self, ctx):
+        """Context manager that makes this object's f_ptr the context's USERF
+        function for the calls inside it (nothing to do for EXP)."""
+        if self.fname == "USERF" and self.f_ptr is not None:
+            return ctx.userf(self.f_ptr)
+        return contextlib.nullcontext()
 
     def apply(self, a, y, x, ctxt=None):
-        """y[:] = f(scaling*a) x for a dense k x k array `a` (host numpy).
-        Returns info."""
+        """y[:] = f(scaling*a) x for a k x k matrix `a`: a dense host array
+        (psfun_d_serial_apply_array) or a psb_dspmat_type / (rowptr, colind,
+        val) CSR triple, densified first (psfun_d_serial_apply_sparse,
+        serial_mod:526-573).  Returns info."""
+        if isinstance(a, psb.psb_dspmat_type):
+            a = a.host_csr()
+        if isinstance(a, tuple):
+            rp, ci, va = (np.asarray(t) for t in a)
+            nrow = len(rp) - 1
+            dense = np.zeros((nrow, nrow))
+            for i in range(nrow):
+                for q in range(int(rp[i]), int(rp[i + 1])):
+                    dense[i, int(ci[q])] += va[q]   # repeated entries summed, as the SpMV sees them
+            a = dense
         a = np.asarray(a, dtype=np.float64)
         if a.ndim != 2 or a.shape[0] != a.shape[1]:
             return psb.psb_err_from_subroutine_
-        st, var = self.device_variant()
+        st, fn, var = self.device_fn()
         if st != _lib.PSFUN_SUCCESS:
             return psb.psb_err_from_subroutine_
         k = a.shape[0]
@@ -77,9 +114,10 @@ class psfun_d_serial:
         xx = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros(k)
         ctx = ctxt if ctxt is not None else psb.default_context()
-        st = _lib.lib().psfun_serial_apply_host(ctx.handle, _lib.PSFUN_FN_EXP, var, float(self.scaling), k,
-                                                H.ctypes.data_as(C.c_void_p), xx.ctypes.data_as(C.c_void_p),
-                                                out.ctypes.data_as(C.c_void_p))
+        with self.bound(ctx):
+            st = _lib.lib().psfun_serial_apply_host(ctx.handle, fn, var, float(self.scaling), k,
+                                                    H.ctypes.data_as(C.c_void_p), xx.ctypes.data_as(C.c_void_p),
+                                                    out.ctypes.data_as(C.c_void_p))
         if st != _lib.PSFUN_SUCCESS:
             return psb.psb_err_from_subroutine_
         y[:] = out

@@ -31,7 +31,7 @@ def build(verbose=True):
     mod_obj = os.path.join(OUT, "psfun_hip_mod.o")
     steps = [
         [fc, "-O2", "-module-dir", OUT, "-c", mod_src, "-o", mod_obj],
-        [fc, "-O2", "-I", OUT, os.path.join(HERE, "psfun_driver.f90"), mod_obj, "-o", EXE,
+        [fc, "-O2", "-I", OUT, "-module-dir", OUT, os.path.join(HERE, "psfun_driver.f90"), mod_obj, "-o", EXE,
          "-L", os.path.join(REPO, "psfun_amd"), "-lpsfun_hip", "-Wl,-rpath,$ORIGIN/../../../psfun_amd"],
     ]
     for cmd in steps:

@@ -5,11 +5,26 @@
 !   LANCZOS  (EXP, CHBSYM, scaling 1, eps 1e-6, itmax 100, istop 1)
 !   ARNOLDI  (EXP, GENPADE, scaling 1, eps 1e-6, itmax 100, istop 1)
 ! and one shifted solve as in examples/shiftedkrylov/shifted.inp:
-!   CG       ((-A + I) x = 1, DIAG, eps 1e-7, istop 2).
+!   CG       ((-A + I) x = 1, DIAG, eps 1e-7, istop 2),
+! and the LANCZOS deck again with fname USERF/SYM and a Fortran user function
+! f(x) = exp(x) handed over with c_funloc (USERF).
 ! Prints "<METHOD> <iter> <err>" followed by the n entries of the result.
+module driver_userf
+  use iso_c_binding
+  implicit none
+contains
+  function userf_exp(x, user) bind(C) result(fx)
+    real(c_double), value :: x
+    type(c_ptr), value :: user
+    real(c_double) :: fx
+    fx = exp(x)
+  end function
+end module driver_userf
+
 program psfun_driver
   use iso_c_binding
   use psfun_hip_mod
+  use driver_userf
   implicit none
   integer, parameter :: n = 50
   integer(c_int32_t) :: rowptr(n + 1), colind(3 * n - 2)
@@ -66,6 +81,13 @@ program psfun_driver
   call check(st, 'psfun_cg_shifted')
   call check(psfun_vec_download(ctx, y, dy, int(n, c_int64_t)), 'psfun_vec_download')
   call emit('CG', iter, err, y)
+
+  call check(psfun_set_userf(ctx, c_funloc(userf_exp), c_null_ptr), 'psfun_set_userf')
+  st = psfun_lanczos_apply(ctx, a, PSFUN_FN_USERF, 1.0d0, 1.0d0, dx, dy, 1.0d-6, 100_c_int, 1_c_int, iter, err, &
+       & res, nres)
+  call check(st, 'psfun_lanczos_apply (USERF)')
+  call check(psfun_vec_download(ctx, y, dy, int(n, c_int64_t)), 'psfun_vec_download')
+  call emit('USERF', iter, err, y)
 
   call check(psfun_vec_free(ctx, dx), 'psfun_vec_free')
   call check(psfun_vec_free(ctx, dy), 'psfun_vec_free')

@@ -1,7 +1,8 @@
 """The iso_c_binding boundary from Fortran: tests/fortran/psfun_driver (built
 by tests/fortran/build.py against include/psfun_hip_mod.f90) runs the
 reference example decks lanczos.inp / arnoldi.inp / shifted.inp on the 50x50
-example matrix; its results are checked against the oracle and the fixture."""
+example matrix, and the Lanczos deck with a Fortran USERF function; its
+results are checked against the oracle and the fixture."""
 import os
 import subprocess
 
@@ -44,6 +45,10 @@ def test_fortran_driver_matches_oracle(gpu_ctx):
     ref = O.arnoldi(A, ones, 1e-6, 100, 1)
     y = np.array(got["ARNOLDI"]["y"])
     assert got["ARNOLDI"]["iter"] == ref["iter"]
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    ref = O.lanczos(A, ones, 1e-6, 100, 1, f=np.exp)      # USERF/SYM with the Fortran f(x) = exp(x)
+    y = np.array(got["USERF"]["y"])
+    assert got["USERF"]["iter"] == ref["iter"] == got["LANCZOS"]["iter"]
     assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
     x = np.array(got["CG"]["y"])
     assert np.linalg.norm(x - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])

@@ -318,3 +318,77 @@ def test_sai_arnoldi_closed_form(tau):
     ra = O.arnoldi(A, x, 1e-10, 150, 1)
     assert r["iter"] < 0.5 * ra["iter"]
     assert abs(r["res"][0] - np.linalg.norm(x)) <= 1e-12 * np.linalg.norm(x)
+
+
+# ---- USERF/SYM f(H) step (serial/psfun_d_serial_mod.F90:420-485) ----------
+@pytest.mark.parametrize("k", [1, 2, 5, 33])
+def test_sym_eig_matches_lapack(k):
+    """Eigenvalues of the symmetric matrix of H's upper triangle against
+    numpy's LAPACK eigh; Z orthonormal; Z diag(w) Z^T reproduces it."""
+    rng = np.random.default_rng(100 + k)
+    H = rng.normal(size=(k, k))
+    S = np.triu(H) + np.triu(H, 1).T
+    w, Z = O.sym_eig(H)
+    we = np.linalg.eigh(S)[0]
+    scale = max(1.0, np.abs(we).max())
+    assert np.abs(np.sort(w) - we).max() <= 1e-14 * k * scale
+    assert np.linalg.norm(Z.T @ Z - np.eye(k)) <= 1e-13 * k
+    assert np.linalg.norm(Z @ np.diag(w) @ Z.T - S) <= 1e-14 * k * np.linalg.norm(S)
+
+
+def test_userf_sym_apply_closed_forms():
+    """f = exp against scipy expm, f = sqrt against scipy sqrtm (SPD), f = 1/z
+    against a dense solve, each of the upper-triangle matrix with a scaling."""
+    rng = np.random.default_rng(7)
+    k = 24
+    H = rng.normal(size=(k, k))
+    S = np.triu(H) + np.triu(H, 1).T
+    x = rng.normal(size=k)
+    y = O.userf_sym_apply(H, -0.4, np.exp, x)
+    assert np.linalg.norm(y - sla.expm(-0.4 * S) @ x) <= 1e-13 * np.linalg.norm(y)
+    B = rng.normal(size=(k, k))
+    P = B @ B.T + k * np.eye(k)
+    Pu = np.triu(P) + 3.0 * np.tril(P, -1)     # lower triangle must be ignored
+    y = O.userf_sym_apply(Pu, 2.0, np.sqrt, x)
+    assert np.linalg.norm(y - np.real(sla.sqrtm(2.0 * P)) @ x) <= 1e-12 * np.linalg.norm(y)
+    y = O.userf_sym_apply(P, 1.0, lambda z: 1.0 / z, x)
+    assert np.linalg.norm(y - np.linalg.solve(P, x)) <= 1e-12 * np.linalg.norm(y)
+
+
+def test_lanczos_userf_exp_equals_exp():
+    """USERF with f = exp reproduces the EXP Lanczos: same iterations, same
+    estimates, same result to rounding."""
+    A = matgen.laplacian((12, 12), scale=-1.0)
+    x = matgen.uniform_pm1(3, 144)
+    a = O.lanczos(A, x, 1e-10, 60, 1)
+    b = O.lanczos(A, x, 1e-10, 60, 1, f=np.exp)
+    assert a["iter"] == b["iter"]
+    assert np.allclose(a["res"], b["res"], rtol=1e-8, atol=1e-13)
+    assert np.linalg.norm(a["y"] - b["y"]) <= 1e-12 * np.linalg.norm(a["y"])
+
+
+def test_lanczos_userf_sqrt_against_sqrtm():
+    """f = sqrt of the SPD Laplacian (opscale +1) against scipy sqrtm."""
+    A = matgen.laplacian((9, 9), scale=1.0)
+    x = matgen.uniform_pm1(4, 81)
+    r = O.lanczos(A, x, 1e-12, 81, 1, f=np.sqrt)
+    Ad = matgen.to_scipy(A).toarray()
+    ref = np.real(sla.sqrtm(Ad)) @ x
+    assert np.linalg.norm(r["y"] - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+def test_arnoldi_and_sai_userf_exp_equal_exp():
+    """On a symmetric A, H_k is symmetric up to rounding, so USERF/SYM with
+    f = exp reproduces the EXP Arnoldi and SAI projections."""
+    A = matgen.laplacian((10, 10), scale=-1.0)
+    x = matgen.uniform_pm1(5, 100)
+    a = O.arnoldi(A, x, 1e-10, 60, 1)
+    b = O.arnoldi(A, x, 1e-10, 60, 1, f=np.exp)
+    assert a["iter"] == b["iter"]
+    assert np.linalg.norm(a["y"] - b["y"]) <= 1e-11 * np.linalg.norm(a["y"])
+    A = matgen.laplacian((10, 10), scale=-10.0)
+    a = O.sai_arnoldi(A, x, 1e-10, 60, tau=0.0, istop=1, in_itmax=2000)
+    b = O.sai_arnoldi(A, x, 1e-10, 60, tau=0.0, istop=1, in_itmax=2000, f=np.exp)
+    assert abs(a["iter"] - b["iter"]) <= 1
+    assert np.linalg.norm(a["y"] - b["y"]) <= 1e-9 * np.linalg.norm(a["y"])
+    assert np.linalg.norm(b["y"] - matgen.dst_expm_laplacian((10, 10), x, 10.0)) <= 1e-8 * np.linalg.norm(b["y"])
