@@ -176,7 +176,8 @@ def headline(args):
                 "traffic": (pmc_traffic("k_lz_spmv_sell", "lz_spmv_sell_128cube_pmc.txt")
                             if fmt == "SELL" and m == 128 else None),
                 "kernel": ("k_lz_spmv_sell" if fmt == "SELL" else f"k_lz_spmv<{lanes}>")
-                + " (SpMV 12*nnz+4(n+1) algorithmic matrix bytes + w read, v and u written: 24n)",
+                + " (SpMV 12*nnz+4(n+1) algorithmic matrix bytes + w read and u written: 16n;"
+                " the first step also copies x into the basis: 24n)",
                 "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_us, 3),
                 "launches": k["launches"], "launches_per_event_pair": args.prof_reps,
                 "update_kernel": {"name": "k_lz_update (3-term update + ||w||^2 partials, 32n bytes)",

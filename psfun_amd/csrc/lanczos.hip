@@ -7,17 +7,23 @@
 // rs(i) = rs(1) beta |yk(i-1)| / ||yk||, exit if rs(i) < eps.
 // Assembly: y = rs(1) (yk(1) v_1 + sum_{l=2}^{itx-1} yk(l) v_l).
 //
+// Basis storage: slot l of V holds the unnormalised w_l (slot 0: x), and
+// v_{l+1} = w_l * (1/beta_l) is formed wherever it is read, with the same
+// 1/beta the SpMV step used, so every v value is the one a stored v_{l+1}
+// would hold (beta_0 = rs1).
+//
 // Device schedule, per Krylov index j = 1, 2, ...:
 //   k_lz_spmv(j):   every streaming block sums the partials of ||w||^2 left by
 //                   the previous kernel (fixed order: identical in all blocks),
-//                   beta = sqrt(.), writes v_j = (1/beta) w once, computes
-//                   u = opscale*A v_j gathering w and scaling on the fly, and
-//                   leaves per-block partials of alpha_j = u.v_j.  One extra
-//                   control block runs the reference's loop control for
-//                   index i = j (breakdown test and the a-posteriori estimate
-//                   from T_{j-1}) concurrently with the streaming blocks.
+//                   beta = sqrt(.), computes u = opscale*A v_j gathering
+//                   w_{j-1} and scaling by 1/beta on the fly, writes u into
+//                   slot j, and leaves per-block partials of alpha_j = u.v_j.
+//                   One extra control block runs the reference's loop control
+//                   for index i = j (breakdown test and the a-posteriori
+//                   estimate from T_{j-1}) concurrently with the streaming
+//                   blocks.
 //   k_lz_update(j): every block sums the alpha partials, w = u - alpha_j v_j -
-//                   beta_{j-1} v_{j-1} in place, partials of ||w||^2.
+//                   beta_{j-1} v_{j-1} in place in slot j, partials of ||w||^2.
 // Scalars cross kernels only through kernel boundaries, so no grid-wide
 // atomics or fences are needed.  When the control block ends the loop at
 // index j, the SpMV/update it overlapped with feed nothing that is returned
@@ -122,11 +128,11 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
     if (row < n && lane == 0) {
-      const double v = src[row] * inv;
+      const double w = src[row];
       const double u = opscale * sum;
-      v_out[row] = v;
+      if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
       u_out[row] = u;
-      dot = fma(u, v, dot);
+      dot = fma(u, w * inv, dot);
     }
   }
   dot = block_sum<PSFUN_BS>(dot, lds);
@@ -191,11 +197,11 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
     if (row < n) {
       const bool first = (ch == xr.begin);
       const double sum = first ? acc0 : sell_row_dot(row, sp, col, val, src, 1.0);
-      const double v = (first ? src0 : src[row]) * inv;
+      const double w = first ? src0 : src[row];
       const double u = uscale * sum;
-      v_out[row] = v;
+      if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
       u_out[row] = u;
-      dot = fma(u, v, dot);
+      dot = fma(u, w * inv, dot);
     }
   }
   dot = block_sum<PSFUN_BS>(dot, lds);
@@ -214,43 +220,76 @@ __global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in,
   lz_control<false>(ctl, j, beta, nl, istop, eps, fscale, &W, hf);
 }
 
-// The stop flag, the partials and this thread's first (u, v_j, v_{j-1})
-// triple are loaded before anything waits on them, so the fold's latency hides
-// behind the first loads.  No block of the grid writes the flag, so every
-// thread reads the same value.
+// The basis slots hold the unnormalised w_{l} (slot 0: x), and v_{l+1} =
+// w_l * (1/beta_l) is formed where it is read, with the same 1/beta the SpMV
+// step used, so every v value equals the one a stored v would hold.
+// beta_0 = rs1, beta_l = T(l, l+1).
+__device__ __forceinline__ double lz_inv_beta(const LzCtl *ctl, int l) {
+  const double b = (l == 0) ? ctl->rs1 : ctl->beta[l];
+  return b != 0.0 ? 1.0 / b : 0.0;
+}
+
+// The stop flag, the partials and this thread's first LZ_UB (u, w_{j-1},
+// w_{j-2}) triples are loaded before anything waits on them, so the fold's
+// latency hides behind the first loads; later triples are also loaded LZ_UB
+// at a time (a thread keeps 3*LZ_UB loads in flight, which the bandwidth
+// needs).  Each thread still visits its rows in grid-stride order, so the
+// ||w||^2 partials do not depend on LZ_UB.  No block of the grid writes the
+// flag, so every thread reads the same value.
+#define LZ_UB 4
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_update(int n, double *__restrict__ u,
                                                         const double *__restrict__ vj,
                                                         const double *__restrict__ vjm1, int j,
                                                         const double *__restrict__ p_in, int g_in,
                                                         double *__restrict__ p_out, LzCtl *ctl) {
+  // u: slot j (u_j in, w_j out); vj: slot j-1 (w_{j-1}); vjm1: slot j-2 (w_{j-2})
   __shared__ double lds[PSFUN_BS / 64 + 1];
   const int stopped = ctl->done;
   double pacc = 0.0;
 #pragma unroll 4
   for (int b = threadIdx.x; b < g_in; b += PSFUN_BS) pacc += p_in[b];
+  const int64_t stride = (int64_t)gridDim.x * PSFUN_BS;
   const int64_t i0 = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
-  double u0 = 0.0, a0 = 0.0, b0 = 0.0;
-  if (i0 < n) {
-    u0 = u[i0];
-    a0 = vj[i0];
-    if (j > 1) b0 = vjm1[i0];
+  double uu[LZ_UB], aa[LZ_UB], bb[LZ_UB];
+#pragma unroll
+  for (int q = 0; q < LZ_UB; ++q) {
+    const int64_t i = i0 + q * stride;
+    uu[q] = aa[q] = bb[q] = 0.0;
+    if (i < n) {
+      uu[q] = u[i];
+      aa[q] = vj[i];
+      if (j > 1) bb[q] = vjm1[i];
+    }
   }
-  const double be = (j > 1) ? ctl->beta[j - 1] : 0.0;
   if (stopped) return;
+  const double be = (j > 1) ? ctl->beta[j - 1] : 0.0;
+  const double ia = lz_inv_beta(ctl, j - 1);                 // v_j = w_{j-1} * ia
+  const double ib = (j > 1) ? lz_inv_beta(ctl, j - 2) : 0.0;  // v_{j-1} = w_{j-2} * ib
   const double al = block_sum<PSFUN_BS>(pacc, lds);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha[j] = al;
   double acc = 0.0;
-  if (i0 < n) {
-    double w = fma(-al, a0, u0);
-    if (j > 1) w = fma(-be, b0, w);
-    u[i0] = w;
-    acc = fma(w, w, acc);
-  }
-  for (int64_t i = i0 + (int64_t)gridDim.x * PSFUN_BS; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    double w = fma(-al, vj[i], u[i]);
-    if (j > 1) w = fma(-be, vjm1[i], w);
-    u[i] = w;
-    acc = fma(w, w, acc);
+  for (int64_t base = i0; base < n; base += LZ_UB * stride) {
+    if (base != i0) {
+#pragma unroll
+      for (int q = 0; q < LZ_UB; ++q) {
+        const int64_t i = base + q * stride;
+        if (i < n) {
+          uu[q] = u[i];
+          aa[q] = vj[i];
+          if (j > 1) bb[q] = vjm1[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < LZ_UB; ++q) {
+      const int64_t i = base + q * stride;
+      if (i < n) {
+        double w = fma(-al, aa[q] * ia, uu[q]);
+        if (j > 1) w = fma(-be, bb[q] * ib, w);
+        u[i] = w;
+        acc = fma(w, w, acc);
+      }
+    }
   }
   acc = block_sum<PSFUN_BS>(acc, lds);
   if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
@@ -290,14 +329,18 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_assemble(int n, const double *_
   if (done == LZ_DONE_CONVERGED) m = (m - 1 < 1) ? 1 : m - 1;
   if (done == LZ_DONE_ZERO) m = 0;
   const double rs1 = ctl->rs1;
-  __shared__ double yk[LZ_NL_MAX];
-  for (int l = threadIdx.x; l < m; l += PSFUN_BS) yk[l] = ctl->yk[l];
+  __shared__ double yk[LZ_NL_MAX], ib[LZ_NL_MAX];
+  for (int l = threadIdx.x; l < m; l += PSFUN_BS) {
+    yk[l] = ctl->yk[l];
+    ib[l] = lz_inv_beta(ctl, l);  // v_{l+1} = w_l * ib[l]
+  }
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
     double acc = 0.0;
     if (m > 0) {
-      acc = yk[0] * V[i];
-      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * ldv + i], acc);
+      acc = yk[0] * (V[i] * ib[0]);
+#pragma unroll 8
+      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * ldv + i] * ib[l], acc);
     }
     y[i] = rs1 * acc;
   }
@@ -320,8 +363,9 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
                           const double *p_in, int g_in, double *v_out, double *u_out, double *p_out, LzCtl *ctl,
                           int nl, int nl_ctl, int istop, double eps, double fscale) {
   const int L = A->lanes;
-  // algorithmic bytes: val+colind, rowptr, src rows, v and u written
-  const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
+  // algorithmic bytes: val+colind, rowptr, src rows read, u written (and x
+  // copied into basis slot 0 at j = 1)
+  const double bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + (j == 1 ? 24.0 : 16.0) * A->n;
   // The step is idempotent (it reads src and the partials and rewrites v_j, u
   // and its own partials and control fields with the same values), so timing
   // may repeat it back to back inside one event pair to amortise the events.
@@ -391,12 +435,13 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   PSFUN_HIP_TRY(hipSetDevice(c->device));
   const int64_t ldv = (int64_t)align_up((size_t)n, 64);
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
-  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)(nl + 2);  // v_1..v_{nl-1}, u0, u1
+  // basis slots 0..nl-1: slot 0 holds x, slot l the unnormalised w_l (u_l
+  // until the update of step l has run)
+  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)nl;
   void *ws = nullptr;
   PSFUN_TRY(psfun_workspace(c, ctl_bytes + vbytes, &ws));
   LzCtl *ctl = (LzCtl *)ws;
   double *V = (double *)((char *)ws + ctl_bytes);
-  double *U[2] = {V + (size_t)ldv * nl, V + (size_t)ldv * (nl + 1)};
   // USERF: the kernels' control blocks keep the loop bookkeeping and the
   // breakdown test; with istop = 1 the estimate (and with it the iteration
   // limit) moves to a host-synchronous step after each control index, since
@@ -429,14 +474,14 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; partials of T(1,1) = u.v_1
   hipLaunchKernelGGL(k_sumsq_partials, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, Pw);
   PSFUN_HIP_TRY(hipGetLastError());
-  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, U[0], Pa, ctl, nl, nl_k, istop_k, eps, fscale));
+  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, V + ldv, Pa, ctl, nl, nl_k, istop_k, eps, fscale));
 
   // Pacing: before queueing step j+2, wait until step j-1 has reported
   // (h_flags[0] >= j-1).  The stores from a running kernel may only reach the
   // host when that kernel ends, so this keeps two steps queued ahead of the
   // GPU; at most two steps run as no-ops after a stop.
   for (int j = 1; j <= nl - 1; ++j) {
-    double *u = U[(j - 1) & 1];
+    double *u = V + (size_t)ldv * j;
     const double *vj = V + (size_t)ldv * (j - 1);
     const double *vjm1 = (j > 1) ? V + (size_t)ldv * (j - 2) : nullptr;
     // algorithmic bytes: u read + w written, v_j (and v_{j-1}) read
@@ -445,7 +490,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     if (j + 1 <= nl - 1) {
-      PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V + (size_t)ldv * j, U[j & 1], Pa, ctl, nl,
+      PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V, V + (size_t)ldv * (j + 1), Pa, ctl, nl,
                                nl_k, istop_k, eps, fscale));
     } else {
       hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl_k, istop_k, eps, fscale,
