@@ -111,6 +111,58 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents(int64_t nrows, i
   }
 }
 
+// Same sum with R elements per thread that share one column: for K | PSFUN_BS
+// the elements e, e + PSFUN_BS, ... of a tile of R*PSFUN_BS have the same
+// column, so each coefficient load serves R rows (the coefficient table is
+// read through the caches, and one load per V value made it the limit).
+// Each element's sum runs over l in the same order as k_krylov_resolvents.
+template <int S, int R>
+__global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nrows, int K, int mmax, int nshift,
+                                                                     const double *__restrict__ V,
+                                                                     const double *__restrict__ coef,
+                                                                     double2 *__restrict__ out) {
+  const int64_t tot = nrows * K;
+  const int64_t vs = tot;
+  const double2 *cf = reinterpret_cast<const double2 *>(coef);
+  for (int64_t t0 = (int64_t)blockIdx.x * (R * PSFUN_BS); t0 < tot; t0 += (int64_t)gridDim.x * (R * PSFUN_BS)) {
+    const int64_t e0 = t0 + threadIdx.x;
+    const int col = (int)(e0 % K);
+    double ar[R][S], ai[R][S];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < S; ++q) ar[r][q] = ai[r][q] = 0.0;
+    for (int l = 0; l < mmax; ++l) {
+      double v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t e = e0 + (int64_t)r * PSFUN_BS;
+        v[r] = (e < tot) ? V[(int64_t)l * vs + e] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        if (q < nshift) {
+          const double2 c = cf[((int64_t)q * mmax + l) * K + col];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            ar[r][q] = fma(c.x, v[r], ar[r][q]);
+            ai[r][q] = fma(c.y, v[r], ai[r][q]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t e = e0 + (int64_t)r * PSFUN_BS;
+      if (e < tot) {
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          if (q < nshift) out[(int64_t)q * tot + e] = make_double2(ar[r][q], ai[r][q]);
+      }
+    }
+  }
+}
+
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_sddmm_cols(int n, const int *__restrict__ rowptr,
                                                          const int *__restrict__ col, int K, int nshift,
@@ -167,7 +219,18 @@ extern "C" int psfun_krylov_resolvents(psfun_ctx_t c, int64_t nrows, int K, int 
                      rs1, (const double *)dsh, coef, scr);
   PSFUN_HIP_TRY(hipGetLastError());
   const int G = psfun_grid(c, (nrows * K + PSFUN_BS - 1) / PSFUN_BS, 8);
-  if (nshift <= 8)
+  if (PSFUN_BS % K == 0 && nshift <= 16) {
+    // several rows per thread share each coefficient load
+    if (nshift <= 8) {
+      const int Gr = psfun_grid(c, (nrows * K + 4 * PSFUN_BS - 1) / (4 * PSFUN_BS), 8);
+      hipLaunchKernelGGL((k_krylov_resolvents_rows<8, 4>), dim3(Gr), dim3(PSFUN_BS), 0, c->stream, nrows, K, mmax,
+                         nshift, V, (const double *)coef, (double2 *)out);
+    } else {
+      const int Gr = psfun_grid(c, (nrows * K + 2 * PSFUN_BS - 1) / (2 * PSFUN_BS), 8);
+      hipLaunchKernelGGL((k_krylov_resolvents_rows<16, 2>), dim3(Gr), dim3(PSFUN_BS), 0, c->stream, nrows, K, mmax,
+                         nshift, V, (const double *)coef, (double2 *)out);
+    }
+  } else if (nshift <= 8)
     hipLaunchKernelGGL(k_krylov_resolvents<8>, dim3(G), dim3(PSFUN_BS), 0, c->stream, nrows, K, mmax, nshift, V,
                        (const double *)coef, (double2 *)out);
   else if (nshift <= 16)
