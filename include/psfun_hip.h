@@ -240,6 +240,21 @@ int psfun_sai_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int varia
 typedef struct psfun_batch *psfun_batch_t;
 int psfun_batch_create(psfun_ctx_t ctx, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
                        const int32_t *colind, const double *val, psfun_batch_t *B);
+/* Row order a batch keeps on the device.  RCM renumbers each graph's rows by
+ * reverse Cuthill-McKee (psfun_rcm_order) so that the neighbour gathers of
+ * the SpMM stay close together; X, Y and values stay in the caller's
+ * numbering (the batch gathers X, scatters Y and permutes values itself), and
+ * each row keeps its entries in the given order, so results equal the
+ * natural order's up to the order of the dot-product reductions.  Calls that
+ * would expose the internal numbering (psfun_batch_csr,
+ * psfun_batch_lanczos_basis) return PSFUN_ERR_UNSUPPORTED on such a batch. */
+enum { PSFUN_ORDER_NATURAL = 0, PSFUN_ORDER_RCM = 1 };
+int psfun_batch_create_ordered(psfun_ctx_t ctx, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
+                               const int32_t *colind, const double *val, int order, psfun_batch_t *B);
+/* Reverse Cuthill-McKee order of one graph (host): n rows, rowptr[0..n] into
+ * colind, column indices base..base+n-1; perm[p] = original local row of new
+ * row p.  No device work. */
+int psfun_rcm_order(int n, const int32_t *rowptr, const int32_t *colind, int base, int32_t *perm);
 int psfun_batch_destroy(psfun_batch_t B);
 /* Tuning knobs (set by the autotuner): rows per block-chunk, and whether the
  * whole launch sequence is captured once in a hipGraph and replayed. */

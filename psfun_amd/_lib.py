@@ -33,6 +33,8 @@ PSFUN_FORMAT_CSR = 1
 PSFUN_FORMAT_SELL = 2
 PSFUN_PREC_NONE = 0
 PSFUN_PREC_DIAG = 1
+PSFUN_ORDER_NATURAL = 0
+PSFUN_ORDER_RCM = 1
 PSFUN_QUAD_RE_SUM = 0
 PSFUN_QUAD_A_IM_SUM = 1
 
@@ -85,6 +87,8 @@ SIGNATURES = {
     "psfun_sai_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _i, _i, _i, _vp, _vp, _d, _i, _i, _pi, _pd, _pd,
                                      _pi]),
     "psfun_batch_create": (_i, [_vp, _i, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "psfun_batch_create_ordered": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
+    "psfun_rcm_order": (_i, [_i, _vp, _vp, _i, _vp]),
     "psfun_batch_destroy": (_i, [_vp]),
     "psfun_batch_set_tuning": (_i, [_vp, _i, _i]),
     "psfun_batch_info": (_i, [_vp, _pi, _pi64, _pi64, _pi, _pi]),

@@ -41,18 +41,38 @@ def concat_block_diagonal(mats):
 _TUNE_CACHE = {}
 
 
+_ORDERS = {"natural": _lib.PSFUN_ORDER_NATURAL, "rcm": _lib.PSFUN_ORDER_RCM}
+
+
+def rcm_order(rowptr, colind, base=0):
+    """Reverse Cuthill-McKee order of one graph (psfun_rcm_order, host):
+    perm[p] = original row of new row p."""
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colind, np.int32)
+    n = len(rp) - 1
+    perm = np.zeros(max(n, 1), np.int32)
+    _lib.check(_lib.lib().psfun_rcm_order(n, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p), int(base),
+                                          perm.ctypes.data_as(C.c_void_p)), "psfun_rcm_order")
+    return perm[:n]
+
+
 class psfun_batch:
-    def __init__(self, mats, ctxt=None):
+    def __init__(self, mats, ctxt=None, order="natural"):
+        """order 'natural' keeps every graph's row numbering on the device;
+        'rcm' renumbers each graph by reverse Cuthill-McKee for gather
+        locality (X, Y and values stay in the caller's numbering)."""
         self.ctxt = ctxt if ctxt is not None else psb.default_context()
         self.row_offsets, rp, ci, va = concat_block_diagonal(mats)
         self.ngraph = len(mats)
         self.nrows = int(self.row_offsets[-1])
         self.nnz = int(rp[-1])
+        self.order = str(order).lower()
         h = C.c_void_p()
-        st = _lib.lib().psfun_batch_create(self.ctxt.handle, self.ngraph, self.row_offsets.ctypes.data_as(C.c_void_p),
-                                           rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
-                                           va.ctypes.data_as(C.c_void_p), C.byref(h))
-        _lib.check(st, "psfun_batch_create")
+        st = _lib.lib().psfun_batch_create_ordered(self.ctxt.handle, self.ngraph,
+                                                   self.row_offsets.ctypes.data_as(C.c_void_p),
+                                                   rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                                   va.ctypes.data_as(C.c_void_p), _ORDERS[self.order], C.byref(h))
+        _lib.check(st, "psfun_batch_create_ordered")
         self.handle = h
         self.rows_per_chunk = 512
         self.use_graph = False
@@ -90,7 +110,7 @@ class psfun_batch:
         """Time every (rows_per_chunk, hipGraph) candidate on this batch and
         keep the fastest.  Results are cached per (nrows, nnz, ngraph, K)."""
         K = 1 if X.dim() == 1 else int(X.shape[1])
-        key = (self.nrows, self.nnz, self.ngraph, K, itmax)
+        key = (self.nrows, self.nnz, self.ngraph, K, itmax, self.order)
         if key in _TUNE_CACHE:
             self.set_tuning(*_TUNE_CACHE[key][0])
             return _TUNE_CACHE[key]

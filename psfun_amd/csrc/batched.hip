@@ -48,6 +48,11 @@ struct psfun_batch {
   double *d_part = nullptr;  // partials: 2 * nchunks * BT_KMAX
   size_t part_cap = 0;
   int use_graph = 0;
+  // row order (PSFUN_ORDER_*): with RCM the device CSR is in the reordered
+  // numbering, d_perm[r'] = original global row of row r', d_nzperm[q'] =
+  // original position of stored entry q'
+  int order = PSFUN_ORDER_NATURAL;
+  int *d_perm = nullptr, *d_nzperm = nullptr;
   std::map<std::tuple<const void *, void *, const void *, int, int, int, double, double, double>, hipGraphExec_t>
       graphs;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -156,10 +161,24 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
   double dot = 0.0;
   if (s_any && k < K && s_act[k]) {
     const double inv = s_inv[k];
+    // entries four at a time: their column indices, then their gathers, are
+    // issued together (a one-at-a-time col -> src chain leaves the kernel
+    // waiting on latency); the sum keeps the entry order
     for (int r = r0 + sub; r < r1; r += RP) {
       const int s = rowptr[r], e = rowptr[r + 1];
       double sum = 0.0;
-      for (int q = s; q < e; ++q) sum = fma(val[q], src[(int64_t)col[q] * K + k] * inv, sum);
+      int q = s;
+      for (; q + 4 <= e; q += 4) {
+        const int c0 = col[q], c1 = col[q + 1], c2 = col[q + 2], c3 = col[q + 3];
+        const double a0 = val[q], a1 = val[q + 1], a2 = val[q + 2], a3 = val[q + 3];
+        const double x0 = src[(int64_t)c0 * K + k], x1 = src[(int64_t)c1 * K + k];
+        const double x2 = src[(int64_t)c2 * K + k], x3 = src[(int64_t)c3 * K + k];
+        sum = fma(a0, x0 * inv, sum);
+        sum = fma(a1, x1 * inv, sum);
+        sum = fma(a2, x2 * inv, sum);
+        sum = fma(a3, x3 * inv, sum);
+      }
+      for (; q < e; ++q) sum = fma(val[q], src[(int64_t)col[q] * K + k] * inv, sum);
       const double v = src[(int64_t)r * K + k] * inv;
       const double u = opscale * sum;
       v_out[(int64_t)r * K + k] = v;
@@ -285,10 +304,12 @@ __global__ __launch_bounds__(64) void k_bt_final(BtInst in, double fscale) {
   if (threadIdx.x == 0) in.m[I] = kk;
 }
 
+// perm != nullptr: the batch is reordered and row r of the result goes to
+// row perm[r] of Y (the caller's numbering).
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict__ chunk, int K,
                                                           const double *__restrict__ V, int64_t vstride, BtInst in,
-                                                          double *__restrict__ Y) {
+                                                          const int *__restrict__ perm, double *__restrict__ Y) {
   constexpr int RP = PSFUN_BS / KP;
   __shared__ int s_m[KP];
   __shared__ double s_rs1[KP];
@@ -321,8 +342,38 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict_
       acc = yk[0] * V[i];
       for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * vstride + i], acc);
     }
-    Y[i] = s_rs1[k] * acc;
+    Y[perm ? (int64_t)perm[r] * K + k : i] = s_rs1[k] * acc;
   }
+}
+
+// Xp[r] = X[perm[r]] row by row (K doubles per row): a reordered batch's
+// input in its own numbering.
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_gather_rows(int64_t nrows, int K, const int *__restrict__ perm,
+                                                            const double *__restrict__ X, double *__restrict__ Xp) {
+  const int64_t tot = nrows * K;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < tot; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const int64_t r = i / K;
+    Xp[i] = X[(int64_t)perm[r] * K + (i - r * K)];
+  }
+}
+
+// Y[perm[r]] = Yp[r] + beta * Y[perm[r]] row by row.
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_scatter_rows(int64_t nrows, int K, const int *__restrict__ perm,
+                                                             const double *__restrict__ Yp, double beta,
+                                                             double *__restrict__ Y) {
+  const int64_t tot = nrows * K;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < tot; i += (int64_t)gridDim.x * PSFUN_BS) {
+    const int64_t r = i / K;
+    const int64_t o = (int64_t)perm[r] * K + (i - r * K);
+    Y[o] = (beta != 0.0) ? fma(beta, Y[o], Yp[i]) : Yp[i];
+  }
+}
+
+// val[q] = src[nzperm[q]]: values given in the caller's entry order.
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_gather_vals(int64_t nnz, const int *__restrict__ nzperm,
+                                                            const double *__restrict__ src, double *__restrict__ val) {
+  for (int64_t q = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * PSFUN_BS)
+    val[q] = src[nzperm[q]];
 }
 
 __global__ void k_bt_init(BtInst in, int ninst, BtCtl *bctl) {
@@ -371,10 +422,9 @@ static int bt_build_chunks(psfun_batch *B) {
   return PSFUN_SUCCESS;
 }
 
-extern "C" int psfun_batch_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
-                                  const int32_t *colind, const double *val, psfun_batch_t *out) {
-  if (!c || !out || ngraph < 0 || (ngraph && (!row_offsets || !rowptr))) return PSFUN_ERR_INVALID_ARG;
-  *out = nullptr;
+static int bt_validate(int ngraph, const int64_t *row_offsets, const int32_t *rowptr, const int32_t *colind,
+                       const double *val) {
+  if (ngraph < 0 || (ngraph && (!row_offsets || !rowptr))) return PSFUN_ERR_INVALID_ARG;
   const int64_t nrows = ngraph ? row_offsets[ngraph] : 0;
   if (nrows > INT32_MAX - 1 || (ngraph && row_offsets[0] != 0)) return PSFUN_ERR_INVALID_ARG;
   const int64_t nnz = nrows ? rowptr[nrows] : 0;
@@ -388,6 +438,14 @@ extern "C" int psfun_batch_create(psfun_ctx_t c, int ngraph, const int64_t *row_
         if (colind[q] < row_offsets[g] || colind[q] >= row_offsets[g + 1]) return PSFUN_ERR_INVALID_ARG;
     }
   }
+  return PSFUN_SUCCESS;
+}
+
+// Device batch from validated host arrays in the order given.
+static int bt_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
+                     const int32_t *colind, const double *val, psfun_batch_t *out) {
+  const int64_t nrows = ngraph ? row_offsets[ngraph] : 0;
+  const int64_t nnz = nrows ? rowptr[nrows] : 0;
   PSFUN_HIP_TRY(hipSetDevice(c->device));
   psfun_batch *B = new psfun_batch();
   B->ctx = c;
@@ -415,6 +473,60 @@ extern "C" int psfun_batch_create(psfun_ctx_t c, int ngraph, const int64_t *row_
   return PSFUN_SUCCESS;
 }
 
+extern "C" int psfun_batch_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, const int32_t *rowptr,
+                                  const int32_t *colind, const double *val, psfun_batch_t *out) {
+  return psfun_batch_create_ordered(c, ngraph, row_offsets, rowptr, colind, val, PSFUN_ORDER_NATURAL, out);
+}
+
+extern "C" int psfun_batch_create_ordered(psfun_ctx_t c, int ngraph, const int64_t *row_offsets,
+                                          const int32_t *rowptr, const int32_t *colind, const double *val, int order,
+                                          psfun_batch_t *out) {
+  if (!c || !out) return PSFUN_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (order != PSFUN_ORDER_NATURAL && order != PSFUN_ORDER_RCM) return PSFUN_ERR_INVALID_ARG;
+  PSFUN_TRY(bt_validate(ngraph, row_offsets, rowptr, colind, val));
+  if (order == PSFUN_ORDER_NATURAL) return bt_create(c, ngraph, row_offsets, rowptr, colind, val, out);
+  // reverse Cuthill-McKee inside every graph; each row keeps its entries in
+  // the given order (so a row sum is the same sum as without reordering)
+  const int64_t nrows = ngraph ? row_offsets[ngraph] : 0;
+  const int64_t nnz = nrows ? rowptr[nrows] : 0;
+  std::vector<int32_t> perm((size_t)nrows), iperm((size_t)nrows), rp2((size_t)nrows + 1), ci2((size_t)nnz),
+      nzp((size_t)nnz);
+  std::vector<double> va2((size_t)nnz);
+  for (int g = 0; g < ngraph; ++g) {
+    const int64_t r0 = row_offsets[g], ng = row_offsets[g + 1] - r0;
+    PSFUN_TRY(psfun_rcm_order((int)ng, rowptr + r0, colind, (int)r0, perm.data() + r0));
+    for (int64_t p = 0; p < ng; ++p) perm[r0 + p] += (int32_t)r0;
+  }
+  for (int64_t r = 0; r < nrows; ++r) iperm[perm[r]] = (int32_t)r;
+  int64_t pos = 0;
+  rp2[0] = 0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    const int32_t o = perm[r];
+    for (int q = rowptr[o]; q < rowptr[o + 1]; ++q) {
+      ci2[pos] = iperm[colind[q]];
+      va2[pos] = val[q];
+      nzp[pos] = q;
+      ++pos;
+    }
+    rp2[r + 1] = (int32_t)pos;
+  }
+  psfun_batch_t B = nullptr;
+  PSFUN_TRY(bt_create(c, ngraph, row_offsets, rp2.data(), ci2.data(), va2.data(), &B));
+  hipError_t e = hipMalloc(&B->d_perm, sizeof(int) * std::max<int64_t>(nrows, 1));
+  if (e == hipSuccess) e = hipMalloc(&B->d_nzperm, sizeof(int) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess && nrows) e = hipMemcpy(B->d_perm, perm.data(), sizeof(int) * nrows, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(B->d_nzperm, nzp.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    psfun_record_hip_error(e, "psfun_batch_create_ordered", __FILE__, __LINE__);
+    psfun_batch_destroy(B);
+    return PSFUN_ERR_HIP;
+  }
+  B->order = PSFUN_ORDER_RCM;
+  *out = B;
+  return PSFUN_SUCCESS;
+}
+
 extern "C" int psfun_batch_destroy(psfun_batch_t B) {
   if (!B) return PSFUN_SUCCESS;
   hipSetDevice(B->ctx->device);
@@ -427,6 +539,8 @@ extern "C" int psfun_batch_destroy(psfun_batch_t B) {
   if (B->val) hipFree(B->val);
   if (B->d_chunk) hipFree(B->d_chunk);
   if (B->d_part) hipFree(B->d_part);
+  if (B->d_perm) hipFree(B->d_perm);
+  if (B->d_nzperm) hipFree(B->d_nzperm);
   delete B;
   return PSFUN_SUCCESS;
 }
@@ -455,10 +569,15 @@ extern "C" int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, in
 template <int KP>
 static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double opscale, const double *X, double *Y,
                       double eps, int itmax, int istop, const BtInst &in, BtCtl *bctl, double *V, double *U0,
-                      double *U1, bool poll_host) {
+                      double *U1, double *Xp, bool poll_host) {
   const int NC = B->nchunks;
   const int ninst = B->ngraph * K;
   const int64_t vs = B->nrows * K;
+  if (B->d_perm) {  // reordered batch: X into its numbering first
+    hipLaunchKernelGGL(k_bt_gather_rows, dim3(psfun_grid(c, (vs + PSFUN_BS - 1) / PSFUN_BS, 8)), dim3(PSFUN_BS), 0,
+                       c->stream, (int64_t)B->nrows, K, (const int *)B->d_perm, X, Xp);
+    X = Xp;
+  }
   double *P0 = B->d_part, *P1 = B->d_part + (size_t)NC * BT_KMAX;
   hipLaunchKernelGGL(k_bt_init, dim3(std::min(1024, (ninst * in.ld + 255) / 256 + 1)), dim3(256), 0, c->stream, in,
                      ninst, bctl);
@@ -502,7 +621,8 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     }
   }
   hipLaunchKernelGGL(k_bt_final, dim3(ninst), dim3(64), 0, c->stream, in, fscale);
-  hipLaunchKernelGGL((k_bt_assemble<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, V, vs, in, Y);
+  hipLaunchKernelGGL((k_bt_assemble<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, V, vs, in,
+                     (const int *)B->d_perm, Y);
   PSFUN_HIP_TRY(hipGetLastError());
   return PSFUN_SUCCESS;
 }
@@ -546,6 +666,8 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
   if (!X || !Y) return PSFUN_ERR_INVALID_VECT_STATE;
   if (itmax < 2 || itmax > PSFUN_FT_KMAX) return PSFUN_ERR_INVALID_ARG;
   if (B->nrows == 0 || B->ngraph == 0) return PSFUN_SUCCESS;
+  // an exported basis would be in the batch's own row numbering
+  if (V_ext && B->d_perm) return PSFUN_ERR_UNSUPPORTED;
   PSFUN_HIP_TRY(hipSetDevice(c->device));
   const int ninst = B->ngraph * K;
   const int ld = itmax + 2;
@@ -554,7 +676,8 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
   const size_t inst_bytes = align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256) +
                             align_up(sizeof(int) * (size_t)ninst * 3, 256);
   // the basis lives in the caller's buffer when one is given
-  const size_t bytes = 256 + inst_bytes + sizeof(double) * (size_t)vs * (V_ext ? 2 : itmax + 1);
+  // (a reordered batch also keeps X in its own numbering)
+  const size_t bytes = 256 + inst_bytes + sizeof(double) * (size_t)vs * ((V_ext ? 2 : itmax + 1) + (B->d_perm ? 1 : 0));
   void *ws = nullptr;
   PSFUN_TRY(psfun_workspace(c, bytes, &ws));
   BtCtl *bctl = (BtCtl *)ws;
@@ -581,6 +704,7 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
     U0 = V + (size_t)vs * (itmax - 1);
   }
   double *U1 = U0 + vs;
+  double *Xp = B->d_perm ? U1 + vs : nullptr;
 
   int KP = 1;
   while (KP < K) KP <<= 1;
@@ -606,7 +730,7 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
     if (graph && hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) st = PSFUN_ERR_HIP;
     if (st == PSFUN_SUCCESS) {
 #define BT_CASE(KPV) \
-  case KPV: st = bt_enqueue<KPV>(c, B, K, fscale, opscale, X, Y, eps, itmax, istop, in, bctl, V, U0, U1, !graph); break;
+  case KPV: st = bt_enqueue<KPV>(c, B, K, fscale, opscale, X, Y, eps, itmax, istop, in, bctl, V, U0, U1, Xp, !graph); break;
       switch (KP) {
         BT_CASE(1) BT_CASE(2) BT_CASE(4) BT_CASE(8) BT_CASE(16) BT_CASE(32) BT_CASE(64)
         default: st = PSFUN_ERR_INVALID_ARG;
@@ -690,10 +814,25 @@ extern "C" int psfun_batch_spmm(psfun_ctx_t c, psfun_batch_t B, int K, double al
   if (B->nrows == 0 || B->nchunks == 0) return PSFUN_SUCCESS;
   int KP = 1;
   while (KP < K) KP <<= 1;
+  // reordered batch: X into its numbering, the product there, Y scattered back
+  const int64_t vs = B->nrows * K;
+  const double *Xs = X;
+  double *Ys = Y;
+  double beta_s = beta;
+  if (B->d_perm) {
+    void *ws = nullptr;
+    PSFUN_TRY(psfun_workspace(c, sizeof(double) * 2 * (size_t)vs, &ws));
+    double *Xp = (double *)ws, *Yp = Xp + vs;
+    hipLaunchKernelGGL(k_bt_gather_rows, dim3(psfun_grid(c, (vs + PSFUN_BS - 1) / PSFUN_BS, 8)), dim3(PSFUN_BS), 0,
+                       c->stream, (int64_t)B->nrows, K, (const int *)B->d_perm, X, Xp);
+    Xs = Xp;
+    Ys = Yp;
+    beta_s = 0.0;
+  }
 #define SP_CASE(KPV)                                                                                       \
   case KPV:                                                                                                \
     hipLaunchKernelGGL((k_bt_spmm_plain<KPV>), dim3(B->nchunks), dim3(PSFUN_BS), 0, c->stream, B->rowptr, \
-                       B->colind, B->val, B->d_chunk, K, alpha, X, beta, Y);                              \
+                       B->colind, B->val, B->d_chunk, K, alpha, Xs, beta_s, Ys);                          \
     break;
   switch (KP) {
     SP_CASE(1) SP_CASE(2) SP_CASE(4) SP_CASE(8) SP_CASE(16) SP_CASE(32) SP_CASE(64)
@@ -701,18 +840,31 @@ extern "C" int psfun_batch_spmm(psfun_ctx_t c, psfun_batch_t B, int K, double al
   }
 #undef SP_CASE
   PSFUN_HIP_TRY(hipGetLastError());
+  if (B->d_perm) {
+    hipLaunchKernelGGL(k_bt_scatter_rows, dim3(psfun_grid(c, (vs + PSFUN_BS - 1) / PSFUN_BS, 8)), dim3(PSFUN_BS), 0,
+                       c->stream, (int64_t)B->nrows, K, (const int *)B->d_perm, (const double *)Ys, beta, Y);
+    PSFUN_HIP_TRY(hipGetLastError());
+  }
   return PSFUN_SUCCESS;
 }
 
 extern "C" int psfun_batch_set_values(psfun_batch_t B, const double *val) {
   if (!B || (B->nnz && !val)) return PSFUN_ERR_INVALID_ARG;
-  if (B->nnz)
+  if (!B->nnz) return PSFUN_SUCCESS;
+  if (B->d_nzperm) {
+    psfun_ctx *c = B->ctx;
+    hipLaunchKernelGGL(k_bt_gather_vals, dim3(psfun_grid(c, (B->nnz + PSFUN_BS - 1) / PSFUN_BS, 8)), dim3(PSFUN_BS), 0,
+                       c->stream, (int64_t)B->nnz, (const int *)B->d_nzperm, val, B->val);
+    PSFUN_HIP_TRY(hipGetLastError());
+  } else {
     PSFUN_HIP_TRY(hipMemcpyAsync(B->val, val, sizeof(double) * B->nnz, hipMemcpyDeviceToDevice, B->ctx->stream));
+  }
   return PSFUN_SUCCESS;
 }
 
 extern "C" int psfun_batch_csr(psfun_batch_t B, const int32_t **rowptr, const int32_t **colind, const double **val) {
   if (!B) return PSFUN_ERR_INVALID_ARG;
+  if (B->d_perm) return PSFUN_ERR_UNSUPPORTED;  // its arrays are in the batch's own numbering
   if (rowptr) *rowptr = B->rowptr;
   if (colind) *colind = B->colind;
   if (val) *val = B->val;

@@ -78,3 +78,44 @@ def test_batched_zero_column_and_autotune(gpu_ctx):
     assert len(table) == 4 and best in [c for c, _ in table]
     Y2, _, _ = B.expm(Xd, t=0.5)
     assert (torch.linalg.norm(Y - Y2) / torch.linalg.norm(Y)).item() <= 1e-10
+
+
+def test_batched_rcm_order_matches_natural(gpu_ctx):
+    """A batch kept in reverse Cuthill-McKee order takes and returns X, Y and
+    values in the caller's numbering: exp(-tA)X equal to the natural-order
+    batch up to the dot products' summation order; SpMM bitwise equal (each
+    row keeps its entries in order); set_values takes the caller's entry
+    order; calls that would expose the internal numbering refuse."""
+    import ctypes as C
+    from psfun_amd import _lib
+    sizes = [700, 2500, 40, 1800]
+    mats = _graphs(sizes, 31)
+    Bn = psfun_batch(mats)
+    Br = psfun_batch(mats, order="rcm")
+    N, K = sum(sizes), 5
+    Xd = torch.as_tensor(matgen.normal(5, N * K).reshape(N, K), device="cuda:0")
+    Yn, itn, _ = Bn.expm(Xd, t=0.5)
+    Yr, itr, _ = Br.expm(Xd, t=0.5)
+    assert np.abs(itn - itr).max() <= 1
+    assert torch.linalg.norm(Yr - Yn) <= 1e-12 * torch.linalg.norm(Yn)
+    for rpc, g in ((256, True), (1024, False)):        # tuning and graph replay keep the permutation
+        Br.set_tuning(rpc, g)                           # (other chunking: other fold order of the partials)
+        Y2, _, _ = Br.expm(Xd, t=0.5)
+        assert torch.linalg.norm(Y2 - Yr) <= 1e-10 * torch.linalg.norm(Yr)
+    L = _lib.lib()
+    Y0 = torch.as_tensor(matgen.normal(6, N * K).reshape(N, K), device="cuda:0")
+    Za, Zb = Y0.clone(), Y0.clone()
+    for B, Z in ((Bn, Za), (Br, Zb)):
+        B.ctxt.bind_stream()
+        _lib.check(L.psfun_batch_spmm(B.ctxt.handle, B.handle, K, -0.5, C.c_void_p(Xd.data_ptr()), 0.25,
+                                      C.c_void_p(Z.data_ptr())))
+    torch.cuda.synchronize()
+    assert torch.equal(Za, Zb)
+    vals = torch.as_tensor(np.concatenate([m[2] for m in mats]) * 1.7, device="cuda:0")
+    for B in (Bn, Br):
+        _lib.check(L.psfun_batch_set_values(B.handle, C.c_void_p(vals.data_ptr())))
+    Yn, _, _ = Bn.expm(Xd, t=0.5)
+    Yr, _, _ = Br.expm(Xd, t=0.5)
+    assert torch.linalg.norm(Yr - Yn) <= 1e-12 * torch.linalg.norm(Yn)
+    rp = C.c_void_p()
+    assert L.psfun_batch_csr(Br.handle, C.byref(rp), None, None) == _lib.PSFUN_ERR_UNSUPPORTED
