@@ -145,6 +145,11 @@ int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
 int psfun_csr_set_format(psfun_csr_t A, int format);
 /* Format in use and the padded sliced-ELL entry count (0 when not built). */
 int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_entries);
+/* Width of the column indices the Lanczos sliced-ELL step reads: 16 when
+ * every column is within [-32768, 32767] of its row (16-bit offsets are kept
+ * beside the 32-bit columns; results are the same), 32 otherwise, 0 without
+ * a sliced-ELL copy. */
+int psfun_csr_sell_index_bits(psfun_csr_t A, int *bits);
 
 /* ---- device vectors ------------------------------------------------------ */
 int psfun_vec_alloc(psfun_ctx_t ctx, int64_t n, double **dptr);

@@ -70,6 +70,7 @@ SIGNATURES = {
     "psfun_csr_info": (_i, [_vp, _pi64, _pi64, _pi]),
     "psfun_csr_set_format": (_i, [_vp, _i]),
     "psfun_csr_get_format": (_i, [_vp, _pi, _pi64]),
+    "psfun_csr_sell_index_bits": (_i, [_vp, _pi]),
     "psfun_vec_alloc": (_i, [_vp, _i64, C.POINTER(_vp)]),
     "psfun_vec_free": (_i, [_vp, _vp]),
     "psfun_vec_upload": (_i, [_vp, _vp, _vp, _i64]),

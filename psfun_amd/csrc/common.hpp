@@ -111,6 +111,10 @@ struct psfun_csr {
   int *sell_ptr = nullptr;
   int *sell_col = nullptr;
   double *sell_val = nullptr;
+  // optional 16-bit column offsets of the sliced-ELL entries (col - row),
+  // built when every offset fits: half the column bytes for the kernels
+  // that read them (nullptr otherwise)
+  int16_t *sell_dcol = nullptr;
   int format = PSFUN_FORMAT_CSR;  // format the kernels use
 };
 #define PSFUN_SELL_C 64
@@ -259,6 +263,22 @@ __device__ __forceinline__ double sell_row_dot(int row, const int *__restrict__ 
   double sum = 0.0;
 #pragma unroll 4
   for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], x[cp[k * 64]] * xs, sum);
+  return sum;
+}
+
+// sell_row_dot with 16-bit column offsets: entry k of the row reads
+// x[row + dcol_k].
+__device__ __forceinline__ double sell_row_dot16(int row, const int *__restrict__ sp,
+                                                 const int16_t *__restrict__ dcol, const double *__restrict__ val,
+                                                 const double *__restrict__ x, double xs) {
+  const int s = row >> 6, l = row & 63;
+  const int b = sp[s];
+  const int w = (sp[s + 1] - b) >> 6;
+  const int16_t *cp = dcol + b + l;
+  const double *vp = val + b + l;
+  double sum = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], x[row + (int)cp[k * 64]] * xs, sum);
   return sum;
 }
 

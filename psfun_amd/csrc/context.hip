@@ -194,12 +194,36 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sell_fill(int n, int nslice, const
   }
 }
 
+// 16-bit offsets col - row of every sliced-ELL entry; *bad is set when an
+// offset does not fit.  Padding entries have offset 0, and so do the rows
+// beyond n that fill the last slice (no kernel computes them).
+__global__ __launch_bounds__(PSFUN_BS) void k_sell_delta(int n, int nslice, const int *__restrict__ sp,
+                                                         const int *__restrict__ scol, int16_t *__restrict__ dcol,
+                                                         int *bad) {
+  const int64_t r = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  if (r >= (int64_t)nslice * PSFUN_SELL_C) return;
+  const int s = (int)(r / PSFUN_SELL_C), l = (int)(r % PSFUN_SELL_C);
+  const int b = sp[s];
+  const int w = (sp[s + 1] - b) / PSFUN_SELL_C;
+  int miss = 0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t idx = (int64_t)b + (int64_t)k * PSFUN_SELL_C + l;
+    const int64_t d = (r < n) ? (int64_t)scol[idx] - r : 0;
+    const bool fits = d >= -32768 && d <= 32767;
+    if (!fits) miss = 1;
+    dcol[idx] = (int16_t)(fits ? d : 0);
+  }
+  if (miss) atomicOr(bad, 1);
+}
+
 static void sell_free(psfun_csr *A) {
   if (A->sell_ptr) hipFree(A->sell_ptr);
   if (A->sell_col) hipFree(A->sell_col);
   if (A->sell_val) hipFree(A->sell_val);
+  if (A->sell_dcol) hipFree(A->sell_dcol);
   A->sell_ptr = A->sell_col = nullptr;
   A->sell_val = nullptr;
+  A->sell_dcol = nullptr;
   A->sell_nnz = 0;
   A->nslice = 0;
 }
@@ -243,6 +267,25 @@ int psfun_sell_build(psfun_csr *A, const int *rp) {
   A->nslice = ns;
   A->sell_nnz = tot;
   PSFUN_TRY(sell_launch_fill(A, 1));
+  // 16-bit column offsets when all of them fit (kept beside the 32-bit columns)
+  if (hipMalloc(&A->sell_dcol, sizeof(int16_t) * (size_t)std::max<int64_t>(tot, 1)) == hipSuccess) {
+    int *bad = (int *)A->ctx->d_scalar;
+    int hbad = 0;
+    PSFUN_HIP_TRY(hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, A->ctx->stream));
+    const int64_t threads = (int64_t)ns * PSFUN_SELL_C;
+    hipLaunchKernelGGL(k_sell_delta, dim3((unsigned)((threads + PSFUN_BS - 1) / PSFUN_BS)), dim3(PSFUN_BS), 0,
+                       A->ctx->stream, n, ns, A->sell_ptr, A->sell_col, A->sell_dcol, bad);
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_HIP_TRY(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, A->ctx->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+    if (hbad) {
+      hipFree(A->sell_dcol);
+      A->sell_dcol = nullptr;
+    }
+  } else {
+    (void)hipGetLastError();
+    A->sell_dcol = nullptr;
+  }
   PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
   return PSFUN_SUCCESS;
 }
@@ -409,6 +452,12 @@ extern "C" int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_en
   if (!A) return PSFUN_ERR_INVALID_ARG;
   if (format) *format = A->format;
   if (sell_entries) *sell_entries = A->sell_nnz;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_sell_index_bits(psfun_csr_t A, int *bits) {
+  if (!A || !bits) return PSFUN_ERR_INVALID_ARG;
+  *bits = A->sell_nnz == 0 ? 0 : (A->sell_dcol ? 16 : 32);
   return PSFUN_SUCCESS;
 }
 

@@ -145,9 +145,11 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
 // so the fold's latency hides behind the first chunk's loads; each row sum is
 // taken unscaled and multiplied by 1/beta afterwards (u = opscale (1/beta)
 // sum_k a_k w_k, equal to opscale sum_k a_k v_k up to rounding).
-template <bool SMALLT>
+// D16: the columns come as 16-bit offsets from the row (A->sell_dcol).
+template <bool SMALLT, bool D16>
 __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *__restrict__ sp,
                                                            const int *__restrict__ col,
+                                                           const int16_t *__restrict__ dcol,
                                                            const double *__restrict__ val, double opscale,
                                                            const double *__restrict__ src, int j, int G,
                                                            const double *__restrict__ p_in, int g_in,
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
   if (xr.begin < xr.end) {
     const int row0 = (int)(xr.begin * PSFUN_BS + threadIdx.x);
     if (row0 < n) {
-      acc0 = sell_row_dot(row0, sp, col, val, src, 1.0);
+      acc0 = D16 ? sell_row_dot16(row0, sp, dcol, val, src, 1.0) : sell_row_dot(row0, sp, col, val, src, 1.0);
       src0 = src[row0];
     }
   }
@@ -196,7 +198,9 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
     const int row = (int)(ch * PSFUN_BS + threadIdx.x);
     if (row < n) {
       const bool first = (ch == xr.begin);
-      const double sum = first ? acc0 : sell_row_dot(row, sp, col, val, src, 1.0);
+      const double sum = first ? acc0
+                               : (D16 ? sell_row_dot16(row, sp, dcol, val, src, 1.0)
+                                      : sell_row_dot(row, sp, col, val, src, 1.0));
       const double w = first ? src0 : src[row];
       const double u = uscale * sum;
       if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
@@ -372,16 +376,21 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   const int reps = psfun_prof_reps(c);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
   if (A->format == PSFUN_FORMAT_SELL) {
+#define LZ_SELL(SM, D)                                                                                           \
+  hipLaunchKernelGGL((k_lz_spmv_sell<SM, D>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,      \
+                     A->sell_col, A->sell_dcol, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, \
+                     ctl, nl_ctl, istop, eps, fscale, c->d_flags)
+    const bool d16 = A->sell_dcol != nullptr;
     for (int r = 0; r < reps; ++r) {
-      if (nl <= 65)
-        hipLaunchKernelGGL(k_lz_spmv_sell<true>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
-                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
-                           istop, eps, fscale, c->d_flags);
-      else
-        hipLaunchKernelGGL(k_lz_spmv_sell<false>, dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,
-                           A->sell_col, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
-                           istop, eps, fscale, c->d_flags);
+      if (nl <= 65) {
+        if (d16) LZ_SELL(true, true);
+        else LZ_SELL(true, false);
+      } else {
+        if (d16) LZ_SELL(false, true);
+        else LZ_SELL(false, false);
+      }
     }
+#undef LZ_SELL
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;

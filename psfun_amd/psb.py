@@ -204,6 +204,13 @@ class psb_dspmat_type:
         """'CSR' (vector CSR), 'SELL' (sliced ELL, 64-row slices) or 'AUTO'."""
         _lib.check(_lib.lib().psfun_csr_set_format(self.handle, self._FORMATS[str(fmt).upper()]))
 
+    def sell_index_bits(self):
+        """16 or 32: width of the column indices the Lanczos SELL step reads
+        (0 without a sliced-ELL copy)."""
+        b = C.c_int(0)
+        _lib.check(_lib.lib().psfun_csr_sell_index_bits(self.handle, C.byref(b)))
+        return b.value
+
     def get_format(self):
         """(format name, padded sliced-ELL entry count)."""
         f = C.c_int(0)

@@ -94,6 +94,35 @@ def test_format_auto_choice(gpu_ctx):
     assert M.get_format()[0] == "SELL"
 
 
+def test_lanczos_sell_index_widths(gpu_ctx):
+    """The Lanczos sliced-ELL step reads 16-bit column offsets when every
+    column is within 32767 of its row, 32-bit columns otherwise; both match
+    the oracle and the CSR path."""
+    A = matgen.laplacian((20, 20, 20), scale=-1.0)               # offsets up to 400
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format("SELL")
+    assert M.sell_index_bits() == 16
+    # a path graph of 40 000 nodes closed by one long edge (offset 39 999)
+    n = 40000
+    rows = np.concatenate([np.arange(n - 1), np.arange(1, n), np.arange(n), [0, n - 1]])
+    cols = np.concatenate([np.arange(1, n), np.arange(n - 1), np.arange(n), [n - 1, 0]])
+    vals = np.concatenate([np.ones(2 * n - 2), -2.5 * np.ones(n), [0.5, 0.5]])
+    B = matgen._coo_to_csr(n, rows, cols, vals)
+    MB = psb.psb_dspmat_type.from_csr(B)
+    MB.set_format("SELL")
+    assert MB.sell_index_bits() == 32
+    for mat, nn in ((A, 8000), (B, n)):
+        x = matgen.uniform_pm1(5, nn)
+        ref = O.lanczos(mat, x, 1e-10, 50, 1)
+        ys = {}
+        for fmt in ("SELL", "CSR"):
+            info, it, err, res, y = _lanczos_gpu(mat, x, 1e-10, 50, fmt=fmt)
+            assert info == 0 and abs(it - ref["iter"]) <= 1
+            assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+            ys[fmt] = y
+        assert np.linalg.norm(ys["SELL"] - ys["CSR"]) <= 1e-12 * np.linalg.norm(ys["CSR"])
+
+
 def test_spmv_one_based_input(gpu_ctx):
     g = load_golden("laplacian50.npz")
     M = psb.psb_dspmat_type(g["rowptr"], g["colind"], g["val"], index_base=1)
