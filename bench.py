@@ -21,6 +21,10 @@ The same JSON line also carries:
   quadrature    -- exp(-A)x on the 96^3 Laplacian by the 16-pole Talbot rule
                    with one multi-shift CG over all poles (end-to-end error
                    <= 1e-8 against the closed form)
+  blas1         -- the PSBLAS BLAS-1 primitives (dot, nrm2, amax, axpby) on
+                   1 GiB vectors
+  cg            -- the shifted PCG (A + 0.01 I) x = b on the 96^3 Laplacian,
+                   DIAG preconditioner, eps 1e-8
 """
 import argparse
 import json
@@ -55,7 +59,7 @@ def parse():
     p.add_argument("--copy-probe", action="store_true", help="also run the 1 GiB copy that scales PMC counters")
     p.add_argument("--prof-reps", type=int, default=4, help="back-to-back launches per timed SpMV step")
     p.add_argument("--no-extra", action="store_true", help="headline only (no blas1/batched/adjoint/quadrature)")
-    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature",
+    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg",
                    help="comma-separated extra sections to run after the headline")
     p.add_argument("--format", default="", help="SELL, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     return p.parse_args()
@@ -414,6 +418,60 @@ def quadrature_section(args):
             "seed_spmv_GBs": round(ks["bytes"] / max(ks["ms"], 1e-9) / 1e6, 1)}
 
 
+def cg_section(args):
+    """The shifted PCG of psfun_dcg_vect on the 96^3 Laplacian: (A + 0.01 I) x
+    = b, DIAG preconditioner, istop 2, eps 1e-8, zero initial guess."""
+    import torch
+    from psfun_amd import _lib, matgen, psb
+    from psfun_amd.shifted import psb_dprec_type, psb_krylov
+
+    dims = (96, 96, 96)
+    A = matgen.laplacian(dims)
+    n = 96 ** 3
+    ctx = psb.default_context(0)
+    M = psb.psb_dspmat_type.from_csr(A, ctx)
+    desc = psb.psb_desc_type(n, ctx)
+    prec = psb_dprec_type()
+    prec.init(ctx, "DIAG")
+    prec.build(M, desc)
+    b = psb.psb_d_vect_type(torch.as_tensor(matgen.uniform_pm1(9, n), device="cuda:0"))
+    xv = psb.psb_d_vect_type(torch.zeros(n, dtype=torch.float64, device="cuda:0"))
+    eta, zeta, eps = 1.0, 0.01, 1e-8
+
+    def solve():
+        xv.v.zero_()
+        return psb_krylov("CG", M, prec, b, eta, zeta, xv, eps, desc, itmax=4000, istop=2)
+
+    solve()
+    torch.cuda.synchronize()
+    steps = 3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info, it, err, _ = solve()
+    torch.cuda.synchronize()
+    ts = (time.perf_counter() - t0) / steps
+    L = _lib.lib()
+    L.psfun_prof_reset(ctx.handle)
+    L.psfun_prof_enable(ctx.handle, 1)
+    solve()
+    torch.cuda.synchronize()
+    L.psfun_prof_enable(ctx.handle, 0)
+    ka = read_prof(L, _lib, ctx, "cg")
+    # residual check as shiftedtest.f90 does: ||b - (eta A + zeta I) x|| / ||b||
+    xh = xv.get_vect()
+    r = b.get_vect() - (eta * matgen.to_scipy(A) @ xh + zeta * xh)
+    rel = float(np.linalg.norm(r) / np.linalg.norm(b.get_vect()))
+    us_it = ts / max(it, 1) * 1e6
+    log(f"cg: {1 / ts:.2f} solves/s, {it} iterations, {us_it:.1f} us/iteration, residual {rel:.2e}")
+    return {"workload": "(A + 0.01 I) x = b, 3-D Laplacian 96^3, PCG with DIAG, istop 2, eps 1e-8",
+            "value": round(1.0 / ts, 3), "unit": "solves/s", "ms_per_solve": round(ts * 1e3, 3),
+            "iterations": int(it), "us_per_iteration": round(us_it, 2), "relative_residual": rel,
+            "spmv_kernel": "k_cg_a (p update + (eta A + zeta I) p + sigma partials; "
+                           "12*nnz + 4(n+1) + 40n bytes)",
+            "spmv_kernel_GBs": round(ka["bytes"] / max(ka["ms"], 1e-9) / 1e6, 1),
+            "spmv_kernel_us": round(1e3 * ka["ms"] / max(ka["launches"], 1), 2)}
+
+
 def blas1_section(args):
     """The PSBLAS BLAS-1 primitives behind the Krylov loops (psb_gedot,
     psb_genrm2, psb_geamax, psb_geaxpby) on 1 GiB vectors: per-kernel device
@@ -488,7 +546,7 @@ def main():
     if not args.no_extra:
         wanted = set(args.sections.split(","))
         for name, fn in (("blas1", blas1_section), ("batched", batched_section), ("adjoint", adjoint_section),
-                         ("quadrature", quadrature_section)):
+                         ("quadrature", quadrature_section), ("cg", cg_section)):
             if name not in wanted:
                 continue
             try:
