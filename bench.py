@@ -25,6 +25,8 @@ The same JSON line also carries:
                    1 GiB vectors
   cg            -- the shifted PCG (A + 0.01 I) x = b on the 96^3 Laplacian,
                    DIAG preconditioner, eps 1e-8
+  arnoldi       -- Arnoldi exp(-A)x on the headline's matrix (modified
+                   Gram-Schmidt, GENPADE f(H) step)
 """
 import argparse
 import json
@@ -59,7 +61,7 @@ def parse():
     p.add_argument("--copy-probe", action="store_true", help="also run the 1 GiB copy that scales PMC counters")
     p.add_argument("--prof-reps", type=int, default=4, help="back-to-back launches per timed SpMV step")
     p.add_argument("--no-extra", action="store_true", help="headline only (no blas1/batched/adjoint/quadrature)")
-    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg",
+    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg,arnoldi",
                    help="comma-separated extra sections to run after the headline")
     p.add_argument("--format", default="", help="SELL, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     return p.parse_args()
@@ -472,6 +474,55 @@ def cg_section(args):
             "spmv_kernel_us": round(1e3 * ka["ms"] / max(ka["launches"], 1), 2)}
 
 
+def arnoldi_section(args):
+    """psfun_d_arnoldi (modified Gram-Schmidt, GENPADE-variant f(H) step) on
+    the headline's 128^3 Laplacian: exp(-A)x, eps 1e-10."""
+    import torch
+    from psfun_amd import _lib, matgen, psb
+    from psfun_amd.arnoldi import arnoldi_apply
+    from psfun_amd.serial import psfun_d_serial
+
+    m = args.grid
+    dims = (m, m, m)
+    n = m ** 3
+    A = matgen.laplacian(dims)
+    ctx = psb.default_context(0)
+    M = psb.psb_dspmat_type.from_csr(A, ctx)
+    x = matgen.uniform_pm1(1, n)
+    xv = psb.psb_d_vect_type(torch.as_tensor(x, device="cuda:0"))
+    yv = psb.psb_d_vect_type(torch.zeros(n, dtype=torch.float64, device="cuda:0"))
+    fun = psfun_d_serial()
+    fun.set("VARIANT", "GENPADE")
+    fun.set("SCALING", -1.0)
+    desc = psb.psb_desc_type(n, ctx)
+    arnoldi_apply(fun, M, desc, yv, xv, 1e-10, itmax=50)
+    torch.cuda.synchronize()
+    steps = 5
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info, it, err, res = arnoldi_apply(fun, M, desc, yv, xv, 1e-10, itmax=50)
+    torch.cuda.synchronize()
+    ta = (time.perf_counter() - t0) / steps
+    L = _lib.lib()
+    L.psfun_prof_reset(ctx.handle)
+    L.psfun_prof_enable(ctx.handle, 1)
+    arnoldi_apply(fun, M, desc, yv, xv, 1e-10, itmax=50)
+    torch.cuda.synchronize()
+    L.psfun_prof_enable(ctx.handle, 0)
+    ks = read_prof(L, _lib, ctx, "ar_spmv")
+    km = read_prof(L, _lib, ctx, "ar_mgs")
+    yex = matgen.dst_expm_laplacian(dims, x, 1.0)
+    rel = float(np.linalg.norm(yv.get_vect() - yex) / np.linalg.norm(yex))
+    log(f"arnoldi: {1 / ta:.2f} f(A)x/s, {it} iterations, rel err {rel:.2e}")
+    return {"workload": f"Arnoldi exp(-A)x (SpMV scaled by -1, GENPADE f(H)), 3-D Laplacian {m}^3, eps 1e-10",
+            "value": round(1.0 / ta, 3), "unit": "f(A)x/s", "ms_per_apply": round(ta * 1e3, 3),
+            "iterations": int(it), "rel_err_vs_closed_form": rel,
+            "spmv_kernel_us": round(1e3 * ks["ms"] / max(ks["launches"], 1), 2),
+            "spmv_kernel_GBs": round(ks["bytes"] / max(ks["ms"], 1e-9) / 1e6, 1),
+            "mgs_step_us": round(1e3 * km["ms"] / max(km["launches"], 1), 2),
+            "mgs_GBs": round(km["bytes"] / max(km["ms"], 1e-9) / 1e6, 1)}
+
+
 def blas1_section(args):
     """The PSBLAS BLAS-1 primitives behind the Krylov loops (psb_gedot,
     psb_genrm2, psb_geamax, psb_geaxpby) on 1 GiB vectors: per-kernel device
@@ -546,7 +597,7 @@ def main():
     if not args.no_extra:
         wanted = set(args.sections.split(","))
         for name, fn in (("blas1", blas1_section), ("batched", batched_section), ("adjoint", adjoint_section),
-                         ("quadrature", quadrature_section), ("cg", cg_section)):
+                         ("quadrature", quadrature_section), ("cg", cg_section), ("arnoldi", arnoldi_section)):
             if name not in wanted:
                 continue
             try:

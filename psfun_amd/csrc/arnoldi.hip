@@ -27,46 +27,83 @@
 #include "lanczos.hpp"
 #include "userf.hpp"
 
-// Reference control at loop index i = j >= 2, with h(j,j-1) = sigma.  All
-// threads of the control block.
-__device__ void ar_control(LzCtl *ctl, double *H, int ldh, int model, int j, double sigma, int nl, int istop,
-                           double eps, double fscale, DenseLds *W, int *hf) {
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    host_flag_store(hf, j);
-    H[(int64_t)(j - 2) * ldh + (j - 1)] = sigma;  // h(j, j-1)
-    ctl->itx = j - 1;
-  }
-  if (istop == 1) {
-    const int k = j - 1;
-    for (int i = tid; i < k; i += PSFUN_BS) W->y[i] = (i == 0) ? 1.0 : 0.0;
-    __syncthreads();
-    block_expmv(k, H, ldh, model, fscale, W);
-    double ss = 0.0;
-    for (int i = tid; i < k; i += PSFUN_BS) {
-      const double v = W->y[i];
-      ss = fma(v, v, ss);
-      ctl->yk[i] = v;
-    }
-    const double nrm = sqrt(dense_block_sum(ss, W));
-    const double errnum = ctl->rs1 * sigma * fabs(W->y[k - 1]);
-    const double rsj = errnum / nrm;
-    if (tid == 0) {
-      ctl->errnum = errnum;
-      ctl->errden = nrm;
-      ctl->rs[j] = rsj;
-      if (rsj < eps) {
-        ctl->m = k;
-        ctl->done = LZ_DONE_CONVERGED;
-        host_flag_store(hf + 1, LZ_DONE_CONVERGED);
-      }
-    }
-    if (rsj < eps) return;
-  }
-  if (j >= nl && tid == 0) {
+// Reference loop bookkeeping at index i = j >= 2, with h(j,j-1) = sigma:
+// the subdiagonal entry, itx, the host progress flag and, when no estimate is
+// taken (istop 2), the iteration limit.  The istop-1 estimate runs in
+// k_ar_est right after the step, so the dense f(H) code stays out of the
+// streaming kernel (and out of its register budget).
+__device__ void ar_control(LzCtl *ctl, double *H, int ldh, int j, double sigma, int nl, int istop, int *hf) {
+  if (threadIdx.x != 0) return;
+  host_flag_store(hf, j);
+  H[(int64_t)(j - 2) * ldh + (j - 1)] = sigma;  // h(j, j-1)
+  ctl->itx = j - 1;
+  if (istop != 1 && j >= nl) {
     ctl->done = LZ_DONE_MAXIT;
     host_flag_store(hf + 1, LZ_DONE_MAXIT);
   }
+}
+
+// The reference's estimate at index j (istop 1, psfun_d_arnoldi.f90:215-257):
+// yk = f(H(1:j-1,1:j-1)) e1, rs(j) = rs1 h(j,j-1) |yk(j-1)| / ||yk||, then
+// the convergence test and the iteration limit.  WAVE: one wavefront
+// (order <= 64); otherwise one block.
+template <bool WAVE>
+__global__ __launch_bounds__(WAVE ? 64 : PSFUN_BS) void k_ar_est(LzCtl *ctl, const double *H, int ldh, int model,
+                                                                 int j, int nl, double eps, double fscale, int *hf) {
+  if (ctl->done) return;
+  const int tid = threadIdx.x;
+  const int k = j - 1;
+  double nrm, last;
+  if (WAVE) {
+    __shared__ WaveDenseLds L;
+    const double y = wave_expmv(k, H, ldh, model, fscale, tid == 0 ? 1.0 : 0.0, &L);
+    if (tid < k) ctl->yk[tid] = y;
+    nrm = sqrt(wave_sum(tid < k ? y * y : 0.0));
+    last = __shfl(y, k - 1, 64);
+  } else {
+    __shared__ DenseLds W;
+    for (int i = tid; i < k; i += PSFUN_BS) W.y[i] = (i == 0) ? 1.0 : 0.0;
+    __syncthreads();
+    block_expmv(k, H, ldh, model, fscale, &W);
+    double ss = 0.0;
+    for (int i = tid; i < k; i += PSFUN_BS) {
+      const double v = W.y[i];
+      ss = fma(v, v, ss);
+      ctl->yk[i] = v;
+    }
+    nrm = sqrt(dense_block_sum(ss, &W));
+    last = W.y[k - 1];
+  }
+  if (tid != 0) return;
+  const double sigma = H[(int64_t)(j - 2) * ldh + (j - 1)];
+  const double errnum = ctl->rs1 * sigma * fabs(last);
+  const double rsj = errnum / nrm;
+  ctl->errnum = errnum;
+  ctl->errden = nrm;
+  ctl->rs[j] = rsj;
+  int st = 0;
+  if (rsj < eps) {
+    ctl->m = k;
+    st = LZ_DONE_CONVERGED;
+  } else if (j >= nl) {
+    st = LZ_DONE_MAXIT;
+  }
+  if (st) {
+    ctl->done = st;
+    host_flag_store(hf + 1, st);
+  }
+}
+
+static int ar_launch_est(psfun_ctx *c, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl, double eps,
+                         double fscale) {
+  if (j - 1 <= PSFUN_WDENSE_K)
+    hipLaunchKernelGGL(k_ar_est<true>, dim3(1), dim3(64), 0, c->stream, ctl, H, ldh, model, j, nl, eps, fscale,
+                       c->d_flags);
+  else
+    hipLaunchKernelGGL(k_ar_est<false>, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, j, nl, eps,
+                       fscale, c->d_flags);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
 }
 
 template <int L, bool SELL>
@@ -95,8 +132,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
       }
       return;
     }
-    __shared__ DenseLds W;
-    ar_control(ctl, H, ldh, model, j, sigma, nl, istop, eps, fscale, &W, hf);
+    ar_control(ctl, H, ldh, j, sigma, nl, istop, hf);
     return;
   }
   if (sigma == 0.0 && j == 1) return;
@@ -133,22 +169,56 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
   if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
 }
 
-// One modified Gram-Schmidt step k of column j (1-based).
+// One modified Gram-Schmidt step k of column j (1-based).  Each thread keeps
+// AR_UB rows' (w, v_k, v_{k+1}) loads in flight and visits its rows in
+// grid-stride order, so the partials do not depend on AR_UB.
+#define AR_UB 4
 __global__ __launch_bounds__(PSFUN_BS) void k_ar_mgs(int n, int k, int j, const double *__restrict__ V, int64_t ldv,
                                                      double *__restrict__ w, const double *__restrict__ p_in, int g_in,
                                                      double *__restrict__ p_out, LzCtl *ctl, double *H, int ldh) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
-  const double h = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) H[(int64_t)(j - 1) * ldh + (k - 1)] = h;  // h(k, j)
   const double *vk = V + (int64_t)(k - 1) * ldv;
   const double *vn = V + (int64_t)k * ldv;  // v_{k+1} (read only when k < j)
   const bool last = (k == j);
+  const int64_t stride = (int64_t)gridDim.x * PSFUN_BS;
+  const int64_t i0 = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  // the first rows' loads are issued before the fold of the partials waits
+  double ww[AR_UB], aa[AR_UB], bb[AR_UB];
+#pragma unroll
+  for (int q = 0; q < AR_UB; ++q) {
+    const int64_t i = i0 + q * stride;
+    ww[q] = aa[q] = bb[q] = 0.0;
+    if (i < n) {
+      ww[q] = w[i];
+      aa[q] = vk[i];
+      if (!last) bb[q] = vn[i];
+    }
+  }
+  const double h = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) H[(int64_t)(j - 1) * ldh + (k - 1)] = h;  // h(k, j)
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    const double x = fma(-h, vk[i], w[i]);
-    w[i] = x;
-    acc = last ? fma(x, x, acc) : fma(vn[i], x, acc);
+  for (int64_t base = i0; base < n; base += AR_UB * stride) {
+    if (base != i0) {
+#pragma unroll
+      for (int q = 0; q < AR_UB; ++q) {
+        const int64_t i = base + q * stride;
+        if (i < n) {
+          ww[q] = w[i];
+          aa[q] = vk[i];
+          if (!last) bb[q] = vn[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < AR_UB; ++q) {
+      const int64_t i = base + q * stride;
+      if (i < n) {
+        const double x = fma(-h, aa[q], ww[q]);
+        w[i] = x;
+        acc = last ? fma(x, x, acc) : fma(bb[q], x, acc);
+      }
+    }
   }
   acc = block_sum<PSFUN_BS>(acc, lds);
   if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
@@ -159,10 +229,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_ctrl(const double *__restrict__
                                                       double *H, int ldh, int model, int nl, int istop, double eps,
                                                       double fscale, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  __shared__ DenseLds W;
   if (ctl->done) return;
   const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
-  ar_control(ctl, H, ldh, model, j, sigma, nl, istop, eps, fscale, &W, hf);
+  ar_control(ctl, H, ldh, j, sigma, nl, istop, hf);
 }
 
 // Coefficients when the loop did not end by convergence:
@@ -331,6 +400,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     PSFUN_TRY(psfun_prof_end(c));
     cur = 1 - cur;
     gcur = Gs;
+    if (istop_k == 1 && j >= 2) PSFUN_TRY(ar_launch_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
     if (fsync && j >= 2) {
       // control index j: yk = f(H(1:j-1,1:j-1)) e1, rs(j) from h(j, j-1)
       bool stopped = false;
@@ -362,6 +432,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model,
                        nl_k, istop_k, eps, fscale, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
+    if (istop_k == 1) PSFUN_TRY(ar_launch_est(c, ctl, H, ldh, model, nl, nl_k, eps, fscale));
     if (fsync) {
       bool stopped = false;
       PSFUN_TRY(userf_eval(c, nl - 1, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc, &stopped));

@@ -152,6 +152,14 @@ static inline int psfun_grid(const psfun_ctx *ctx, int64_t chunks, int per_cu = 
 // Device helpers
 // ---------------------------------------------------------------------------
 
+// LDS written by some lanes of a wavefront, then read by others: order the
+// accesses within the wavefront (no block barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -162,6 +170,30 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
+// Max over the wavefront by DPP row moves (row_shr 1/2/4/8 within each
+// 16-lane row, then row_bcast 15 and 31 across rows), the result read from
+// lane 63: register-to-register steps instead of the cross-lane permutes of
+// __shfl_xor.  Lanes whose DPP source lies outside the row keep their own
+// value, which a max ignores.
+template <int CTRL, int RM, int BM>
+__device__ __forceinline__ double dpp_mov_f64(double old, double src) {
+  const long long o = __double_as_longlong(old), s = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)s, CTRL, RM, BM, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), CTRL, RM, BM, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, dpp_mov_f64<0x111, 0xf, 0xf>(v, v));  // row_shr:1
+  v = fmax(v, dpp_mov_f64<0x112, 0xf, 0xf>(v, v));  // row_shr:2
+  v = fmax(v, dpp_mov_f64<0x114, 0xf, 0xf>(v, v));  // row_shr:4
+  v = fmax(v, dpp_mov_f64<0x118, 0xf, 0xf>(v, v));  // row_shr:8
+  v = fmax(v, dpp_mov_f64<0x142, 0xa, 0xf>(v, v));  // row_bcast:15
+  v = fmax(v, dpp_mov_f64<0x143, 0xc, 0xf>(v, v));  // row_bcast:31
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));

@@ -3,12 +3,12 @@
 //
 // Method (the Taylor scheme of Al-Mohy & Higham, "Computing the action of the
 // matrix exponential", SIAM J. Sci. Comput. 33 (2011)): shift by
-// mu = s*trace(H)/k, split into p substeps with ||B||_1 / p <= 1 for
+// mu = s*trace(H)/k, split into p substeps with ||B||_1 / p <= 2 for
 // B = s*H - mu*I, and on each substep sum the Taylor series of exp(B/p)
 // applied to the current vector until two successive terms fall below
 // 2^-53 of the partial sum (max norms); then multiply by exp(mu/p).  With
-// substep norm <= 1 the terms never exceed the result by more than about e^2,
-// so the cancellation error stays near the unit roundoff.
+// substep norm <= 2 the terms never exceed the result by more than about e^4,
+// so the cancellation error stays within a few dozen unit roundoffs.
 //
 // H is read from memory (column-major, leading dimension ldh) through the
 // caches; how it is read is the variant's matrix model:
@@ -98,7 +98,7 @@ __device__ void block_expmv(int k, const double *H, int ldh, int model, double s
   }
   const double nrm = dense_block_max(cmax, W);
   int p = 1;
-  if (nrm > 1.0) p = (nrm < 1e6) ? (int)ceil(nrm) : 1000000;
+  if (nrm > 2.0) p = (nrm < 2e6) ? (int)ceil(0.5 * nrm) : 1000000;
   const double emu = exp(mu / p);
   const double tiny = 0x1p-53;
   for (int step = 0; step < p; ++step) {
@@ -126,6 +126,74 @@ __device__ void block_expmv(int k, const double *H, int ldh, int model, double s
     for (int i = tid; i < k; i += PSFUN_BS) W->y[i] *= emu;
     __syncthreads();
   }
+}
+
+// The same Taylor scheme by ONE wavefront for k <= 64: lane r owns row r, H
+// sits in LDS (row stride 65, so a column read is conflict-free), and the
+// norms are wavefront reductions, so no step waits on a block barrier.
+#define PSFUN_WDENSE_K 64
+struct WaveDenseLds {
+  double h[PSFUN_WDENSE_K * (PSFUN_WDENSE_K + 1)];
+  double t[2][PSFUN_WDENSE_K];
+};
+
+// y (lane r holds y_r, r < k) <- exp(s*H) y.  Called by the 64 lanes of one
+// wavefront.
+__device__ inline double wave_expmv(int k, const double *H, int ldh, int model, double s, double y,
+                                    WaveDenseLds *L) {
+  const int lane = threadIdx.x & 63;
+  constexpr int LD = PSFUN_WDENSE_K + 1;
+  for (int idx = lane; idx < k * k; idx += 64) {
+    const int r = idx % k, c = idx / k;
+    L->h[r * LD + c] = dense_h(H, ldh, model, r, c);
+  }
+  wave_lds_sync();
+  const double mu = s * wave_sum(lane < k ? L->h[lane * LD + lane] : 0.0) / k;
+  double cs = 0.0;
+  if (lane < k) {
+    for (int r = 0; r < k; ++r) {
+      double b = s * L->h[r * LD + lane];
+      if (r == lane) b -= mu;
+      cs += fabs(b);
+    }
+  }
+  const double nrm = wave_max_dpp(cs);
+  int p = 1;
+  if (nrm > 2.0) p = (nrm < 2e6) ? (int)ceil(0.5 * nrm) : 1000000;
+  const double emu = exp(mu / p);
+  const double tiny = 0x1p-53;
+  const double *hr = L->h + lane * LD;
+  for (int step = 0; step < p; ++step) {
+    int cur = 0;
+    if (lane < k) L->t[0][lane] = y;
+    wave_lds_sync();
+    double prev = INFINITY;
+    for (int m = 1; m <= 64; ++m) {
+      const double *t = L->t[cur];
+      double tn = 0.0;
+      if (lane < k) {
+        double acc = 0.0, acc2 = 0.0;  // two chains: half the dependent latency
+        int c = 0;
+        for (; c + 1 < k; c += 2) {
+          acc = fma(hr[c], t[c], acc);
+          acc2 = fma(hr[c + 1], t[c + 1], acc2);
+        }
+        if (c < k) acc = fma(hr[c], t[c], acc);
+        acc += acc2;
+        tn = fma(s, acc, -mu * t[lane]) / ((double)p * m);
+        y += tn;
+      }
+      const double tm = wave_max_dpp(fabs(tn));
+      const double ym = wave_max_dpp(lane < k ? fabs(y) : 0.0);
+      if (lane < k) L->t[cur ^ 1][lane] = tn;
+      wave_lds_sync();
+      cur ^= 1;
+      if (prev + tm <= tiny * ym) break;
+      prev = tm;
+    }
+    y *= emu;
+  }
+  return y;
 }
 
 // Hi = H_k^{-1} (k x k, leading dimension k) by Gauss-Jordan elimination with

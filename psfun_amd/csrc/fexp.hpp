@@ -32,11 +32,6 @@ struct ChebLds {
   int M;
 };
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Bessel table b_m = e^{-r} I_m(r), m = 0..M, with the factor 2 folded into
 // m >= 1.  Returns M (number of terms - 1).  Lane 0 only.
