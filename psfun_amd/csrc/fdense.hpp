@@ -3,12 +3,15 @@
 //
 // Method (the Taylor scheme of Al-Mohy & Higham, "Computing the action of the
 // matrix exponential", SIAM J. Sci. Comput. 33 (2011)): shift by
-// mu = s*trace(H)/k, split into p substeps with ||B||_1 / p <= 2 for
+// mu = s*trace(H)/k, split into p substeps with ||B||_1 / p <= 1 for
 // B = s*H - mu*I, and on each substep sum the Taylor series of exp(B/p)
 // applied to the current vector until two successive terms fall below
 // 2^-53 of the partial sum (max norms); then multiply by exp(mu/p).  With
-// substep norm <= 2 the terms never exceed the result by more than about e^4,
-// so the cancellation error stays within a few dozen unit roundoffs.
+// substep norm <= 1 the terms never exceed the result by more than about e^2,
+// so the cancellation error stays near the unit roundoff.  (A bound of 2
+// saves a third of the terms but lets the error grow to ~e^4 roundoffs,
+// which the a-posteriori estimate, a component near 1e-14 of ||yk|| at tight
+// tolerances, can no longer resolve.)
 //
 // H is read from memory (column-major, leading dimension ldh) through the
 // caches; how it is read is the variant's matrix model:
@@ -98,7 +101,7 @@ __device__ void block_expmv(int k, const double *H, int ldh, int model, double s
   }
   const double nrm = dense_block_max(cmax, W);
   int p = 1;
-  if (nrm > 2.0) p = (nrm < 2e6) ? (int)ceil(0.5 * nrm) : 1000000;
+  if (nrm > 1.0) p = (nrm < 1e6) ? (int)ceil(nrm) : 1000000;
   const double emu = exp(mu / p);
   const double tiny = 0x1p-53;
   for (int step = 0; step < p; ++step) {
@@ -159,7 +162,7 @@ __device__ inline double wave_expmv(int k, const double *H, int ldh, int model, 
   }
   const double nrm = wave_max_dpp(cs);
   int p = 1;
-  if (nrm > 2.0) p = (nrm < 2e6) ? (int)ceil(0.5 * nrm) : 1000000;
+  if (nrm > 1.0) p = (nrm < 1e6) ? (int)ceil(nrm) : 1000000;
   const double emu = exp(mu / p);
   const double tiny = 0x1p-53;
   const double *hr = L->h + lane * LD;

@@ -170,3 +170,20 @@ def test_serial_variant_matrix_models(gpu_ctx):
         out = np.zeros(k)
         assert fun.apply(H, out, x) == 0
         assert _rel(out, sla.expm(s * Hm) @ x) <= 1e-12, v
+
+
+def test_arnoldi_long_run_block_estimate(gpu_ctx):
+    """exp(-30 L) on 16^3 needs ~90 iterations, so the estimates past order
+    64 take the block f(H) routine instead of the wavefront one.  (At eps
+    1e-12 the estimate compares a component near 1e-14 of ||yk|| with eps,
+    which is the rounding floor of either side's f(H); 1e-10 keeps the
+    iteration counts comparable.)"""
+    dims = (16, 16, 16)
+    A = matgen.laplacian(dims, scale=1.0)
+    x = matgen.uniform_pm1(3, 4096)
+    info, it, err, res, y = _arnoldi_gpu(A, x, 1e-10, 150, scaling=-30.0, variant="GENPADE")
+    assert info == 0 and it > 66
+    ref = O.arnoldi(A, x, 1e-10, 150, 1, scaling=-30.0)
+    assert abs(it - ref["iter"]) <= 1
+    assert _rel(y, ref["y"]) <= 1e-10
+    assert _rel(y, matgen.dst_expm_laplacian(dims, x, 30.0)) <= 1e-9
