@@ -27,6 +27,15 @@ __global__ __launch_bounds__(PSFUN_BS) void k_serial_dense(int k, const double *
   for (int i = threadIdx.x; i < k; i += PSFUN_BS) y[i] = W.y[i];
 }
 
+// The same for k <= 64 by one wavefront (wave_expmv).
+__global__ __launch_bounds__(64) void k_serial_dense_wave(int k, const double *H, int model, double s,
+                                                         const double *x, double *y) {
+  __shared__ WaveDenseLds L;
+  const int lane = threadIdx.x;
+  const double v = wave_expmv(k, H, k, model, s, lane < k ? x[lane] : 0.0, &L);
+  if (lane < k) y[lane] = v;
+}
+
 int launch_serial_tridiag(psfun_ctx *c, int k, const double *d, const double *e, double s, const double *x,
                           double *y) {
   hipLaunchKernelGGL(k_serial_tridiag, dim3(1), dim3(64), 0, c->stream, k, d, e, s, x, y);
@@ -97,7 +106,10 @@ extern "C" int psfun_serial_apply_host(psfun_ctx_t c, int fname, int variant, do
     double *dx = dH + hk, *dy = dx + k;
     PSFUN_HIP_TRY(hipMemcpyAsync(dH, H, sizeof(double) * hk, hipMemcpyHostToDevice, c->stream));
     PSFUN_HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_serial_dense, dim3(1), dim3(PSFUN_BS), 0, c->stream, k, dH, model, scaling, dx, dy);
+    if (k <= PSFUN_WDENSE_K)
+      hipLaunchKernelGGL(k_serial_dense_wave, dim3(1), dim3(64), 0, c->stream, k, dH, model, scaling, dx, dy);
+    else
+      hipLaunchKernelGGL(k_serial_dense, dim3(1), dim3(PSFUN_BS), 0, c->stream, k, dH, model, scaling, dx, dy);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
   }

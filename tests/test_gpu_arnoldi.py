@@ -187,3 +187,20 @@ def test_arnoldi_long_run_block_estimate(gpu_ctx):
     assert abs(it - ref["iter"]) <= 1
     assert _rel(y, ref["y"]) <= 1e-10
     assert _rel(y, matgen.dst_expm_laplacian(dims, x, 30.0)) <= 1e-9
+
+
+@pytest.mark.parametrize("k", [64, 65, 512])
+def test_serial_dense_largest_orders(gpu_ctx, k):
+    """The dense f(H) step at the wavefront/block boundary and at the
+    largest order the C-ABI takes (512), against scipy's expm."""
+    rng = np.random.default_rng(600 + k)
+    H = rng.normal(size=(k, k)) * (2.0 / np.sqrt(k))
+    x = rng.normal(size=k)
+    for v in ("GENPADE", "CHBHES"):
+        fun = psfun_d_serial()
+        fun.set("VARIANT", v)
+        fun.set("SCALING", -0.5)
+        out = np.zeros(k)
+        assert fun.apply(H, out, x) == 0
+        Hm = H if v == "GENPADE" else np.triu(H, -1)
+        assert _rel(out, sla.expm(-0.5 * Hm) @ x) <= 1e-12, v

@@ -16,7 +16,7 @@ def _graphs(sizes, seed):
     return [matgen.rgg_laplacian(int(n), seed + i) for i, n in enumerate(sizes)]
 
 
-@pytest.mark.parametrize("K", [1, 3, 16])
+@pytest.mark.parametrize("K", [1, 3, 16, 64])
 def test_batched_matches_oracle_per_instance(gpu_ctx, K):
     sizes = [50, 300, 1000, 37, 2000, 5]
     mats = _graphs(sizes, 100)

@@ -213,6 +213,21 @@ def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     assert len(res) == it + 1 and err < 1e-10
 
 
+@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+def test_lanczos_long_run_lds_fT(gpu_ctx, fmt):
+    """exp(-30 L) on 16^3 takes ~90 iterations: the loop runs with itmax > 65,
+    so the control block's f(T) step is the LDS routine for orders past 64."""
+    dims = (16, 16, 16)
+    A = matgen.laplacian(dims, scale=-30.0)
+    x = matgen.uniform_pm1(8, 4096)
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, 200, fmt=fmt)
+    assert info == 0 and it > 66
+    ref = O.lanczos(A, x, 1e-10, 200, 1)
+    assert abs(it - ref["iter"]) <= 1
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    assert np.linalg.norm(y - matgen.dst_expm_laplacian(dims, x, 30.0)) <= 1e-9 * np.linalg.norm(y)
+
+
 def test_lanczos_reference_example(gpu_ctx):
     g = load_golden("laplacian50.npz")
     A = (g["rowptr"] - 1, g["colind"] - 1, g["val"])
