@@ -25,6 +25,7 @@ info = psb_err_from_subroutine_ with a message on stderr.
 mm_mat_read(filename, ctxt=None) -> (a, info)    a: psb.psb_dspmat_type or None
 mm_mat_read_csr(filename)        -> (rowptr, colind, val, nrows, ncols)   (host, 0-based)
 mm_array_read(filename)          -> (x, info)    x: numpy float64 (M,) or (M, N)
+mm_array_write(x, title, filename) -> info
 """
 import sys
 
@@ -136,6 +137,25 @@ def mm_array_read(filename):
         return None, psb.psb_err_from_subroutine_
     x = vals[: m * n].reshape((n, m)).T
     return (x[:, 0].copy() if n == 1 else np.ascontiguousarray(x)), psb.psb_success_
+
+
+def mm_array_write(x, title, filename):
+    """Dense vector or column-major matrix as a MatrixMarket array file (the
+    serial driver's mm_array_write(y, title, info, iunit, filename)); returns
+    info."""
+    x = np.asarray(x, dtype=np.float64)
+    X = x.reshape(-1, 1) if x.ndim == 1 else x
+    try:
+        with open(filename, "w") as fh:
+            fh.write("%%MatrixMarket matrix array real general\n")
+            fh.write(f"% {title}\n")
+            fh.write(f"{X.shape[0]} {X.shape[1]}\n")
+            for v in X.T.reshape(-1):
+                fh.write(f"{v:.17e}\n")
+    except OSError as e:
+        sys.stderr.write(f"mm_array_write: {e}\n")
+        return psb.psb_err_from_subroutine_
+    return psb.psb_success_
 
 
 def hb_read(filename, ctxt=None):
