@@ -6,6 +6,7 @@ examples/shiftedkrylov/shifted.inp):
   python -m psfun_amd.drivers arnoldi  [deck]     (arnolditest.F90)
   python -m psfun_amd.drivers serial   [deck]     (serialtest.f90)
   python -m psfun_amd.drivers shifted  [deck]     (shiftedtest.f90)
+  python -m psfun_amd.drivers quadrature           (quadraturetest.F90: the poles)
 
 As the Fortran programs do, the deck comes from the file named as the first
 argument or else from standard input: a count line, then one value per line
@@ -221,11 +222,29 @@ def shifted_driver(deck, out=sys.stdout):
             "residual_unshifted": resmx, "residual_shifted": ressys}
 
 
+def quadrature_driver(out=sys.stdout):
+    """quadraturetest.F90: the N = 10 poles of HHT method 1 for
+    f(z) = sqrt(1 + z) on [0.01, 4] (lines 60-67), listed (the reference
+    plots them through gnuplot)."""
+    from . import quadrature as Q
+    q = Q.psfun_z_quadrature()
+    info = q.computepoles(Q.hhtmethod1, lambda z: np.sqrt(1.0 + z), 10, rparams=[0.01, 4.0])
+    w = out.write
+    w(f"HHT method 1, N = 10, [0.01, 4], f(z) = sqrt(1+z): eta = {q.eta:.16e}, sign = {q.sign:+.1f}\n")
+    w(f"{'j':>3s} {'Re xi':>24s} {'Im xi':>24s} {'Re c':>24s} {'Im c':>24s}\n")
+    for j, (xi, c) in enumerate(zip(q.xi, q.c), 1):
+        w(f"{j:3d} {xi.real:24.16e} {xi.imag:24.16e} {c.real:24.16e} {c.imag:24.16e}\n")
+    return {"info": info, "xi": np.asarray(q.xi), "c": np.asarray(q.c), "eta": q.eta, "sign": q.sign}
+
+
 def main(argv=None):
     argv = list(sys.argv[1:] if argv is None else argv)
-    if not argv or argv[0] not in ("lanczos", "arnoldi", "serial", "shifted"):
-        sys.stderr.write("usage: python -m psfun_amd.drivers {lanczos|arnoldi|serial|shifted} [deck]\n")
+    if not argv or argv[0] not in ("lanczos", "arnoldi", "serial", "shifted", "quadrature"):
+        sys.stderr.write("usage: python -m psfun_amd.drivers {lanczos|arnoldi|serial|shifted} [deck]\n"
+                         "       python -m psfun_amd.drivers quadrature\n")
         return 2
+    if argv[0] == "quadrature":
+        return 0 if quadrature_driver()["info"] == 0 else 1
     deck = argv[1] if len(argv) > 1 else sys.stdin
     try:
         if argv[0] in ("lanczos", "arnoldi"):

@@ -110,3 +110,15 @@ def test_driver_command_line(gpu_ctx, tmp_path):
                        capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Number of iterations" in r.stdout and "Info  on exit" in r.stdout
+
+
+def test_quadrature_driver_poles():
+    """quadraturetest.F90's poles (N = 10, [0.01, 4], sqrt(1+z)) equal the
+    oracle's HHT method 1."""
+    from oracle import oracle as O
+    r = drivers.quadrature_driver(out=io.StringIO())
+    xi, c, eta, sign = O.hhtmethod1(lambda z: np.sqrt(1.0 + z), 10, [0.01, 4.0])
+    assert r["info"] == 0
+    assert np.allclose(r["xi"], xi, rtol=1e-13, atol=0)
+    assert np.allclose(r["c"], c, rtol=1e-12, atol=0)
+    assert abs(r["eta"] - eta) <= 1e-14 * abs(eta) and r["sign"] == sign == -1.0
