@@ -288,3 +288,50 @@ def test_lanczos_full_size_3d_128(gpu_ctx):
     yex = matgen.dst_expm_laplacian(dims, x, 1.0)
     assert info == 0
     assert np.linalg.norm(y - yex) / np.linalg.norm(yex) <= 1e-10
+
+
+def test_two_contexts_in_two_threads(gpu_ctx):
+    """One context per host thread, each on its own stream, running at the
+    same time (the ctypes calls release the GIL): every result equals the
+    single-context run bitwise (contexts share no state)."""
+    import threading
+    dims = (40, 40, 40)
+    A = matgen.laplacian(dims, scale=-1.0)
+    n = 64000
+    xs = [matgen.uniform_pm1(50 + i, n) for i in range(2)]
+    refs = [_lanczos_gpu(A, x, 1e-10, 50)[4] for x in xs]
+    out, errors = {}, []
+
+    def work(i):
+        try:
+            ctx = psb.Context(0)
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                M = psb.psb_dspmat_type.from_csr(A, ctx)
+                fun = psfun_d_serial()
+                fun.set("VARIANT", "CHBSYM")
+                km = psfun_d_krylov()
+                km.set("KNAME", "LANCZOS")
+                x = _vec(xs[i])
+                ys = []
+                for _ in range(6):
+                    y = _vec(np.zeros(n))
+                    info, it, err, res = km.apply(fun, M, psb.psb_desc_type(n, ctx), y, x, 1e-10, itmax=50)
+                    assert info == 0
+                    ys.append(y)
+                s.synchronize()
+                out[i] = [y.get_vect() for y in ys]
+            ctx.close()
+        except BaseException as e:  # re-raised in the main thread
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(2):
+        assert len(out[i]) == 6
+        for y in out[i]:
+            assert np.array_equal(y, refs[i])
