@@ -268,14 +268,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_assemble(int n, const double *_
   __shared__ double yk[LZ_NL_MAX];
   for (int l = threadIdx.x; l < m; l += PSFUN_BS) yk[l] = ctl->yk[l];
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    double acc = 0.0;
-    if (m > 0) {
-      acc = yk[0] * V[i];
-      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * ldv + i], acc);
-    }
-    y[i] = rs1 * acc;
-  }
+  basis_combine<false>(n, V, ldv, m, yk, nullptr, rs1, y);
 }
 
 __global__ void k_ar_init(LzCtl *ctl) {

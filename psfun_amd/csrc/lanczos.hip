@@ -339,15 +339,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_assemble(int n, const double *_
     ib[l] = lz_inv_beta(ctl, l);  // v_{l+1} = w_l * ib[l]
   }
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    double acc = 0.0;
-    if (m > 0) {
-      acc = yk[0] * (V[i] * ib[0]);
-#pragma unroll 8
-      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * ldv + i] * ib[l], acc);
-    }
-    y[i] = rs1 * acc;
-  }
+  basis_combine<true>(n, V, ldv, m, yk, ib, rs1, y);
 }
 
 __global__ void k_lz_init(LzCtl *ctl) {
