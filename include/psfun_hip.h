@@ -110,7 +110,7 @@ enum {
   PSFUN_PROF_BATCH = 7,      /* batched Lanczos step kernels             */
   PSFUN_PROF_SDDMM = 8,      /* gradient SDDMM                           */
   PSFUN_PROF_AR_SPMV = 9,    /* Arnoldi SpMV step (+ control block)      */
-  PSFUN_PROF_AR_MGS = 10,    /* Arnoldi modified Gram-Schmidt step       */
+  PSFUN_PROF_AR_MGS = 10,    /* Arnoldi Gram-Schmidt of one column       */
   PSFUN_PROF_REDUCE = 11,    /* dot / nrm2 / amax partial sums           */
   PSFUN_PROF_AXPBY = 12      /* axpby                                    */
 };

@@ -11,13 +11,13 @@
 // Device schedule per column j = 1, 2, ...:
 //   k_ar_spmv(j):   all blocks fold the ||w||^2 partials into h(j,j-1) (||x||
 //                   for j = 1), write v_j = (1/h) w once and u = scaling*A v_j
-//                   (gathering w*(1/h) on the fly), and leave partials of
-//                   <v_1, u>.  Block 0 is the control block: it runs the
+//                   (gathering w*(1/h) on the fly).  Block 0 is the control block: it runs the
 //                   reference's control for loop index i = j (the a-posteriori
 //                   estimate from f(H_{j-1}) e1, by the dense block exp of
 //                   fdense.hpp) while the streaming blocks work.
-//   k_ar_mgs(k), k = 1..j: fold <v_k, w> into h(k,j), w -= h(k,j) v_k in place,
-//                   leave partials of <v_{k+1}, w> (k < j) or ||w||^2 (k = j).
+//   k_ar_gram, k_ar_hsolve, k_ar_lsu: the column's modified Gram-Schmidt in
+//                   its one-reduction form (below): h(1:j, j), w in place of
+//                   u, partials of ||w||^2.
 // Scalars cross kernels through kernel boundaries only; the control block
 // stores its loop index and stop state to host-visible flags on which the
 // host paces its launches (as in the Lanczos path).
@@ -94,15 +94,31 @@ __global__ __launch_bounds__(WAVE ? 64 : PSFUN_BS) void k_ar_est(LzCtl *ctl, con
   }
 }
 
-static int ar_launch_est(psfun_ctx *c, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl, double eps,
-                         double fscale) {
+static int ar_launch_est(psfun_ctx *c, hipStream_t st, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl,
+                         double eps, double fscale) {
   if (j - 1 <= PSFUN_WDENSE_K)
-    hipLaunchKernelGGL(k_ar_est<true>, dim3(1), dim3(64), 0, c->stream, ctl, H, ldh, model, j, nl, eps, fscale,
-                       c->d_flags);
+    hipLaunchKernelGGL(k_ar_est<true>, dim3(1), dim3(64), 0, st, ctl, H, ldh, model, j, nl, eps, fscale, c->d_flags);
   else
-    hipLaunchKernelGGL(k_ar_est<false>, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, j, nl, eps,
-                       fscale, c->d_flags);
+    hipLaunchKernelGGL(k_ar_est<false>, dim3(1), dim3(PSFUN_BS), 0, st, ctl, H, ldh, model, j, nl, eps, fscale,
+                       c->d_flags);
   PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+// The estimate of loop index j reads only H(1:j-1, 1:j-1) and h(j, j-1),
+// which the orthogonalisation of column j does not write, so it runs on the
+// auxiliary stream beside that orthogonalisation: forked after the SpMV step
+// of index j, joined before the next SpMV step (which then sees a stop).
+static int ar_fork_est(psfun_ctx *c, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl, double eps,
+                       double fscale) {
+  PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+  PSFUN_HIP_TRY(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
+  PSFUN_TRY(ar_launch_est(c, c->aux_stream, ctl, H, ldh, model, j, nl, eps, fscale));
+  PSFUN_HIP_TRY(hipEventRecord(c->ev_join, c->aux_stream));
+  return PSFUN_SUCCESS;
+}
+static int ar_join_est(psfun_ctx *c) {
+  PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   return PSFUN_SUCCESS;
 }
 
@@ -112,7 +128,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
                                                       const double *__restrict__ src, int j, int G,
                                                       const double *__restrict__ p_in, int g_in,
                                                       double *__restrict__ V, int64_t ldv, double *__restrict__ u_out,
-                                                      double *__restrict__ p_out, LzCtl *ctl, double *H, int ldh,
+                                                      LzCtl *ctl, double *H, int ldh,
                                                       int model, int nl, int istop, double eps, double fscale,
                                                       int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
@@ -142,7 +158,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
   const int64_t nchunks = (n + RPB - 1) / RPB;
   const XcdRange xr = xcd_range_c0(nchunks, G);
   const int sub = SELL ? threadIdx.x : threadIdx.x / L, lane = SELL ? 0 : threadIdx.x % L;
-  double dot = 0.0;
   for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
     const int row = (int)(ch * RPB + sub);
     double sum = 0.0;
@@ -161,67 +176,229 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
       const double u = scaling * sum;
       vj[row] = v;
       u_out[row] = u;
-      const double v1 = (j == 1) ? v : V[row];
-      dot = fma(v1, u, dot);
     }
   }
-  dot = block_sum<PSFUN_BS>(dot, lds);
-  if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
 }
 
-// One modified Gram-Schmidt step k of column j (1-based).  Each thread keeps
-// AR_UB rows' (w, v_k, v_{k+1}) loads in flight and visits its rows in
-// grid-stride order, so the partials do not depend on AR_UB.
-#define AR_UB 4
-__global__ __launch_bounds__(PSFUN_BS) void k_ar_mgs(int n, int k, int j, const double *__restrict__ V, int64_t ldv,
-                                                     double *__restrict__ w, const double *__restrict__ p_in, int g_in,
-                                                     double *__restrict__ p_out, LzCtl *ctl, double *H, int ldh) {
-  __shared__ double lds[PSFUN_BS / 64 + 1];
+// ---------------------------------------------------------------------------
+// Orthogonalisation of column j against v_1..v_j.  The reference runs modified
+// Gram-Schmidt (psfun_d_arnoldi.f90:209-213, psfun_d_sai.f90 likewise):
+// for k = 1..j: h(k,j) = <v_k, w>; w -= h(k,j) v_k.  Substituting the earlier
+// updates into each inner product gives
+//   h(k,j) = <v_k, u> - sum_{i<k} <v_k, v_i> h(i,j)      (u = the SpMV output),
+// i.e. h = (I + L)^{-1} V^T u with L the strictly lower part of the basis Gram
+// matrix: the "inverse compact WY" form of MGS (Swirydowicz, Langou,
+// Ananthan, Yang, Thomas, "Low synchronization Gram-Schmidt and generalized
+// minimal residual algorithms", Numer. Linear Algebra Appl. 28 (2021) e2343),
+// which keeps MGS's bound on the loss of orthogonality.  One reduction per
+// column instead of j:
+//   k_ar_gram:   partials of <v_l, u> (l = 1..j) and <v_l, v_j> (l < j);
+//   k_ar_hsolve: one block folds them in a fixed order, keeps row j of the
+//                Gram matrix and solves (I + L) h = V^T u by forward
+//                substitution; h is column j of H;
+//   k_ar_lsu:    w = u - h(1,j) v_1 - ... - h(j,j) v_j, row by row in that
+//                order (the reference's sequence of updates), and the
+//                partials of ||w||^2.
+// Each basis vector is read twice per column instead of twice per MGS step.
+// ---------------------------------------------------------------------------
+
+// Partial sums of one gram launch over basis vectors l0 <= l < l1.  A block
+// takes tiles of AR_GR*PSFUN_BS rows (u and v_j of the tile in registers),
+// streams v_l of the tile for every l (the next v_l's loads issued before the
+// current one is reduced), reduces both products over the wavefront and adds
+// them to per-wave LDS sums; the block's sums go to the partials, list-major:
+// list l-1 holds <v_l, u>, list j+l-1 holds <v_l, v_j>.  Fixed order
+// throughout, so the partials do not depend on timing.
+#define AR_GR 8
+#define AR_LG 128
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_gram(int n, const double *__restrict__ V, int64_t ldv, int j,
+                                                      int l0, int l1, const double *__restrict__ u,
+                                                      double *__restrict__ P, int Gg, const LzCtl *ctl) {
+  __shared__ double az[PSFUN_BS / 64][AR_LG], ag[PSFUN_BS / 64][AR_LG];
   if (ctl->done) return;
-  const double *vk = V + (int64_t)(k - 1) * ldv;
-  const double *vn = V + (int64_t)k * ldv;  // v_{k+1} (read only when k < j)
-  const bool last = (k == j);
-  const int64_t stride = (int64_t)gridDim.x * PSFUN_BS;
-  const int64_t i0 = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
-  // the first rows' loads are issued before the fold of the partials waits
-  double ww[AR_UB], aa[AR_UB], bb[AR_UB];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nv = l1 - l0;
+  for (int i = lane; i < nv; i += 64) {
+    az[wv][i] = 0.0;
+    ag[wv][i] = 0.0;
+  }
+  wave_lds_sync();  // lane 0 of each wave owns its row of sums from here on
+  const double *vj = V + (int64_t)(j - 1) * ldv;
+  constexpr int64_t TILE = (int64_t)AR_GR * PSFUN_BS;
+  const int64_t ntiles = ((int64_t)n + TILE - 1) / TILE;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t r0 = t * TILE + threadIdx.x;
+    double uu[AR_GR], jj[AR_GR], a[AR_GR];
 #pragma unroll
-  for (int q = 0; q < AR_UB; ++q) {
-    const int64_t i = i0 + q * stride;
-    ww[q] = aa[q] = bb[q] = 0.0;
-    if (i < n) {
-      ww[q] = w[i];
-      aa[q] = vk[i];
-      if (!last) bb[q] = vn[i];
+    for (int q = 0; q < AR_GR; ++q) {
+      const int64_t r = r0 + q * PSFUN_BS;
+      uu[q] = r < n ? u[r] : 0.0;
+      jj[q] = r < n ? vj[r] : 0.0;
+      a[q] = r < n ? V[(int64_t)(l0 - 1) * ldv + r] : 0.0;
+    }
+    for (int l = l0; l < l1; ++l) {
+      double b[AR_GR];
+      const double *vn = V + (int64_t)l * ldv;  // v_{l+1}
+#pragma unroll
+      for (int q = 0; q < AR_GR; ++q) {
+        const int64_t r = r0 + q * PSFUN_BS;
+        b[q] = (l + 1 < l1 && r < n) ? vn[r] : 0.0;
+      }
+      double z = 0.0, g = 0.0;
+#pragma unroll
+      for (int q = 0; q < AR_GR; ++q) {
+        z = fma(a[q], uu[q], z);
+        g = fma(a[q], jj[q], g);
+      }
+      z = wave_sum_dpp(z);
+      g = wave_sum_dpp(g);
+      if (lane == 0) {
+        az[wv][l - l0] += z;
+        ag[wv][l - l0] += g;
+      }
+#pragma unroll
+      for (int q = 0; q < AR_GR; ++q) a[q] = b[q];
     }
   }
-  const double h = block_sum_partials<PSFUN_BS>(p_in, g_in, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) H[(int64_t)(j - 1) * ldh + (k - 1)] = h;  // h(k, j)
+  __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += PSFUN_BS) {
+    const int l = l0 + i;
+    double z = 0.0, g = 0.0;
+#pragma unroll
+    for (int w = 0; w < PSFUN_BS / 64; ++w) {
+      z += az[w][i];
+      g += ag[w][i];
+    }
+    P[(int64_t)(l - 1) * Gg + blockIdx.x] = z;
+    if (l < j) P[(int64_t)(j + l - 1) * Gg + blockIdx.x] = g;
+  }
+}
+
+// Fold of the gram partials and the triangular solve for h(1:j, j), one block.
+// Gm keeps the Gram matrix's strictly lower part, G(k,i) = <v_k, v_i> at
+// Gm[(i-1)*ldg + (k-1)] (row k written at column k).  Forward substitution by
+// columns in one wavefront: h_i = z_i - acc_i, then acc_k += G(k,i) h_i for
+// k > i, the Gram columns staged through LDS AR_HS_STAGE doubles at a time.
+#define AR_HS_BS 1024
+#define AR_HS_STAGE 4096
+__global__ __launch_bounds__(AR_HS_BS) void k_ar_hsolve(int j, const double *__restrict__ P, int Gg,
+                                                        double *__restrict__ Gm, int ldg, const LzCtl *ctl, double *H,
+                                                        int ldh) {
+  __shared__ double z[LZ_NL_MAX], gj[LZ_NL_MAX], acc[LZ_NL_MAX], h[LZ_NL_MAX];
+  __shared__ double gs[AR_HS_STAGE];
+  if (ctl->done) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = AR_HS_BS / 64;
+  const int nlist = 2 * j - 1;
+  for (int q = wv; q < nlist; q += NW) {
+    const double *p = P + (int64_t)q * Gg;
+    double sum = 0.0;
+    for (int b = lane; b < Gg; b += 64) sum += p[b];
+    sum = wave_sum_dpp(sum);
+    if (lane == 0) {
+      if (q < j)
+        z[q] = sum;
+      else
+        gj[q - j] = sum;
+    }
+  }
+  for (int k = threadIdx.x; k < j; k += AR_HS_BS) acc[k] = 0.0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < j - 1; q += AR_HS_BS) Gm[(int64_t)q * ldg + (j - 1)] = gj[q];  // row j
+  const int cw = (j * j <= AR_HS_STAGE) ? j : AR_HS_STAGE / j;
+  for (int i0 = 0; i0 < j; i0 += cw) {
+    const int i1 = min(j, i0 + cw);
+    for (int e = threadIdx.x; e < (i1 - i0) * j; e += AR_HS_BS) {
+      const int i = i0 + e / j, k = e % j;
+      double g = 0.0;
+      if (k > i) g = (k == j - 1) ? gj[i] : Gm[(int64_t)i * ldg + k];
+      gs[e] = g;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      for (int i = i0; i < i1; ++i) {
+        const double hi = z[i] - acc[i];
+        if (lane == 0) h[i] = hi;
+        for (int k = i + 1 + lane; k < j; k += 64) acc[k] = fma(gs[(i - i0) * j + k], hi, acc[k]);
+        wave_lds_sync();
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < j; k += AR_HS_BS) H[(int64_t)(j - 1) * ldh + k] = h[k];  // h(k+1, j)
+}
+
+// w = u - sum_{l=1..j} h(l,j) v_l in place (row by row, l ascending) and the
+// partials of ||w||^2.  Rows in pairs (16-byte loads; w and the slots are
+// 16-byte aligned), the basis with nontemporal loads; the odd last row goes
+// to thread 0 of the last block.
+__global__ __launch_bounds__(PSFUN_BS) void k_ar_lsu(int n, int j, const double *__restrict__ V, int64_t ldv,
+                                                     double *__restrict__ w, const double *__restrict__ H, int ldh,
+                                                     double *__restrict__ p_out, const LzCtl *ctl) {
+  __shared__ double h[LZ_NL_MAX];
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  for (int k = threadIdx.x; k < j; k += PSFUN_BS) h[k] = H[(int64_t)(j - 1) * ldh + k];
+  __syncthreads();
+  const int64_t n2 = n >> 1;
+  psfun_d2 *w2 = (psfun_d2 *)w;
   double acc = 0.0;
-  for (int64_t base = i0; base < n; base += AR_UB * stride) {
-    if (base != i0) {
-#pragma unroll
-      for (int q = 0; q < AR_UB; ++q) {
-        const int64_t i = base + q * stride;
-        if (i < n) {
-          ww[q] = w[i];
-          aa[q] = vk[i];
-          if (!last) bb[q] = vn[i];
-        }
-      }
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n2; i += (int64_t)gridDim.x * PSFUN_BS) {
+    psfun_d2 x = w2[i];
+#pragma unroll 8
+    for (int l = 0; l < j; ++l) {
+      const psfun_d2 v = __builtin_nontemporal_load((const psfun_d2 *)(V + (int64_t)l * ldv) + i);
+      x.x = fma(-h[l], v.x, x.x);
+      x.y = fma(-h[l], v.y, x.y);
     }
-#pragma unroll
-    for (int q = 0; q < AR_UB; ++q) {
-      const int64_t i = base + q * stride;
-      if (i < n) {
-        const double x = fma(-h, aa[q], ww[q]);
-        w[i] = x;
-        acc = last ? fma(x, x, acc) : fma(bb[q], x, acc);
-      }
-    }
+    w2[i] = x;
+    acc = fma(x.x, x.x, acc);
+    acc = fma(x.y, x.y, acc);
+  }
+  if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    const int64_t i = n - 1;
+    double x = w[i];
+    for (int l = 0; l < j; ++l) x = fma(-h[l], V[(int64_t)l * ldv + i], x);
+    w[i] = x;
+    acc = fma(x, x, acc);
   }
   acc = block_sum<PSFUN_BS>(acc, lds);
   if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
+}
+
+// Host side: orthogonalise the device vector w (u on entry) against basis
+// slots 1..j; leaves ||w||^2 partials (gl of them) in p_w.
+struct ArOrtho {
+  double *Pg;   // gram partials, 2*nl lists of Gg
+  double *Gm;   // Gram matrix, nl x nl
+  int Gg, Gl, ldg;
+};
+static size_t ar_ortho_bytes(const psfun_ctx *c, int n, int nl) {
+  const int64_t tiles = ((int64_t)n + AR_GR * PSFUN_BS - 1) / (AR_GR * PSFUN_BS);
+  const int Gg = psfun_grid(c, tiles, 4);
+  return align_up(sizeof(double) * 2 * (size_t)nl * Gg, 256) + align_up(sizeof(double) * (size_t)nl * nl, 256);
+}
+static ArOrtho ar_ortho_carve(const psfun_ctx *c, void *base, int n, int nl) {
+  ArOrtho o;
+  const int64_t tiles = ((int64_t)n + AR_GR * PSFUN_BS - 1) / (AR_GR * PSFUN_BS);
+  o.Gg = psfun_grid(c, tiles, 4);
+  o.Gl = psfun_grid(c, ((int64_t)(n >> 1) + PSFUN_BS - 1) / PSFUN_BS, 4);
+  o.ldg = nl;
+  o.Pg = (double *)base;
+  o.Gm = (double *)((char *)base + align_up(sizeof(double) * 2 * (size_t)nl * o.Gg, 256));
+  return o;
+}
+static int ar_orthogonalize(psfun_ctx *c, const ArOrtho &o, int n, int j, const double *V, int64_t ldv, double *w,
+                            double *p_w, LzCtl *ctl, double *H, int ldh) {
+  for (int l0 = 1; l0 <= j; l0 += AR_LG) {
+    const int l1 = min(j + 1, l0 + AR_LG);
+    hipLaunchKernelGGL(k_ar_gram, dim3(o.Gg), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, j, l0, l1, w, o.Pg, o.Gg,
+                       ctl);
+  }
+  hipLaunchKernelGGL(k_ar_hsolve, dim3(1), dim3(AR_HS_BS), 0, c->stream, j, o.Pg, o.Gg, o.Gm, o.ldg, ctl, H, ldh);
+  hipLaunchKernelGGL(k_ar_lsu, dim3(o.Gl), dim3(PSFUN_BS), 0, c->stream, n, j, V, ldv, w, H, ldh, p_w, ctl);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
 }
 
 // Control at the last loop index i = nl (no SpMV needed there).
@@ -339,14 +516,15 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   const int64_t ldv = (int64_t)align_up((size_t)n, 64);
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
   const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
-  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)(nl + 2);  // v_1..v_{nl-1}, u0, u1
+  const size_t vbytes = align_up(sizeof(double) * (size_t)ldv * (size_t)(nl + 2), 256);  // v_1..v_{nl-1}, u0, u1
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace(c, ctl_bytes + h_bytes + vbytes, &ws));
+  PSFUN_TRY(psfun_workspace(c, ctl_bytes + h_bytes + vbytes + ar_ortho_bytes(c, n, nl), &ws));
   LzCtl *ctl = (LzCtl *)ws;
   double *H = (double *)((char *)ws + ctl_bytes);
   double *V = (double *)((char *)ws + ctl_bytes + h_bytes);
   double *U[2] = {V + (size_t)ldv * nl, V + (size_t)ldv * (nl + 1)};
   double *P[2] = {c->d_partials, c->d_partials + PSFUN_MAX_GRID};
+  const ArOrtho orth = ar_ortho_carve(c, (char *)ws + ctl_bytes + h_bytes + vbytes, n, nl);
 
   const bool sell = A->format == PSFUN_FORMAT_SELL;
   const int L = sell ? 1 : A->lanes;
@@ -368,16 +546,16 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   for (int j = 1; j <= nl - 1; ++j) {
     const double *src = (j == 1) ? x : U[j & 1];
     double *u = U[(j + 1) & 1];
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_SPMV, spmv_bytes + (j > 1 ? 8.0 * n : 0.0), j));
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_SPMV, spmv_bytes, j));
     if (sell) {
       hipLaunchKernelGGL((k_ar_spmv<1, true>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->sell_ptr,
-                         A->sell_col, A->sell_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H,
-                         ldh, model, nl_k, istop_k, eps, fscale, c->d_flags);
+                         A->sell_col, A->sell_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, ctl, H, ldh,
+                         model, nl_k, istop_k, eps, fscale, c->d_flags);
     } else {
 #define AR_LAUNCH(LL)                                                                                               \
   hipLaunchKernelGGL((k_ar_spmv<LL, false>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, \
-                     A->val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, P[1 - cur], ctl, H, ldh, model, nl_k,   \
-                     istop_k, eps, fscale, c->d_flags)
+                     A->val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, ctl, H, ldh, model, nl_k, istop_k, eps, \
+                     fscale, c->d_flags)
       switch (L) {
         case 1: AR_LAUNCH(1); break;
         case 2: AR_LAUNCH(2); break;
@@ -391,9 +569,8 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     }
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
-    cur = 1 - cur;
-    gcur = Gs;
-    if (istop_k == 1 && j >= 2) PSFUN_TRY(ar_launch_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
+    const bool forked = istop_k == 1 && j >= 2;
+    if (forked) PSFUN_TRY(ar_fork_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
     if (fsync && j >= 2) {
       // control index j: yk = f(H(1:j-1,1:j-1)) e1, rs(j) from h(j, j-1)
       bool stopped = false;
@@ -404,15 +581,14 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
       }
       PSFUN_TRY(userf_launch_est(c, ctl, j, H + (size_t)(j - 2) * ldh + (j - 1), sc, nl, eps));
     }
-    for (int k = 1; k <= j; ++k) {
-      PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_MGS, (k < j ? 32.0 : 24.0) * n, j));
-      hipLaunchKernelGGL(k_ar_mgs, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, k, j, V, ldv, u, P[cur], gcur,
-                         P[1 - cur], ctl, H, ldh);
-      PSFUN_HIP_TRY(hipGetLastError());
-      PSFUN_TRY(psfun_prof_end(c));
-      cur = 1 - cur;
-      gcur = Gu;
-    }
+    // orthogonalise u against v_1..v_j: gram pass (u and j slots read),
+    // update pass (j slots and u read, w written)
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_MGS, (2.0 * j + 3.0) * 8.0 * n, j));
+    PSFUN_TRY(ar_orthogonalize(c, orth, n, j, V, ldv, u, P[1 - cur], ctl, H, ldh));
+    PSFUN_TRY(psfun_prof_end(c));
+    if (forked) PSFUN_TRY(ar_join_est(c));
+    cur = 1 - cur;
+    gcur = orth.Gl;
     if (!fsync && j >= 2 && j < nl - 1) {
       PSFUN_TRY(psfun_host_wait(c, j - 1));
       if (c->h_flags[1] != 0) {
@@ -425,7 +601,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model,
                        nl_k, istop_k, eps, fscale, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
-    if (istop_k == 1) PSFUN_TRY(ar_launch_est(c, ctl, H, ldh, model, nl, nl_k, eps, fscale));
+    if (istop_k == 1) PSFUN_TRY(ar_launch_est(c, c->stream, ctl, H, ldh, model, nl, nl_k, eps, fscale));
     if (fsync) {
       bool stopped = false;
       PSFUN_TRY(userf_eval(c, nl - 1, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc, &stopped));
@@ -492,16 +668,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sai_normalize(int n, const double 
   const double inv = 1.0 / sigma;
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
     v_out[i] = w[i] * inv;
-}
-
-__global__ __launch_bounds__(PSFUN_BS) void k_sai_dot(int64_t n, const double *__restrict__ a,
-                                                      const double *__restrict__ b, double *__restrict__ p_out) {
-  __shared__ double lds[PSFUN_BS / 64 + 1];
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
-    acc = fma(a[i], b[i], acc);
-  acc = block_sum<PSFUN_BS>(acc, lds);
-  if (threadIdx.x == 0) p_out[blockIdx.x] = acc;
 }
 
 // yk = f(H_k^{-1} - tau I) e1 into W->y; false when H_k is singular.
@@ -624,12 +790,13 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
   const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
   const size_t m_bytes = align_up(sizeof(double) * (size_t)nl * nl, 256);
   const size_t vbytes = align_up(sizeof(double) * (size_t)ldv * (size_t)(nl + 1), 256);  // v_1..v_nl, w
-  const size_t u_bytes = userf ? userf_scratch_bytes(nl) : 0;
+  const size_t u_bytes = align_up(userf ? userf_scratch_bytes(nl) : 0, 256);
   // the basis lives in the second workspace: the inner CG takes the first
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes, &ws));
+  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes + ar_ortho_bytes(c, n, nl), &ws));
   UserfScratch sc;
   if (userf) sc = userf_scratch_carve((char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes, nl);
+  const ArOrtho orth = ar_ortho_carve(c, (char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes, n, nl);
   LzCtl *ctl = (LzCtl *)ws;
   double *H = (double *)((char *)ws + ctl_bytes);
   double *M = (double *)((char *)ws + ctl_bytes + h_bytes);
@@ -661,16 +828,10 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
                                       inner_istop, &it_in, &err_in, nullptr);
       if (st != PSFUN_SUCCESS) return st;
       // modified Gram-Schmidt against v_1..v_j, then v_{j+1} = w / h(j+1, j)
-      hipLaunchKernelGGL(k_sai_dot, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, V, w, P[0]);
-      int cur = 0;
-      for (int k = 1; k <= j; ++k) {
-        hipLaunchKernelGGL(k_ar_mgs, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, k, j, V, ldv, w, P[cur], Gu,
-                           P[1 - cur], ctl, H, ldh);
-        cur = 1 - cur;
-      }
+      PSFUN_TRY(ar_orthogonalize(c, orth, n, j, V, ldv, w, P[0], ctl, H, ldh));
       hipLaunchKernelGGL(k_sai_normalize, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, w, V + (size_t)ldv * j,
-                         P[cur], Gu, ctl, 0);
-      hipLaunchKernelGGL(k_sai_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], Gu, i, ctl, H, ldh, model, M,
+                         P[0], orth.Gl, ctl, 0);
+      hipLaunchKernelGGL(k_sai_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[0], orth.Gl, i, ctl, H, ldh, model, M,
                          Hi, tau, nl_k, istop_k, eps, fscale);
       PSFUN_HIP_TRY(hipGetLastError());
       if (fsync) {

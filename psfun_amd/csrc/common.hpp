@@ -55,6 +55,10 @@ struct psfun_ctx {
   volatile int *h_flags = nullptr;
   int *d_flags = nullptr;         // device address of h_flags
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // auxiliary stream (default priority) for work that overlaps the main
+  // stream's inside one call, forked and joined by these events
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only workspaces: ws for each solver's own state; ws2 for an outer
   // method whose state must survive inner solver calls (shift-and-invert Arnoldi)
   void *ws = nullptr;
@@ -231,6 +235,20 @@ __device__ __forceinline__ double wave_max_dpp(double v) {
   v = fmax(v, dpp_mov_f64<0x118, 0xf, 0xf>(v, v));  // row_shr:8
   v = fmax(v, dpp_mov_f64<0x142, 0xa, 0xf>(v, v));  // row_bcast:15
   v = fmax(v, dpp_mov_f64<0x143, 0xc, 0xf>(v, v));  // row_bcast:31
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the wavefront by the same DPP row moves (lanes with no source
+// add 0), read from lane 63: the result is wavefront-uniform.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_mov_f64<0x111, 0xf, 0xf>(0.0, v);  // row_shr:1
+  v += dpp_mov_f64<0x112, 0xf, 0xf>(0.0, v);  // row_shr:2
+  v += dpp_mov_f64<0x114, 0xf, 0xf>(0.0, v);  // row_shr:4
+  v += dpp_mov_f64<0x118, 0xf, 0xf>(0.0, v);  // row_shr:8
+  v += dpp_mov_f64<0x142, 0xa, 0xf>(0.0, v);  // row_bcast:15
+  v += dpp_mov_f64<0x143, 0xc, 0xf>(0.0, v);  // row_bcast:31
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);

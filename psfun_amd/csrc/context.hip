@@ -51,6 +51,9 @@ extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
   if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&c->d_flags, (void *)c->h_flags, 0);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     psfun_record_hip_error(e, "psfun_ctx_create", __FILE__, __LINE__);
@@ -67,6 +70,7 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->own_stream) hipStreamSynchronize(c->own_stream);
+  if (c->aux_stream) hipStreamSynchronize(c->aux_stream);
   if (c->ws) hipFree(c->ws);
   if (c->ws2) hipFree(c->ws2);
   if (c->d_partials) hipFree(c->d_partials);
@@ -76,6 +80,9 @@ extern "C" int psfun_ctx_destroy(psfun_ctx_t c) {
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   for (auto &e : c->prof_ev) hipEventDestroy(e);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->aux_stream) hipStreamDestroy(c->aux_stream);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
   return PSFUN_SUCCESS;
