@@ -289,17 +289,29 @@ __global__ __launch_bounds__(AR_HS_BS) void k_ar_hsolve(int j, const double *__r
   if (ctl->done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = AR_HS_BS / 64;
+  // the partials come from blocks on every XCD, so each wave keeps the loads
+  // of four lists (and four of each lane's partials) in flight at once
   const int nlist = 2 * j - 1;
-  for (int q = wv; q < nlist; q += NW) {
-    const double *p = P + (int64_t)q * Gg;
-    double sum = 0.0;
-    for (int b = lane; b < Gg; b += 64) sum += p[b];
-    sum = wave_sum_dpp(sum);
-    if (lane == 0) {
-      if (q < j)
-        z[q] = sum;
-      else
-        gj[q - j] = sum;
+  for (int q0 = wv; q0 < nlist; q0 += 4 * NW) {
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int b = lane; b < Gg; b += 64) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = q0 + r * NW;
+        if (q < nlist) sum[r] += P[(int64_t)q * Gg + b];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + r * NW;
+      const double t = wave_sum_dpp(sum[r]);
+      if (lane == 0 && q < nlist) {
+        if (q < j)
+          z[q] = t;
+        else
+          gj[q - j] = t;
+      }
     }
   }
   for (int k = threadIdx.x; k < j; k += AR_HS_BS) acc[k] = 0.0;
