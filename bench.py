@@ -63,7 +63,7 @@ def parse():
     p.add_argument("--no-extra", action="store_true", help="headline only (no blas1/batched/adjoint/quadrature)")
     p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg,arnoldi",
                    help="comma-separated extra sections to run after the headline")
-    p.add_argument("--format", default="", help="SELL, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
+    p.add_argument("--format", default="", help="SELL, HDIA, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     return p.parse_args()
 
 
@@ -120,8 +120,8 @@ def headline(args):
     # storage autotune for this matrix: sliced ELL, or vector CSR with 1..8
     # lanes per row (timed separately, not in `value`)
     def use(choice):
-        if choice == "SELL":
-            M.set_format("SELL")
+        if choice in ("SELL", "HDIA"):
+            M.set_format(choice)
         else:
             M.set_format("CSR")
             M.set_lanes(int(choice[1:]))
@@ -130,8 +130,11 @@ def headline(args):
     if args.format:
         use(args.format)
     else:
-        for choice in ("SELL", "L1", "L2", "L4", "L8"):
-            use(choice)
+        for choice in ("SELL", "HDIA", "L1", "L2", "L4", "L8"):
+            try:
+                use(choice)
+            except _lib.PsfunError:   # diagonal slices refused (too many diagonals per slice)
+                continue
             step()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -174,26 +177,39 @@ def headline(args):
     # counted by the library)
     per_launch_bytes = k["bytes"] / max(k["launches"], 1)
     avg_us = 1e3 * k["ms"] / max(k["launches"], 1)
-    idx_bits = M.sell_index_bits() if fmt == "SELL" else 32
+    idx_bits = M.sell_index_bits() if fmt == "SELL" else (0 if fmt == "HDIA" else 32)
     n_, nnz_ = M.get_nrows(), M.get_nzeros()
-    # (sliced ELL does not read rowptr; its padding, 0.2% here, is left out)
-    kernel_bytes = (per_launch_bytes if fmt != "SELL"
-                    else per_launch_bytes - (4.0 - idx_bits / 8.0) * nnz_ - 4.0 * (n_ + 1))
+    # bytes the kernel itself moves: sliced ELL does not read rowptr (its
+    # padding, 0.2% here, is left out); diagonal slices read 64 values per
+    # slice diagonal, one offset per slice diagonal and the slice pointers,
+    # and no column indices
+    if fmt == "SELL":
+        kernel_bytes = per_launch_bytes - (4.0 - idx_bits / 8.0) * nnz_ - 4.0 * (n_ + 1)
+    elif fmt == "HDIA":
+        nd = M.hdia_diagonals()
+        kernel_bytes = (per_launch_bytes - 12.0 * nnz_ - 4.0 * (n_ + 1)
+                        + (8.0 * 64 + 4.0) * nd + 4.0 * ((n_ + 63) // 64 + 1))
+    else:
+        kernel_bytes = per_launch_bytes
     achieved = per_launch_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     upd_us = 1e3 * ku["ms"] / max(ku["launches"], 1)
     upd_bytes = ku["bytes"] / max(ku["launches"], 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (pmc_traffic("k_lz_spmv_sell", "lz_spmv_sell_128cube_pmc.txt")
-                            if fmt == "SELL" and m == 128 else None),
-                "kernel": ("k_lz_spmv_sell" if fmt == "SELL" else f"k_lz_spmv<{lanes}>")
+                "traffic": (pmc_traffic("k_lz_spmv_" + fmt.lower(), f"lz_spmv_{fmt.lower()}_128cube_pmc.txt")
+                            if fmt in ("SELL", "HDIA") and m == 128 else None),
+                "kernel": ({"SELL": "k_lz_spmv_sell", "HDIA": "k_lz_spmv_hdia"}.get(fmt, f"k_lz_spmv<{lanes}>"))
                 + " (SpMV 12*nnz+4(n+1) algorithmic matrix bytes + w read and u written: 16n;"
                 " the first step also copies x into the basis: 24n)",
-                # the sliced-ELL step reads 16-bit column offsets when they fit, so it moves
-                # 10 bytes per nonzero instead of the 12 the algorithmic count assumes
+                # the sliced-ELL step reads 16-bit column offsets when they fit (10 bytes
+                # per nonzero instead of the 12 the algorithmic count assumes); the
+                # diagonal-slice step reads no column indices at all (8 bytes per stored
+                # value), so `achieved` (algorithmic bytes) can exceed what the kernel
+                # moves; kernel_* give the bytes it does move
                 "index_bits": idx_bits,
                 "kernel_bytes_per_launch": kernel_bytes,
                 "kernel_achieved_GBs": round(kernel_bytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 else None,
+                "kernel_frac": round(kernel_bytes / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if avg_us > 0 else None,
                 "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_us, 3),
                 "launches": k["launches"], "launches_per_event_pair": args.prof_reps,
                 "update_kernel": {"name": "k_lz_update (3-term update + ||w||^2 partials, 32n bytes)",

@@ -75,7 +75,7 @@ typedef struct psfun_csr *psfun_csr_t;
 enum { PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1 };
 
 /* Storage the SpMV kernels read (psfun_csr_set_format). */
-enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2 };
+enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, PSFUN_FORMAT_HDIA = 3 };
 
 /* ---- context / errors --------------------------------------------------- */
 const char *psfun_strerror(int status);
@@ -140,9 +140,18 @@ int psfun_csr_set_lanes(psfun_csr_t A, int lanes_per_row);
 int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
 /* Choose the storage the SpMV kernels read: PSFUN_FORMAT_CSR (vector CSR,
  * lanes_per_row lanes per row), PSFUN_FORMAT_SELL (sliced ELL, 64-row slices,
- * one thread per row; built at creation) or PSFUN_FORMAT_AUTO (SELL when its
- * padding stays within 25% of nnz).  Results differ only in rounding order. */
+ * one thread per row; built at creation), PSFUN_FORMAT_HDIA (diagonal slices,
+ * PSBLAS's "hacked DIA": each 64-row slice stores one value per row for each
+ * distinct offset col - row in it, so the SpMV reads no column indices; built
+ * on first choice, PSFUN_ERR_UNSUPPORTED when its rows would store more than
+ * 1.5x nnz values; the Lanczos step and psfun_spmv read it, the other kernels the
+ * sliced-ELL copy) or PSFUN_FORMAT_AUTO (SELL when its padding stays within
+ * 25% of nnz).  Results differ only in rounding order.  (PSBLAS's
+ * psb_d_hdia_sparse_mat is the reference ecosystem's counterpart.) */
 int psfun_csr_set_format(psfun_csr_t A, int format);
+/* Slice diagonals the diagonal-slice copy stores (values = 64 x that), 0 when
+ * it has not been built. */
+int psfun_csr_hdia_diagonals(psfun_csr_t A, int64_t *ndiag);
 /* Format in use and the padded sliced-ELL entry count (0 when not built). */
 int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_entries);
 /* Width of the column indices the Lanczos sliced-ELL step reads: 16 when

@@ -31,6 +31,7 @@ PSFUN_MEM_DEVICE = 1
 PSFUN_FORMAT_AUTO = 0
 PSFUN_FORMAT_CSR = 1
 PSFUN_FORMAT_SELL = 2
+PSFUN_FORMAT_HDIA = 3
 PSFUN_PREC_NONE = 0
 PSFUN_PREC_DIAG = 1
 PSFUN_ORDER_NATURAL = 0
@@ -71,6 +72,7 @@ SIGNATURES = {
     "psfun_csr_set_format": (_i, [_vp, _i]),
     "psfun_csr_get_format": (_i, [_vp, _pi, _pi64]),
     "psfun_csr_sell_index_bits": (_i, [_vp, _pi]),
+    "psfun_csr_hdia_diagonals": (_i, [_vp, _pi64]),
     "psfun_vec_alloc": (_i, [_vp, _i64, C.POINTER(_vp)]),
     "psfun_vec_free": (_i, [_vp, _vp]),
     "psfun_vec_upload": (_i, [_vp, _vp, _vp, _i64]),

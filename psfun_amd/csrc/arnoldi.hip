@@ -538,7 +538,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   double *P[2] = {c->d_partials, c->d_partials + PSFUN_MAX_GRID};
   const ArOrtho orth = ar_ortho_carve(c, (char *)ws + ctl_bytes + h_bytes + vbytes, n, nl);
 
-  const bool sell = A->format == PSFUN_FORMAT_SELL;
+  const bool sell = psfun_reads_sell(A);
   const int L = sell ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
   int Gs = psfun_grid(c, chunks, sell ? 8 : 4);

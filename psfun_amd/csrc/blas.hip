@@ -58,6 +58,25 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sell_spmv(int n, const int *__rest
   }
 }
 
+// The same on the diagonal slices (chunks of PSFUN_BS rows keep every
+// wavefront inside one 64-row slice).
+__global__ __launch_bounds__(PSFUN_BS) void k_hdia_spmv(int n, const int *__restrict__ hp,
+                                                        const int *__restrict__ off,
+                                                        const double *__restrict__ hv, double alpha,
+                                                        const double *__restrict__ x, double beta,
+                                                        double *__restrict__ y) {
+  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
+  const XcdRange xr = xcd_range(nchunks);
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * PSFUN_BS + threadIdx.x);
+    if (row < n) {
+      double r = alpha * hdia_row_dot(row, n, hp, off, hv, x);
+      if (beta != 0.0) r = fma(beta, y[row], r);
+      y[row] = r;
+    }
+  }
+}
+
 // Column-major multi-RHS: Y(:,c) = alpha*A*X(:,c) + beta*Y(:,c), KB columns per pass.
 template <int L, int KB>
 __global__ __launch_bounds__(PSFUN_BS) void k_csr_spmm(int n, const int *__restrict__ rowptr,
@@ -119,7 +138,15 @@ int launch_spmv(psfun_ctx *c, const psfun_csr *A, double alpha, const double *x,
   if (A->n == 0) return PSFUN_SUCCESS;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_SPMV,
                              12.0 * A->nnz + 4.0 * (A->n + 1) + (beta != 0.0 ? 24.0 : 16.0) * A->n));
-  if (A->format == PSFUN_FORMAT_SELL) {
+  if (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr) {
+    const int G = psfun_grid(c, ((int64_t)A->n + PSFUN_BS - 1) / PSFUN_BS);
+    hipLaunchKernelGGL(k_hdia_spmv, dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n, A->hdia_ptr, A->hdia_off,
+                       A->hdia_val, alpha, x, beta, y);
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    return PSFUN_SUCCESS;
+  }
+  if (psfun_reads_sell(A)) {
     const int G = psfun_grid(c, ((int64_t)A->n + PSFUN_BS - 1) / PSFUN_BS);
     hipLaunchKernelGGL(k_sell_spmv, dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr, A->sell_col,
                        A->sell_val, alpha, x, beta, y);

@@ -119,6 +119,16 @@ struct psfun_csr {
   // built when every offset fits: half the column bytes for the kernels
   // that read them (nullptr otherwise)
   int16_t *sell_dcol = nullptr;
+  // Optional diagonal slices (PSBLAS's "hacked DIA"), built when the format is
+  // first chosen: slice s of PSFUN_SELL_C rows keeps the distinct offsets
+  // col - row of its entries, ascending, in hdia_off[hdia_ptr[s] ..
+  // hdia_ptr[s+1]), and for each one a value per row (0 where the row has no
+  // such entry) at hdia_val[(hdia_ptr[s] + k) * C + l].  The SpMV reads no
+  // column indices and x[row + offset] contiguously.
+  int64_t hdia_ndiag = 0;  // slice diagonals stored
+  int *hdia_ptr = nullptr;  // nullptr: not built
+  int *hdia_off = nullptr;
+  double *hdia_val = nullptr;
   int format = PSFUN_FORMAT_CSR;  // format the kernels use
 };
 #define PSFUN_SELL_C 64
@@ -374,6 +384,26 @@ __device__ __forceinline__ double sell_row_dot16(int row, const int *__restrict_
   return sum;
 }
 
+// Row sum of `row` in the diagonal slices: one term per diagonal of the row's
+// slice, in ascending offset order (the row's CSR order for sorted columns;
+// the slice's other diagonals add 0).  Every lane of the calling wavefront
+// must hold a row of the same slice (64 consecutive rows from a multiple of
+// 64), so the slice's offsets are scalar loads.  Out-of-range columns of the
+// zero padding read a clamped, in-range x.
+__device__ __forceinline__ double hdia_row_dot(int row, int n, const int *__restrict__ hp,
+                                               const int *__restrict__ off, const double *__restrict__ val,
+                                               const double *__restrict__ x) {
+  const int s = __builtin_amdgcn_readfirstlane(row >> 6), l = row & 63;
+  const int b = hp[s], e = hp[s + 1];
+  double sum = 0.0;
+#pragma unroll 4
+  for (int k = b; k < e; ++k) {
+    const int c = min(max(row + off[k], 0), n - 1);
+    sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], x[c], sum);
+  }
+  return sum;
+}
+
 // Store to a host-visible flag word (coherent pinned memory): a relaxed
 // system-scope atomic store, so the value leaves the GPU caches at once and
 // the host sees it while the kernel still runs.
@@ -410,11 +440,17 @@ __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict_
   return sum;
 }
 
+// Kernels without a diagonal-slice variant read the sliced-ELL copy (always
+// built beside it when it fits) when the diagonal-slice format is chosen.
+static inline bool psfun_reads_sell(const psfun_csr *A) {
+  return A->format == PSFUN_FORMAT_SELL || (A->format == PSFUN_FORMAT_HDIA && A->sell_nnz > 0);
+}
+
 // Launch-side dispatch over the matrix storage: defines LL (lanes) and SS
 // (sliced ELL) as constants and the matrix arrays MRP/MCOL/MVAL for __VA_ARGS__.
 #define PSFUN_DISPATCH_STORAGE(A, ...)                                                           \
   do {                                                                                           \
-    if ((A)->format == PSFUN_FORMAT_SELL) {                                                      \
+    if (psfun_reads_sell(A)) {                                                                   \
       constexpr int LL = 1;                                                                      \
       constexpr bool SS = true;                                                                  \
       const int *MRP = (A)->sell_ptr, *MCOL = (A)->sell_col;                                     \
@@ -438,6 +474,6 @@ __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict_
 
 // Rows per chunk of a streaming SpMV kernel for the storage.
 static inline int64_t psfun_spmv_chunks(const psfun_csr *A) {
-  const int rpb = (A->format == PSFUN_FORMAT_SELL) ? PSFUN_BS : PSFUN_BS / A->lanes;
+  const int rpb = psfun_reads_sell(A) ? PSFUN_BS : PSFUN_BS / A->lanes;
   return ((int64_t)A->n + rpb - 1) / rpb;
 }

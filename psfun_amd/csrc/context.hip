@@ -297,7 +297,101 @@ int psfun_sell_build(psfun_csr *A, const int *rp) {
   return PSFUN_SUCCESS;
 }
 
-int psfun_sell_refill(psfun_csr *A) { return sell_launch_fill(A, 0); }
+// Values of the diagonal slices from the CSR arrays: thread r zeroes its
+// slots, then adds its entries in CSR order at their offsets' slots (binary
+// search in the slice's ascending offsets).  Threads of the rows beyond n that
+// fill the last slice only zero theirs.
+__global__ __launch_bounds__(PSFUN_BS) void k_hdia_fill(int n, int nslice, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const double *__restrict__ va,
+                                                        const int *__restrict__ hp, const int *__restrict__ off,
+                                                        double *__restrict__ hv) {
+  const int64_t r = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  if (r >= (int64_t)nslice * PSFUN_SELL_C) return;
+  const int s = (int)(r / PSFUN_SELL_C), l = (int)(r % PSFUN_SELL_C);
+  const int b = hp[s], e = hp[s + 1];
+  for (int k = b; k < e; ++k) hv[(int64_t)k * PSFUN_SELL_C + l] = 0.0;
+  if (r >= n) return;
+  for (int q = rp[r]; q < rp[r + 1]; ++q) {
+    const int d = ci[q] - (int)r;
+    int lo = b, hi = e;  // first k with off[k] >= d (the list holds d)
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (off[mid] < d)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    hv[(int64_t)lo * PSFUN_SELL_C + l] += va[q];
+  }
+}
+
+static void hdia_free(psfun_csr *A) {
+  if (A->hdia_ptr) hipFree(A->hdia_ptr);
+  if (A->hdia_off) hipFree(A->hdia_off);
+  if (A->hdia_val) hipFree(A->hdia_val);
+  A->hdia_ptr = A->hdia_off = nullptr;
+  A->hdia_val = nullptr;
+  A->hdia_ndiag = 0;
+}
+
+static int hdia_launch_fill(psfun_csr *A) {
+  if (!A->hdia_ptr || A->n == 0) return PSFUN_SUCCESS;
+  const int ns = (A->n + PSFUN_SELL_C - 1) / PSFUN_SELL_C;
+  const int64_t threads = (int64_t)ns * PSFUN_SELL_C;
+  hipLaunchKernelGGL(k_hdia_fill, dim3((unsigned)((threads + PSFUN_BS - 1) / PSFUN_BS)), dim3(PSFUN_BS), 0,
+                     A->ctx->stream, A->n, ns, A->rowptr, A->colind, A->val, A->hdia_ptr, A->hdia_off, A->hdia_val);
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
+// Diagonal slices: the offsets of each slice on the host (from the device
+// CSR pattern), the values by k_hdia_fill.  PSFUN_ERR_UNSUPPORTED when the
+// slices would hold more than 1.5 x nnz values.
+static int hdia_build(psfun_csr *A) {
+  hdia_free(A);
+  const int n = A->n;
+  const int ns = (n + PSFUN_SELL_C - 1) / PSFUN_SELL_C;
+  const double limit = 1.5 * (double)std::max<int64_t>(A->nnz, 1);
+  std::vector<int> rp((size_t)n + 1), ci((size_t)std::max<int64_t>(A->nnz, 1));
+  PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+  PSFUN_HIP_TRY(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+  if (A->nnz) PSFUN_HIP_TRY(hipMemcpy(ci.data(), A->colind, sizeof(int) * (size_t)A->nnz, hipMemcpyDeviceToHost));
+  std::vector<int> hp((size_t)ns + 1), offs, tmp;
+  double stored = 0.0;  // values of the rows that exist (the last slice may be partial)
+  for (int s = 0; s < ns; ++s) {
+    tmp.clear();
+    const int r1 = std::min(n, (s + 1) * PSFUN_SELL_C);
+    for (int r = s * PSFUN_SELL_C; r < r1; ++r)
+      for (int q = rp[r]; q < rp[r + 1]; ++q) tmp.push_back(ci[q] - r);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    hp[s] = (int)offs.size();
+    offs.insert(offs.end(), tmp.begin(), tmp.end());
+    stored += (double)tmp.size() * (r1 - s * PSFUN_SELL_C);
+    if (stored > limit || offs.size() > (size_t)INT32_MAX / PSFUN_SELL_C) return PSFUN_ERR_UNSUPPORTED;
+  }
+  hp[ns] = (int)offs.size();
+  const size_t nd = offs.size();
+  hipError_t e = hipMalloc(&A->hdia_ptr, sizeof(int) * ((size_t)ns + 1));
+  if (e == hipSuccess) e = hipMalloc(&A->hdia_off, sizeof(int) * std::max<size_t>(nd, 1));
+  if (e == hipSuccess) e = hipMalloc(&A->hdia_val, sizeof(double) * std::max<size_t>(nd * PSFUN_SELL_C, 1));
+  if (e == hipSuccess) e = hipMemcpy(A->hdia_ptr, hp.data(), sizeof(int) * ((size_t)ns + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nd) e = hipMemcpy(A->hdia_off, offs.data(), sizeof(int) * nd, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    hdia_free(A);
+    return PSFUN_ERR_ALLOC;
+  }
+  A->hdia_ndiag = (int64_t)nd;
+  PSFUN_TRY(hdia_launch_fill(A));
+  PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+  return PSFUN_SUCCESS;
+}
+
+int psfun_sell_refill(psfun_csr *A) {
+  PSFUN_TRY(sell_launch_fill(A, 0));
+  return hdia_launch_fill(A);
+}
 
 static int auto_format(const psfun_csr *A) {
   if (A->sell_nnz > 0 && (double)A->sell_nnz <= 1.25 * (double)std::max<int64_t>(A->nnz, 1))
@@ -400,10 +494,11 @@ extern "C" int psfun_csr_destroy(psfun_csr_t A) {
     if (A->colind) hipFree(A->colind);
     if (A->val) hipFree(A->val);
   }
-  if (A->sell_nnz || A->sell_ptr) {
+  if (A->sell_nnz || A->sell_ptr || A->hdia_ptr) {
     hipSetDevice(A->ctx->device);
     hipStreamSynchronize(A->ctx->stream);
     sell_free(A);
+    hdia_free(A);
   }
   delete A;
   return PSFUN_SUCCESS;
@@ -449,6 +544,12 @@ extern "C" int psfun_csr_set_format(psfun_csr_t A, int format) {
   } else if (format == PSFUN_FORMAT_SELL) {
     if (A->sell_nnz == 0 && A->n > 0) return PSFUN_ERR_UNSUPPORTED;
     A->format = PSFUN_FORMAT_SELL;
+  } else if (format == PSFUN_FORMAT_HDIA) {
+    if (!A->hdia_ptr) {
+      PSFUN_HIP_TRY(hipSetDevice(A->ctx->device));
+      PSFUN_TRY(hdia_build(A));
+    }
+    A->format = PSFUN_FORMAT_HDIA;
   } else {
     return PSFUN_ERR_INVALID_ARG;
   }
@@ -459,6 +560,12 @@ extern "C" int psfun_csr_get_format(psfun_csr_t A, int *format, int64_t *sell_en
   if (!A) return PSFUN_ERR_INVALID_ARG;
   if (format) *format = A->format;
   if (sell_entries) *sell_entries = A->sell_nnz;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_csr_hdia_diagonals(psfun_csr_t A, int64_t *ndiag) {
+  if (!A || !ndiag) return PSFUN_ERR_INVALID_ARG;
+  *ndiag = A->hdia_ptr ? A->hdia_ndiag : 0;
   return PSFUN_SUCCESS;
 }
 

@@ -212,6 +212,71 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
   if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
 }
 
+// Same step on the diagonal slices (hdia_row_dot: no column indices, x read
+// contiguously at each of the slice's offsets).
+template <bool SMALLT>
+__global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_hdia(int n, const int *__restrict__ hp,
+                                                           const int *__restrict__ off,
+                                                           const double *__restrict__ hv, double opscale,
+                                                           const double *__restrict__ src, int j, int G,
+                                                           const double *__restrict__ p_in, int g_in,
+                                                           double *__restrict__ v_out, double *__restrict__ u_out,
+                                                           double *__restrict__ p_out, LzCtl *ctl, int nl,
+                                                           int istop, double eps, double fscale,
+                                                           int *hf) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  const int stopped = ctl->done;
+  double pacc = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < g_in; b += PSFUN_BS) pacc += p_in[b];
+  const int64_t nchunks = (n + PSFUN_BS - 1) / PSFUN_BS;
+  XcdRange xr = {0, 0, 1};
+  if (blockIdx.x > 0) xr = xcd_range_c0(nchunks, G);
+  double acc0 = 0.0, src0 = 0.0;
+  if (xr.begin < xr.end) {
+    const int row0 = (int)(xr.begin * PSFUN_BS + threadIdx.x);
+    if (row0 < n) {
+      acc0 = hdia_row_dot(row0, n, hp, off, hv, src);
+      src0 = src[row0];
+    }
+  }
+  if (__syncthreads_or(stopped)) return;  // block-uniform (the control block may set it meanwhile)
+  const double sigma = sqrt(block_sum<PSFUN_BS>(pacc, lds));
+  if (blockIdx.x == 0) {  // control block
+    if (threadIdx.x >= 64) return;
+    if (j == 1) {
+      if (threadIdx.x == 0) {
+        host_flag_store(hf, 1);
+        ctl->rs1 = sigma;
+        ctl->rs[1] = sigma;
+        if (sigma == 0.0) lz_stop(ctl, hf, LZ_DONE_ZERO);
+      }
+      return;
+    }
+    __shared__ ChebLds W;
+    lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
+    return;
+  }
+  if (sigma == 0.0 && j == 1) return;
+  const double inv = 1.0 / sigma;
+  const double uscale = opscale * inv;
+  double dot = 0.0;
+  for (int64_t ch = xr.begin; ch < xr.end; ch += xr.stride) {
+    const int row = (int)(ch * PSFUN_BS + threadIdx.x);
+    if (row < n) {
+      const bool first = (ch == xr.begin);
+      const double sum = first ? acc0 : hdia_row_dot(row, n, hp, off, hv, src);
+      const double w = first ? src0 : src[row];
+      const double u = uscale * sum;
+      if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
+      u_out[row] = u;
+      dot = fma(u, w * inv, dot);
+    }
+  }
+  dot = block_sum<PSFUN_BS>(dot, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x - 1] = dot;
+}
+
 // Loop control at the last index i = nl (no SpMV needed there).
 __global__ __launch_bounds__(64) void k_lz_ctrl(const double *__restrict__ p_in, int g_in, int j, LzCtl *ctl,
                                                 int nl, int istop, double eps, double fscale, int *hf) {
@@ -367,7 +432,22 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   // may repeat it back to back inside one event pair to amortise the events.
   const int reps = psfun_prof_reps(c);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
-  if (A->format == PSFUN_FORMAT_SELL) {
+  if (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr) {
+    for (int r = 0; r < reps; ++r) {
+      if (nl <= 65)
+        hipLaunchKernelGGL((k_lz_spmv_hdia<true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->hdia_ptr,
+                           A->hdia_off, A->hdia_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
+                           istop, eps, fscale, c->d_flags);
+      else
+        hipLaunchKernelGGL((k_lz_spmv_hdia<false>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->hdia_ptr,
+                           A->hdia_off, A->hdia_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl,
+                           istop, eps, fscale, c->d_flags);
+    }
+    PSFUN_HIP_TRY(hipGetLastError());
+    PSFUN_TRY(psfun_prof_end(c));
+    return PSFUN_SUCCESS;
+  }
+  if (psfun_reads_sell(A)) {
 #define LZ_SELL(SM, D)                                                                                           \
   hipLaunchKernelGGL((k_lz_spmv_sell<SM, D>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,      \
                      A->sell_col, A->sell_dcol, A->sell_val, opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, \
@@ -459,7 +539,8 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   double *Pw = c->d_partials;                      // ||w||^2 (and ||x||^2) partials
   double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
 
-  const bool sell = A->format == PSFUN_FORMAT_SELL;
+  // sliced ELL and diagonal slices: one thread per row, chunks of PSFUN_BS rows
+  const bool sell = psfun_reads_sell(A) || (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr);
   const int L = sell ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
   // SpMV streaming blocks; with sliced ELL 8 blocks per CU are resident, and

@@ -198,11 +198,22 @@ class psb_dspmat_type:
         torch.cuda.current_stream().synchronize()
         self._src = self._src[:2] + (v, self._src[3])
 
-    _FORMATS = {"AUTO": _lib.PSFUN_FORMAT_AUTO, "CSR": _lib.PSFUN_FORMAT_CSR, "SELL": _lib.PSFUN_FORMAT_SELL}
+    _FORMATS = {"AUTO": _lib.PSFUN_FORMAT_AUTO, "CSR": _lib.PSFUN_FORMAT_CSR, "SELL": _lib.PSFUN_FORMAT_SELL,
+                "HDIA": _lib.PSFUN_FORMAT_HDIA}
 
     def set_format(self, fmt):
-        """'CSR' (vector CSR), 'SELL' (sliced ELL, 64-row slices) or 'AUTO'."""
+        """'CSR' (vector CSR), 'SELL' (sliced ELL, 64-row slices), 'HDIA'
+        (diagonal slices, as PSBLAS's hacked DIA; raises PsfunError with
+        PSFUN_ERR_UNSUPPORTED when the slices would hold more than 1.5 x nnz
+        values) or 'AUTO'."""
+        self.ctxt.bind_stream()
         _lib.check(_lib.lib().psfun_csr_set_format(self.handle, self._FORMATS[str(fmt).upper()]))
+
+    def hdia_diagonals(self):
+        """Slice diagonals of the diagonal-slice copy (0 when not built)."""
+        d = C.c_int64(0)
+        _lib.check(_lib.lib().psfun_csr_hdia_diagonals(self.handle, C.byref(d)))
+        return d.value
 
     def sell_index_bits(self):
         """16 or 32: width of the column indices the Lanczos SELL step reads

@@ -85,6 +85,86 @@ def test_spmv_sliced_ell_matches_oracle(gpu_ctx, n):
     assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
 
 
+def _banded(n, half, drop, seed, dup=False):
+    """Band matrix with offsets -half..half, a fraction `drop` of the entries
+    removed at random (so slices hold diagonals some rows lack, and the first
+    and last rows' slices hold offsets that run off the matrix); with dup, one
+    entry per 97 rows is stored twice."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        for d in range(-half, half + 1):
+            c = i + d
+            if 0 <= c < n and rng.random() >= drop:
+                rows.append(i)
+                cols.append(c)
+                if dup and i % 97 == 0 and d == 0:
+                    rows.append(i)
+                    cols.append(c)
+    rows, cols = np.array(rows, np.int64), np.array(cols, np.int64)
+    return rows, cols, rng.normal(size=len(rows))
+
+
+def _csr_keep_duplicates(n, rows, cols, vals):
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp).astype(np.int32), cols.astype(np.int32), vals
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 3001])
+def test_spmv_diagonal_slices_match_oracle(gpu_ctx, n):
+    """Diagonal slices (hacked DIA): diagonals absent from some rows, offsets
+    running off both ends of the matrix (clamped reads of the zero padding), a
+    partial last slice, repeated entries summed, and a value refresh."""
+    rows, cols, vals = _banded(n, 3, 0.1, 21, dup=True)
+    A = _csr_keep_duplicates(n, rows, cols, vals)
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format("HDIA")
+    assert M.get_format()[0] == "HDIA"
+    nslice = (n + 63) // 64
+    expect = sum(len(np.unique(cols[(rows >= 64 * s) & (rows < 64 * s + 64)] -
+                               rows[(rows >= 64 * s) & (rows < 64 * s + 64)])) for s in range(nslice))
+    assert M.hdia_diagonals() == expect
+    x = matgen.uniform_pm1(22, n)
+    y0 = matgen.uniform_pm1(23, n)
+    desc = psb.psb_desc_type(n)
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        y = _vec(y0)
+        assert psb.psb_spmm(alpha, M, _vec(x), beta, y, desc) == 0
+        ref = O.spmv(A, x, alpha, beta, y0)
+        assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
+    v2 = matgen.normal(24, len(A[2]))
+    M.set_values(_dev(v2))
+    y = _vec(np.zeros(n))
+    psb.psb_spmm(1.0, M, _vec(x), 0.0, y, desc)
+    ref = O.spmv((A[0], A[1], v2), x)
+    assert np.max(np.abs(y.get_vect() - ref)) <= 1e-13 * (np.max(np.abs(ref)) + 1)
+
+
+def test_diagonal_slices_refused_for_scattered_pattern(gpu_ctx):
+    """Random columns give every slice hundreds of diagonals: the format is
+    refused (PSFUN_ERR_UNSUPPORTED) and the previous choice stays."""
+    from psfun_amd import _lib
+    M = psb.psb_dspmat_type.from_csr(_ragged(3001, 1))
+    M.set_format("SELL")
+    with pytest.raises(_lib.PsfunError) as e:
+        M.set_format("HDIA")
+    assert e.value.status == _lib.PSFUN_ERR_UNSUPPORTED
+    assert M.get_format()[0] == "SELL" and M.hdia_diagonals() == 0
+
+
+def test_lanczos_diagonal_slices_match_sliced_ell(gpu_ctx):
+    """The Lanczos step on diagonal slices adds each row's terms in the same
+    order as the sliced-ELL step: the same iterations and y to rounding."""
+    A = matgen.laplacian((20, 20, 20), scale=-1.0)
+    x = matgen.uniform_pm1(25, 8000)
+    r = {f: _lanczos_gpu(A, x, 1e-10, 50, fmt=f) for f in ("SELL", "HDIA")}
+    assert r["HDIA"][0] == 0 and r["HDIA"][1] == r["SELL"][1]
+    assert np.linalg.norm(r["HDIA"][4] - r["SELL"][4]) <= 1e-14 * np.linalg.norm(r["SELL"][4])
+
+
 def test_format_auto_choice(gpu_ctx):
     M = psb.psb_dspmat_type.from_csr(matgen.laplacian((40, 40, 3)))
     assert M.get_format()[0] == "SELL"        # uniform rows: padding ~0
@@ -196,7 +276,7 @@ def _lanczos_gpu(A, x, eps, itmax, istop=1, fmt="AUTO"):
     return info, it, err, res, y.get_vect()
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
 @pytest.mark.parametrize("dims,itmax", [((100, 100), 30), ((12, 10, 9), 50), ((64, 64, 64), 50)])
 def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     A = matgen.laplacian(dims, scale=-1.0)
@@ -213,7 +293,7 @@ def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     assert len(res) == it + 1 and err < 1e-10
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
 def test_lanczos_long_run_lds_fT(gpu_ctx, fmt):
     """exp(-30 L) on 16^3 takes ~90 iterations: the loop runs with itmax > 65,
     so the control block's f(T) step is the LDS routine for orders past 64."""
