@@ -19,7 +19,8 @@ module psfun_hip_mod
   integer(c_int), parameter :: PSFUN_VAR_CHBSYM = 0, PSFUN_VAR_CHBHES = 1, PSFUN_VAR_CHBGEN = 2, &
        & PSFUN_VAR_GENPADE = 3, PSFUN_VAR_TAYLOR = 4, PSFUN_VAR_SASQ = 5, PSFUN_VAR_SYM = 6
   integer(c_int), parameter :: PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1
-  integer(c_int), parameter :: PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2
+  integer(c_int), parameter :: PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, &
+                                PSFUN_FORMAT_HDIA = 3
   integer(c_int), parameter :: PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1
   integer(c_int), parameter :: PSFUN_QUAD_RE_SUM = 0, PSFUN_QUAD_A_IM_SUM = 1
 
@@ -72,6 +73,12 @@ module psfun_hip_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: a
       integer(c_int), value :: format
+      integer(c_int) :: st
+    end function
+    function psfun_csr_hdia_diagonals(a, ndiag) bind(C, name='psfun_csr_hdia_diagonals') result(st)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: a
+      integer(c_int64_t), intent(out) :: ndiag
       integer(c_int) :: st
     end function
 
