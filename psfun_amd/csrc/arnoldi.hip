@@ -162,7 +162,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_spmv(int n, const int *__restri
     const int row = (int)(ch * RPB + sub);
     double sum = 0.0;
     if (SELL) {
-      if (row < n) sum = sell_row_dot(row, rp, col, val, src, inv);
+      if (row < n)
+        sum = (L == PSFUN_L_HDIA) ? hdia_row_dot<true>(row, n, rp, col, val, src, inv)
+                                  : sell_row_dot(row, rp, col, val, src, inv);
     } else {
       if (row < n) {
         const int s = rp[row], e = rp[row + 1];
@@ -539,9 +541,9 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   const ArOrtho orth = ar_ortho_carve(c, (char *)ws + ctl_bytes + h_bytes + vbytes, n, nl);
 
   const bool sell = psfun_reads_sell(A);
-  const int L = sell ? 1 : A->lanes;
+  const int L = psfun_row_per_thread(A) ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  int Gs = psfun_grid(c, chunks, sell ? 8 : 4);
+  int Gs = psfun_grid(c, chunks, psfun_row_per_thread(A) ? 8 : 4);
   if (Gs + 1 > 8 * c->num_cu) Gs = (8 * c->num_cu - 1) / 8 * 8;
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
@@ -559,7 +561,11 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     const double *src = (j == 1) ? x : U[j & 1];
     double *u = U[(j + 1) & 1];
     PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_SPMV, spmv_bytes, j));
-    if (sell) {
+    if (psfun_reads_hdia(A)) {
+      hipLaunchKernelGGL((k_ar_spmv<PSFUN_L_HDIA, true>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n,
+                         A->hdia_ptr, A->hdia_off, A->hdia_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, ctl, H,
+                         ldh, model, nl_k, istop_k, eps, fscale, c->d_flags);
+    } else if (sell) {
       hipLaunchKernelGGL((k_ar_spmv<1, true>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n, A->sell_ptr,
                          A->sell_col, A->sell_val, scaling, src, j, Gs, P[cur], gcur, V, ldv, u, ctl, H, ldh,
                          model, nl_k, istop_k, eps, fscale, c->d_flags);

@@ -138,7 +138,7 @@ int launch_spmv(psfun_ctx *c, const psfun_csr *A, double alpha, const double *x,
   if (A->n == 0) return PSFUN_SUCCESS;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_SPMV,
                              12.0 * A->nnz + 4.0 * (A->n + 1) + (beta != 0.0 ? 24.0 : 16.0) * A->n));
-  if (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr) {
+  if (psfun_reads_hdia(A)) {
     const int G = psfun_grid(c, ((int64_t)A->n + PSFUN_BS - 1) / PSFUN_BS);
     hipLaunchKernelGGL(k_hdia_spmv, dim3(G), dim3(PSFUN_BS), 0, c->stream, A->n, A->hdia_ptr, A->hdia_off,
                        A->hdia_val, alpha, x, beta, y);

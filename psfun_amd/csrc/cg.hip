@@ -323,7 +323,7 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
 
   double *Psig = c->d_partials, *Prho = c->d_partials + PSFUN_MAX_GRID,
          *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID;
-  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_reads_sell(A) ? 8 : 4);
+  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_row_per_thread(A) ? 8 : 4);
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
   hipLaunchKernelGGL(k_cg_rowstats, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val,

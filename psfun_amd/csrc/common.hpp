@@ -132,6 +132,8 @@ struct psfun_csr {
   int format = PSFUN_FORMAT_CSR;  // format the kernels use
 };
 #define PSFUN_SELL_C 64
+// lanes-per-row template value that selects the diagonal-slice row sum
+#define PSFUN_L_HDIA (-1)
 
 // (Re)build the sliced-ELL copy from the device CSR arrays; `rp_host` is the
 // 0-based row pointer on the host.  Leaves sell_nnz = 0 when the padded size
@@ -390,16 +392,19 @@ __device__ __forceinline__ double sell_row_dot16(int row, const int *__restrict_
 // must hold a row of the same slice (64 consecutive rows from a multiple of
 // 64), so the slice's offsets are scalar loads.  Out-of-range columns of the
 // zero padding read a clamped, in-range x.
+// With SCALED, each x entry is multiplied by xs before its product (as
+// sell_row_dot does), for callers that form a scaled vector on the fly.
+template <bool SCALED = false>
 __device__ __forceinline__ double hdia_row_dot(int row, int n, const int *__restrict__ hp,
                                                const int *__restrict__ off, const double *__restrict__ val,
-                                               const double *__restrict__ x) {
+                                               const double *__restrict__ x, double xs = 1.0) {
   const int s = __builtin_amdgcn_readfirstlane(row >> 6), l = row & 63;
   const int b = hp[s], e = hp[s + 1];
   double sum = 0.0;
 #pragma unroll 4
   for (int k = b; k < e; ++k) {
     const int c = min(max(row + off[k], 0), n - 1);
-    sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], x[c], sum);
+    sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], SCALED ? x[c] * xs : x[c], sum);
   }
   return sum;
 }
@@ -413,13 +418,23 @@ __device__ __forceinline__ void host_flag_store(int *p, int v) {
 
 // Row sum  sum_k val_k * g(col_k)  of `row` for the storage chosen at compile
 // time: SELL (sliced ELL, one thread per row; rp = sell_ptr, entries in CSR
-// order) or vector CSR with L lanes per row (every lane of the group calls it;
-// the sum is valid in every lane).  g(c) yields the gathered operand entry.
+// order), diagonal slices (L = PSFUN_L_HDIA with SELL: rp/col/val = hdia_ptr,
+// hdia_off, hdia_val; one thread per row, the wavefront's rows in one slice;
+// padding columns off the matrix are clamped and meet a stored 0) or vector
+// CSR with L lanes per row (every lane of the group calls it; the sum is
+// valid in every lane).  g(c) yields the gathered operand entry.
 template <int L, bool SELL, typename Gather>
 __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict__ rp, const int *__restrict__ col,
                                           const double *__restrict__ val, int lane, Gather g) {
   double sum = 0.0;
-  if (SELL) {
+  if (L == PSFUN_L_HDIA) {
+    if (row < n) {
+      const int s = __builtin_amdgcn_readfirstlane(row >> 6), l = row & 63;
+      const int b = rp[s], e = rp[s + 1];
+#pragma unroll 4
+      for (int k = b; k < e; ++k) sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], g(min(max(row + col[k], 0), n - 1)), sum);
+    }
+  } else if (SELL) {
     if (row < n) {
       const int s = row >> 6, l = row & 63;
       const int b = rp[s];
@@ -445,12 +460,23 @@ __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict_
 static inline bool psfun_reads_sell(const psfun_csr *A) {
   return A->format == PSFUN_FORMAT_SELL || (A->format == PSFUN_FORMAT_HDIA && A->sell_nnz > 0);
 }
+static inline bool psfun_reads_hdia(const psfun_csr *A) {
+  return A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr != nullptr;
+}
+// Sliced ELL and diagonal slices: one thread per row, chunks of PSFUN_BS rows.
+static inline bool psfun_row_per_thread(const psfun_csr *A) { return psfun_reads_sell(A) || psfun_reads_hdia(A); }
 
 // Launch-side dispatch over the matrix storage: defines LL (lanes) and SS
 // (sliced ELL) as constants and the matrix arrays MRP/MCOL/MVAL for __VA_ARGS__.
 #define PSFUN_DISPATCH_STORAGE(A, ...)                                                           \
   do {                                                                                           \
-    if (psfun_reads_sell(A)) {                                                                   \
+    if (psfun_reads_hdia(A)) {                                                                   \
+      constexpr int LL = PSFUN_L_HDIA;                                                           \
+      constexpr bool SS = true;                                                                  \
+      const int *MRP = (A)->hdia_ptr, *MCOL = (A)->hdia_off;                                     \
+      const double *MVAL = (A)->hdia_val;                                                        \
+      __VA_ARGS__;                                                                               \
+    } else if (psfun_reads_sell(A)) {                                                            \
       constexpr int LL = 1;                                                                      \
       constexpr bool SS = true;                                                                  \
       const int *MRP = (A)->sell_ptr, *MCOL = (A)->sell_col;                                     \
@@ -474,6 +500,6 @@ static inline bool psfun_reads_sell(const psfun_csr *A) {
 
 // Rows per chunk of a streaming SpMV kernel for the storage.
 static inline int64_t psfun_spmv_chunks(const psfun_csr *A) {
-  const int rpb = psfun_reads_sell(A) ? PSFUN_BS : PSFUN_BS / A->lanes;
+  const int rpb = psfun_row_per_thread(A) ? PSFUN_BS : PSFUN_BS / A->lanes;
   return ((int64_t)A->n + rpb - 1) / rpb;
 }

@@ -432,7 +432,7 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   // may repeat it back to back inside one event pair to amortise the events.
   const int reps = psfun_prof_reps(c);
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
-  if (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr) {
+  if (psfun_reads_hdia(A)) {
     for (int r = 0; r < reps; ++r) {
       if (nl <= 65)
         hipLaunchKernelGGL((k_lz_spmv_hdia<true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->hdia_ptr,
@@ -540,7 +540,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
 
   // sliced ELL and diagonal slices: one thread per row, chunks of PSFUN_BS rows
-  const bool sell = psfun_reads_sell(A) || (A->format == PSFUN_FORMAT_HDIA && A->hdia_ptr);
+  const bool sell = psfun_row_per_thread(A);
   const int L = sell ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
   // SpMV streaming blocks; with sliced ELL 8 blocks per CU are resident, and

@@ -324,7 +324,7 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
   PSFUN_HIP_TRY(hipMemcpyAsync(w.dshift, shifts_host, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice,
                                c->stream));
   double *Ppq = c->d_partials, *Prr = c->d_partials + PSFUN_MAX_GRID;
-  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_reads_sell(A) ? 8 : 4);
+  const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_row_per_thread(A) ? 8 : 4);
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
   c->h_flags[0] = 0;

@@ -56,7 +56,7 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
 def test_arnoldi_laplacian_parity_and_closed_form(gpu_ctx, fmt):
     dims = (30, 30)
     A = matgen.laplacian(dims, scale=-1.0)
@@ -70,12 +70,14 @@ def test_arnoldi_laplacian_parity_and_closed_form(gpu_ctx, fmt):
     assert len(res) == it + 1 and abs(res[0] - np.linalg.norm(x)) <= 1e-12 * np.linalg.norm(x)
 
 
-def test_arnoldi_nonsymmetric_with_scaling(gpu_ctx):
+@pytest.mark.parametrize("fmt", ["AUTO", "HDIA"])
+def test_arnoldi_nonsymmetric_with_scaling(gpu_ctx, fmt):
     """SpMV multiplied by fun%scaling (arnoldi:202): exp(-0.2 A) x for a
-    nonsymmetric A, against the oracle and dense scipy expm."""
+    nonsymmetric A, against the oracle and dense scipy expm (also on the
+    diagonal slices of the 5-point convection-diffusion stencil)."""
     A = _convdiff(20)
     x = matgen.normal(32, 400)
-    info, it, err, res, y = _arnoldi_gpu(A, x, 1e-11, 80, scaling=-0.2, variant="GENPADE")
+    info, it, err, res, y = _arnoldi_gpu(A, x, 1e-11, 80, scaling=-0.2, variant="GENPADE", fmt=fmt)
     assert info == 0
     ref = O.arnoldi(A, x, 1e-11, 80, 1, scaling=-0.2)
     assert _rel(y, ref["y"]) <= 1e-10

@@ -41,7 +41,7 @@ def test_cg_reference_example(gpu_ctx):
     assert np.linalg.norm(x.get_vect() - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
 @pytest.mark.parametrize("istop,prec", [(1, "NONE"), (2, "NONE"), (2, "DIAG"), (1, "DIAG")])
 def test_cg_parity(gpu_ctx, istop, prec, fmt):
     A = matgen.anisotropic_2d(24, seed=3)
@@ -90,7 +90,7 @@ def test_cg_zero_rhs(gpu_ctx):
     assert info == 0 and it == ref["iter"] and np.all(x.get_vect() == 0.0)
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
 def test_mscg_dense_and_oracle(gpu_ctx, fmt):
     A = matgen.laplacian((9, 8))
     Ad = matgen.to_scipy(A).toarray()
