@@ -144,8 +144,8 @@ int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
  * PSBLAS's "hacked DIA": each 64-row slice stores one value per row for each
  * distinct offset col - row in it, so the SpMV reads no column indices; built
  * on first choice, PSFUN_ERR_UNSUPPORTED when its rows would store more than
- * 1.5x nnz values; the Lanczos step and psfun_spmv read it, the other kernels the
- * sliced-ELL copy) or PSFUN_FORMAT_AUTO (SELL when its padding stays within
+ * 1.5x nnz values; the Lanczos, Arnoldi, CG and multi-shift CG steps and
+ * psfun_spmv read it) or PSFUN_FORMAT_AUTO (SELL when its padding stays within
  * 25% of nnz).  Results differ only in rounding order.  (PSBLAS's
  * psb_d_hdia_sparse_mat is the reference ecosystem's counterpart.) */
 int psfun_csr_set_format(psfun_csr_t A, int format);
