@@ -232,6 +232,12 @@ def headline(args):
                 "kernel_bytes_per_launch": kernel_bytes,
                 "kernel_achieved_GBs": round(kernel_bytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 else None,
                 "kernel_frac": round(kernel_bytes / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if avg_us > 0 else None,
+                "note": ("achieved/frac count the CSR bytes with int32 column indices (12*nnz + 4(n+1) + 16n); "
+                         "the diagonal-slice step reads no indices, so frac can exceed 1; kernel_frac is the "
+                         "fraction of peak of the bytes it moves, which the PMC traffic confirms"
+                         if fmt == "HDIA" else
+                         "achieved/frac count the CSR bytes with int32 column indices; kernel_frac is the "
+                         "fraction of peak of the bytes the kernel moves"),
                 "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_us, 3),
                 "launches": k["launches"], "launches_per_event_pair": args.prof_reps,
                 "update_kernel": {"name": "k_lz_update (3-term update + ||w||^2 partials, 32n bytes)",

@@ -195,12 +195,15 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_resid(int n, const int *__restr
 }
 
 // Row statistics of A: dinv = 1/diag(A) (1 where the diagonal is 0), and the
-// per-block maxima of the absolute row sums (||A||_inf partials).
+// per-block maxima of the absolute row sums (||A||_inf partials); in the same
+// pass the per-block partials of ||b||_inf (p_b[block]) and ||b||_2^2
+// (p_b[gridDim + block]).
 __global__ __launch_bounds__(PSFUN_BS) void k_cg_rowstats(int n, const int *__restrict__ rowptr,
                                                           const int *__restrict__ col, const double *__restrict__ val,
-                                                          double *__restrict__ dinv, double *__restrict__ p_ani) {
+                                                          const double *__restrict__ b, double *__restrict__ dinv,
+                                                          double *__restrict__ p_ani, double *__restrict__ p_b) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  double mx = 0.0;
+  double mx = 0.0, bmx = 0.0, bss = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
     double s = 0.0, dg = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
@@ -209,22 +212,32 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_rowstats(int n, const int *__re
     }
     mx = fmax(mx, s);
     if (dinv) dinv[i] = (dg != 0.0) ? 1.0 / dg : 1.0;
+    const double bi = b[i];
+    bmx = fmax(bmx, fabs(bi));
+    bss = fma(bi, bi, bss);
   }
   mx = block_max<PSFUN_BS>(mx, lds);
-  if (threadIdx.x == 0) p_ani[blockIdx.x] = mx;
+  bmx = block_max<PSFUN_BS>(bmx, lds);
+  bss = block_sum<PSFUN_BS>(bss, lds);
+  if (threadIdx.x == 0) {
+    p_ani[blockIdx.x] = mx;
+    p_b[blockIdx.x] = bmx;
+    p_b[gridDim.x + blockIdx.x] = bss;
+  }
 }
 
-// psb_init_conv: ||A||_inf, ||b||_inf, ||b||_2 from partials (one block).
-__global__ __launch_bounds__(PSFUN_BS) void k_cg_initconv(int64_t n, const double *__restrict__ b,
-                                                          const double *__restrict__ p_ani, int g_ani, CgCtl *ctl,
+// psb_init_conv: ||A||_inf, ||b||_inf, ||b||_2 from the k_cg_rowstats
+// partials (g of each; one block, a fixed fold order).
+__global__ __launch_bounds__(PSFUN_BS) void k_cg_initconv(const double *__restrict__ p_ani,
+                                                          const double *__restrict__ p_b, int g, CgCtl *ctl,
                                                           int reset) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   double mx = 0.0, s2 = 0.0, a = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += PSFUN_BS) {
-    mx = fmax(mx, fabs(b[i]));
-    s2 = fma(b[i], b[i], s2);
+  for (int i = threadIdx.x; i < g; i += PSFUN_BS) {
+    mx = fmax(mx, p_b[i]);
+    s2 += p_b[g + i];
+    a = fmax(a, p_ani[i]);
   }
-  for (int i = threadIdx.x; i < g_ani; i += PSFUN_BS) a = fmax(a, p_ani[i]);
   mx = block_max<PSFUN_BS>(mx, lds);
   s2 = block_sum<PSFUN_BS>(s2, lds);
   a = block_max<PSFUN_BS>(a, lds);
@@ -322,14 +335,15 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
   if (prec != PSFUN_PREC_DIAG) dinv = nullptr;
 
   double *Psig = c->d_partials, *Prho = c->d_partials + PSFUN_MAX_GRID,
-         *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID;
+         *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID,
+         *Pb = c->d_partials + 7 * PSFUN_MAX_GRID;  // two lists of Gu (no BLAS-1 call runs inside)
   const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_row_per_thread(A) ? 8 : 4);
-  const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
+  const int Gu = std::min(psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4), PSFUN_MAX_GRID / 2);
 
-  hipLaunchKernelGGL(k_cg_rowstats, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val,
-                     dinv, Pani);
+  hipLaunchKernelGGL(k_cg_rowstats, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, b,
+                     dinv, Pani, Pb);
   PSFUN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, Pani, Gu, ctl, 1);
+  hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, Pani, Pb, Gu, ctl, 1);
   PSFUN_HIP_TRY(hipGetLastError());
 
   CgCtl *hctl = (CgCtl *)((char *)c->h_pinned + 256);
@@ -342,7 +356,7 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
                                                  MRP, MCOL, MVAL, eta, zeta, b, x, r, dinv, Prho));
     PSFUN_HIP_TRY(hipGetLastError());
     if (restart > 0) {
-      hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, b, Pani, Gu, ctl, 0);
+      hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, Pani, Pb, Gu, ctl, 0);
       PSFUN_HIP_TRY(hipGetLastError());
     }
     c->h_flags[0] = 0;
