@@ -163,14 +163,24 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_seed(int n, double *__restrict_
 
 // x^s += alpha^s p^s;  p^s = zeta_{k+1} r_{k+1} + beta^s p^s.  At k = 0 the
 // starting values x^s = 0, p^s = b are used without being read.
+// ACC: the x^s are not kept; instead acc += sum_s Re(wq_s alpha^s p^s)
+// (form 0) or Im(...) (form 1), the quadrature's combination of the x^s
+// accumulated one iteration at a time (acc is not read at k = 0).  A shift
+// whose x^s stops changing stops contributing, as its x^s would.
+template <bool ACC>
 __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__restrict__ r,
                                                        const double *__restrict__ b, double2 *__restrict__ X,
-                                                       int64_t ldx, double2 *__restrict__ P, int64_t ldp, int k, int G,
+                                                       int64_t ldx, double *__restrict__ acc, const double *__restrict__ wq,
+                                                       int form, double2 *__restrict__ P, int64_t ldp, int k, int G,
                                                        const double *__restrict__ p_rr, int g_rr, MsCtl *ctl,
                                                        double eps, int itmax, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2];
+  __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2], s_w[MS_MAX][2];
   __shared__ int s_upd_x[MS_MAX], s_upd_p[MS_MAX];
+  if (ACC && threadIdx.x < MS_MAX) {
+    s_w[threadIdx.x][0] = threadIdx.x < ctl->nshift ? wq[2 * threadIdx.x] : 0.0;
+    s_w[threadIdx.x][1] = threadIdx.x < ctl->nshift ? wq[2 * threadIdx.x + 1] : 0.0;
+  }
   // block-uniform: the control block of this same grid may set the flag
   if (__syncthreads_or(ctl->done)) return;
   const double rr1 = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
@@ -248,39 +258,26 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + t; i < n; i += (int64_t)G * PSFUN_BS) {
     const double ri = r[i];
     const double bi = (k == 0) ? b[i] : 0.0;
+    double a = (ACC && k > 0) ? acc[i] : 0.0;
     for (int s = 0; s < nshift; ++s) {
       if (!s_upd_x[s]) continue;
-      double2 *xp = X + (int64_t)s * ldx + i;
       double2 *pp = P + (int64_t)s * ldp + i;
       const double2 pv = (k == 0) ? make_double2(bi, 0.0) : *pp;
-      const double2 xv = (k == 0) ? make_double2(0.0, 0.0) : *xp;
       const double ar = s_alpha[s][0], ai = s_alpha[s][1];
-      *xp = make_double2(xv.x + ar * pv.x - ai * pv.y, xv.y + ar * pv.y + ai * pv.x);
+      const double dxr = ar * pv.x - ai * pv.y, dxi = ar * pv.y + ai * pv.x;  // alpha^s p^s
+      if (ACC) {
+        a += (form == 0) ? (s_w[s][0] * dxr - s_w[s][1] * dxi) : (s_w[s][0] * dxi + s_w[s][1] * dxr);
+      } else {
+        double2 *xp = X + (int64_t)s * ldx + i;
+        const double2 xv = (k == 0) ? make_double2(0.0, 0.0) : *xp;
+        *xp = make_double2(xv.x + dxr, xv.y + dxi);
+      }
       if (s_upd_p[s]) {
         const double br = s_beta[s][0], bim = s_beta[s][1];
         *pp = make_double2(fma(s_zn[s][0], ri, br * pv.x - bim * pv.y), fma(s_zn[s][1], ri, br * pv.y + bim * pv.x));
       }
     }
-  }
-}
-
-// y = gamma * Re(sum_s w_s x^s)  (form 0)  or  t = gamma * Im(sum_s w_s x^s) (form 1).
-__global__ __launch_bounds__(PSFUN_BS) void k_quad_combine(int64_t n, int nshift, const double2 *__restrict__ X,
-                                                           int64_t ld, const double *__restrict__ w, double gamma,
-                                                           int form, double *__restrict__ y) {
-  __shared__ double sw[MS_MAX][2];
-  if (threadIdx.x < nshift) {
-    sw[threadIdx.x][0] = w[2 * threadIdx.x];
-    sw[threadIdx.x][1] = w[2 * threadIdx.x + 1];
-  }
-  __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    double acc = 0.0;
-    for (int s = 0; s < nshift; ++s) {
-      const double2 v = X[(int64_t)s * ld + i];
-      acc += (form == 0) ? (sw[s][0] * v.x - sw[s][1] * v.y) : (sw[s][0] * v.y + sw[s][1] * v.x);
-    }
-    y[i] = gamma * acc;
+    if (ACC) acc[i] = a;
   }
 }
 
@@ -315,8 +312,11 @@ static int ms_workspace(psfun_ctx *c, int n, int nshift, bool with_x, MsWork *w)
   return PSFUN_SUCCESS;
 }
 
+// With acc != nullptr the solutions are not stored: acc receives the
+// combination sum_s Re (form 0) or Im (form 1) of wq_s x^s (wq on the device).
 static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *shifts_host, const double *b,
-                  double2 *X, int64_t ldx, const MsWork &w, double eps, int itmax, int *iters, double *resid) {
+                  double2 *X, int64_t ldx, const MsWork &w, double eps, int itmax, int *iters, double *resid,
+                  double *acc = nullptr, const double *wq = nullptr, int form = 0) {
   const int n = A->n;
   MsCtl *ctl = w.ctl;
   double *r = w.r, *q = w.q;
@@ -341,10 +341,16 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
     PSFUN_TRY(psfun_prof_end(c));
     hipLaunchKernelGGL(k_ms_seed, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, q, k, Ppq, Gs, Prr, ctl);
     PSFUN_HIP_TRY(hipGetLastError());
-    // algorithmic bytes of the shift kernel: r read, x^s and p^s read+written per running shift
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE, 8.0 * n + 64.0 * n * nshift, k));
-    hipLaunchKernelGGL(k_ms_shift, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, P, w.ld, k, Gu, Prr,
-                       Gu, ctl, eps, itmax, c->d_flags);
+    // algorithmic bytes of the shift kernel: r read, p^s read+written per running shift, and
+    // x^s read+written per shift, or the one real accumulator read+written
+    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE,
+                               acc ? 8.0 * n + 32.0 * n * nshift + 16.0 * n : 8.0 * n + 64.0 * n * nshift, k));
+    if (acc)
+      hipLaunchKernelGGL(k_ms_shift<true>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, acc, wq, form,
+                         P, w.ld, k, Gu, Prr, Gu, ctl, eps, itmax, c->d_flags);
+    else
+      hipLaunchKernelGGL(k_ms_shift<false>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, acc, wq, form,
+                         P, w.ld, k, Gu, Prr, Gu, ctl, eps, itmax, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     std::swap(pold, pnew);
@@ -398,15 +404,18 @@ extern "C" int psfun_quad_apply(psfun_ctx_t c, psfun_csr_t A, int nshift, const 
     PSFUN_HIP_TRY(hipMemsetAsync(y, 0, sizeof(double) * n, c->stream));
     return psfun_ctx_synchronize(c);
   }
+  // the solutions are not stored: the shift kernel accumulates
+  // gamma * (Re or Im) sum_s w_s x^s itself, into y (form 0) or into a work
+  // vector that one SpMV then maps to y (form 1)
   MsWork w;
-  PSFUN_TRY(ms_workspace(c, n, nshift, true, &w));
-  PSFUN_TRY(ms_run(c, A, nshift, shifts, x, w.X, w.ld, w, eps, itmax, iters, resid));
-  PSFUN_HIP_TRY(hipMemcpyAsync(w.dw, weights, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice, c->stream));
-  const int G = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
-  hipLaunchKernelGGL(k_quad_combine, dim3(G), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, nshift,
-                     (const double2 *)w.X, w.ld, (const double *)w.dw, gamma, form,
-                     form == PSFUN_QUAD_RE_SUM ? y : w.tmp);
-  PSFUN_HIP_TRY(hipGetLastError());
+  PSFUN_TRY(ms_workspace(c, n, nshift, false, &w));
+  double *wg = (double *)c->h_pinned;  // gamma * w_s, staged in pinned memory
+  PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < 2 * nshift; ++s) wg[s] = gamma * weights[s];
+  PSFUN_HIP_TRY(hipMemcpyAsync(w.dw, wg, sizeof(double) * 2 * nshift, hipMemcpyHostToDevice, c->stream));
+  double *acc = form == PSFUN_QUAD_RE_SUM ? y : w.tmp;
+  PSFUN_TRY(ms_run(c, A, nshift, shifts, x, nullptr, 0, w, eps, itmax, iters, resid, acc, w.dw,
+                   form == PSFUN_QUAD_RE_SUM ? 0 : 1));
   if (form == PSFUN_QUAD_A_IM_SUM) PSFUN_TRY(launch_spmv(c, A, 1.0, w.tmp, 0.0, y));
   return psfun_ctx_synchronize(c);
 }
