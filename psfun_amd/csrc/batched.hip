@@ -119,6 +119,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_norm(const int *__restrict__ ch
   }
 }
 
+#define BT_SR 64    // rows per LDS-staged sub-chunk of the SpMM
+#define BT_CAP 768  // entries staged per sub-chunk (9 KiB with the row pointers)
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ rowptr, const int *__restrict__ col,
                                                       const double *__restrict__ val, double opscale,
@@ -132,6 +134,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
   __shared__ int s_act[KP];
   __shared__ int s_any;
   __shared__ double red[PSFUN_BS];
+  __shared__ int s_rp[BT_SR + 1];
+  __shared__ int s_col[BT_CAP];
+  __shared__ double s_val[BT_CAP];
   if (bctl->remaining == 0) return;
   const int c = blockIdx.x;
   const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
@@ -158,28 +163,63 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
   }
   __syncthreads();
   const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
+  const bool active = s_any && k < K && s_act[k];
+  const double inv = active ? s_inv[k] : 0.0;
   double dot = 0.0;
-  if (s_any && k < K && s_act[k]) {
-    const double inv = s_inv[k];
-    // entries four at a time: their column indices, then their gathers, are
-    // issued together (a one-at-a-time col -> src chain leaves the kernel
-    // waiting on latency); the sum keeps the entry order
-    for (int r = r0 + sub; r < r1; r += RP) {
-      const int s = rowptr[r], e = rowptr[r + 1];
+  // The chunk is taken BT_SR rows at a time: their row pointers, and their
+  // column indices and values when they fit in BT_CAP entries, are staged in
+  // LDS by coalesced loads of the whole block, so a row's gathers issue after
+  // one LDS read instead of after two dependent global loads.  A sub-chunk
+  // with more entries reads them from global memory.  Each row's terms are
+  // added in entry order either way.
+  for (int sr0 = r0; sr0 < r1; sr0 += BT_SR) {
+    const int sr1 = min(r1, sr0 + BT_SR);
+    __syncthreads();  // the previous sub-chunk is done with the LDS
+    for (int i = threadIdx.x; i <= sr1 - sr0; i += PSFUN_BS) s_rp[i] = rowptr[sr0 + i];
+    __syncthreads();
+    const int e0 = s_rp[0], ne = s_rp[sr1 - sr0] - e0;
+    const bool staged = ne <= BT_CAP;
+    if (staged && s_any) {
+      for (int i = threadIdx.x; i < ne; i += PSFUN_BS) {
+        s_col[i] = col[e0 + i];
+        s_val[i] = val[e0 + i];
+      }
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int r = sr0 + sub; r < sr1; r += RP) {
+      const int s = s_rp[r - sr0], e = s_rp[r - sr0 + 1];
+      const double xr = src[(int64_t)r * K + k];
       double sum = 0.0;
       int q = s;
-      for (; q + 4 <= e; q += 4) {
-        const int c0 = col[q], c1 = col[q + 1], c2 = col[q + 2], c3 = col[q + 3];
-        const double a0 = val[q], a1 = val[q + 1], a2 = val[q + 2], a3 = val[q + 3];
-        const double x0 = src[(int64_t)c0 * K + k], x1 = src[(int64_t)c1 * K + k];
-        const double x2 = src[(int64_t)c2 * K + k], x3 = src[(int64_t)c3 * K + k];
-        sum = fma(a0, x0 * inv, sum);
-        sum = fma(a1, x1 * inv, sum);
-        sum = fma(a2, x2 * inv, sum);
-        sum = fma(a3, x3 * inv, sum);
+      if (staged) {
+        const int *cl = s_col - e0;
+        const double *vl = s_val - e0;
+        for (; q + 4 <= e; q += 4) {
+          const double x0 = src[(int64_t)cl[q] * K + k], x1 = src[(int64_t)cl[q + 1] * K + k];
+          const double x2 = src[(int64_t)cl[q + 2] * K + k], x3 = src[(int64_t)cl[q + 3] * K + k];
+          sum = fma(vl[q], x0 * inv, sum);
+          sum = fma(vl[q + 1], x1 * inv, sum);
+          sum = fma(vl[q + 2], x2 * inv, sum);
+          sum = fma(vl[q + 3], x3 * inv, sum);
+        }
+        for (; q < e; ++q) sum = fma(vl[q], src[(int64_t)cl[q] * K + k] * inv, sum);
+      } else {
+        // entries four at a time: their column indices, then their gathers,
+        // are issued together; the sum keeps the entry order
+        for (; q + 4 <= e; q += 4) {
+          const int c0 = col[q], c1 = col[q + 1], c2 = col[q + 2], c3 = col[q + 3];
+          const double a0 = val[q], a1 = val[q + 1], a2 = val[q + 2], a3 = val[q + 3];
+          const double x0 = src[(int64_t)c0 * K + k], x1 = src[(int64_t)c1 * K + k];
+          const double x2 = src[(int64_t)c2 * K + k], x3 = src[(int64_t)c3 * K + k];
+          sum = fma(a0, x0 * inv, sum);
+          sum = fma(a1, x1 * inv, sum);
+          sum = fma(a2, x2 * inv, sum);
+          sum = fma(a3, x3 * inv, sum);
+        }
+        for (; q < e; ++q) sum = fma(val[q], src[(int64_t)col[q] * K + k] * inv, sum);
       }
-      for (; q < e; ++q) sum = fma(val[q], src[(int64_t)col[q] * K + k] * inv, sum);
-      const double v = src[(int64_t)r * K + k] * inv;
+      const double v = xr * inv;
       const double u = opscale * sum;
       v_out[(int64_t)r * K + k] = v;
       u_out[(int64_t)r * K + k] = u;
