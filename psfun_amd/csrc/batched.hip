@@ -377,10 +377,13 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict_
   const double *yk = in.yk + (int64_t)(g * K + k) * in.ld;
   for (int r = r0 + sub; r < r1; r += RP) {
     const int64_t i = (int64_t)r * K + k;
+    // slots in order (the basis is not read again: nontemporal loads; eight
+    // slots' loads in flight per thread)
     double acc = 0.0;
     if (m > 0) {
-      acc = yk[0] * V[i];
-      for (int l = 1; l < m; ++l) acc = fma(yk[l], V[(int64_t)l * vstride + i], acc);
+      acc = yk[0] * __builtin_nontemporal_load(V + i);
+#pragma unroll 8
+      for (int l = 1; l < m; ++l) acc = fma(yk[l], __builtin_nontemporal_load(V + (int64_t)l * vstride + i), acc);
     }
     Y[perm ? (int64_t)perm[r] * K + k : i] = s_rs1[k] * acc;
   }
