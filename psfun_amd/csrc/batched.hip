@@ -13,7 +13,7 @@
 //   k_bt_spmm(j):   v_j = (1/sigma) src, u = opscale*A v_j, alpha partials
 //   k_bt_fold(j):   one wavefront per instance folds its alpha partials
 //   k_bt_update(j): w = u - alpha v_j - beta v_{j-1}; ||w||^2 partials
-//   k_bt_ctrl(j+1): one wavefront per instance: beta = ||w||, breakdown test,
+//   k_bt_beta/k_bt_est(j+1): one wavefront per instance: beta = ||w||, breakdown test,
 //                   a-posteriori estimate from T_j, stop decision.
 // Instances stop independently; a stopped instance's rows are skipped.  The
 // whole sequence (itmax steps + assembly) can be captured once in a hipGraph
@@ -288,11 +288,19 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
 
 // Loop control of instance blockIdx.x at the reference's index i = j
 // (beta = T(j-1, j) from the ||w||^2 partials).
+// Loop control at index j+1, split in two so that the estimate does not hold
+// up the next SpMM (which needs only beta and the run state):
+//   k_bt_beta (stream order): one wavefront per instance folds ||w||^2 into
+//     beta = T(j, j+1), records it and itx, applies the breakdown test and,
+//     when no estimate is taken (istop 2), the iteration limit;
+//   k_bt_est (beside the SpMM, on the auxiliary stream): the reference's
+//     estimate yk = f(T_j) e1, rs(j+1) = rs1 beta |yk(j)| / ||yk||, the
+//     convergence test and the iteration limit.  An instance it stops may
+//     have its SpMM work of the same index wasted, which feeds nothing.
 template <int KP>
-__global__ __launch_bounds__(64) void k_bt_ctrl(const int *__restrict__ gchunk, int K, int j,
+__global__ __launch_bounds__(64) void k_bt_beta(const int *__restrict__ gchunk, int K, int j,
                                                 const double *__restrict__ p_in, BtInst in, BtCtl *bctl, int nl,
-                                                int istop, double eps, double fscale) {
-  __shared__ ChebLds W;
+                                                int istop) {
   const int I = blockIdx.x;
   if (in.done[I] != BT_RUN) return;
   const int g = I / K, k = I % K;
@@ -300,30 +308,40 @@ __global__ __launch_bounds__(64) void k_bt_ctrl(const int *__restrict__ gchunk, 
   double acc = 0.0;
   for (int cc = c0 + threadIdx.x; cc < c1; cc += 64) acc += p_in[(int64_t)cc * KP + k];
   const double beta = sqrt(wave_sum(acc));
+  if (threadIdx.x != 0) return;
   const int64_t base = (int64_t)I * in.ld;
+  in.beta[base + j - 1] = beta;
+  in.itx[I] = j - 1;
+  int fin = BT_RUN;
+  if (fabs(beta) < 1e-16)
+    fin = BT_BRK;
+  else if (istop != 1 && j >= nl)
+    fin = BT_MAXIT;
+  if (fin != BT_RUN) {
+    in.done[I] = fin;
+    atomicSub(&bctl->remaining, 1);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_bt_est(int j, BtInst in, BtCtl *bctl, int nl, double eps, double fscale) {
+  __shared__ ChebLds W;
+  const int I = blockIdx.x;
+  if (in.done[I] != BT_RUN) return;
+  const int64_t base = (int64_t)I * in.ld;
+  const double beta = in.beta[base + j - 1];
   const int lane = threadIdx.x;
   int fin = BT_RUN;
+  double nrm, last;
+  wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+  const double errnum = in.rs1[I] * beta * fabs(last);
+  const double rsj = errnum / nrm;
   if (lane == 0) {
-    in.beta[base + j - 1] = beta;
-    in.itx[I] = j - 1;
+    in.errnum[I] = errnum;
+    in.errden[I] = nrm;
+    in.rs[base + j] = rsj;
   }
-  if (fabs(beta) < 1e-16) {
-    fin = BT_BRK;
-  } else {
-    if (istop == 1) {
-      double nrm, last;
-      wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
-      const double errnum = in.rs1[I] * beta * fabs(last);
-      const double rsj = errnum / nrm;
-      if (lane == 0) {
-        in.errnum[I] = errnum;
-        in.errden[I] = nrm;
-        in.rs[base + j] = rsj;
-      }
-      if (rsj < eps) fin = BT_CONV;
-    }
-    if (fin == BT_RUN && j >= nl) fin = BT_MAXIT;
-  }
+  if (rsj < eps) fin = BT_CONV;
+  if (fin == BT_RUN && j >= nl) fin = BT_MAXIT;
   if (fin != BT_RUN && lane == 0) {
     if (fin == BT_CONV) in.m[I] = j - 1;
     in.done[I] = fin;
@@ -641,8 +659,14 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     hipLaunchKernelGGL((k_bt_fold<KP, 0>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j, P1, in, bctl);
     hipLaunchKernelGGL((k_bt_update<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, B->d_gchunk, K, j, u,
                        vj, vjm1, P1, P0, in, bctl);
-    hipLaunchKernelGGL((k_bt_ctrl<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
-                       itmax, istop, eps, fscale);
+    hipLaunchKernelGGL((k_bt_beta<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
+                       itmax, istop);
+    if (istop == 1) {  // the estimate of index j+1 beside the SpMM of that index
+      PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+      PSFUN_HIP_TRY(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
+      hipLaunchKernelGGL(k_bt_est, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps, fscale);
+      PSFUN_HIP_TRY(hipEventRecord(c->ev_join, c->aux_stream));
+    }
     if (j + 1 <= itmax - 1) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
       hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val,
@@ -650,6 +674,7 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
                          in, bctl);
       PSFUN_TRY(psfun_prof_end(c));
     }
+    if (istop == 1) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     PSFUN_HIP_TRY(hipGetLastError());
     if (poll_host && j % 2 == 0) {
       const int slot = (j / 2) & 1;
