@@ -289,11 +289,13 @@ int psfun_batch_set_values(psfun_batch_t B, const double *val);
 int psfun_batch_csr(psfun_batch_t B, const int32_t **rowptr, const int32_t **colind, const double **val);
 
 /* As psfun_batch_lanczos, but the Krylov basis goes to the caller's device
- * buffer V ((itmax-1)*nrows*K doubles; basis vector l of every instance at
+ * buffer V ((itmax-1)*nrows*K doubles; basis slot l of every instance at
  * V + l*nrows*K, row-major like X) and each instance's Lanczos data to caller
  * device arrays: T_alpha/T_beta (ninst*(itmax+2); instance I at I*(itmax+2),
  * 1-based as T(i,i) and T(i,i+1)), nbasis (ninst ints: valid basis vectors)
- * and rs1 (ninst: ||x||).  Used to reuse the basis in an adjoint pass. */
+ * and rs1 (ninst: ||x||).  The basis is kept unnormalised: slot 0 holds x and
+ * slot l >= 1 holds w_l = T(l,l+1) v_{l+1}, so v_{l+1} = slot l / T_beta(l)
+ * (v_1 = x / rs1).  Used to reuse the basis in an adjoint pass. */
 int psfun_batch_lanczos_basis(psfun_ctx_t ctx, psfun_batch_t B, int K, double fscale, double opscale, const double *X,
                               double *Y, double eps, int itmax, int istop, double *V, double *T_alpha, double *T_beta,
                               int *nbasis, double *rs1);

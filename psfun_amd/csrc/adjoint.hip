@@ -70,9 +70,12 @@ __global__ void k_fom_coef(int K, int nshift, int mmax, const double *__restrict
     const double nyr = rp[2 * l] - (cr * yr - ci * yi), nyi = rp[2 * l + 1] - (cr * yi + ci * yr);
     yr = nyr;
     yi = nyi;
+    // the stored basis is unnormalised: v_{l+1} = slot l / beta_l (beta_0 = rs1)
+    const double bl = (l == 0) ? sc : be[l - 1];
+    const double ib = (bl != 0.0) ? 1.0 / bl : 0.0;
     double *o = coef + 2 * (((int64_t)s * mmax + l) * K + col);
-    o[0] = sc * yr;
-    o[1] = sc * yi;
+    o[0] = sc * yr * ib;
+    o[1] = sc * yi * ib;
   }
   for (int l = m; l < mmax; ++l) {
     double *o = coef + 2 * (((int64_t)s * mmax + l) * K + col);

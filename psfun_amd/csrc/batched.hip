@@ -10,9 +10,12 @@
 // order by every block that needs it (deterministic, no atomics on data).
 //
 // Per Krylov index j (one launch sequence for all instances):
-//   k_bt_spmm(j):   v_j = (1/sigma) src, u = opscale*A v_j, alpha partials
+//   k_bt_spmm(j):   u = opscale*A v_j with v_j = (1/sigma) src formed on the
+//                   fly (src = basis slot j-1; slot 0 is x), alpha partials
 //   k_bt_fold(j):   one wavefront per instance folds its alpha partials
-//   k_bt_update(j): w = u - alpha v_j - beta v_{j-1}; ||w||^2 partials
+//   k_bt_update(j): w = u - alpha v_j - beta v_{j-1} into basis slot j (the
+//                   basis keeps the unnormalised w: v_{l+1} = w_l / beta_l is
+//                   formed where a slot is read); ||w||^2 partials
 //   k_bt_beta/k_bt_est(j+1): one wavefront per instance: beta = ||w||, breakdown test,
 //                   a-posteriori estimate from T_j, stop decision.
 // Instances stop independently; a stopped instance's rows are skipped.  The
@@ -60,6 +63,9 @@ struct psfun_batch {
 
 struct BtInst {
   double *alpha, *beta, *rs, *yk;  // [ninst][ld]
+  // [ninst][ld]: 1/beta_l, the factor of basis slot l (1/rs1 for slot 0),
+  // written by k_bt_final for the slots the assembly reads
+  double *ib;
   double *rs1, *errnum, *errden;   // [ninst]
   int *done, *itx, *m;             // [ninst]
   int ld;
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
       }
       const double v = xr * inv;
       const double u = opscale * sum;
-      v_out[(int64_t)r * K + k] = v;
+      if (j == 1) v_out[(int64_t)r * K + k] = xr;  // basis slot 0 keeps x (v_1 = x / rs1)
       u_out[(int64_t)r * K + k] = u;
       dot = fma(u, v, dot);
     }
@@ -237,12 +243,13 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
 
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ chunk, const int *__restrict__ gchunk,
-                                                        int K, int j, double *__restrict__ u,
+                                                        int K, int j, const double *__restrict__ u,
+                                                        double *__restrict__ w_out,
                                                         const double *__restrict__ vj, const double *__restrict__ vjm1,
                                                         const double *__restrict__ p_in, double *__restrict__ p_out,
                                                         BtInst in, const BtCtl *bctl) {
   constexpr int RP = PSFUN_BS / KP;
-  __shared__ double s_al[KP], s_be[KP];
+  __shared__ double s_al[KP], s_be[KP], s_ia[KP], s_ib[KP];
   __shared__ int s_act[KP];
   __shared__ double red[PSFUN_BS];
   if (bctl->remaining == 0) return;
@@ -251,29 +258,37 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
   if (threadIdx.x < KP) {
     const int k = threadIdx.x;
     int act = 0;
-    double al = 0.0, be = 0.0;
+    double al = 0.0, be = 0.0, ia = 0.0, ib = 0.0;
     if (k < K) {
       const int I = g * K + k;
+      const int64_t base = (int64_t)I * in.ld;
       act = (in.done[I] == BT_RUN);
       if (act) {
-        al = in.alpha[(int64_t)I * in.ld + j];  // folded by k_bt_fold<KP, 0>
-        be = (j > 1) ? in.beta[(int64_t)I * in.ld + j - 1] : 0.0;
+        al = in.alpha[base + j];  // folded by k_bt_fold<KP, 0>
+        be = (j > 1) ? in.beta[base + j - 1] : 0.0;
+        // v_j = w_{j-1} / beta_{j-1}, v_{j-1} = w_{j-2} / beta_{j-2} (beta_0 = rs1),
+        // with the same reciprocals the SpMM steps used
+        const double bj = (j == 1) ? in.rs1[I] : in.beta[base + j - 1];
+        ia = 1.0 / bj;
+        if (j > 1) ib = 1.0 / ((j == 2) ? in.rs1[I] : in.beta[base + j - 2]);
       }
     }
     s_al[k] = al;
     s_be[k] = be;
+    s_ia[k] = ia;
+    s_ib[k] = ib;
     s_act[k] = act;
   }
   __syncthreads();
   const int k = threadIdx.x % KP, sub = threadIdx.x / KP;
   double acc = 0.0;
   if (k < K && s_act[k]) {
-    const double al = s_al[k], be = s_be[k];
+    const double al = s_al[k], be = s_be[k], ia = s_ia[k], ib = s_ib[k];
     for (int r = r0 + sub; r < r1; r += RP) {
       const int64_t i = (int64_t)r * K + k;
-      double w = fma(-al, vj[i], u[i]);
-      if (j > 1) w = fma(-be, vjm1[i], w);
-      u[i] = w;
+      double w = fma(-al, vj[i] * ia, u[i]);
+      if (j > 1) w = fma(-be, vjm1[i] * ib, w);
+      w_out[i] = w;
       acc = fma(w, w, acc);
     }
   }
@@ -286,8 +301,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
   }
 }
 
-// Loop control of instance blockIdx.x at the reference's index i = j
-// (beta = T(j-1, j) from the ||w||^2 partials).
 // Loop control at index j+1, split in two so that the estimate does not hold
 // up the next SpMM (which needs only beta and the run state):
 //   k_bt_beta (stream order): one wavefront per instance folds ||w||^2 into
@@ -353,13 +366,23 @@ __global__ __launch_bounds__(64) void k_bt_final(BtInst in, double fscale) {
   __shared__ ChebLds W;
   const int I = blockIdx.x;
   const int done = in.done[I];
-  if (done == BT_CONV || done == BT_ZERO) return;
-  int kk = in.itx[I] - 1;
-  if (kk < 1) kk = 1;
   const int64_t base = (int64_t)I * in.ld;
-  double nrm, last;
-  wave_exp_tridiag_e1(kk, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
-  if (threadIdx.x == 0) in.m[I] = kk;
+  int m = 0;
+  if (done == BT_CONV) {
+    m = in.m[I];
+  } else if (done != BT_ZERO) {
+    int kk = in.itx[I] - 1;
+    if (kk < 1) kk = 1;
+    double nrm, last;
+    wave_exp_tridiag_e1(kk, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+    if (threadIdx.x == 0) in.m[I] = kk;
+    m = kk;
+  }
+  // factors of the basis slots the assembly reads
+  for (int l = threadIdx.x; l < m; l += 64) {
+    const double b = (l == 0) ? in.rs1[I] : in.beta[base + l];
+    in.ib[base + l] = (b != 0.0) ? 1.0 / b : 0.0;
+  }
 }
 
 // perm != nullptr: the batch is reordered and row r of the result goes to
@@ -393,15 +416,17 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict_
   if (k >= K) return;
   const int m = s_m[k];
   const double *yk = in.yk + (int64_t)(g * K + k) * in.ld;
+  const double *ibk = in.ib + (int64_t)(g * K + k) * in.ld;  // v_{l+1} = slot l * ibk[l]
   for (int r = r0 + sub; r < r1; r += RP) {
     const int64_t i = (int64_t)r * K + k;
     // slots in order (the basis is not read again: nontemporal loads; eight
     // slots' loads in flight per thread)
     double acc = 0.0;
     if (m > 0) {
-      acc = yk[0] * __builtin_nontemporal_load(V + i);
+      acc = yk[0] * (__builtin_nontemporal_load(V + i) * ibk[0]);
 #pragma unroll 8
-      for (int l = 1; l < m; ++l) acc = fma(yk[l], __builtin_nontemporal_load(V + (int64_t)l * vstride + i), acc);
+      for (int l = 1; l < m; ++l)
+        acc = fma(yk[l], __builtin_nontemporal_load(V + (int64_t)l * vstride + i) * ibk[l], acc);
     }
     Y[perm ? (int64_t)perm[r] * K + k : i] = s_rs1[k] * acc;
   }
@@ -644,8 +669,9 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
                      ninst, bctl);
   hipLaunchKernelGGL((k_bt_norm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, X, P0);
   hipLaunchKernelGGL((k_bt_fold<KP, 1>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, 0, P0, in, bctl);
-  const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 24.0 * B->nrows * K;
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
+  // src read and u written (the first step also copies x into basis slot 0)
+  const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 16.0 * B->nrows * K;
+  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes + 8.0 * B->nrows * K));
   hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val, opscale,
                      B->d_chunk, B->d_gchunk, K, 1, X, V, U0, P0, P1, in, bctl);
   PSFUN_TRY(psfun_prof_end(c));
@@ -657,8 +683,10 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     const double *vj = V + (size_t)vs * (j - 1);
     const double *vjm1 = (j > 1) ? V + (size_t)vs * (j - 2) : nullptr;
     hipLaunchKernelGGL((k_bt_fold<KP, 0>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j, P1, in, bctl);
+    // w_j goes to basis slot j; the last step's w only feeds its norm
+    double *w_out = (j + 1 <= itmax - 1) ? V + (size_t)vs * j : u;
     hipLaunchKernelGGL((k_bt_update<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, B->d_gchunk, K, j, u,
-                       vj, vjm1, P1, P0, in, bctl);
+                       w_out, vj, vjm1, P1, P0, in, bctl);
     hipLaunchKernelGGL((k_bt_beta<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
                        itmax, istop);
     if (istop == 1) {  // the estimate of index j+1 beside the SpMM of that index
@@ -670,8 +698,8 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     if (j + 1 <= itmax - 1) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
       hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val,
-                         opscale, B->d_chunk, B->d_gchunk, K, j + 1, u, V + (size_t)vs * j, (j & 1) ? U1 : U0, P0, P1,
-                         in, bctl);
+                         opscale, B->d_chunk, B->d_gchunk, K, j + 1, V + (size_t)vs * j, nullptr, (j & 1) ? U1 : U0,
+                         P0, P1, in, bctl);
       PSFUN_TRY(psfun_prof_end(c));
     }
     if (istop == 1) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -741,7 +769,7 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
   const int ld = itmax + 2;
   const int64_t vs = B->nrows * K;
   // workspace: control, per-instance arrays, basis v_1..v_{itmax-1}, two u buffers
-  const size_t inst_bytes = align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256) +
+  const size_t inst_bytes = align_up(sizeof(double) * ((size_t)ninst * ld * 5 + (size_t)ninst * 3), 256) +
                             align_up(sizeof(int) * (size_t)ninst * 3, 256);
   // the basis lives in the caller's buffer when one is given
   // (a reordered batch also keeps X in its own numbering)
@@ -756,10 +784,11 @@ static int bt_lanczos(psfun_ctx_t c, psfun_batch_t B, int K, double fscale, doub
   in.beta = d + (size_t)ninst * ld;
   in.rs = d + 2 * (size_t)ninst * ld;
   in.yk = d + 3 * (size_t)ninst * ld;
-  in.rs1 = d + 4 * (size_t)ninst * ld;
+  in.ib = d + 4 * (size_t)ninst * ld;
+  in.rs1 = d + 5 * (size_t)ninst * ld;
   in.errnum = in.rs1 + ninst;
   in.errden = in.errnum + ninst;
-  int *ip = (int *)((char *)d + align_up(sizeof(double) * ((size_t)ninst * ld * 4 + (size_t)ninst * 3), 256));
+  int *ip = (int *)((char *)d + align_up(sizeof(double) * ((size_t)ninst * ld * 5 + (size_t)ninst * 3), 256));
   in.done = ip;
   in.itx = ip + ninst;
   in.m = ip + 2 * ninst;

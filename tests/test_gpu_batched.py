@@ -119,3 +119,30 @@ def test_batched_rcm_order_matches_natural(gpu_ctx):
     assert torch.linalg.norm(Yr - Yn) <= 1e-12 * torch.linalg.norm(Yn)
     rp = C.c_void_p()
     assert L.psfun_batch_csr(Br.handle, C.byref(rp), None, None) == _lib.PSFUN_ERR_UNSUPPORTED
+
+
+def test_exported_basis_is_unnormalised_and_orthonormal_after_scaling(gpu_ctx):
+    """psfun_batch_lanczos_basis keeps the basis unnormalised: slot 0 holds x
+    bitwise, and v_{l+1} = slot l / T_beta(l) (v_1 = x / rs1) are orthonormal
+    to rounding over a short run; y equals the plain batched result."""
+    from psfun_amd.autograd import ExpmvOperator
+    A = matgen.laplacian((30, 30))
+    n, K = 900, 3
+    op = ExpmvOperator(A[0], A[1], eps=1e-10, itmax=40)
+    op.set_values(torch.as_tensor(A[2], device="cuda:0"))
+    X = torch.as_tensor(matgen.normal(41, n * K).reshape(n, K), device="cuda:0")
+    Y, (V, Ta, Tb, nb, rs1, K_) = op.expm_basis(X, 0.05)
+    Yp = op.expm(X, 0.05)
+    assert torch.equal(Y, Yp)
+    ld = op.itmax + 2
+    V = V.cpu().numpy().reshape(op.itmax - 1, n, K)
+    Tb = Tb.cpu().numpy().reshape(K, ld)
+    nb = nb.cpu().numpy()
+    rs1 = rs1.cpu().numpy()
+    Xh = X.cpu().numpy()
+    for k in range(K):
+        m = int(nb[k])
+        assert m >= 5
+        assert np.array_equal(V[0, :, k], Xh[:, k])
+        Q = np.stack([V[l, :, k] / (rs1[k] if l == 0 else Tb[k, l]) for l in range(m)], axis=1)
+        assert np.max(np.abs(Q.T @ Q - np.eye(m))) <= 1e-10
