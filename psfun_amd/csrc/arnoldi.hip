@@ -283,6 +283,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_gram(int n, const double *__res
 // k > i, the Gram columns staged through LDS AR_HS_STAGE doubles at a time.
 #define AR_HS_BS 1024
 #define AR_HS_STAGE 4096
+#define AR_HS_QB 16  // lists whose loads are in flight together in the fold
 __global__ __launch_bounds__(AR_HS_BS) void k_ar_hsolve(int j, const double *__restrict__ P, int Gg,
                                                         double *__restrict__ Gm, int ldg, const LzCtl *ctl, double *H,
                                                         int ldh) {
@@ -291,34 +292,73 @@ __global__ __launch_bounds__(AR_HS_BS) void k_ar_hsolve(int j, const double *__r
   if (ctl->done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = AR_HS_BS / 64;
-  // the partials come from blocks on every XCD, so each wave keeps the loads
-  // of four lists (and four of each lane's partials) in flight at once
+  // Fold: thread t takes partial t (and t + AR_HS_BS, ...) of AR_HS_QB lists at
+  // a time, all its loads issued before any sum (the partials come from
+  // blocks on every XCD, so each round trip is long); each list is then
+  // summed over the wavefront (DPP) and over the 16 wavefronts in order.
+  __shared__ double wsum[NW][64];
   const int nlist = 2 * j - 1;
-  for (int q0 = wv; q0 < nlist; q0 += 4 * NW) {
-    double sum[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int b = lane; b < Gg; b += 64) {
+  for (int q0 = 0; q0 < nlist; q0 += 64) {
+    const int nq = min(64, nlist - q0);
+    for (int qb = 0; qb < nq; qb += AR_HS_QB) {
+      double v[AR_HS_QB];
+      if (Gg <= AR_HS_BS) {  // one partial per thread and list: the loads issue together
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = q0 + r * NW;
-        if (q < nlist) sum[r] += P[(int64_t)q * Gg + b];
+        for (int r = 0; r < AR_HS_QB; ++r)
+          v[r] = (qb + r < nq && (int)threadIdx.x < Gg) ? P[(int64_t)(q0 + qb + r) * Gg + threadIdx.x] : 0.0;
+      } else {
+#pragma unroll
+        for (int r = 0; r < AR_HS_QB; ++r) {
+          v[r] = 0.0;
+          if (qb + r < nq)
+            for (int b = threadIdx.x; b < Gg; b += AR_HS_BS) v[r] += P[(int64_t)(q0 + qb + r) * Gg + b];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < AR_HS_QB; ++r) {
+        const double t = wave_sum_dpp(v[r]);
+        if (lane == 0 && qb + r < nq) wsum[wv][qb + r] = t;
       }
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = q0 + r * NW;
-      const double t = wave_sum_dpp(sum[r]);
-      if (lane == 0 && q < nlist) {
-        if (q < j)
-          z[q] = t;
-        else
-          gj[q - j] = t;
-      }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += AR_HS_BS) {
+      double t = 0.0;
+      for (int w = 0; w < NW; ++w) t += wsum[w][q];
+      if (q0 + q < j)
+        z[q0 + q] = t;
+      else
+        gj[q0 + q - j] = t;
     }
+    __syncthreads();
   }
   for (int k = threadIdx.x; k < j; k += AR_HS_BS) acc[k] = 0.0;
   __syncthreads();
   for (int q = threadIdx.x; q < j - 1; q += AR_HS_BS) Gm[(int64_t)q * ldg + (j - 1)] = gj[q];  // row j
+  if (j <= 64) {
+    // all columns staged at once (j*j <= AR_HS_STAGE); lane k of wavefront 0
+    // keeps z_k and acc_k in registers and h_i comes from lane i, so a step is
+    // one broadcast and one fma (the sums run in the same order as below)
+    for (int e = threadIdx.x; e < j * j; e += AR_HS_BS) {
+      const int i = e / j, k = e % j;
+      double g = 0.0;
+      if (k > i) g = (k == j - 1) ? gj[i] : Gm[(int64_t)i * ldg + k];
+      gs[e] = g;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const bool own = lane < j;
+      const double zk = own ? z[lane] : 0.0;
+      double acck = 0.0, hk = 0.0;
+      for (int i = 0; i < j; ++i) {
+        const double g = (own && lane > i) ? gs[i * j + lane] : 0.0;
+        const double hi = wave_read_f64(zk - acck, i);
+        if (lane == i) hk = hi;
+        acck = fma(g, hi, acck);
+      }
+      if (own) H[(int64_t)(j - 1) * ldh + lane] = hk;  // h(lane+1, j)
+    }
+    return;
+  }
   const int cw = (j * j <= AR_HS_STAGE) ? j : AR_HS_STAGE / j;
   for (int i0 = 0; i0 < j; i0 += cw) {
     const int i1 = min(j, i0 + cw);
