@@ -106,9 +106,12 @@ static int ar_launch_est(psfun_ctx *c, hipStream_t st, LzCtl *ctl, const double 
 }
 
 // The estimate of loop index j reads only H(1:j-1, 1:j-1) and h(j, j-1),
-// which the orthogonalisation of column j does not write, so it runs on the
-// auxiliary stream beside that orthogonalisation: forked after the SpMV step
-// of index j, joined before the next SpMV step (which then sees a stop).
+// which neither the orthogonalisation of column j nor the SpMV step of index
+// j+1 writes, so it runs on the auxiliary stream beside both: forked after
+// the SpMV step of index j, joined before the orthogonalisation of column j+1.
+// A stop it finds leaves that one SpMV step's work unused, and the step's
+// control block may have advanced itx past the stop index; the host then
+// takes itx = m (the reference exits with itx = m on convergence).
 static int ar_fork_est(psfun_ctx *c, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl, double eps,
                        double fscale) {
   PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
@@ -597,6 +600,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   int cur = 0, gcur = Gu;  // slot holding the newest partials, and their count
   const double spmv_bytes = 12.0 * A->nnz + 4.0 * (A->n + 1) + 24.0 * A->n;
   bool ran_all = true;
+  bool est_pending = false;
   for (int j = 1; j <= nl - 1; ++j) {
     const double *src = (j == 1) ? x : U[j & 1];
     double *u = U[(j + 1) & 1];
@@ -627,8 +631,14 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     }
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
-    const bool forked = istop_k == 1 && j >= 2;
-    if (forked) PSFUN_TRY(ar_fork_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
+    if (est_pending) {  // the estimate of index j-1 ends before column j's work
+      PSFUN_TRY(ar_join_est(c));
+      est_pending = false;
+    }
+    if (istop_k == 1 && j >= 2) {
+      PSFUN_TRY(ar_fork_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
+      est_pending = true;
+    }
     if (fsync && j >= 2) {
       // control index j: yk = f(H(1:j-1,1:j-1)) e1, rs(j) from h(j, j-1)
       bool stopped = false;
@@ -644,7 +654,6 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_AR_MGS, (2.0 * j + 3.0) * 8.0 * n, j));
     PSFUN_TRY(ar_orthogonalize(c, orth, n, j, V, ldv, u, P[1 - cur], ctl, H, ldh));
     PSFUN_TRY(psfun_prof_end(c));
-    if (forked) PSFUN_TRY(ar_join_est(c));
     cur = 1 - cur;
     gcur = orth.Gl;
     if (!fsync && j >= 2 && j < nl - 1) {
@@ -655,6 +664,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
       }
     }
   }
+  if (est_pending) PSFUN_TRY(ar_join_est(c));
   if (ran_all) {
     hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model,
                        nl_k, istop_k, eps, fscale, c->d_flags);
@@ -687,7 +697,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   LzCtl *h = (LzCtl *)((char *)c->h_pinned + 256);
   PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(LzCtl), hipMemcpyDeviceToHost, c->stream));
   PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
-  const int itx = h->itx;
+  const int itx = (h->done == LZ_DONE_CONVERGED) ? h->m : h->itx;
   psfun_prof_discard_after(c, PSFUN_PROF_AR_SPMV, itx + 1);
   psfun_prof_discard_after(c, PSFUN_PROF_AR_MGS, itx);
   const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
