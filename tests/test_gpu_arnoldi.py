@@ -206,3 +206,22 @@ def test_serial_dense_largest_orders(gpu_ctx, k):
         assert fun.apply(H, out, x) == 0
         Hm = H if v == "GENPADE" else np.triu(H, -1)
         assert _rel(out, sla.expm(-0.5 * Hm) @ x) <= 1e-12, v
+
+
+def test_arnoldi_eigenvector_start_stops_at_first_estimate(gpu_ctx):
+    """x an eigenvector: h(2,1) is at rounding level, so the first estimate
+    (index 2) converges: iter 1, res of length 2, y = exp(-lambda) x.  The
+    SpMV step of index 3 runs beside that estimate and may advance its loop
+    counter; the result keeps the reference's exit index."""
+    m = 20
+    A = matgen.laplacian((m, m), scale=-1.0)
+    i = np.arange(1, m + 1)
+    p, q = 3, 5
+    x = np.outer(np.sin(np.pi * i * p / (m + 1)), np.sin(np.pi * i * q / (m + 1))).reshape(-1)
+    lam = (2 - 2 * np.cos(np.pi * p / (m + 1))) + (2 - 2 * np.cos(np.pi * q / (m + 1)))
+    for variant in ("GENPADE", "CHBHES"):
+        info, it, err, res, y = _arnoldi_gpu(A, x, 1e-10, 30, variant=variant)
+        ref = O.arnoldi(A, x, 1e-10, 30, 1)
+        assert info == 0 and it == 1 == ref["iter"] and len(res) == 2
+        assert _rel(y, np.exp(-lam) * x) <= 1e-12
+        assert _rel(y, ref["y"]) <= 1e-12
