@@ -415,3 +415,20 @@ def test_two_contexts_in_two_threads(gpu_ctx):
         assert len(out[i]) == 6
         for y in out[i]:
             assert np.array_equal(y, refs[i])
+
+
+def test_diagonal_slices_empty_matrix(gpu_ctx):
+    """No entries: every slice has no diagonal; the SpMV gives beta*y and the
+    Lanczos apply of the zero operator gives y = exp(0) x = x."""
+    n = 100
+    A = (np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format("HDIA")
+    assert M.get_format()[0] == "HDIA" and M.hdia_diagonals() == 0
+    x = matgen.uniform_pm1(61, n)
+    y = _vec(np.full(n, 2.0))
+    psb.psb_spmm(1.0, M, _vec(x), 0.5, y, psb.psb_desc_type(n))
+    assert np.array_equal(y.get_vect(), np.full(n, 1.0))
+    info, it, err, res, yl = _lanczos_gpu(A, x, 1e-10, 30, fmt="HDIA")
+    assert info == 0
+    assert np.allclose(yl, x, rtol=0, atol=1e-14 * np.linalg.norm(x))
