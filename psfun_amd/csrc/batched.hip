@@ -193,6 +193,50 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
     }
     __syncthreads();
     if (!active) continue;
+    if (staged) {
+      // two rows per thread at a time (r and r + RP): their gathers are in
+      // flight together; each row's terms keep their entry order
+      const int *cl = s_col - e0;
+      const double *vl = s_val - e0;
+      for (int r = sr0 + sub; r < sr1; r += 2 * RP) {
+        const int r2 = r + RP;
+        const bool two = r2 < sr1;
+        const int s1 = s_rp[r - sr0], e1 = s_rp[r - sr0 + 1];
+        const int s2 = two ? s_rp[r2 - sr0] : 0, e2 = two ? s_rp[r2 - sr0 + 1] : 0;
+        const int lm = max(e1 - s1, e2 - s2);
+        const double xr1 = src[(int64_t)r * K + k];
+        const double xr2 = two ? src[(int64_t)r2 * K + k] : 0.0;
+        double sum1 = 0.0, sum2 = 0.0;
+        for (int t = 0; t < lm; t += 4) {
+          double a1[4], a2[4], g1[4], g2[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            const int q1 = s1 + t + m, q2 = s2 + t + m;
+            const bool ok1 = q1 < e1, ok2 = q2 < e2;
+            a1[m] = ok1 ? vl[q1] : 0.0;
+            a2[m] = ok2 ? vl[q2] : 0.0;
+            g1[m] = ok1 ? src[(int64_t)cl[q1] * K + k] : 0.0;
+            g2[m] = ok2 ? src[(int64_t)cl[q2] * K + k] : 0.0;
+          }
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            if (s1 + t + m < e1) sum1 = fma(a1[m], g1[m] * inv, sum1);
+            if (s2 + t + m < e2) sum2 = fma(a2[m], g2[m] * inv, sum2);
+          }
+        }
+        const double v1 = xr1 * inv, u1 = opscale * sum1;
+        if (j == 1) v_out[(int64_t)r * K + k] = xr1;  // basis slot 0 keeps x (v_1 = x / rs1)
+        u_out[(int64_t)r * K + k] = u1;
+        dot = fma(u1, v1, dot);
+        if (two) {
+          const double v2 = xr2 * inv, u2 = opscale * sum2;
+          if (j == 1) v_out[(int64_t)r2 * K + k] = xr2;
+          u_out[(int64_t)r2 * K + k] = u2;
+          dot = fma(u2, v2, dot);
+        }
+      }
+      continue;
+    }
     for (int r = sr0 + sub; r < sr1; r += RP) {
       const int s = s_rp[r - sr0], e = s_rp[r - sr0 + 1];
       const double xr = src[(int64_t)r * K + k];
