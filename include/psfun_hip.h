@@ -80,7 +80,11 @@ enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, PSFUN
 /* ---- context / errors --------------------------------------------------- */
 const char *psfun_strerror(int status);
 /* Create a context on HIP device `device`.  The context owns a default-priority
- * stream, a pinned host staging area and a grow-only device workspace. */
+ * stream, a pinned host staging area and a grow-only device workspace.
+ * Every call reuses them (and the loop flags the host paces on), so one
+ * context, with the matrices and batches created on it, serves one host
+ * thread at a time; concurrent callers use one context each (contexts on one
+ * device are independent). */
 int psfun_ctx_create(int device, psfun_ctx_t *ctx);
 int psfun_ctx_destroy(psfun_ctx_t ctx);
 /* Order all work on an external hipStream_t (e.g. the caller's current
