@@ -586,7 +586,9 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   const bool sell = psfun_reads_sell(A);
   const int L = psfun_row_per_thread(A) ? 1 : A->lanes;
   const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
-  int Gs = psfun_grid(c, chunks, psfun_row_per_thread(A) ? 8 : 4);
+  // diagonal slices: 6 streaming blocks per CU measured about 2% faster per
+  // f(A)x than 8 at 128^3
+  int Gs = psfun_grid(c, chunks, psfun_reads_hdia(A) ? 6 : (psfun_row_per_thread(A) ? 8 : 4));
   if (Gs + 1 > 8 * c->num_cu) Gs = (8 * c->num_cu - 1) / 8 * 8;
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);
 
