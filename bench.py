@@ -213,73 +213,90 @@ def headline(args):
                         + (8.0 * 64 + 4.0) * nd + 4.0 * ((n_ + 63) // 64 + 1))
     else:
         kernel_bytes = per_launch_bytes
-    achieved = per_launch_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    # `achieved`/`frac` count the bytes the timed kernel moves; the CSR-equivalent
+    # count of the same step (int32 column indices and rowptr: 12*nnz + 4(n+1) + 16n)
+    # is reported beside it under csr_equiv_*
+    achieved = kernel_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    csr_equiv = per_launch_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     upd_us = 1e3 * ku["ms"] / max(ku["launches"], 1)
     upd_bytes = ku["bytes"] / max(ku["launches"], 1)
+    kname = {"SELL": "k_lz_spmv_sell", "HDIA": "k_lz_spmv_hdia"}.get(fmt, f"k_lz_spmv<{lanes}>")
+    per_unit = {"SELL": f"{idx_bits // 8 + 8}*nnz (values + {idx_bits}-bit column offsets) + 16n",
+                "HDIA": "(8*64 + 4)*slice_diagonals + 4*(slices + 1) + 16n (no column indices)"}.get(
+                    fmt, "12*nnz + 4(n+1) + 16n")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (pmc_traffic("k_lz_spmv_" + fmt.lower(), f"lz_spmv_{fmt.lower()}_128cube_pmc.txt")
-                            if fmt in ("SELL", "HDIA") and m == 128 else None),
-                "kernel": ({"SELL": "k_lz_spmv_sell", "HDIA": "k_lz_spmv_hdia"}.get(fmt, f"k_lz_spmv<{lanes}>"))
-                + " (SpMV 12*nnz+4(n+1) algorithmic matrix bytes + w read and u written: 16n;"
-                " the first step also copies x into the basis: 24n)",
-                # the sliced-ELL step reads 16-bit column offsets when they fit (10 bytes
-                # per nonzero instead of the 12 the algorithmic count assumes); the
-                # diagonal-slice step reads no column indices at all (8 bytes per stored
-                # value), so `achieved` (algorithmic bytes) can exceed what the kernel
-                # moves; kernel_* give the bytes it does move
-                "index_bits": idx_bits,
-                "kernel_bytes_per_launch": kernel_bytes,
-                "kernel_achieved_GBs": round(kernel_bytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 else None,
-                "kernel_frac": round(kernel_bytes / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if avg_us > 0 else None,
-                "note": ("achieved/frac count the CSR bytes with int32 column indices (12*nnz + 4(n+1) + 16n); "
-                         "the diagonal-slice step reads no indices, so frac can exceed 1; kernel_frac is the "
-                         "fraction of peak of the bytes it moves, which the PMC traffic confirms"
-                         if fmt == "HDIA" else
-                         "achieved/frac count the CSR bytes with int32 column indices; kernel_frac is the "
-                         "fraction of peak of the bytes the kernel moves"),
-                "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_us, 3),
-                "launches": k["launches"], "launches_per_event_pair": args.prof_reps,
+                "traffic": (pmc_traffic(kname, f"lz_spmv_{fmt.lower()}_{m}cube_pmc.txt")
+                            if fmt in ("SELL", "HDIA") else None),
+                "kernel": kname + " (Lanczos SpMV step, alpha partials fused)",
+                "bytes_per_launch": kernel_bytes, "bytes_per_unit": per_unit,
+                "avg_launch_us": round(avg_us, 3), "launches": k["launches"],
+                "launches_per_event_pair": args.prof_reps, "index_bits": idx_bits,
+                "slice_diagonals": (M.hdia_diagonals() if fmt == "HDIA" else None),
+                "csr_equiv_bytes_per_launch": per_launch_bytes,
+                "csr_equiv_GBs": round(csr_equiv, 1),
+                "csr_equiv_frac": round(csr_equiv / HBM_PEAK_GBS, 4),
+                "note": ("achieved/frac: bytes the kernel moves per launch over its HIP-event-timed launch "
+                         "duration (the first step of each apply also copies x: +8n, included); csr_equiv_*: "
+                         "the same step charged the CSR count 12*nnz + 4(n+1) + 16n, which exceeds what a "
+                         "format without column indices reads"),
                 "update_kernel": {"name": "k_lz_update (3-term update + ||w||^2 partials, 32n bytes)",
                                   "avg_launch_us": round(upd_us, 3),
                                   "achieved_GBs": round(upd_bytes / (upd_us * 1e-6) / 1e9, 1) if upd_us else None}}
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
-        O.set_threads(threads)
-        runs = 0
-        tc0 = time.perf_counter()
-        ref = None
-        while True:
-            ref = O.lanczos(A, x, args.eps, args.itmax, 1)
-            runs += 1
-            if time.perf_counter() - tc0 > args.cpu_seconds or runs >= 50:
-                break
-        tc = time.perf_counter() - tc0
-        # the same restatement on one core, a shorter sample (about 5 s)
-        O.set_threads(1)
-        runs1 = 0
-        t10 = time.perf_counter()
-        while True:
-            O.lanczos(A, x, args.eps, args.itmax, 1)
-            runs1 += 1
-            if time.perf_counter() - t10 > args.cpu_seconds / 3 or runs1 >= 20:
-                break
-        rate1 = runs1 / (time.perf_counter() - t10)
-        O.set_threads(threads)
-        cpu = {"value": round(runs / tc, 4), "unit": "f(A)x/s", "cores": threads, "kind": "port",
-               "value_1core": round(rate1, 4), "sample_1core": f"{runs1} full f(A)x on 1 core",
-               "sample": f"{runs} full f(A)x of the same {m}^3 workload (oracle/psfun_oracle.c: unfused "
-                         f"PSBLAS-style passes, OpenMP over rows, f(T) by tridiagonal QL every iteration)",
-               "gpu_vs_cpu_rel_diff": float(np.linalg.norm(y - ref["y"]) / np.linalg.norm(ref["y"])),
-               "cpu_iter": ref["iter"]}
-        log(f"cpu baseline: {cpu['value']} f(A)x/s on {threads} threads")
+        cpu = cpu_baseline(args, A, x, y, m)
     cfg = {"workload": f"lanczos exp(-tA)x, 3-D 7-point Dirichlet Laplacian {m}^3", "n": n, "nnz": nnz, "t": t,
            "itmax": args.itmax, "eps": args.eps, "format": fmt, "lanes_per_row": lanes, "global_batch": 1,
            "parallelism": "single-gpu"}
     return value, elapsed, it, rel_err, roofline, cpu, cfg
+
+
+def cpu_baseline(args, A, x, y, m):
+    """The oracle (the reference's Lanczos restated in C: unfused PSBLAS-style
+    passes, OpenMP over rows, f(T) every iteration) on this host's cores: one
+    warm-up run, then the median of at least 5 timed runs, on every core the
+    process may use (bounded by OMP_NUM_THREADS when that is set) and on 1 core."""
+    import statistics
+    from oracle import oracle as O
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    threads = max(1, min(threads, avail))
+
+    def timed(nthreads, min_runs, budget_s):
+        O.set_threads(nthreads)
+        ref = O.lanczos(A, x, args.eps, args.itmax, 1)           # warm-up
+        times = []
+        t_all = time.perf_counter()
+        while len(times) < min_runs or (time.perf_counter() - t_all < budget_s and len(times) < 101):
+            t0 = time.perf_counter()
+            ref = O.lanczos(A, x, args.eps, args.itmax, 1)
+            times.append(time.perf_counter() - t0)
+        return ref, times
+
+    ref, tm = timed(threads, 5, args.cpu_seconds * 2 / 3)
+    _, t1 = timed(1, 5, args.cpu_seconds / 3)
+    O.set_threads(threads)
+    med, med1 = statistics.median(tm), statistics.median(t1)
+    out = {"value": round(1.0 / med, 4), "unit": "f(A)x/s", "cores": threads, "kind": "port",
+           "nproc": nproc, "cpus_available": avail, "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+           "protocol": "1 warm-up, then the median of the timed runs (wall clock around the apply)",
+           "runs": len(tm), "run_ms_min_median_max": [round(1e3 * min(tm), 2), round(1e3 * med, 2),
+                                                      round(1e3 * max(tm), 2)],
+           "value_1core": round(1.0 / med1, 4), "runs_1core": len(t1),
+           "sample": f"{len(tm)} full f(A)x of the same {m}^3 workload after one warm-up "
+                     f"(oracle/psfun_oracle.c: unfused PSBLAS-style passes, OpenMP over rows, f(T) by tridiagonal "
+                     f"QL every iteration)",
+           "gpu_vs_cpu_rel_diff": float(np.linalg.norm(y - ref["y"]) / np.linalg.norm(ref["y"])),
+           "cpu_iter": ref["iter"]}
+    log(f"cpu baseline: {out['value']} f(A)x/s on {threads} threads (nproc {nproc}, available {avail}), "
+        f"{out['value_1core']} on 1 core")
+    return out
 
 
 def batched_section(args):

@@ -49,6 +49,10 @@ def lib():
         L.or_lanczos.restype = C.c_int
         L.or_arnoldi.argtypes = kry
         L.or_arnoldi.restype = C.c_int
+        _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.or_lanczos_batch.argtypes = [C.c_int, _lp, _ip, _ip, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                                       C.c_double, C.c_int, C.c_int, _ip, _dp]
+        L.or_lanczos_batch.restype = C.c_int
         L.or_lanczos_f.argtypes = kry + [DFUN]
         L.or_lanczos_f.restype = C.c_int
         L.or_arnoldi_f.argtypes = kry + [DFUN]
@@ -164,6 +168,24 @@ def lanczos(A, x, eps, itmax, istop=1, fscale=1.0, opscale=1.0, f=None):
     if f is None:
         return _krylov(lib().or_lanczos, A, x, eps, itmax, istop, fscale, opscale)
     return _krylov(lib().or_lanczos_f, A, x, eps, itmax, istop, fscale, opscale, _dfun(f))
+
+
+def lanczos_batch(row_offsets, A, X, eps, itmax, istop=1, fscale=1.0, opscale=1.0):
+    """or_lanczos on every (graph, column) instance of a block-diagonal batch
+    (row_offsets, global-column CSR A, row-major X of nrows x K), the
+    instances in parallel; returns (Y, iters (ngraph, K), errs (ngraph, K))."""
+    rp, ci, va = _csr(A)
+    off = np.ascontiguousarray(row_offsets, np.int64)
+    ng = len(off) - 1
+    X = np.ascontiguousarray(X, np.float64)
+    K = 1 if X.ndim == 1 else X.shape[1]
+    Y = np.zeros_like(X)
+    iters = np.zeros(ng * K, np.int32)
+    errs = np.zeros(ng * K)
+    rc = lib().or_lanczos_batch(ng, off, rp, ci, va, K, fscale, opscale, X, Y, eps, itmax, istop, iters, errs)
+    if rc:
+        raise RuntimeError(f"or_lanczos_batch failed: {rc}")
+    return Y, iters.reshape(ng, K), errs.reshape(ng, K)
 
 
 def arnoldi(A, x, eps, itmax, istop=1, fscale=1.0, scaling=1.0, f=None):

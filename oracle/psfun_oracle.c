@@ -21,6 +21,7 @@
  *                                         scipy sqrtm on small SPD matrices
  *   or_lanczos                         -- DST closed form exp(-tA) for Dirichlet Laplacians,
  *                                         scipy dense expm on small matrices, breakdown cases
+ *   or_lanczos_batch                   -- bitwise equal to or_lanczos run instance by instance
  *   or_arnoldi                         -- same oracles as or_lanczos
  *   or_sai_arnoldi / or_dense_inverse  -- DST closed form, dense expm, A*inverse = I
  *   or_cg_shifted                      -- dense solve, residual check, PSBLAS stop rules
@@ -511,6 +512,46 @@ int or_lanczos(int n, const int *rp, const int *ci, const double *va, double fsc
                const double *x, double *y, double eps, int itmax, int istop, int *iter, double *err,
                double *res, int *nres) {
   return or_lanczos_f(n, rp, ci, va, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, nres, NULL);
+}
+
+/* Many independent or_lanczos runs: graph g owns rows off[g]..off[g+1]-1 of
+ * one block-diagonal 0-based CSR (columns global, inside the graph's rows);
+ * X and Y are row-major nrows x K, instance g*K+k takes column k of graph g.
+ * Each instance is exactly or_lanczos on the graph's own CSR (local column
+ * numbers); the loop over instances is what runs in parallel, every instance
+ * itself single-threaded.  iters/errs (ngraph*K) may be NULL.  Returns the
+ * first nonzero status, else 0. */
+int or_lanczos_batch(int ngraph, const int64_t *off, const int *rp, const int *ci, const double *va, int K,
+                     double fscale, double opscale, const double *X, double *Y, double eps, int itmax, int istop,
+                     int *iters, double *errs) {
+  int status = 0;
+  const int saved = g_threads;
+  g_threads = 1; /* the primitives run serially inside each instance */
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : status)
+  for (int64_t inst = 0; inst < (int64_t)ngraph * K; ++inst) {
+    const int g = (int)(inst / K), k = (int)(inst % K);
+    const int64_t r0 = off[g];
+    const int n = (int)(off[g + 1] - r0);
+    const int base = rp[r0], nz = rp[r0 + n] - base;
+    int *lrp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *lci = (int *)malloc(sizeof(int) * (size_t)(nz > 0 ? nz : 1));
+    double *x = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *res = (double *)malloc(sizeof(double) * (size_t)(itmax + 2));
+    for (int i = 0; i <= n; ++i) lrp[i] = rp[r0 + i] - base;
+    for (int j = 0; j < nz; ++j) lci[j] = ci[base + j] - (int)r0;
+    for (int i = 0; i < n; ++i) x[i] = X[(r0 + i) * K + k];
+    int it = 0, nres = 0;
+    double er = 0.0;
+    const int rc = or_lanczos(n, lrp, lci, va + base, fscale, opscale, x, y, eps, itmax, istop, &it, &er, res, &nres);
+    for (int i = 0; i < n; ++i) Y[(r0 + i) * K + k] = y[i];
+    if (iters) iters[inst] = it;
+    if (errs) errs[inst] = er;
+    if (rc > status) status = rc;
+    free(lrp); free(lci); free(x); free(y); free(res);
+  }
+  g_threads = saved;
+  return status;
 }
 
 /* ------------------------------------------------------------------ */

@@ -84,7 +84,10 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
         const int ok = (cden == 0.0) ? (cnum <= eps) : (cnum <= eps * cden);
         if (ok) ctl->done = CG_CONV;
         else if (ctl->itx >= itmax) ctl->done = CG_MAXIT;
-        else if (rho_old == 0.0) ctl->done = CG_BRK;  // psfun_dcg.F90:290-295
+        else if (rho_old == 0.0) {  // psfun_dcg.F90:290-295
+          ctl->done = CG_BRK;
+          ctl->itx += 1;  // the reference counts the iteration that breaks down (itx = itx + 1 at 281)
+        }
       }
       if (!ctl->done) ctl->itx += 1;
       host_flag_store(hf, it);
@@ -123,13 +126,16 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
                                                    const double *__restrict__ dinv, int it,
                                                    const double *__restrict__ p_sig, int g_sig,
                                                    double *__restrict__ p_rho, double *__restrict__ p_conv,
-                                                   CgCtl *ctl, int istop) {
+                                                   CgCtl *ctl, int istop, int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
   const double sigma = block_sum_partials<PSFUN_BS>(p_sig, g_sig, lds);
   const double rho = ctl->rho[it & 1];
   if (sigma == 0.0) {  // psfun_dcg.F90:305-310
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = CG_BRK;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->done = CG_BRK;
+      host_flag_store(hf + 1, CG_BRK);  // the later k_cg_a launches return before their control block
+    }
     return;
   }
   const double alpha = rho / sigma;
@@ -372,7 +378,7 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
       PSFUN_HIP_TRY(hipGetLastError());
       PSFUN_TRY(psfun_prof_end(c));
       hipLaunchKernelGGL(k_cg_b, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, x, r, pnew, q, dinv, it, Psig, Gs, Prho,
-                         Pconv, ctl, istop);
+                         Pconv, ctl, istop, c->d_flags);
       PSFUN_HIP_TRY(hipGetLastError());
       std::swap(pold, pnew);
       // pacing on the host flags: about two iterations queued ahead of the

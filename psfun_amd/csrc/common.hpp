@@ -362,6 +362,14 @@ __device__ __forceinline__ XcdRange xcd_range_c0(int64_t nchunks, int G) {
   return r;
 }
 
+// Operand of a padded-format product v * x: the padding slots of sliced ELL
+// and diagonal slices hold v = 0 and point at some in-range x entry, which
+// CSR would never read; taking x as 0 there keeps a non-finite x entry from
+// turning unrelated rows into NaN (0 * Inf), so these formats agree with CSR
+// on such input.  For finite x the product and the sum are bitwise unchanged.
+// (An explicitly stored 0 of the matrix is treated like padding.)
+__device__ __forceinline__ double pad_x(double v, double x) { return v != 0.0 ? x : 0.0; }
+
 // Sum over the sliced-ELL entries of `row`, gathering x[col] * xs.
 __device__ __forceinline__ double sell_row_dot(int row, const int *__restrict__ sp, const int *__restrict__ col,
                                                const double *__restrict__ val, const double *__restrict__ x,
@@ -373,7 +381,10 @@ __device__ __forceinline__ double sell_row_dot(int row, const int *__restrict__ 
   const double *vp = val + b + l;
   double sum = 0.0;
 #pragma unroll 4
-  for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], x[cp[k * 64]] * xs, sum);
+  for (int k = 0; k < w; ++k) {
+    const double v = vp[k * 64];
+    sum = fma(v, pad_x(v, x[cp[k * 64]] * xs), sum);
+  }
   return sum;
 }
 
@@ -389,7 +400,10 @@ __device__ __forceinline__ double sell_row_dot16(int row, const int *__restrict_
   const double *vp = val + b + l;
   double sum = 0.0;
 #pragma unroll 4
-  for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], x[row + (int)cp[k * 64]] * xs, sum);
+  for (int k = 0; k < w; ++k) {
+    const double v = vp[k * 64];
+    sum = fma(v, pad_x(v, x[row + (int)cp[k * 64]] * xs), sum);
+  }
   return sum;
 }
 
@@ -411,7 +425,8 @@ __device__ __forceinline__ double hdia_row_dot(int row, int n, const int *__rest
 #pragma unroll 4
   for (int k = b; k < e; ++k) {
     const int c = min(max(row + off[k], 0), n - 1);
-    sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], SCALED ? x[c] * xs : x[c], sum);
+    const double v = val[(int64_t)k * PSFUN_SELL_C + l];
+    sum = fma(v, pad_x(v, SCALED ? x[c] * xs : x[c]), sum);
   }
   return sum;
 }
@@ -439,7 +454,10 @@ __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict_
       const int s = __builtin_amdgcn_readfirstlane(row >> 6), l = row & 63;
       const int b = rp[s], e = rp[s + 1];
 #pragma unroll 4
-      for (int k = b; k < e; ++k) sum = fma(val[(int64_t)k * PSFUN_SELL_C + l], g(min(max(row + col[k], 0), n - 1)), sum);
+      for (int k = b; k < e; ++k) {
+        const double v = val[(int64_t)k * PSFUN_SELL_C + l];
+        sum = fma(v, pad_x(v, g(min(max(row + col[k], 0), n - 1))), sum);
+      }
     }
   } else if (SELL) {
     if (row < n) {
@@ -449,7 +467,10 @@ __device__ __forceinline__ double row_dot(int row, int n, const int *__restrict_
       const int *cp = col + b + l;
       const double *vp = val + b + l;
 #pragma unroll 4
-      for (int k = 0; k < w; ++k) sum = fma(vp[k * 64], g(cp[k * 64]), sum);
+      for (int k = 0; k < w; ++k) {
+        const double v = vp[k * 64];
+        sum = fma(v, pad_x(v, g(cp[k * 64])), sum);
+      }
     }
   } else {
     if (row < n) {

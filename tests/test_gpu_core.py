@@ -203,6 +203,32 @@ def test_lanczos_sell_index_widths(gpu_ctx):
         assert np.linalg.norm(ys["SELL"] - ys["CSR"]) <= 1e-12 * np.linalg.norm(ys["CSR"])
 
 
+def test_spmv_nonfinite_x_same_in_every_format(gpu_ctx):
+    """One Inf and one NaN entry of x: in every storage format exactly the
+    rows whose stored entries meet them are non-finite (the padding of
+    sliced ELL and diagonal slices never multiplies a real x entry); the
+    Lanczos SpMV steps read the same row sums."""
+    dims = (20, 13)
+    A = matgen.laplacian(dims)
+    n = 260
+    x = matgen.uniform_pm1(12, n)
+    x[57], x[200] = np.inf, np.nan
+    Asp = matgen.to_scipy(A)
+    hit = np.asarray(Asp[:, [57, 200]].getnnz(axis=1) > 0).ravel()
+    ref = O.spmv(A, x)
+    assert np.array_equal(~np.isfinite(ref), hit)
+    desc = psb.psb_desc_type(n)
+    for fmt in ("CSR", "SELL", "HDIA"):
+        M = psb.psb_dspmat_type.from_csr(A)
+        M.set_format(fmt)
+        y = _vec(np.zeros(n))
+        assert psb.psb_spmm(1.0, M, _vec(x), 0.0, y, desc) == 0
+        got = y.get_vect()
+        assert np.array_equal(~np.isfinite(got), hit), fmt
+        assert np.array_equal(np.isnan(got), np.isnan(ref)), fmt
+        assert np.max(np.abs(got[~hit] - ref[~hit])) <= 1e-13, fmt
+
+
 def test_spmv_one_based_input(gpu_ctx):
     g = load_golden("laplacian50.npz")
     M = psb.psb_dspmat_type(g["rowptr"], g["colind"], g["val"], index_base=1)
@@ -358,16 +384,22 @@ def test_lanczos_errors(gpu_ctx):
     assert km.apply(bad, M, desc, y, _vec(np.ones(6)), 1e-8)[0] == psb.psb_err_from_subroutine_non_
 
 
-def test_lanczos_full_size_3d_128(gpu_ctx):
-    """128^3 Dirichlet Laplacian (n = 2,097,152): size-independent check
-    against the closed-form DST solution."""
+@pytest.mark.parametrize("fmt", ["AUTO", "HDIA"])
+def test_lanczos_full_size_3d_128(gpu_ctx, fmt):
+    """128^3 Dirichlet Laplacian (n = 2,097,152), on the automatic choice
+    (sliced ELL) and on the diagonal slices the headline runs: against the
+    closed-form DST solution and the oracle at 1e-10, iterations within 1."""
     dims = (128, 128, 128)
     A = matgen.laplacian(dims, scale=-1.0)
     x = matgen.uniform_pm1(2024, 128 ** 3)
-    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, 50)
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, 50, fmt=fmt)
     yex = matgen.dst_expm_laplacian(dims, x, 1.0)
     assert info == 0
     assert np.linalg.norm(y - yex) / np.linalg.norm(yex) <= 1e-10
+    ref = O.lanczos(A, x, 1e-10, 50, 1)
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    assert abs(it - ref["iter"]) <= 1
+    assert len(res) == it + 1 and err < 1e-10
 
 
 def test_two_contexts_in_two_threads(gpu_ctx):

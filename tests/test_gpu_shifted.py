@@ -80,6 +80,23 @@ def test_cg_itmax_and_unknown_method(gpu_ctx):
     assert np.linalg.norm(x.get_vect() - ref["x"]) <= 1e-12 * np.linalg.norm(ref["x"])
 
 
+@pytest.mark.parametrize("itmax", [1, 2, 7, 8])
+def test_cg_breakdown_restarts_match_oracle(gpu_ctx, itmax):
+    """Indefinite DIAG preconditioner on 2x2 systems whose CG breaks down at
+    rho_old = 0 (psfun_dcg.F90:290-295) and at sigma = 0 (305-310): the
+    restarts count the breaking iteration as the reference does, so the
+    iteration count, x and err equal the oracle's."""
+    from test_oracle import CG_RHO_BRK, CG_SIG_BRK
+    for A in (CG_RHO_BRK, CG_SIG_BRK):
+        M = psb.psb_dspmat_type.from_csr(A)
+        x = _vec(np.zeros(2))
+        info, it, err, _ = psb_krylov("CG", M, _prec("DIAG"), _vec(np.ones(2)), 1.0, 0.0, x, 1e-8,
+                                      psb.psb_desc_type(2), itmax=itmax, istop=2)
+        ref = O.cg_shifted(A, np.ones(2), 1.0, 0.0, eps=1e-8, itmax=itmax, istop=2, prec=1)
+        assert info == 0 and it == ref["iter"] == itmax
+        assert np.array_equal(x.get_vect(), ref["x"]) and err == ref["err"]
+
+
 def test_cg_zero_rhs(gpu_ctx):
     A = matgen.laplacian((10, 10))
     M = psb.psb_dspmat_type.from_csr(A)
@@ -137,13 +154,17 @@ def test_quad_hht_sqrt_fixture(gpu_ctx):
     assert np.linalg.norm(y.get_vect() - g["y_sqrt"]) <= 1e-8 * np.linalg.norm(g["y_sqrt"])
 
 
-def test_quad_talbot16_3d_96_closed_form(gpu_ctx):
+@pytest.mark.parametrize("fmt", ["AUTO", "HDIA"])
+def test_quad_talbot16_3d_96_closed_form(gpu_ctx, fmt):
     """exp(-A)x on the 96^3 Dirichlet Laplacian (n = 884,736) with the
-    16-pole Talbot rule: relative error <= 1e-8 against the DST solution."""
+    16-pole Talbot rule, on the automatic storage choice and on the diagonal
+    slices the benchmark runs: relative error <= 1e-8 against the DST
+    solution and against the oracle's multi-shift restatement."""
     dims = (96, 96, 96)
     A = matgen.laplacian(dims)
     x = matgen.uniform_pm1(4, 96 ** 3)
     M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format(fmt)
     q = Q.psfun_z_quadrature()
     q.computepoles(Q.talbot, None, 16, rparams=[1.0])
     q.set(M)
@@ -152,3 +173,5 @@ def test_quad_talbot16_3d_96_closed_form(gpu_ctx):
     yex = matgen.dst_expm_laplacian(dims, x, 1.0)
     assert info == 0
     assert np.linalg.norm(y.get_vect() - yex) / np.linalg.norm(yex) <= 1e-8
+    ref = O.quad_apply(A, q.sign * q.xi, q.c, q.eta * q.sign, 0, x, eps=1e-10, itmax=3000)
+    assert np.linalg.norm(y.get_vect() - ref["y"]) <= 1e-8 * np.linalg.norm(ref["y"])

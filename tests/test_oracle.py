@@ -189,6 +189,36 @@ def test_lanczos_invalid_istop():
     assert O.lanczos(A, np.ones(5), 1e-8, 10, 3)["info"] == 2
 
 
+def test_lanczos_batch_is_or_lanczos_per_instance():
+    """The batched restatement runs or_lanczos on every (graph, column)
+    instance: bitwise the single-threaded per-instance result, also for a
+    1-node graph, an eigenvector column (breakdown) and a zero column (NaN,
+    as the reference's 1/||x|| gives)."""
+    from psfun_amd.batched import concat_block_diagonal
+    sizes = [1000, 1, 50, 30, 2000]
+    mats = [matgen.rgg_laplacian(n, 300 + i) for i, n in enumerate(sizes)]
+    mats[3] = matgen.laplacian((30,))
+    off, rp, ci, va = concat_block_diagonal(mats)
+    K = 3
+    X = matgen.normal(8, sum(sizes) * K).reshape(sum(sizes), K)
+    X[off[3]:off[4], 1] = np.sin(np.arange(1, 31) * 2 * np.pi / 31)
+    X[off[2]:off[3], 2] = 0.0
+    saved = O.max_threads()
+    try:
+        O.set_threads(4)
+        Y, it, er = O.lanczos_batch(off, (rp, ci, va), X, 1e-10, 40, 1, opscale=-0.5)
+        O.set_threads(1)
+        for g, A in enumerate(mats):
+            for k in range(K):
+                r = O.lanczos(A, X[off[g]:off[g + 1], k], 1e-10, 40, 1, opscale=-0.5)
+                assert np.array_equal(r["y"], Y[off[g]:off[g + 1], k], equal_nan=True), (g, k)
+                assert r["iter"] == it[g, k]
+                assert r["err"] == er[g, k] or (np.isnan(r["err"]) and np.isnan(er[g, k]))
+    finally:
+        O.set_threads(saved)
+    assert it[3, 1] == 1
+
+
 def test_arnoldi_dst_closed_form():
     A, x, yex = _dirichlet_case((30, 30), 2)
     r = O.arnoldi(A, x, 1e-10, 40, 1)
@@ -230,6 +260,27 @@ def test_cg_itmax_stops():
     A = matgen.laplacian((20, 20))
     r = O.cg_shifted(A, np.ones(400), 1.0, 0.0, eps=1e-30, itmax=7, istop=2)
     assert r["iter"] == 7
+
+
+# Breakdowns with an indefinite DIAG preconditioner (psfun_dcg.F90:280-310),
+# traced by hand.  b = (1, 1), dinv = (1, -1) gives rho = r.z = 0 at it = 1.
+#  * A = [[1, .5], [.5, -1]]: sigma = p.Ap = -1, alpha = 0 (x, r unchanged);
+#    it = 2 has rho_old = 0: breakdown, counted (itx = itx + 1 comes first);
+#    so every restart costs 2 iterations, and a check at itx = itmax ends it.
+#  * A = diag(1, -1): p = z = (1, -1), sigma = p.Ap = 0 at it = 1: each
+#    restart costs one iteration until itx = itmax.
+CG_RHO_BRK = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([1.0, 0.5, 0.5, -1.0]))
+CG_SIG_BRK = (np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([1.0, -1.0]))
+
+
+@pytest.mark.parametrize("itmax", [1, 2, 7, 8])
+def test_cg_breakdown_restarts_count_iterations(itmax):
+    r = O.cg_shifted(CG_RHO_BRK, np.ones(2), 1.0, 0.0, eps=1e-8, itmax=itmax, istop=2, prec=1)
+    assert r["iter"] == itmax and np.all(r["x"] == 0.0)
+    assert r["err"] == 1.0       # ||r|| / ||b|| of the last check (x = 0)
+    r = O.cg_shifted(CG_SIG_BRK, np.ones(2), 1.0, 0.0, eps=1e-8, itmax=itmax, istop=2, prec=1)
+    assert r["iter"] == itmax and np.all(r["x"] == 0.0)
+    assert r["err"] == 0.0       # no convergence check ran (psb_init_conv's errnum = 0)
 
 
 # ---- multi-shift CG, quadrature --------------------------------------------
