@@ -86,6 +86,9 @@ const char *psfun_strerror(int status);
  * thread at a time; concurrent callers use one context each (contexts on one
  * device are independent). */
 int psfun_ctx_create(int device, psfun_ctx_t *ctx);
+/* Matrices and batches created on a context may still be destroyed after it
+ * (they keep their device, not the context); any other use of them after
+ * their context is gone is an error. */
 int psfun_ctx_destroy(psfun_ctx_t ctx);
 /* Order all work on an external hipStream_t (e.g. the caller's current
  * stream); NULL selects the HIP null stream. */

@@ -39,6 +39,7 @@ struct BtCtl {
 
 struct psfun_batch {
   psfun_ctx *ctx = nullptr;
+  int device = 0;  // ctx->device, kept so destroy never reads the context
   int ngraph = 0;
   int64_t nrows = 0, nnz = 0;
   int *rowptr = nullptr, *colind = nullptr;
@@ -579,6 +580,7 @@ static int bt_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, cons
   PSFUN_HIP_TRY(hipSetDevice(c->device));
   psfun_batch *B = new psfun_batch();
   B->ctx = c;
+  B->device = c->device;
   B->ngraph = ngraph;
   B->nrows = nrows;
   B->nnz = nnz;
@@ -659,8 +661,8 @@ extern "C" int psfun_batch_create_ordered(psfun_ctx_t c, int ngraph, const int64
 
 extern "C" int psfun_batch_destroy(psfun_batch_t B) {
   if (!B) return PSFUN_SUCCESS;
-  hipSetDevice(B->ctx->device);
-  hipStreamSynchronize(B->ctx->stream);
+  hipSetDevice(B->device);  // the context may already be gone
+  hipDeviceSynchronize();
   for (auto &kv : B->graphs) hipGraphExecDestroy(kv.second);
   if (B->ev_in) hipEventDestroy(B->ev_in);
   if (B->ev_out) hipEventDestroy(B->ev_out);

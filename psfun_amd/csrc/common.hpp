@@ -96,6 +96,7 @@ int psfun_prof_collect(psfun_ctx *c);
 
 struct psfun_csr {
   psfun_ctx *ctx = nullptr;
+  int device = 0;  // ctx->device, kept so destroy never reads the context
   int n = 0;
   int64_t nnz = 0;
   int *rowptr = nullptr;

@@ -1,5 +1,6 @@
 // Context, workspace, CSR handles and device vectors of the C-ABI.
 #include "common.hpp"
+#include "kernels.hpp"
 
 #include <stdio.h>
 #include <string.h>
@@ -409,6 +410,7 @@ extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int
   PSFUN_HIP_TRY(hipSetDevice(c->device));
   psfun_csr *A = new psfun_csr();
   A->ctx = c;
+  A->device = c->device;
   A->n = (int)n;
   A->nnz = nnz;
   std::vector<int> host_rp;
@@ -487,16 +489,19 @@ extern "C" int psfun_csr_create(psfun_ctx_t c, int64_t n, int64_t nnz, const int
 
 extern "C" int psfun_csr_destroy(psfun_csr_t A) {
   if (!A) return PSFUN_SUCCESS;
+  // The context may already be gone (a caller that destroys it first): only
+  // the device recorded at creation is used here, and a device-wide wait
+  // stands in for the context stream's.
+  if (A->owns || A->sell_nnz || A->sell_ptr || A->hdia_ptr) {
+    hipSetDevice(A->device);
+    hipDeviceSynchronize();
+  }
   if (A->owns) {
-    hipSetDevice(A->ctx->device);
-    hipStreamSynchronize(A->ctx->stream);
     if (A->rowptr) hipFree(A->rowptr);
     if (A->colind) hipFree(A->colind);
     if (A->val) hipFree(A->val);
   }
   if (A->sell_nnz || A->sell_ptr || A->hdia_ptr) {
-    hipSetDevice(A->ctx->device);
-    hipStreamSynchronize(A->ctx->stream);
     sell_free(A);
     hdia_free(A);
   }

@@ -422,7 +422,7 @@ __global__ void k_lz_init(LzCtl *ctl) {
 // ---------------------------------------------------------------------------
 static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, const double *src, int j, int G,
                           const double *p_in, int g_in, double *v_out, double *u_out, double *p_out, LzCtl *ctl,
-                          int nl, int nl_ctl, int istop, double eps, double fscale) {
+                          int nl, int nl_ctl, int istop, double eps, double fscale, bool prof) {
   const int L = A->lanes;
   // algorithmic bytes: val+colind, rowptr, src rows read, u written (and x
   // copied into basis slot 0 at j = 1)
@@ -430,8 +430,8 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   // The step is idempotent (it reads src and the partials and rewrites v_j, u
   // and its own partials and control fields with the same values), so timing
   // may repeat it back to back inside one event pair to amortise the events.
-  const int reps = psfun_prof_reps(c);
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
+  const int reps = prof ? psfun_prof_reps(c) : 1;
+  if (prof) PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_SPMV, bytes, j, reps));
   if (psfun_reads_hdia(A)) {
     for (int r = 0; r < reps; ++r) {
       if (nl <= 65)
@@ -444,7 +444,7 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
                            istop, eps, fscale, c->d_flags);
     }
     PSFUN_HIP_TRY(hipGetLastError());
-    PSFUN_TRY(psfun_prof_end(c));
+    if (prof) PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;
   }
   if (psfun_reads_sell(A)) {
@@ -464,7 +464,7 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
     }
 #undef LZ_SELL
     PSFUN_HIP_TRY(hipGetLastError());
-    PSFUN_TRY(psfun_prof_end(c));
+    if (prof) PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;
   }
 #define LZ_LAUNCH(LL)                                                                                          \
@@ -489,38 +489,26 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
   }
 #undef LZ_LAUNCH
   PSFUN_HIP_TRY(hipGetLastError());
-  PSFUN_TRY(psfun_prof_end(c));
+  if (prof) PSFUN_TRY(psfun_prof_end(c));
   return PSFUN_SUCCESS;
 }
 
-extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, double fscale, double opscale,
-                                   const double *x, double *y, double eps, int itmax, int istop, int *iter,
-                                   double *err, double *res, int *nres) {
-  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
-  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
-  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
-  const bool userf = fname == PSFUN_FN_USERF;
-  if (userf)
-    PSFUN_TRY(userf_check(c, PSFUN_VAR_SYM));
-  else if (fname != PSFUN_FN_EXP)
-    return PSFUN_ERR_UNSUPPORTED;
-  if (itmax < 2 || itmax > LZ_NL_MAX) return PSFUN_ERR_INVALID_ARG;
+// Bytes of the loop workspace of one run: the control block, then the basis
+// slots 0..nl-1 (slot 0 holds x, slot l the unnormalised w_l, u_l until the
+// update of step l has run), each ldv = align64(n) doubles.
+static size_t lz_ws_bytes(int64_t n, int nl) {
+  return align_up(sizeof(LzCtl), 256) + sizeof(double) * align_up((size_t)n, 64) * (size_t)nl;
+}
+
+// The Lanczos run on a caller-provided workspace `ws` (lz_ws_bytes) and
+// partial-sum lists Pw/Pa (PSFUN_MAX_GRID doubles each); `prof` enables the
+// per-kernel event timing.
+static int lz_apply(psfun_ctx *c, psfun_csr *A, bool userf, double fscale, double opscale, const double *x,
+                    double *y, double eps, int nl, int istop, int *iter, double *err, double *res, int *nres,
+                    void *ws, double *Pw, double *Pa, bool prof) {
   const int n = A->n;
-  const int nl = itmax;
-  if (n == 0) {
-    if (iter) *iter = 0;
-    if (err) *err = 0.0;
-    if (nres) *nres = 0;
-    return PSFUN_SUCCESS;
-  }
-  PSFUN_HIP_TRY(hipSetDevice(c->device));
   const int64_t ldv = (int64_t)align_up((size_t)n, 64);
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
-  // basis slots 0..nl-1: slot 0 holds x, slot l the unnormalised w_l (u_l
-  // until the update of step l has run)
-  const size_t vbytes = sizeof(double) * (size_t)ldv * (size_t)nl;
-  void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace(c, ctl_bytes + vbytes, &ws));
   LzCtl *ctl = (LzCtl *)ws;
   double *V = (double *)((char *)ws + ctl_bytes);
   // USERF: the kernels' control blocks keep the loop bookkeeping and the
@@ -536,8 +524,7 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   const bool fsync = userf && istop == 1;
   const int istop_k = userf ? 2 : istop;
   const int nl_k = fsync ? (1 << 30) : nl;
-  double *Pw = c->d_partials;                      // ||w||^2 (and ||x||^2) partials
-  double *Pa = c->d_partials + PSFUN_MAX_GRID;     // alpha partials
+  // Pw: ||w||^2 (and ||x||^2) partials; Pa: alpha partials
 
   // sliced ELL and diagonal slices: one thread per row, chunks of PSFUN_BS rows
   const bool sell = psfun_row_per_thread(A);
@@ -558,7 +545,8 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   // rs(1) = ||x||; v_1 = (1/rs1) x; u = A v_1; partials of T(1,1) = u.v_1
   hipLaunchKernelGGL(k_sumsq_partials, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, (int64_t)n, x, Pw);
   PSFUN_HIP_TRY(hipGetLastError());
-  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, V + ldv, Pa, ctl, nl, nl_k, istop_k, eps, fscale));
+  PSFUN_TRY(lz_launch_spmv(c, A, opscale, x, 1, Gs, Pw, Gu, V, V + ldv, Pa, ctl, nl, nl_k, istop_k, eps, fscale,
+                           prof));
 
   // Pacing: before queueing step j+2, wait until step j-1 has reported
   // (h_flags[0] >= j-1).  The stores from a running kernel may only reach the
@@ -569,13 +557,13 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     const double *vj = V + (size_t)ldv * (j - 1);
     const double *vjm1 = (j > 1) ? V + (size_t)ldv * (j - 2) : nullptr;
     // algorithmic bytes: u read + w written, v_j (and v_{j-1}) read
-    PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n, j));
+    if (prof) PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n, j));
     hipLaunchKernelGGL(k_lz_update, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, Pa, Gs, Pw, ctl);
     PSFUN_HIP_TRY(hipGetLastError());
-    PSFUN_TRY(psfun_prof_end(c));
+    if (prof) PSFUN_TRY(psfun_prof_end(c));
     if (j + 1 <= nl - 1) {
       PSFUN_TRY(lz_launch_spmv(c, A, opscale, u, j + 1, Gs, Pw, Gu, V, V + (size_t)ldv * (j + 1), Pa, ctl, nl,
-                               nl_k, istop_k, eps, fscale));
+                               nl_k, istop_k, eps, fscale, prof));
     } else {
       hipLaunchKernelGGL(k_lz_ctrl, dim3(1), dim3(64), 0, c->stream, Pw, Gu, nl, ctl, nl_k, istop_k, eps, fscale,
                          c->d_flags);
@@ -609,10 +597,10 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
     PSFUN_HIP_TRY(hipGetLastError());
   }
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
-  PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_ASSEMBLE, 0.0));
+  if (prof) PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_ASSEMBLE, 0.0));
   hipLaunchKernelGGL(k_lz_assemble, dim3(Ga), dim3(PSFUN_BS), 0, c->stream, n, V, ldv, ctl, y);
   PSFUN_HIP_TRY(hipGetLastError());
-  PSFUN_TRY(psfun_prof_end(c));
+  if (prof) PSFUN_TRY(psfun_prof_end(c));
 
   // results back to the host
   LzCtl *h = (LzCtl *)((char *)c->h_pinned + 256);
@@ -622,8 +610,10 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   // timing pairs of launches that came after the stop did no work: SpMV steps
   // 1..itx+1 and updates 1..itx ran (the last SpMV overlapped the stop test)
   // (with repeats, the repeats of the stopping step itself return at once)
-  psfun_prof_discard_after(c, PSFUN_PROF_LZ_SPMV, psfun_prof_reps(c) > 1 ? itx : itx + 1);
-  psfun_prof_discard_after(c, PSFUN_PROF_LZ_UPDATE, itx);
+  if (prof) {
+    psfun_prof_discard_after(c, PSFUN_PROF_LZ_SPMV, psfun_prof_reps(c) > 1 ? itx : itx + 1);
+    psfun_prof_discard_after(c, PSFUN_PROF_LZ_UPDATE, itx);
+  }
   const double e = (h->errden == 0.0) ? h->errnum : h->errnum / h->errden;
   if (iter) *iter = itx;
   if (err) *err = e;
@@ -634,3 +624,29 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
   if (nres) *nres = itx + 1;
   return PSFUN_SUCCESS;
 }
+
+extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, double fscale, double opscale,
+                                   const double *x, double *y, double eps, int itmax, int istop, int *iter,
+                                   double *err, double *res, int *nres) {
+  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
+  const bool userf = fname == PSFUN_FN_USERF;
+  if (userf)
+    PSFUN_TRY(userf_check(c, PSFUN_VAR_SYM));
+  else if (fname != PSFUN_FN_EXP)
+    return PSFUN_ERR_UNSUPPORTED;
+  if (itmax < 2 || itmax > LZ_NL_MAX) return PSFUN_ERR_INVALID_ARG;
+  if (A->n == 0) {
+    if (iter) *iter = 0;
+    if (err) *err = 0.0;
+    if (nres) *nres = 0;
+    return PSFUN_SUCCESS;
+  }
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  void *ws = nullptr;
+  PSFUN_TRY(psfun_workspace(c, lz_ws_bytes(A->n, itmax), &ws));
+  return lz_apply(c, A, userf, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, nres, ws, c->d_partials,
+                  c->d_partials + PSFUN_MAX_GRID, true);
+}
+

@@ -449,6 +449,24 @@ def test_two_contexts_in_two_threads(gpu_ctx):
             assert np.array_equal(y, refs[i])
 
 
+def test_objects_destroyed_after_their_context(gpu_ctx):
+    """A matrix (with its sliced-ELL and diagonal-slice copies) and a batch
+    destroyed after the context they were created on: destroy uses only the
+    device they recorded, leaves no HIP error behind, and later calls on
+    another context work."""
+    from psfun_amd.batched import psfun_batch
+    ctx = psb.Context(0)
+    A = matgen.laplacian((30, 30))
+    M = psb.psb_dspmat_type.from_csr(A, ctx)
+    M.set_format("HDIA")
+    B = psfun_batch([matgen.rgg_laplacian(300, 5)], ctxt=ctx)
+    ctx.close()
+    M.free()
+    B.free()
+    info, it, err, res, y = _lanczos_gpu(matgen.laplacian((30, 30), scale=-1.0), np.ones(900), 1e-10, 30)
+    assert info == 0 and it > 1
+
+
 def test_diagonal_slices_empty_matrix(gpu_ctx):
     """No entries: every slice has no diagonal; the SpMV gives beta*y and the
     Lanczos apply of the zero operator gives y = exp(0) x = x."""
