@@ -43,7 +43,8 @@ enum {
   PSFUN_ERR_ALLOC = 7,                /* device or pinned allocation failed */
   PSFUN_ERR_HIP = 8,                  /* a HIP runtime call failed */
   PSFUN_ERR_NO_DEVICE = 9,
-  PSFUN_ERR_BREAKDOWN = 10            /* solver breakdown reported as an error (CG/COCG only) */
+  PSFUN_ERR_BREAKDOWN = 10,           /* solver breakdown reported as an error (CG/COCG only) */
+  PSFUN_ERR_TOO_LARGE = 11            /* input exceeds the fixed scratch of psfun_lanczos_apply_noalloc */
 };
 
 /* f(H) step selection: function names follow psfun_d_serial%fname ('EXP',
@@ -217,6 +218,24 @@ int psfun_serial_apply_host(psfun_ctx_t ctx, int fname, int variant, double scal
 int psfun_lanczos_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
                         const double *x, double *y, double eps, int itmax, int istop, int *iter,
                         double *err, double *res, int *nres);
+
+/* The same Lanczos apply without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * kernels are psfun_lanczos_apply's; the control block, partial sums and
+ * Krylov basis live in a fixed 64 MiB device array of the library (one copy
+ * per device, held until the process ends) instead of the context's
+ * workspace.  Limit: 64 KiB + align256(control block) + 8 * align64(n) *
+ * itmax bytes <= 64 MiB (n * itmax up to about 8.3 M, e.g. n = 160 000 at
+ * itmax 50); larger inputs return PSFUN_ERR_TOO_LARGE before any work.
+ * fname must be PSFUN_FN_EXP (USERF returns PSFUN_ERR_UNSUPPORTED).  Every
+ * call on a device shares that device's array, so calls that overlap on one
+ * device (two contexts, two host threads) race: serialise them.  Calls after
+ * one another are fine (each clears what it reads first).  Results are
+ * bitwise those of psfun_lanczos_apply; per-kernel timing is not recorded. */
+int psfun_lanczos_apply_noalloc(psfun_ctx_t ctx, psfun_csr_t A, int fname, double fscale, double opscale,
+                                const double *x, double *y, double eps, int itmax, int istop, int *iter,
+                                double *err, double *res, int *nres);
 
 /* y = ||x|| V_k f(fscale*H_k) e1 by Arnoldi (modified Gram-Schmidt) with the
  * reference's stopping rule and assembly (psfun_d_arnoldi.f90:183-329): the

@@ -21,6 +21,7 @@ PSFUN_ERR_ALLOC = 7
 PSFUN_ERR_HIP = 8
 PSFUN_ERR_NO_DEVICE = 9
 PSFUN_ERR_BREAKDOWN = 10
+PSFUN_ERR_TOO_LARGE = 11
 
 PSFUN_FN_EXP = 0
 PSFUN_FN_USERF = 1
@@ -86,6 +87,7 @@ SIGNATURES = {
     "psfun_scal": (_i, [_vp, _i64, _d, _vp]),
     "psfun_serial_apply_host": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _vp]),
     "psfun_lanczos_apply": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
+    "psfun_lanczos_apply_noalloc": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
     "psfun_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _vp, _d, _i, _i, _pi, _pd, _pd, _pi]),
     "psfun_sai_arnoldi_apply": (_i, [_vp, _vp, _i, _i, _d, _d, _i, _i, _i, _vp, _vp, _d, _i, _i, _pi, _pd, _pd,
                                      _pi]),

@@ -27,6 +27,7 @@ extern "C" const char *psfun_strerror(int status) {
     case PSFUN_ERR_HIP: return g_last_hip_error[0] ? g_last_hip_error : "HIP runtime error";
     case PSFUN_ERR_NO_DEVICE: return "no HIP device";
     case PSFUN_ERR_BREAKDOWN: return "solver breakdown";
+    case PSFUN_ERR_TOO_LARGE: return "input exceeds the 64 MiB fixed scratch of the allocation-free entry point";
     default: return "unknown status";
   }
 }
@@ -60,6 +61,12 @@ extern "C" int psfun_ctx_create(int device, psfun_ctx_t *out) {
     psfun_record_hip_error(e, "psfun_ctx_create", __FILE__, __LINE__);
     psfun_ctx_destroy(c);
     return PSFUN_ERR_HIP;
+  }
+  // the device address of this device's allocation-free scratch (once per device)
+  const int st = psfun_noalloc_setup(device);
+  if (st != PSFUN_SUCCESS) {
+    psfun_ctx_destroy(c);
+    return st;
   }
   c->stream = c->own_stream;
   *out = c;

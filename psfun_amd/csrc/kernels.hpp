@@ -5,3 +5,6 @@
 int launch_spmv(psfun_ctx *c, const psfun_csr *A, double alpha, const double *x, double beta, double *y);
 int launch_reduce(psfun_ctx *c, int op, int64_t n, const double *x, const double *y, double *d_out);
 int launch_axpby(psfun_ctx *c, int64_t n, double alpha, const double *x, double beta, double *y);
+// Looks up the device address of the allocation-free entry point's fixed
+// scratch on `device` (current); defined beside that array in lanczos.hip.
+int psfun_noalloc_setup(int device);

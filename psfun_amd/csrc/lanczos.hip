@@ -650,3 +650,63 @@ extern "C" int psfun_lanczos_apply(psfun_ctx_t c, psfun_csr_t A, int fname, doub
                   c->d_partials + PSFUN_MAX_GRID, true);
 }
 
+// ---- allocation-free variant ------------------------------------------------
+// Fixed device scratch of psfun_lanczos_apply_noalloc: the control block, the
+// two partial-sum lists and the basis, carved as lz_apply expects them.  Each
+// device the library runs on holds its own copy until the process ends.  The
+// array's name is only a host-side handle: kernels get the device address
+// looked up once per device by psfun_noalloc_setup.
+#define PSFUN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) double scratch[PSFUN_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+#define PSFUN_NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[PSFUN_NOALLOC_MAX_DEVICES];
+
+// Called by psfun_ctx_create with `device` current: the device address of
+// this device's copy of `scratch`.
+int psfun_noalloc_setup(int device) {
+  if (device < 0 || device >= PSFUN_NOALLOC_MAX_DEVICES) return PSFUN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PSFUN_SUCCESS;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    psfun_record_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)", __FILE__,
+                           __LINE__);
+    return PSFUN_ERR_HIP;
+  }
+  g_scratch_ptr[device] = p;
+  return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_lanczos_apply_noalloc(psfun_ctx_t c, psfun_csr_t A, int fname, double fscale, double opscale,
+                                           const double *x, double *y, double eps, int itmax, int istop, int *iter,
+                                           double *err, double *res, int *nres) {
+  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!x || !y) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (fname != PSFUN_FN_EXP) return PSFUN_ERR_UNSUPPORTED;  // USERF needs host-visible scratch of its own
+  if (itmax < 2 || itmax > LZ_NL_MAX) return PSFUN_ERR_INVALID_ARG;
+  const size_t part_bytes = sizeof(double) * 2 * PSFUN_MAX_GRID;
+  if (part_bytes + lz_ws_bytes(A->n, itmax) > PSFUN_NOALLOC_SCRATCH_BYTES) return PSFUN_ERR_TOO_LARGE;
+  if (A->n == 0) {
+    if (iter) *iter = 0;
+    if (err) *err = 0.0;
+    if (nres) *nres = 0;
+    return PSFUN_SUCCESS;
+  }
+  if (c->device < 0 || c->device >= PSFUN_NOALLOC_MAX_DEVICES) return PSFUN_ERR_INVALID_ARG;
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[c->device];
+  }
+  if (!base) return PSFUN_ERR_HIP;  // psfun_ctx_create sets it up for the context's device
+  PSFUN_HIP_TRY(hipSetDevice(c->device));
+  // layout: Pw | Pa | control block + basis (256-byte aligned throughout)
+  double *Pw = (double *)base;
+  double *Pa = Pw + PSFUN_MAX_GRID;
+  void *ws = (char *)base + part_bytes;
+  return lz_apply(c, A, false, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, nres, ws, Pw, Pa, false);
+}
