@@ -14,7 +14,7 @@ module psfun_hip_mod
   integer(c_int), parameter :: PSFUN_SUCCESS = 0, PSFUN_ERR_INVALID_ARG = 1, PSFUN_ERR_INVALID_ISTOP = 2, &
        & PSFUN_ERR_INVALID_VECT_STATE = 3, PSFUN_ERR_INVALID_ARGS_COMBINATION = 4, PSFUN_ERR_NOT_SQUARE = 5, &
        & PSFUN_ERR_UNSUPPORTED = 6, PSFUN_ERR_ALLOC = 7, PSFUN_ERR_HIP = 8, PSFUN_ERR_NO_DEVICE = 9, &
-       & PSFUN_ERR_BREAKDOWN = 10
+       & PSFUN_ERR_BREAKDOWN = 10, PSFUN_ERR_TOO_LARGE = 11
   integer(c_int), parameter :: PSFUN_FN_EXP = 0, PSFUN_FN_USERF = 1
   integer(c_int), parameter :: PSFUN_VAR_CHBSYM = 0, PSFUN_VAR_CHBHES = 1, PSFUN_VAR_CHBGEN = 2, &
        & PSFUN_VAR_GENPADE = 3, PSFUN_VAR_TAYLOR = 4, PSFUN_VAR_SASQ = 5, PSFUN_VAR_SYM = 6
@@ -110,6 +110,20 @@ module psfun_hip_mod
       integer(c_int) :: st
     end function
 
+    ! NUL-terminated message of a status code (c_f_pointer it to a character array)
+    function psfun_strerror(status) bind(C, name='psfun_strerror') result(msg)
+      import :: c_int, c_ptr
+      integer(c_int), value :: status
+      type(c_ptr) :: msg
+    end function
+    function psfun_csr_info(a, n, nnz, lanes) bind(C, name='psfun_csr_info') result(st)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: a
+      integer(c_int64_t), intent(out) :: n, nnz
+      integer(c_int), intent(out) :: lanes
+      integer(c_int) :: st
+    end function
+
     ! ---- PSBLAS primitives -------------------------------------------------
     function psfun_spmv(ctx, a, alpha, x, beta, y) bind(C, name='psfun_spmv') result(st)
       import :: c_int, c_double, c_ptr
@@ -129,6 +143,20 @@ module psfun_hip_mod
       type(c_ptr), value :: ctx, x
       integer(c_int64_t), value :: n
       real(c_double), intent(out) :: res
+      integer(c_int) :: st
+    end function
+    function psfun_amax(ctx, n, x, res) bind(C, name='psfun_amax') result(st)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, x
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: res
+      integer(c_int) :: st
+    end function
+    function psfun_scal(ctx, n, alpha, x) bind(C, name='psfun_scal') result(st)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, x
+      integer(c_int64_t), value :: n
+      real(c_double), value :: alpha
       integer(c_int) :: st
     end function
     function psfun_axpby(ctx, n, alpha, x, beta, y) bind(C, name='psfun_axpby') result(st)
@@ -154,6 +182,18 @@ module psfun_hip_mod
     ! ---- Krylov f(A)x (psfun_d_krylov%apply) -------------------------------
     function psfun_lanczos_apply(ctx, a, fname, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, nres) &
          & bind(C, name='psfun_lanczos_apply') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, a, x, y
+      integer(c_int), value :: fname, itmax, istop
+      real(c_double), value :: fscale, opscale, eps
+      integer(c_int), intent(out) :: iter, nres
+      real(c_double), intent(out) :: err
+      real(c_double), intent(out) :: res(*)
+      integer(c_int) :: st
+    end function
+    ! The same apply on the library's fixed device scratch (no allocation; EXP only)
+    function psfun_lanczos_apply_noalloc(ctx, a, fname, fscale, opscale, x, y, eps, itmax, istop, iter, err, res, &
+         & nres) bind(C, name='psfun_lanczos_apply_noalloc') result(st)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx, a, x, y
       integer(c_int), value :: fname, itmax, istop
@@ -249,6 +289,13 @@ module psfun_hip_mod
       type(c_ptr), value :: user
       integer(c_int), value :: n
       real(c_double), intent(in) :: rparams(2)
+      real(c_double), intent(out) :: xi(*), c(*), eta, sign
+      integer(c_int) :: st
+    end function
+    function psfun_talbot(n, t, xi, c, eta, sign) bind(C, name='psfun_talbot') result(st)
+      import :: c_int, c_double
+      integer(c_int), value :: n
+      real(c_double), value :: t
       real(c_double), intent(out) :: xi(*), c(*), eta, sign
       integer(c_int) :: st
     end function

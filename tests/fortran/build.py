@@ -1,7 +1,10 @@
-"""Build the Fortran caller tests/fortran/psfun_driver.f90 against
-include/psfun_hip_mod.f90 and psfun_amd/libpsfun_hip.so with flang (test
-infrastructure: it shows the iso_c_binding boundary working from Fortran).
-Output: tests/fortran/_build/psfun_driver.  Usage: python tests/fortran/build.py"""
+"""Build the Fortran callers with flang (test infrastructure):
+  tests/fortran/psfun_driver.f90 -- flat calls through include/psfun_hip_mod.f90
+  tests/fortran/psfun_typed.f90  -- the reference's example drivers against
+                                    the typed layer psfun_amd/fortran (built
+                                    first by psfun_amd/fortran/build.py)
+Both link psfun_amd/libpsfun_hip.so.  Output: tests/fortran/_build/.
+Usage: python tests/fortran/build.py"""
 import os
 import shutil
 import subprocess
@@ -11,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 OUT = os.path.join(HERE, "_build")
 EXE = os.path.join(OUT, "psfun_driver")
+TYPED = os.path.join(OUT, "psfun_typed")
 
 
 def flang():
@@ -34,12 +38,17 @@ def build(verbose=True):
         [fc, "-O2", "-I", OUT, "-module-dir", OUT, os.path.join(HERE, "psfun_driver.f90"), mod_obj, "-o", EXE,
          "-L", os.path.join(REPO, "psfun_amd"), "-lpsfun_hip", "-Wl,-rpath,$ORIGIN/../../../psfun_amd"],
     ]
+    sys.path.insert(0, REPO)
+    from psfun_amd.fortran import build as fb
+    flib = fb.build(verbose=verbose)
+    steps.append([fc, "-O2", "-I", fb.OUT, os.path.join(HERE, "psfun_typed.f90"), flib, "-o", TYPED,
+                  "-L", os.path.join(REPO, "psfun_amd"), "-lpsfun_hip", "-Wl,-rpath,$ORIGIN/../../../psfun_amd"])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {EXE}")
+        print(f"built {EXE} and {TYPED}")
     return EXE
 
 

@@ -1,0 +1,183 @@
+"""The typed Fortran layer (psfun_amd/fortran: psfun_d_krylov, psfun_d_serial,
+psb_krylov, psfun_z_quadrature over a PSBLAS subset) driven by
+tests/fortran/psfun_typed, whose modes follow the reference's example
+drivers call for call (examples/polynomial/lanczostest.F90:205-240,
+examples/shiftedkrylov/shiftedtest.f90:225-255, examples/serial/serialtest.f90,
+examples/quadrature/quadraturetest.F90).  Results are checked against the
+oracle, the example matrix's fixtures and closed forms."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import REPO, load_golden
+from oracle import oracle as O
+from psfun_amd import matgen
+from test_drivers import LANCZOS, SERIAL, SHIFTED
+from test_mmio import _laplacian50_text
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(REPO, "tests", "fortran", "_build", "psfun_typed")
+
+
+def _run(tmp_path, *args):
+    if not os.path.exists(EXE):
+        pytest.skip("typed Fortran driver not built (flang unavailable when building)")
+    print(f"[fortran-typed] {' '.join(str(a) for a in args)}", flush=True)
+    r = subprocess.run([EXE] + [str(a) for a in args], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("@"):
+            k, *v = line.split()
+            out.setdefault(k[1:], []).append(v)
+    return out, r.stdout
+
+
+def _vec(out, key):
+    return np.array([float(v[0]) for v in out[key]])
+
+
+def _laplacian50(tmp_path):
+    text, (rp, ci, va) = _laplacian50_text()
+    (tmp_path / "laplacian.mtx").write_text(text)
+    return rp, ci, va
+
+
+def _write_mtx(path, A):
+    rp, ci, va = A
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate real general\n{n} {n} {len(ci)}\n")
+        np.savetxt(f, np.column_stack([rows + 1, ci + 1, va]), fmt=["%d", "%d", "%.17g"])
+
+
+def test_typed_lanczostest_deck(gpu_ctx, tmp_path):
+    """lanczostest with lanczos.inp: kmethd%set('KNAME','LANCZOS'), the
+    driver's eps 1e-6 / itmax 100 / itrace 1 / istop 1, res allocated by
+    the callee with iter+1 entries, the convergence trace printed."""
+    A = _laplacian50(tmp_path)
+    (tmp_path / "lanczos.inp").write_text(LANCZOS)
+    out, text = _run(tmp_path, "krylov", "lanczos.inp")
+    assert int(out["INFO"][0][0]) == 0
+    ref = O.lanczos(A, np.ones(50), 1e-6, 100, 1)
+    it = int(out["ITER"][0][0])
+    y = _vec(out, "Y")
+    assert it == ref["iter"]
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    g = load_golden("laplacian50.npz")
+    assert np.linalg.norm(y - g["y_exp"]) <= 1e-6 * np.linalg.norm(g["y_exp"])
+    res = _vec(out, "RES")
+    assert len(res) == it + 1 and abs(res[0] - np.sqrt(50.0)) <= 1e-12 * np.sqrt(50.0)
+    assert np.allclose(res, ref["res"], rtol=1e-8, atol=0)
+    assert "LANCZOS:" in text and "iterations" in text      # itrace = 1 log
+
+
+def test_typed_arnoldi_and_errors(gpu_ctx, tmp_path):
+    """The same deck through KNAME 'ARNOLDI' (the type's default method), and
+    the reference's error paths through the typed objects: an unknown KNAME
+    gives psb_err_invalid_args_combination_ (1126)."""
+    A = _laplacian50(tmp_path)
+    (tmp_path / "lanczos.inp").write_text(LANCZOS)
+    out, _ = _run(tmp_path, "krylov", "lanczos.inp", "arnoldi")
+    ref = O.arnoldi(A, np.ones(50), 1e-6, 100, 1)
+    assert int(out["ITER"][0][0]) == ref["iter"]
+    y = _vec(out, "Y")
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    out, _ = _run(tmp_path, "krylov", "lanczos.inp", "NOPE")
+    assert int(out["INFO"][0][0]) == 1126
+
+
+@pytest.mark.parametrize("afmt", ["CSR", "HDIA"])
+def test_typed_lanczos_plain_config(gpu_ctx, tmp_path, afmt):
+    """The plain CPU-runnable case: exp(-A)x on the 100x100 Dirichlet
+    Laplacian (the matrix file holds -A: Lanczos ignores fun%scaling),
+    x ~ U[-1,1] from a MatrixMarket array, eps 1e-10, itmax 30, istop 1,
+    assembled as afmt: the oracle at 1e-10 and the DST closed form."""
+    dims = (100, 100)
+    A = matgen.laplacian(dims, scale=-1.0)
+    x = matgen.uniform_pm1(1, 10000)
+    _write_mtx(tmp_path / "lap100.mtx", A)
+    with open(tmp_path / "x.mtx", "w") as f:
+        f.write("%%MatrixMarket matrix array real general\n10000 1\n")
+        np.savetxt(f, x, fmt="%.17g")
+    deck = LANCZOS.replace("laplacian.mtx", "lap100.mtx").replace("NONE ", "x.mtx").replace("CSR ", afmt + " ")
+    deck = deck.replace("1e-6 ", "1e-10").replace("100           ! itmax", "30            ! itmax")
+    (tmp_path / "plain.inp").write_text(deck)
+    out, _ = _run(tmp_path, "krylov", "plain.inp", "LANCZOS", "keep")
+    assert int(out["INFO"][0][0]) == 0
+    y = _vec(out, "Y")
+    ref = O.lanczos(A, x, 1e-10, 30, 1)
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    assert abs(int(out["ITER"][0][0]) - ref["iter"]) <= 1
+    yex = matgen.dst_expm_laplacian(dims, x, 1.0)
+    assert np.linalg.norm(y - yex) <= 1e-10 * np.linalg.norm(yex)
+
+
+def test_typed_shiftedtest_deck(gpu_ctx, tmp_path):
+    """shiftedtest with shifted.inp: psb_krylov('CG', a, prec DIAG, b, -1, 1,
+    x, 1e-7, ...), istop 2, itmax 500: x against the dense-solve fixture and
+    the oracle; the driver's residual uses the unshifted A."""
+    A = _laplacian50(tmp_path)
+    (tmp_path / "shifted.inp").write_text(SHIFTED)
+    out, _ = _run(tmp_path, "shifted", "shifted.inp")
+    assert int(out["INFO"][0][0]) == 0
+    x = _vec(out, "Y")
+    g = load_golden("laplacian50.npz")
+    assert np.linalg.norm(x - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
+    ref = O.cg_shifted(A, np.ones(50), -1.0, 1.0, eps=1e-7, itmax=500, istop=2, prec=1, want_cond=True)
+    assert abs(int(out["ITER"][0][0]) - ref["iter"]) <= 1
+    assert np.linalg.norm(x - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
+    assert float(out["ERR"][0][0]) <= 1e-7
+    r = np.ones(50) - matgen.to_scipy(A) @ x
+    assert abs(float(out["RESN2"][0][0]) - np.linalg.norm(r)) <= 1e-10 * max(np.linalg.norm(r), 1.0)
+    cond = float(out["COND"][0][0])
+    if int(out["ITER"][0][0]) == ref["iter"]:       # the ratio of the CG tridiagonal's extreme eigenvalues
+        assert abs(cond - ref["cond"]) <= 1e-6 * abs(ref["cond"])
+
+
+def test_typed_serialtest_deck(gpu_ctx, tmp_path):
+    """serialtest with serial.inp: fun%apply on the sparse example matrix,
+    EXP / GENPADE / scaling 1, x = 1: exp(A)x against the scipy fixture."""
+    _laplacian50(tmp_path)
+    (tmp_path / "serial.inp").write_text(SERIAL)
+    out, _ = _run(tmp_path, "serial", "serial.inp")
+    assert int(out["INFO"][0][0]) == 0
+    g = load_golden("laplacian50.npz")
+    y = _vec(out, "Y")
+    assert np.linalg.norm(y - g["y_exp"]) <= 1e-12 * np.linalg.norm(g["y_exp"])
+    # the reference's default variant 'EXPOKIT' is not a case of its apply
+    (tmp_path / "bad.inp").write_text(SERIAL.replace("GENPADE", "EXPOKIT"))
+    out, _ = _run(tmp_path, "serial", "bad.inp")
+    assert int(out["INFO"][0][0]) == 4010
+
+
+def test_typed_quadrature(gpu_ctx, tmp_path):
+    """psfun_z_quadrature through computepoles(hhtmethod1 | talbot, ...),
+    set(a) and apply: sqrt(A)1 by the HHT rule (poles equal to the oracle's
+    hhtmethod1) and exp(-A)1 by the 24-pole Talbot rule (conjugate pairs
+    folded: 12 solves), against scipy on an SPD 2-D Laplacian."""
+    dims = (12, 10)
+    A = matgen.laplacian(dims)
+    n = 120
+    _write_mtx(tmp_path / "spd.mtx", A)
+    Ad = matgen.to_scipy(A).toarray()
+    ev = np.linalg.eigvalsh(Ad)
+    out, _ = _run(tmp_path, "quad", "spd.mtx", "HHT", 24, repr(float(ev[0])), repr(float(ev[-1])))
+    assert int(out["POLES_INFO"][0][0]) == 0 and int(out["INFO"][0][0]) == 0
+    poles = np.array([[float(t) for t in v] for v in out["POLE"]])
+    xi, c, eta, sign = O.hhtmethod1(np.sqrt, 24, [ev[0], ev[-1]])
+    assert np.allclose(poles[:, 0] + 1j * poles[:, 1], xi, rtol=1e-13, atol=1e-14)
+    assert np.allclose(poles[:, 2] + 1j * poles[:, 3], c, rtol=1e-12, atol=1e-14)
+    y = _vec(out, "Y")
+    ex = np.real(sla.sqrtm(Ad)) @ np.ones(n)
+    assert np.linalg.norm(y - ex) <= 1e-8 * np.linalg.norm(ex)
+    out, _ = _run(tmp_path, "quad", "spd.mtx", "TALBOT", 24, "1.0")
+    assert int(out["INFO"][0][0]) == 0 and int(out["NSOLVE"][0][0]) == 12
+    y = _vec(out, "Y")
+    ex = sla.expm(-Ad) @ np.ones(n)
+    assert np.linalg.norm(y - ex) <= 1e-10 * np.linalg.norm(ex)
