@@ -64,6 +64,8 @@ def parse():
     p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg,arnoldi",
                    help="comma-separated extra sections to run after the headline")
     p.add_argument("--format", default="", help="SELL, HDIA, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
+    p.add_argument("--bt-chunks", default="64,128,256,512,1024",
+                   help="rows-per-chunk candidates of the batched autotuner")
     return p.parse_args()
 
 
@@ -318,7 +320,8 @@ def batched_section(args):
         B = psfun_batch(mats, order=order)
         N = B.nrows
         X = torch.as_tensor(matgen.normal(77, N * K).reshape(N, K), device="cuda:0")
-        best, table = B.autotune(X, t=t, eps=eps, itmax=itmax, candidates=(256, 512, 1024), reps=1)
+        cands = tuple(int(v) for v in args.bt_chunks.split(","))
+        best, table = B.autotune(X, t=t, eps=eps, itmax=itmax, candidates=cands, reps=2)
         Y, iters, errs = B.expm(X, t=t, eps=eps, itmax=itmax)
         torch.cuda.synchronize()
         steps = 3
@@ -328,6 +331,8 @@ def batched_section(args):
         torch.cuda.synchronize()
         tb = (time.perf_counter() - t0) / steps
         timed[order] = (tb, B, best, Y, iters)
+        log(f"batched: order {order}: tuning (rows_per_chunk, hipgraph) -> ms: "
+            + ", ".join(f"{c[0]}/{int(c[1])}: {dt * 1e3:.2f}" for c, dt in table))
         log(f"batched: order {order}: {B.ngraph * K / tb:.1f} f(A)x/s")
     order = min(timed, key=lambda o: timed[o][0])
     tb, B, best, Y, iters = timed[order]

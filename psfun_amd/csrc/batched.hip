@@ -126,6 +126,30 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_norm(const int *__restrict__ ch
   }
 }
 
+#ifndef BT_XCD
+#define BT_XCD 1
+#endif
+// Chunk of this SpMM block.  Blocks are dispatched round-robin over the 8
+// XCDs; block b runs on XCD b % 8 and takes the (b / 8)-th chunk of that
+// XCD's contiguous eighth, so the blocks an XCD runs at a time cover
+// consecutive rows and their neighbour gathers (rows close in the graph's
+// ordering) hit that XCD's L2.  The grid is 8 * ceil(nc / 8); -1 for the
+// padding blocks.
+__device__ __forceinline__ int bt_block_chunk(int nc) {
+  if (!BT_XCD) return blockIdx.x < (unsigned)nc ? (int)blockIdx.x : -1;
+  const int per = (nc + 7) >> 3;
+  const int c = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  return c < nc ? c : -1;
+}
+static inline int bt_spmm_grid(int nc) { return BT_XCD ? 8 * ((nc + 7) / 8) : nc; }
+
+#define BT_UB 4     // rows per thread in flight in the 3-term update
+#ifndef BT_AR
+#define BT_AR 8     // rows per thread in flight in the assembly
+#endif
+#ifndef BT_EB
+#define BT_EB 4     // entries per row whose gathers the staged SpMM issues together
+#endif
 #define BT_SR 64    // rows per LDS-staged sub-chunk of the SpMM
 #define BT_CAP 768  // entries staged per sub-chunk (9 KiB with the row pointers)
 template <int KP>
@@ -135,7 +159,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
                                                       int K, int j, const double *__restrict__ src,
                                                       double *__restrict__ v_out, double *__restrict__ u_out,
                                                       const double *__restrict__ p_in, double *__restrict__ p_out,
-                                                      BtInst in, BtCtl *bctl) {
+                                                      BtInst in, BtCtl *bctl, int nc) {
   constexpr int RP = PSFUN_BS / KP;
   __shared__ double s_inv[KP];
   __shared__ int s_act[KP];
@@ -145,7 +169,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
   __shared__ int s_col[BT_CAP];
   __shared__ double s_val[BT_CAP];
   if (bctl->remaining == 0) return;
-  const int c = blockIdx.x;
+  const int c = bt_block_chunk(nc);
+  if (c < 0) return;
   const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
   if (threadIdx.x == 0) s_any = 0;
   __syncthreads();
@@ -197,8 +222,6 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
     if (staged) {
       // two rows per thread at a time (r and r + RP): their gathers are in
       // flight together; each row's terms keep their entry order
-      const int *cl = s_col - e0;
-      const double *vl = s_val - e0;
       for (int r = sr0 + sub; r < sr1; r += 2 * RP) {
         const int r2 = r + RP;
         const bool two = r2 < sr1;
@@ -208,19 +231,19 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
         const double xr1 = src[(int64_t)r * K + k];
         const double xr2 = two ? src[(int64_t)r2 * K + k] : 0.0;
         double sum1 = 0.0, sum2 = 0.0;
-        for (int t = 0; t < lm; t += 4) {
-          double a1[4], a2[4], g1[4], g2[4];
+        for (int t = 0; t < lm; t += BT_EB) {
+          double a1[BT_EB], a2[BT_EB], g1[BT_EB], g2[BT_EB];
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
+          for (int m = 0; m < BT_EB; ++m) {
             const int q1 = s1 + t + m, q2 = s2 + t + m;
             const bool ok1 = q1 < e1, ok2 = q2 < e2;
-            a1[m] = ok1 ? vl[q1] : 0.0;
-            a2[m] = ok2 ? vl[q2] : 0.0;
-            g1[m] = ok1 ? src[(int64_t)cl[q1] * K + k] : 0.0;
-            g2[m] = ok2 ? src[(int64_t)cl[q2] * K + k] : 0.0;
+            a1[m] = ok1 ? s_val[q1 - e0] : 0.0;
+            a2[m] = ok2 ? s_val[q2 - e0] : 0.0;
+            g1[m] = ok1 ? src[(int64_t)s_col[q1 - e0] * K + k] : 0.0;
+            g2[m] = ok2 ? src[(int64_t)s_col[q2 - e0] * K + k] : 0.0;
           }
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
+          for (int m = 0; m < BT_EB; ++m) {
             if (s1 + t + m < e1) sum1 = fma(a1[m], g1[m] * inv, sum1);
             if (s2 + t + m < e2) sum2 = fma(a2[m], g2[m] * inv, sum2);
           }
@@ -244,17 +267,15 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ ro
       double sum = 0.0;
       int q = s;
       if (staged) {
-        const int *cl = s_col - e0;
-        const double *vl = s_val - e0;
         for (; q + 4 <= e; q += 4) {
-          const double x0 = src[(int64_t)cl[q] * K + k], x1 = src[(int64_t)cl[q + 1] * K + k];
-          const double x2 = src[(int64_t)cl[q + 2] * K + k], x3 = src[(int64_t)cl[q + 3] * K + k];
-          sum = fma(vl[q], x0 * inv, sum);
-          sum = fma(vl[q + 1], x1 * inv, sum);
-          sum = fma(vl[q + 2], x2 * inv, sum);
-          sum = fma(vl[q + 3], x3 * inv, sum);
+          const double x0 = src[(int64_t)s_col[q - e0] * K + k], x1 = src[(int64_t)s_col[q + 1 - e0] * K + k];
+          const double x2 = src[(int64_t)s_col[q + 2 - e0] * K + k], x3 = src[(int64_t)s_col[q + 3 - e0] * K + k];
+          sum = fma(s_val[q - e0], x0 * inv, sum);
+          sum = fma(s_val[q + 1 - e0], x1 * inv, sum);
+          sum = fma(s_val[q + 2 - e0], x2 * inv, sum);
+          sum = fma(s_val[q + 3 - e0], x3 * inv, sum);
         }
-        for (; q < e; ++q) sum = fma(vl[q], src[(int64_t)cl[q] * K + k] * inv, sum);
+        for (; q < e; ++q) sum = fma(s_val[q - e0], src[(int64_t)s_col[q - e0] * K + k] * inv, sum);
       } else {
         // entries four at a time: their column indices, then their gathers,
         // are issued together; the sum keeps the entry order
@@ -329,7 +350,27 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ 
   double acc = 0.0;
   if (k < K && s_act[k]) {
     const double al = s_al[k], be = s_be[k], ia = s_ia[k], ib = s_ib[k];
-    for (int r = r0 + sub; r < r1; r += RP) {
+    // BT_UB rows per thread at a time: their loads are issued together; the
+    // ||w||^2 partial accumulates in row order as before
+    int r = r0 + sub;
+    for (; r + (BT_UB - 1) * RP < r1; r += BT_UB * RP) {
+      double uu[BT_UB], a[BT_UB], b[BT_UB];
+#pragma unroll
+      for (int q = 0; q < BT_UB; ++q) {
+        const int64_t i = (int64_t)(r + q * RP) * K + k;
+        uu[q] = u[i];
+        a[q] = vj[i];
+        b[q] = (j > 1) ? vjm1[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < BT_UB; ++q) {
+        double w = fma(-al, a[q] * ia, uu[q]);
+        if (j > 1) w = fma(-be, b[q] * ib, w);
+        w_out[(int64_t)(r + q * RP) * K + k] = w;
+        acc = fma(w, w, acc);
+      }
+    }
+    for (; r < r1; r += RP) {
       const int64_t i = (int64_t)r * K + k;
       double w = fma(-al, vj[i] * ia, u[i]);
       if (j > 1) w = fma(-be, vjm1[i] * ib, w);
@@ -462,10 +503,35 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict_
   const int m = s_m[k];
   const double *yk = in.yk + (int64_t)(g * K + k) * in.ld;
   const double *ibk = in.ib + (int64_t)(g * K + k) * in.ld;  // v_{l+1} = slot l * ibk[l]
-  for (int r = r0 + sub; r < r1; r += RP) {
+  // BT_AR rows per thread at a time, the slots in order for each (the basis is
+  // not read again: nontemporal loads): the BT_AR loads of a slot are in
+  // flight together and share one load of that slot's two coefficients
+  int r = r0 + sub;
+  for (; r + (BT_AR - 1) * RP < r1; r += BT_AR * RP) {
+    double acc[BT_AR];
+#pragma unroll
+    for (int q = 0; q < BT_AR; ++q) acc[q] = 0.0;
+    if (m > 0) {
+      const double y0 = yk[0], b0 = ibk[0];
+#pragma unroll
+      for (int q = 0; q < BT_AR; ++q)
+        acc[q] = y0 * (__builtin_nontemporal_load(V + (int64_t)(r + q * RP) * K + k) * b0);
+      for (int l = 1; l < m; ++l) {
+        const double yl = yk[l], bl = ibk[l];
+        const double *Vl = V + (int64_t)l * vstride;
+#pragma unroll
+        for (int q = 0; q < BT_AR; ++q)
+          acc[q] = fma(yl, __builtin_nontemporal_load(Vl + (int64_t)(r + q * RP) * K + k) * bl, acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < BT_AR; ++q) {
+      const int rq = r + q * RP;
+      Y[perm ? (int64_t)perm[rq] * K + k : (int64_t)rq * K + k] = s_rs1[k] * acc[q];
+    }
+  }
+  for (; r < r1; r += RP) {
     const int64_t i = (int64_t)r * K + k;
-    // slots in order (the basis is not read again: nontemporal loads; eight
-    // slots' loads in flight per thread)
     double acc = 0.0;
     if (m > 0) {
       acc = yk[0] * (__builtin_nontemporal_load(V + i) * ibk[0]);
@@ -718,8 +784,8 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
   // src read and u written (the first step also copies x into basis slot 0)
   const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 16.0 * B->nrows * K;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes + 8.0 * B->nrows * K));
-  hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val, opscale,
-                     B->d_chunk, B->d_gchunk, K, 1, X, V, U0, P0, P1, in, bctl);
+  hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind,
+                     B->val, opscale, B->d_chunk, B->d_gchunk, K, 1, X, V, U0, P0, P1, in, bctl, NC);
   PSFUN_TRY(psfun_prof_end(c));
   PSFUN_HIP_TRY(hipGetLastError());
   volatile int *hflag = (volatile int *)c->h_pinned;
@@ -743,9 +809,9 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     }
     if (j + 1 <= itmax - 1) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
-      hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind, B->val,
-                         opscale, B->d_chunk, B->d_gchunk, K, j + 1, V + (size_t)vs * j, nullptr, (j & 1) ? U1 : U0,
-                         P0, P1, in, bctl);
+      hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr,
+                         B->colind, B->val, opscale, B->d_chunk, B->d_gchunk, K, j + 1, V + (size_t)vs * j,
+                         nullptr, (j & 1) ? U1 : U0, P0, P1, in, bctl, NC);
       PSFUN_TRY(psfun_prof_end(c));
     }
     if (istop == 1) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
