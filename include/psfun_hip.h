@@ -253,7 +253,8 @@ int psfun_arnoldi_apply(psfun_ctx_t ctx, psfun_csr_t A, int fname, int variant, 
 
 /* Shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90:37-419): the
  * Krylov space of (A + tau I)^{-1}, each inner solve (A + tau I) w = v by
- * psfun_cg_shifted (prec NONE/DIAG, tolerance eps, inner_itmax, inner_istop;
+ * psfun_cg_shifted (prec NONE/DIAG, or BJAC built once per call on the
+ * unshifted A; tolerance eps, inner_itmax, inner_istop;
  * A + tau I must be definite), modified Gram-Schmidt, and
  * y = ||x|| V_k f(fscale (H_k^{-1} - tau I)) e1 with the reference's
  * a-posteriori estimate from that same projection.  Differences from the
@@ -344,15 +345,40 @@ int psfun_sddmm_cols(psfun_ctx_t ctx, int n, const int32_t *rowptr, const int32_
                      const double *weights, const double *U, const double *V, double scale, double *grad);
 
 /* ---- shifted solves (psb_krylov extension) -------------------------------- */
-enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1 };
+/* Preconditioners of the shifted CG (psb_dprec_type%init(ctxt, ptype, info),
+ * applied at krylov/shifted/psfun_dcg.F90:283): NONE, DIAG (inverse diagonal
+ * of the unshifted A) and BJAC (block Jacobi, which with one process is the
+ * whole matrix factored by ILU(0): M = L U on A's sparsity pattern). */
+enum { PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1, PSFUN_PREC_BJAC = 2 };
+typedef struct psfun_prec *psfun_prec_t;
+/* Build a preconditioner on the (unshifted) matrix A, as prec%build(a,
+ * desc_a, info) does: BJAC factors A by ILU(0) on the host (rows sorted by
+ * column, repeated entries summed; a zero pivot returns PSFUN_ERR_BREAKDOWN)
+ * and uploads the factors with the level schedules of their triangular
+ * solves; NONE and DIAG keep no state.  The object may outlive its context
+ * only to be destroyed (as matrices). */
+int psfun_prec_create(psfun_ctx_t ctx, psfun_csr_t A, int kind, psfun_prec_t *P);
+int psfun_prec_destroy(psfun_prec_t P);
+/* kind and the number of level-scheduled steps of one BJAC application */
+int psfun_prec_info(psfun_prec_t P, int *kind, int *levels);
+/* z = M^{-1} r (BJAC: U^{-1} L^{-1} r), device vectors; PSFUN_ERR_UNSUPPORTED
+ * for NONE/DIAG (the solver applies those itself). */
+int psfun_prec_apply(psfun_ctx_t ctx, psfun_prec_t P, const double *r, double *z);
 /* (eta*A + zeta*I) x = b by preconditioned CG with the semantics of
  * psfun_dcg_vect (krylov/shifted/psfun_dcg.F90:129-373): x holds the initial
- * guess on entry; prec NONE or DIAG (diagonal of the unshifted A); istop 1
+ * guess on entry; prec NONE, DIAG (diagonal of the unshifted A) or BJAC
+ * (ILU(0) of the unshifted A, built for the call); istop 1
  * (||r||_inf <= eps(||A||_inf ||x||_inf + ||b||_inf)) or 2 (||r||_2 <= eps ||b||_2);
  * breakdowns restart as in the reference.  cond (may be NULL) receives the
  * extreme-eigenvalue ratio of the CG tridiagonal (lines 314-354). */
 int psfun_cg_shifted(psfun_ctx_t ctx, psfun_csr_t A, int prec, const double *b, double eta, double zeta,
                      double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond);
+/* The same solver with a built preconditioner object (the reference's
+ * prec%apply): prec = PSFUN_PREC_BJAC builds one per call in
+ * psfun_cg_shifted, this entry reuses P across solves. */
+int psfun_cg_shifted_prec(psfun_ctx_t ctx, psfun_csr_t A, psfun_prec_t P, const double *b, double eta,
+                          double zeta, double *x, double eps, int itmax, int istop, int *iter, double *err,
+                          double *cond);
 /* Multi-shift CG: (A + sigma_s I) X(:,s) = b for nshift <= 64 complex shifts
  * (host array of (re,im) pairs), A symmetric positive definite, one shared
  * real Krylov space.  X is device complex (interleaved), column s at

@@ -21,7 +21,7 @@ module psfun_hip_mod
   integer(c_int), parameter :: PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1
   integer(c_int), parameter :: PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, &
                                 PSFUN_FORMAT_HDIA = 3
-  integer(c_int), parameter :: PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1
+  integer(c_int), parameter :: PSFUN_PREC_NONE = 0, PSFUN_PREC_DIAG = 1, PSFUN_PREC_BJAC = 2
   integer(c_int), parameter :: PSFUN_QUAD_RE_SUM = 0, PSFUN_QUAD_A_IM_SUM = 1
 
   interface
@@ -237,6 +237,41 @@ module psfun_hip_mod
       integer(c_int), intent(out) :: iter
       real(c_double), intent(out) :: err
       type(c_ptr), value :: cond     ! c_loc(a real(c_double)) or c_null_ptr
+      integer(c_int) :: st
+    end function
+    function psfun_cg_shifted_prec(ctx, a, p, b, eta, zeta, x, eps, itmax, istop, iter, err, cond) &
+         & bind(C, name='psfun_cg_shifted_prec') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, a, p, b, x
+      integer(c_int), value :: itmax, istop
+      real(c_double), value :: eta, zeta, eps
+      integer(c_int), intent(out) :: iter
+      real(c_double), intent(out) :: err
+      type(c_ptr), value :: cond
+      integer(c_int) :: st
+    end function
+    ! ---- preconditioner objects (prec%init / prec%build / prec%apply) -------
+    function psfun_prec_create(ctx, a, kind, p) bind(C, name='psfun_prec_create') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, a
+      integer(c_int), value :: kind
+      type(c_ptr), intent(out) :: p
+      integer(c_int) :: st
+    end function
+    function psfun_prec_destroy(p) bind(C, name='psfun_prec_destroy') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int) :: st
+    end function
+    function psfun_prec_info(p, kind, levels) bind(C, name='psfun_prec_info') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int), intent(out) :: kind, levels
+      integer(c_int) :: st
+    end function
+    function psfun_prec_apply(ctx, p, r, z) bind(C, name='psfun_prec_apply') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, p, r, z
       integer(c_int) :: st
     end function
     function psfun_mscg(ctx, a, nshift, shifts, b, x, ldx, eps, itmax, iters, resid) &

@@ -73,6 +73,10 @@ def lib():
                                     C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]
         L.or_cg_shifted.restype = C.c_int
+        L.or_ilu0.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, C.POINTER(C.c_int)]
+        L.or_ilu0.restype = C.c_int
+        L.or_ilu0_apply.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.or_ilu0_apply.restype = C.c_int
         L.or_mscg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
                               np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), _dp]
         L.or_mscg.restype = C.c_int
@@ -249,6 +253,31 @@ def dense_inverse(H):
     if rc:
         raise RuntimeError("singular matrix")
     return out.reshape(k, k, order="F")
+
+
+def ilu0(A):
+    """ILU(0) factors of A (the BJAC preconditioner): (status, rowptr,
+    colind, lu) of one sorted CSR holding L's strict lower part (unit
+    diagonal) and U."""
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    orp = np.zeros(n + 1, np.int32)
+    oci = np.zeros(max(len(ci), 1), np.int32)
+    olu = np.zeros(max(len(ci), 1))
+    nz = C.c_int(0)
+    rc = lib().or_ilu0(n, rp, ci, va, orp, oci, olu, C.byref(nz))
+    return rc, orp, oci[:nz.value].copy(), olu[:nz.value].copy()
+
+
+def ilu0_apply(A, r):
+    """z = (LU)^{-1} r with LU the ILU(0) of A; raises on a zero pivot."""
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    z = np.zeros(n)
+    rc = lib().or_ilu0_apply(n, rp, ci, va, np.ascontiguousarray(r, np.float64), z)
+    if rc:
+        raise RuntimeError("ILU(0): zero pivot")
+    return z
 
 
 def cg_shifted(A, b, eta, zeta, x0=None, eps=1e-8, itmax=1000, istop=2, prec=0, want_cond=False):

@@ -25,6 +25,8 @@
  *   or_arnoldi                         -- same oracles as or_lanczos
  *   or_sai_arnoldi / or_dense_inverse  -- DST closed form, dense expm, A*inverse = I
  *   or_cg_shifted                      -- dense solve, residual check, PSBLAS stop rules
+ *   or_ilu0 / or_ilu0_apply            -- L U equals A on A's pattern; exact LU (M^{-1} A = I) for
+ *                                         tridiagonal and dense patterns; the breakdown on a zero pivot
  *   or_mscg                            -- dense complex solve per shift
  *   or_ellipkkp / or_ellipj / horner  -- the control values of examples/utils/utiltest.f90,
  *                                         mpmath elliptic functions
@@ -797,8 +799,126 @@ int or_sai_arnoldi(int n, const int *rp, const int *ci, const double *va, double
 /* ------------------------------------------------------------------ */
 /* Shifted PCG (krylov/shifted/psfun_dcg.F90:249-357)                  */
 /* ------------------------------------------------------------------ */
+/* ILU(0) on A's own sparsity pattern: the BJAC preconditioner PSBLAS builds
+ * for prec%init(ctxt, 'BJAC', info) with one process (one diagonal block, the
+ * whole matrix; default block solver ILU(0)).  PSBLAS is not part of the
+ * reference tree: this follows the textbook IKJ algorithm (Saad, Iterative
+ * Methods for Sparse Linear Systems, 2nd ed., Alg. 10.4) on rows sorted by
+ * column with repeated entries summed in input order, with explicit fma.
+ * f->lu holds the strict lower part of L (unit diagonal) and U in one CSR;
+ * f->dp the position of each diagonal.  Returns 0, or 10 on a zero or
+ * missing pivot. */
+typedef struct {
+  int n;
+  int *rp, *ci, *dp;
+  double *lu;
+} or_ilu_t;
+
+static void ilu0_free(or_ilu_t *f) {
+  free(f->rp); free(f->ci); free(f->dp); free(f->lu);
+}
+
+static int ilu0_build(int n, const int *rp, const int *ci, const double *va, or_ilu_t *f) {
+  f->n = n;
+  f->rp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  f->ci = (int *)malloc(sizeof(int) * (size_t)(rp[n] > 0 ? rp[n] : 1));
+  f->lu = (double *)malloc(sizeof(double) * (size_t)(rp[n] > 0 ? rp[n] : 1));
+  f->dp = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *where = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) where[i] = -1;
+  int nz = 0;
+  f->rp[0] = 0;
+  for (int i = 0; i < n; ++i) { /* sorted row, repeats summed in input order (stable insertion) */
+    const int start = nz;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) {
+      int pos = start;
+      while (pos < nz && f->ci[pos] <= ci[q]) ++pos;
+      if (pos > start && f->ci[pos - 1] == ci[q]) {
+        f->lu[pos - 1] += va[q];
+        continue;
+      }
+      for (int t = nz; t > pos; --t) {
+        f->ci[t] = f->ci[t - 1];
+        f->lu[t] = f->lu[t - 1];
+      }
+      f->ci[pos] = ci[q];
+      f->lu[pos] = va[q];
+      ++nz;
+    }
+    f->rp[i + 1] = nz;
+  }
+  int rc = 0;
+  for (int i = 0; i < n && !rc; ++i) {
+    f->dp[i] = -1;
+    for (int q = f->rp[i]; q < f->rp[i + 1]; ++q) where[f->ci[q]] = q;
+    for (int q = f->rp[i]; q < f->rp[i + 1]; ++q) {
+      const int k = f->ci[q];
+      if (k >= i) break;
+      const int dk = f->dp[k];
+      if (dk < 0 || f->lu[dk] == 0.0) {
+        rc = 10;
+        break;
+      }
+      const double lik = f->lu[q] / f->lu[dk];
+      f->lu[q] = lik;
+      for (int t = dk + 1; t < f->rp[k + 1]; ++t) {
+        const int wj = where[f->ci[t]];
+        if (wj >= 0) f->lu[wj] = fma(-lik, f->lu[t], f->lu[wj]);
+      }
+    }
+    for (int q = f->rp[i]; q < f->rp[i + 1]; ++q) {
+      if (f->ci[q] == i) f->dp[i] = q;
+      where[f->ci[q]] = -1;
+    }
+    if (!rc && (f->dp[i] < 0 || f->lu[f->dp[i]] == 0.0)) rc = 10;
+  }
+  free(where);
+  return rc;
+}
+
+/* z = U^{-1} L^{-1} r: forward then backward substitution, terms in column order */
+static void ilu0_solve(const or_ilu_t *f, const double *r, double *z) {
+  const int n = f->n;
+  for (int i = 0; i < n; ++i) {
+    double t = r[i];
+    for (int q = f->rp[i]; q < f->dp[i]; ++q) t = fma(-f->lu[q], z[f->ci[q]], t);
+    z[i] = t;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double t = z[i];
+    for (int q = f->dp[i] + 1; q < f->rp[i + 1]; ++q) t = fma(-f->lu[q], z[f->ci[q]], t);
+    z[i] = t / f->lu[f->dp[i]];
+  }
+}
+
+/* The factors as a sorted CSR (out_rp n+1, out_ci/out_lu up to nnz); returns
+ * the status of ilu0_build and the stored count in *nnz_out. */
+int or_ilu0(int n, const int *rp, const int *ci, const double *va, int *out_rp, int *out_ci, double *out_lu,
+            int *nnz_out) {
+  or_ilu_t f;
+  const int rc = ilu0_build(n, rp, ci, va, &f);
+  for (int i = 0; i <= n; ++i) out_rp[i] = f.rp[i];
+  for (int q = 0; q < f.rp[n]; ++q) {
+    out_ci[q] = f.ci[q];
+    out_lu[q] = f.lu[q];
+  }
+  *nnz_out = f.rp[n];
+  ilu0_free(&f);
+  return rc;
+}
+
+/* z = M^{-1} r with M the ILU(0) of A */
+int or_ilu0_apply(int n, const int *rp, const int *ci, const double *va, const double *r, double *z) {
+  or_ilu_t f;
+  const int rc = ilu0_build(n, rp, ci, va, &f);
+  if (!rc) ilu0_solve(&f, r, z);
+  ilu0_free(&f);
+  return rc;
+}
+
 /* (eta*A + zeta*I) x = b, x holds the initial guess.  prec: 0 = none,
- * 1 = diagonal of the unshifted A (the DIAG preconditioner built on A).
+ * 1 = diagonal of the unshifted A (the DIAG preconditioner built on A),
+ * 2 = ILU(0) of the unshifted A (BJAC).
  * Stopping follows PSBLAS psb_init_conv/psb_check_conv/psb_end_conv:
  * istop 1: ||r||_inf <= eps (||A||_inf ||x||_inf + ||b||_inf);
  * istop 2: ||r||_2 <= eps ||b||_2; and it >= itmax stops.
@@ -810,6 +930,16 @@ int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int pre
   double *p = (double *)calloc((size_t)n, sizeof(double)), *q = (double *)calloc((size_t)n, sizeof(double)),
          *r = (double *)calloc((size_t)n, sizeof(double)), *z = (double *)calloc((size_t)n, sizeof(double)),
          *dinv = NULL;
+  or_ilu_t ilu;
+  int have_ilu = 0;
+  if (prec == 2) {
+    if (ilu0_build(n, rp, ci, va, &ilu)) {
+      ilu0_free(&ilu);
+      free(p); free(q); free(r); free(z);
+      return 10;
+    }
+    have_ilu = 1;
+  }
   if (prec == 1) {
     dinv = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; ++i) {
@@ -845,6 +975,8 @@ int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int pre
       itx++;
       if (prec == 1)
         for (int i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
+      else if (prec == 2)
+        ilu0_solve(&ilu, r, z);
       else
         or_axpby(n, 1.0, r, 0.0, z);
       rho_old = rho;
@@ -910,6 +1042,7 @@ int or_cg_shifted(int n, const int *rp, const int *ci, const double *va, int pre
   if (err) *err = (errden == 0.0) ? errnum : errnum / errden;
   if (iter) *iter = itx;
   free(p); free(q); free(r); free(z); free(dinv);
+  if (have_ilu) ilu0_free(&ilu);
   return 0;
 }
 

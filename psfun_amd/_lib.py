@@ -35,6 +35,7 @@ PSFUN_FORMAT_SELL = 2
 PSFUN_FORMAT_HDIA = 3
 PSFUN_PREC_NONE = 0
 PSFUN_PREC_DIAG = 1
+PSFUN_PREC_BJAC = 2
 PSFUN_ORDER_NATURAL = 0
 PSFUN_ORDER_RCM = 1
 PSFUN_QUAD_RE_SUM = 0
@@ -106,6 +107,11 @@ SIGNATURES = {
     "psfun_krylov_resolvents": (_i, [_vp, _i64, _i, _i, _vp, _vp, _vp, _i, _d, _vp, _vp, _i, _vp, _vp]),
     "psfun_sddmm_cols": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _d, _vp]),
     "psfun_cg_shifted": (_i, [_vp, _vp, _i, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
+    "psfun_prec_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
+    "psfun_prec_destroy": (_i, [_vp]),
+    "psfun_prec_info": (_i, [_vp, _pi, _pi]),
+    "psfun_prec_apply": (_i, [_vp, _vp, _vp, _vp]),
+    "psfun_cg_shifted_prec": (_i, [_vp, _vp, _vp, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
     "psfun_mscg": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
     "psfun_horner": (_d, [_i, _vp, _d]),
     "psfun_ellipkkp": (_i, [_d, _vp]),

@@ -839,7 +839,7 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
     if (variant < PSFUN_VAR_CHBSYM || variant > PSFUN_VAR_SASQ) return PSFUN_ERR_UNSUPPORTED;
   }
   if (itmax < 2 || itmax > LZ_NL_MAX || itmax > PSFUN_DENSE_KMAX) return PSFUN_ERR_INVALID_ARG;
-  if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG) return PSFUN_ERR_INVALID_ARG;
+  if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG && prec != PSFUN_PREC_BJAC) return PSFUN_ERR_INVALID_ARG;
   if (inner_istop < 1 || inner_istop > 2) return PSFUN_ERR_INVALID_ISTOP;
   const int n = A->n;
   const int nl = itmax;
@@ -850,6 +850,13 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
     return PSFUN_SUCCESS;
   }
   PSFUN_HIP_TRY(hipSetDevice(c->device));
+  // BJAC: the ILU(0) of the unshifted A is built once for all inner solves
+  // (the reference builds its inner preconditioner on A, psfun_d_krylov_mod.F90:326-403)
+  struct PrecGuard {
+    psfun_prec_t P = nullptr;
+    ~PrecGuard() { psfun_prec_destroy(P); }
+  } pobj;
+  if (prec == PSFUN_PREC_BJAC) PSFUN_TRY(psfun_prec_create(c, A, PSFUN_PREC_BJAC, &pobj.P));
   const int model = userf ? DENSE_SYM_UPPER : dense_model(variant);
   const bool fsync = userf && istop == 1;
   const int istop_k = userf ? 2 : istop;
@@ -894,8 +901,10 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
       PSFUN_HIP_TRY(hipGetLastError());
       int it_in = 0;
       double err_in = 0.0;
-      const int st = psfun_cg_shifted(c, A, prec, V + (size_t)ldv * (j - 1), 1.0, tau, w, eps, inner_itmax,
-                                      inner_istop, &it_in, &err_in, nullptr);
+      const int st = pobj.P ? psfun_cg_shifted_prec(c, A, pobj.P, V + (size_t)ldv * (j - 1), 1.0, tau, w, eps,
+                                                    inner_itmax, inner_istop, &it_in, &err_in, nullptr)
+                            : psfun_cg_shifted(c, A, prec, V + (size_t)ldv * (j - 1), 1.0, tau, w, eps,
+                                               inner_itmax, inner_istop, &it_in, &err_in, nullptr);
       if (st != PSFUN_SUCCESS) return st;
       // modified Gram-Schmidt against v_1..v_j, then v_{j+1} = w / h(j+1, j)
       PSFUN_TRY(ar_orthogonalize(c, orth, n, j, V, ldv, w, P[0], ctl, H, ldh));

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
                                                    const double *__restrict__ p_rho, int g_rho,
                                                    const double *__restrict__ p_conv, int g_in,
                                                    double *__restrict__ p_sig, CgCtl *ctl, int istop, double eps,
-                                                   int itmax, int *hf) {
+                                                   int itmax, int *hf, const double *__restrict__ zv) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   // block-uniform: the control block of this same grid may set the flag
   if (__syncthreads_or(ctl->done)) return;
@@ -97,10 +97,10 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
   }
   if (it > 1 && rho_old == 0.0) return;
   const double beta = (it > 1) ? rho / rho_old : 0.0;
-  // direction p = M^{-1} r + beta p_old, formed on the fly at every gathered column
-  auto pk = [&](int c) -> double {
-    return (it > 1) ? fma(beta, p_old[c], prec_apply(dinv, c, r[c])) : prec_apply(dinv, c, r[c]);
-  };
+  // direction p = M^{-1} r + beta p_old, formed on the fly at every gathered
+  // column; z = M^{-1} r is read from zv when a preconditioner object formed it
+  auto zk = [&](int c) -> double { return zv ? zv[c] : prec_apply(dinv, c, r[c]); };
+  auto pk = [&](int c) -> double { return (it > 1) ? fma(beta, p_old[c], zk(c)) : zk(c); };
   constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
   const XcdRange xr = xcd_range_g(nchunks, G);
@@ -172,6 +172,18 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
     p_conv[blockIdx.x] = a1;
     if (istop == 1) p_conv[gridDim.x + blockIdx.x] = a2;
   }
+}
+
+// Partials of rho = r.z (z = M^{-1} r formed by a preconditioner object).
+__global__ __launch_bounds__(PSFUN_BS) void k_cg_rz(int n, const double *__restrict__ r, const double *__restrict__ z,
+                                                    double *__restrict__ p_rho, const CgCtl *ctl) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS)
+    acc = fma(r[i], z[i], acc);
+  acc = block_sum<PSFUN_BS>(acc, lds);
+  if (threadIdx.x == 0) p_rho[blockIdx.x] = acc;
 }
 
 // r = b - eta*A x - zeta*x; partials of rho = r.M^{-1}r (restart entry, lines 259-262).
@@ -316,13 +328,11 @@ static double tridiag_extreme(const std::vector<double> &d, const std::vector<do
   return 0.5 * (lo + hi);
 }
 
-extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const double *b, double eta, double zeta,
-                                double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond) {
-  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
-  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
-  if (!b || !x) return PSFUN_ERR_INVALID_VECT_STATE;
-  if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG) return PSFUN_ERR_INVALID_ARG;
-  if (itmax < 0) return PSFUN_ERR_INVALID_ARG;
+// The solver: prec NONE or DIAG (formed in the kernels), or the
+// preconditioner object P (BJAC: z = M^{-1} r by its triangular solves after
+// every residual update, then the rho partials r.z).
+static int cg_run(psfun_ctx *c, psfun_csr *A, int prec, const psfun_prec *P, const double *b, double eta,
+                  double zeta, double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond) {
   const int n = A->n;
   if (cond) *cond = 0.0;
   if (n == 0) {
@@ -331,14 +341,16 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
     return PSFUN_SUCCESS;
   }
   PSFUN_HIP_TRY(hipSetDevice(c->device));
+  const bool zmode = prec == PSFUN_PREC_BJAC;
   const size_t ctl_bytes = align_up(sizeof(CgCtl), 256);
   const size_t vec = align_up((size_t)n, 64);
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace(c, ctl_bytes + sizeof(double) * vec * 5, &ws));
+  PSFUN_TRY(psfun_workspace(c, ctl_bytes + sizeof(double) * vec * 6, &ws));
   CgCtl *ctl = (CgCtl *)ws;
   double *r = (double *)((char *)ws + ctl_bytes);
-  double *pA = r + vec, *pB = pA + vec, *q = pB + vec, *dinv = q + vec;
+  double *pA = r + vec, *pB = pA + vec, *q = pB + vec, *dinv = q + vec, *zv = dinv + vec;
   if (prec != PSFUN_PREC_DIAG) dinv = nullptr;
+  if (!zmode) zv = nullptr;
 
   double *Psig = c->d_partials, *Prho = c->d_partials + PSFUN_MAX_GRID,
          *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID,
@@ -365,6 +377,11 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
       hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, Pani, Pb, Gu, ctl, 0);
       PSFUN_HIP_TRY(hipGetLastError());
     }
+    if (zmode) {  // z = M^{-1} r and rho = r.z partials over Gu blocks
+      PSFUN_TRY(psfun_prec_launch(c, P, r, zv));
+      hipLaunchKernelGGL(k_cg_rz, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, zv, Prho, ctl);
+      PSFUN_HIP_TRY(hipGetLastError());
+    }
     c->h_flags[0] = 0;
     c->h_flags[1] = 0;
     int it;
@@ -373,13 +390,18 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_CG, 12.0 * A->nnz + 4.0 * (n + 1) + (it > 1 ? 40.0 : 32.0) * n));
       PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_cg_a<LL, SS>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n,
                                                    MRP, MCOL, MVAL, eta, zeta, r, dinv, pold, pnew, q, it, Gs, Prho,
-                                                   it == 1 ? Gs : Gu, Pconv, Gu, Psig, ctl, istop, eps, itmax,
-                                                   c->d_flags));
+                                                   (it == 1 && !zmode) ? Gs : Gu, Pconv, Gu, Psig, ctl, istop, eps,
+                                                   itmax, c->d_flags, zv));
       PSFUN_HIP_TRY(hipGetLastError());
       PSFUN_TRY(psfun_prof_end(c));
       hipLaunchKernelGGL(k_cg_b, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, x, r, pnew, q, dinv, it, Psig, Gs, Prho,
                          Pconv, ctl, istop, c->d_flags);
       PSFUN_HIP_TRY(hipGetLastError());
+      if (zmode) {  // the next iteration's z = M^{-1} r and rho = r.z
+        PSFUN_TRY(psfun_prec_launch(c, P, r, zv));
+        hipLaunchKernelGGL(k_cg_rz, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, zv, Prho, ctl);
+        PSFUN_HIP_TRY(hipGetLastError());
+      }
       std::swap(pold, pnew);
       // pacing on the host flags: about two iterations queued ahead of the
       // one whose control block has reported
@@ -413,4 +435,32 @@ extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const do
     }
   }
   return PSFUN_SUCCESS;
+}
+
+extern "C" int psfun_cg_shifted(psfun_ctx_t c, psfun_csr_t A, int prec, const double *b, double eta, double zeta,
+                                double *x, double eps, int itmax, int istop, int *iter, double *err, double *cond) {
+  if (!c || !A) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!b || !x) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (prec != PSFUN_PREC_NONE && prec != PSFUN_PREC_DIAG && prec != PSFUN_PREC_BJAC) return PSFUN_ERR_INVALID_ARG;
+  if (itmax < 0) return PSFUN_ERR_INVALID_ARG;
+  if (prec != PSFUN_PREC_BJAC || A->n == 0)
+    return cg_run(c, A, prec, nullptr, b, eta, zeta, x, eps, itmax, istop, iter, err, cond);
+  // BJAC without a prebuilt object: build it for this call
+  psfun_prec_t P = nullptr;
+  PSFUN_TRY(psfun_prec_create(c, A, PSFUN_PREC_BJAC, &P));
+  const int st = cg_run(c, A, prec, P, b, eta, zeta, x, eps, itmax, istop, iter, err, cond);
+  psfun_prec_destroy(P);
+  return st;
+}
+
+extern "C" int psfun_cg_shifted_prec(psfun_ctx_t c, psfun_csr_t A, psfun_prec_t P, const double *b, double eta,
+                                     double zeta, double *x, double eps, int itmax, int istop, int *iter, double *err,
+                                     double *cond) {
+  if (!c || !A || !P) return PSFUN_ERR_INVALID_ARG;
+  if (psfun_prec_size(P) != A->n) return PSFUN_ERR_INVALID_ARG;
+  if (istop < 1 || istop > 2) return PSFUN_ERR_INVALID_ISTOP;
+  if (!b || !x) return PSFUN_ERR_INVALID_VECT_STATE;
+  if (itmax < 0) return PSFUN_ERR_INVALID_ARG;
+  return cg_run(c, A, psfun_prec_kind(P), P, b, eta, zeta, x, eps, itmax, istop, iter, err, cond);
 }

@@ -8,3 +8,8 @@ int launch_axpby(psfun_ctx *c, int64_t n, double alpha, const double *x, double 
 // Looks up the device address of the allocation-free entry point's fixed
 // scratch on `device` (current); defined beside that array in lanczos.hip.
 int psfun_noalloc_setup(int device);
+// Preconditioner objects (prec.hip): z = M^{-1} r on the context stream for a
+// BJAC object; the kind and order of an object.
+int psfun_prec_launch(psfun_ctx *c, const psfun_prec *P, const double *r, double *z);
+int psfun_prec_kind(const psfun_prec *P);
+int psfun_prec_size(const psfun_prec *P);

@@ -189,7 +189,7 @@ def shifted_driver(deck, out=sys.stdout):
     desc = psb.psb_desc_type(n, ctx)
     prec = psb_dprec_type()
     if prec.init(ctx, ptype) != 0:
-        raise DeckError(f"preconditioner {ptype!r}: NONE and DIAG are available")
+        raise DeckError(f"preconditioner {ptype!r}: NONE, DIAG and BJAC are available")
     t0 = time.perf_counter()
     prec.build(a, desc)
     tprec = time.perf_counter() - t0

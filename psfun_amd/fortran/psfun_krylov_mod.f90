@@ -64,7 +64,12 @@ contains
     cnd = dzero
     pc = c_null_ptr
     if (want_cond) pc = c_loc(cnd)
-    st = psfun_cg_shifted(a%ctxt%ctx, a%a, prec%get_code(), b%v, eta, zeta, x%v, eps, itmax_, istop_, it, derr, pc)
+    if (c_associated(prec%p)) then      ! a built BJAC object
+      st = psfun_cg_shifted_prec(a%ctxt%ctx, a%a, prec%p, b%v, eta, zeta, x%v, eps, itmax_, istop_, it, derr, pc)
+    else
+      st = psfun_cg_shifted(a%ctxt%ctx, a%a, prec%get_code(), b%v, eta, zeta, x%v, eps, itmax_, istop_, it, &
+           & derr, pc)
+    end if
     if (st /= PSFUN_SUCCESS) then
       info = psb_err_from_subroutine_
       call psb_errpush(info, 'psfun_krylov', a_err=trim(method))

@@ -17,29 +17,71 @@ from . import psb
 
 
 class psb_dprec_type:
-    """Preconditioner object with the two kinds the shifted CG supports on
-    the device: 'NONE' and 'DIAG' (diagonal of the matrix it was built on;
-    the reference's driver builds it on the unshifted A)."""
+    """Preconditioner object with the kinds the shifted CG applies on the
+    device, built on the matrix given to build() (the reference's drivers
+    build it on the unshifted A, shiftedtest.f90:219-223):
+      'NONE' / 'NOPREC'
+      'DIAG'  the inverse diagonal
+      'BJAC'  block Jacobi, which with one process is the whole matrix
+              factored by ILU(0) (host factorization, device triangular
+              solves by level sets; psfun_prec_create)."""
 
     def __init__(self):
         self.ptype = None
         self.built = False
+        self.handle = None
+        self.n = None
 
     def init(self, ctxt, ptype):
         p = str(ptype).upper()
-        if p not in ("NONE", "NOPREC", "DIAG"):
+        if p not in ("NONE", "NOPREC", "DIAG", "BJAC"):
             return psb.psb_err_invalid_args_combination_
-        self.ptype = "NONE" if p in ("NONE", "NOPREC") else "DIAG"
+        self.free()
+        self.ptype = {"NONE": "NONE", "NOPREC": "NONE"}.get(p, p)
+        self.built = False
         return psb.psb_success_
 
-    def build(self, a, desc_a):
+    def build(self, a, desc_a=None):
         if self.ptype is None:
             return psb.psb_err_invalid_args_combination_
+        self.free()
+        if self.ptype == "BJAC":
+            h = C.c_void_p()
+            st = _lib.lib().psfun_prec_create(a.ctxt.handle, a.handle, _lib.PSFUN_PREC_BJAC, C.byref(h))
+            if st != _lib.PSFUN_SUCCESS:
+                return psb.psb_err_from_subroutine_
+            self.handle = h
+        self.n = a.n
         self.built = True
         return psb.psb_success_
 
     def code(self):
-        return _lib.PSFUN_PREC_DIAG if self.ptype == "DIAG" else _lib.PSFUN_PREC_NONE
+        return {"DIAG": _lib.PSFUN_PREC_DIAG, "BJAC": _lib.PSFUN_PREC_BJAC}.get(self.ptype, _lib.PSFUN_PREC_NONE)
+
+    def levels(self):
+        """Level-scheduled steps of one BJAC application (0 otherwise)."""
+        if self.handle is None:
+            return 0
+        k, lv = C.c_int(), C.c_int()
+        _lib.check(_lib.lib().psfun_prec_info(self.handle, C.byref(k), C.byref(lv)))
+        return lv.value
+
+    def apply(self, ctx, r, z):
+        """z = M^{-1} r on device tensors (BJAC objects)."""
+        ctx.bind_stream()
+        _lib.check(_lib.lib().psfun_prec_apply(ctx.handle, self.handle, C.c_void_p(r.data_ptr()),
+                                               C.c_void_p(z.data_ptr())), "psfun_prec_apply")
+
+    def free(self):
+        if getattr(self, "handle", None) is not None:
+            _lib.lib().psfun_prec_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def psb_krylov(method, a, prec, b, eta, zeta, x, eps, desc_a, itmax=None, itrace=None, irst=None, istop=None,
@@ -63,9 +105,15 @@ def psb_krylov(method, a, prec, b, eta, zeta, x, eps, desc_a, itmax=None, itrace
     it = C.c_int(0)
     err = C.c_double(0.0)
     cnd = C.c_double(0.0)
-    st = _lib.lib().psfun_cg_shifted(ctx.handle, a.handle, pc, C.c_void_p(b.v.data_ptr()), float(eta), float(zeta),
-                                     C.c_void_p(x.v.data_ptr()), float(eps), itmax_, istop_, C.byref(it),
-                                     C.byref(err), C.byref(cnd) if want_cond else None)
+    if prec is not None and getattr(prec, "handle", None) is not None:   # a built BJAC object
+        st = _lib.lib().psfun_cg_shifted_prec(ctx.handle, a.handle, prec.handle, C.c_void_p(b.v.data_ptr()),
+                                              float(eta), float(zeta), C.c_void_p(x.v.data_ptr()), float(eps),
+                                              itmax_, istop_, C.byref(it), C.byref(err),
+                                              C.byref(cnd) if want_cond else None)
+    else:
+        st = _lib.lib().psfun_cg_shifted(ctx.handle, a.handle, pc, C.c_void_p(b.v.data_ptr()), float(eta),
+                                         float(zeta), C.c_void_p(x.v.data_ptr()), float(eps), itmax_, istop_,
+                                         C.byref(it), C.byref(err), C.byref(cnd) if want_cond else None)
     if st != _lib.PSFUN_SUCCESS:
         return psb.psb_err_from_subroutine_, 0, 0.0, 0.0
     return psb.psb_success_, it.value, err.value, (cnd.value if want_cond else 0.0)

@@ -41,7 +41,7 @@ def build(verbose=True):
     sys.path.insert(0, REPO)
     from psfun_amd.fortran import build as fb
     flib = fb.build(verbose=verbose)
-    steps.append([fc, "-O2", "-I", fb.OUT, os.path.join(HERE, "psfun_typed.f90"), flib, "-o", TYPED,
+    steps.append([fc, "-O2", "-I", fb.OUT, "-module-dir", OUT, os.path.join(HERE, "psfun_typed.f90"), flib, "-o", TYPED,
                   "-L", os.path.join(REPO, "psfun_amd"), "-lpsfun_hip", "-Wl,-rpath,$ORIGIN/../../../psfun_amd"])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)

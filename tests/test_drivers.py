@@ -92,6 +92,12 @@ def test_drivers_run_reference_decks(gpu_ctx, tmp_path, monkeypatch):
     assert r["info"] == 0
     assert np.linalg.norm(r["x"] - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
     assert r["residual_shifted"] <= 1e-7 * np.sqrt(50)
+    # the same deck with the block-Jacobi preconditioner (ILU(0) of the unshifted A)
+    r = drivers.shifted_driver(SHIFTED.replace("DIAG", "BJAC"), out=out)
+    ref = O.cg_shifted(A, ones, -1.0, 1.0, eps=1e-7, itmax=500, istop=2, prec=2)
+    assert r["info"] == 0 and abs(r["iter"] - ref["iter"]) <= 1
+    assert np.linalg.norm(r["x"] - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
+    assert np.linalg.norm(r["x"] - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
 
 
 @pytest.mark.gpu

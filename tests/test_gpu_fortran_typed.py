@@ -118,18 +118,20 @@ def test_typed_lanczos_plain_config(gpu_ctx, tmp_path, afmt):
     assert np.linalg.norm(y - yex) <= 1e-10 * np.linalg.norm(yex)
 
 
-def test_typed_shiftedtest_deck(gpu_ctx, tmp_path):
-    """shiftedtest with shifted.inp: psb_krylov('CG', a, prec DIAG, b, -1, 1,
-    x, 1e-7, ...), istop 2, itmax 500: x against the dense-solve fixture and
-    the oracle; the driver's residual uses the unshifted A."""
+@pytest.mark.parametrize("ptype,code", [("DIAG", 1), ("BJAC", 2)])
+def test_typed_shiftedtest_deck(gpu_ctx, tmp_path, ptype, code):
+    """shiftedtest with shifted.inp: psb_krylov('CG', a, prec, b, -1, 1,
+    x, 1e-7, ...), istop 2, itmax 500, with the deck's DIAG and with BJAC
+    (ILU(0) of the unshifted A): x against the dense-solve fixture and the
+    oracle; the driver's residual uses the unshifted A."""
     A = _laplacian50(tmp_path)
-    (tmp_path / "shifted.inp").write_text(SHIFTED)
+    (tmp_path / "shifted.inp").write_text(SHIFTED.replace("DIAG", ptype))
     out, _ = _run(tmp_path, "shifted", "shifted.inp")
     assert int(out["INFO"][0][0]) == 0
     x = _vec(out, "Y")
     g = load_golden("laplacian50.npz")
     assert np.linalg.norm(x - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
-    ref = O.cg_shifted(A, np.ones(50), -1.0, 1.0, eps=1e-7, itmax=500, istop=2, prec=1, want_cond=True)
+    ref = O.cg_shifted(A, np.ones(50), -1.0, 1.0, eps=1e-7, itmax=500, istop=2, prec=code, want_cond=True)
     assert abs(int(out["ITER"][0][0]) - ref["iter"]) <= 1
     assert np.linalg.norm(x - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
     assert float(out["ERR"][0][0]) <= 1e-7
@@ -137,7 +139,11 @@ def test_typed_shiftedtest_deck(gpu_ctx, tmp_path):
     assert abs(float(out["RESN2"][0][0]) - np.linalg.norm(r)) <= 1e-10 * max(np.linalg.norm(r), 1.0)
     cond = float(out["COND"][0][0])
     if int(out["ITER"][0][0]) == ref["iter"]:       # the ratio of the CG tridiagonal's extreme eigenvalues
-        assert abs(cond - ref["cond"]) <= 1e-6 * abs(ref["cond"])
+        # with BJAC the preconditioned operator is -I + A^{-1}, whose spectrum
+        # clusters near -1.25; the Ritz value there is not converged at 16
+        # iterations and moves by 1e-3 relative when b is perturbed by 1e-14
+        # (the oracle's own value), so rounding differences move it as much
+        assert abs(cond - ref["cond"]) <= (1e-6 if ptype == "DIAG" else 3e-3) * abs(ref["cond"])
 
 
 def test_typed_serialtest_deck(gpu_ctx, tmp_path):

@@ -90,3 +90,23 @@ def test_sai_istop2_and_zero_vector(gpu_ctx):
     info, it, err, res = saiarnoldi_apply(_fun(), M, "CG", None, -1.0, psb.psb_desc_type(n), 4000, 0, 2, y,
                                           _vec(np.zeros(n)), 1e-10, itmax=12)
     assert info == 0 and it == 0 and np.all(y.get_vect() == 0.0)
+
+
+def test_sai_bjac_inner_solves(gpu_ctx):
+    """Inner solves preconditioned by BJAC (ILU(0) of the unshifted matrix,
+    built once per apply) against the oracle run with the same preconditioner
+    and the DST closed form."""
+    A = matgen.laplacian(DIMS, scale=-SCALE)
+    n = int(np.prod(DIMS))
+    x = matgen.normal(23, n)
+    M = psb.psb_dspmat_type.from_csr(A)
+    p = psb_dprec_type()
+    p.init(None, "BJAC")
+    y = _vec(np.zeros(n))
+    info, it, err, res = saiarnoldi_apply(_fun(), M, "CG", p, -2.0, psb.psb_desc_type(n), 4000, 0, 2, y, _vec(x),
+                                          1e-10, itmax=150, istop=1)
+    assert info == 0 and err < 1e-10
+    ref = O.sai_arnoldi(A, x, 1e-10, 150, tau=-2.0, istop=1, prec=2, in_itmax=4000, in_istop=2)
+    assert abs(it - ref["iter"]) <= 1
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-8
+    assert _rel(y.get_vect(), matgen.dst_expm_laplacian(DIMS, x, SCALE)) <= 2e-8

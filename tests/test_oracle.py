@@ -283,6 +283,93 @@ def test_cg_breakdown_restarts_count_iterations(itmax):
     assert r["err"] == 0.0       # no convergence check ran (psb_init_conv's errnum = 0)
 
 
+# ---- ILU(0), the BJAC preconditioner with one process ----------------------
+def _from_dense(M):
+    import scipy.sparse as sps
+    S = sps.csr_matrix(M)
+    S.sort_indices()
+    return S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64)
+
+
+def _random_sparse(n, density, seed, diag_shift):
+    import scipy.sparse as sps
+    S = sps.random(n, n, density=density, random_state=seed, format="csr") + diag_shift * sps.eye(n, format="csr")
+    return _from_dense(S.toarray())
+
+
+def _ilu_factors(rp, ci, lu):
+    n = len(rp) - 1
+    L, U = np.eye(n), np.zeros((n, n))
+    for i in range(n):
+        for q in range(rp[i], rp[i + 1]):
+            (L if ci[q] < i else U)[i, ci[q]] = lu[q]
+    return L, U
+
+
+@pytest.mark.parametrize("make", [lambda: matgen.anisotropic_2d(7, seed=5), lambda: matgen.laplacian((5, 4, 3)),
+                                  lambda: _random_sparse(40, 0.12, 2, 6.0)])
+def test_ilu0_product_equals_a_on_pattern(make):
+    """ILU(0) is fixed by two conditions: L and U live on A's pattern and
+    (L U)_ij = a_ij on that pattern; a wrong update order or a missed term
+    breaks the second."""
+    A = make()
+    S = matgen.to_scipy(A).toarray()
+    rc, rp, ci, lu = O.ilu0(A)
+    assert rc == 0
+    L, U = _ilu_factors(rp, ci, lu)
+    pat = S != 0
+    assert np.array_equal((L != 0) | (U != 0) | np.eye(len(S), dtype=bool), pat | np.eye(len(S), dtype=bool))
+    P = L @ U
+    assert np.max(np.abs(P[pat] - S[pat])) <= 1e-13 * np.max(np.abs(S))
+    # fill outside the pattern is dropped, so M differs from A there (for a 2-D stencil)
+    if not np.allclose(P, S):
+        assert np.max(np.abs(P[~pat])) > 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 17])
+def test_ilu0_tridiagonal_and_dense_are_exact(n):
+    """No fill for a tridiagonal or a dense pattern: M = A, so M^{-1} A x = x."""
+    rng = np.random.default_rng(n)
+    T = np.diag(4.0 + rng.random(n)) + np.diag(rng.random(n - 1) - 0.5, 1) + np.diag(rng.random(n - 1) - 0.5, -1)
+    D = rng.random((n, n)) + n * np.eye(n)
+    for M in (T, D):
+        sp = _from_dense(M)
+        x = rng.standard_normal(n)
+        z = O.ilu0_apply(sp, M @ x)
+        assert np.linalg.norm(z - x) <= 1e-13 * np.linalg.norm(x) * n
+    # the dense pattern's factors are the unpivoted LU: L U = D everywhere
+    rc, rp, ci, lu = O.ilu0(_from_dense(D))
+    L, U = _ilu_factors(rp, ci, lu)
+    assert rc == 0 and np.max(np.abs(L @ U - D)) <= 1e-13 * n * np.max(np.abs(D))
+
+
+def test_ilu0_zero_pivot_breakdown():
+    """A zero leading entry (or a row without a diagonal) stops the build."""
+    Z = _from_dense(np.array([[0.0, 1.0], [1.0, 2.0]]))
+    assert O.ilu0(Z)[0] == 10
+    with pytest.raises(RuntimeError):
+        O.ilu0_apply(Z, np.ones(2))
+    assert O.cg_shifted(Z, np.ones(2), 1.0, 0.0, prec=2)["info"] == 10
+    nodiag = (np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, 1.0]))
+    assert O.ilu0(nodiag)[0] == 10
+
+
+def test_cg_ilu0_dense_solve_and_fewer_iterations():
+    """BJAC-preconditioned shifted CG solves (eta A + zeta I) x = b, with the
+    factors taken from the unshifted A, and needs fewer iterations than DIAG
+    on an anisotropic operator."""
+    A = matgen.anisotropic_2d(12, seed=4)
+    n = 144
+    Ad = matgen.to_scipy(A).toarray()
+    b = matgen.uniform_pm1(3, n)
+    eta, zeta = 1.0, 0.1
+    ex = np.linalg.solve(eta * Ad + zeta * np.eye(n), b)
+    rb = O.cg_shifted(A, b, eta, zeta, eps=1e-12, itmax=2000, istop=2, prec=2)
+    rd = O.cg_shifted(A, b, eta, zeta, eps=1e-12, itmax=2000, istop=2, prec=1)
+    assert rb["info"] == 0 and np.linalg.norm(rb["x"] - ex) <= 1e-10 * np.linalg.norm(ex)
+    assert rb["iter"] < rd["iter"] / 2
+
+
 # ---- multi-shift CG, quadrature --------------------------------------------
 def test_mscg_dense_solve():
     A = matgen.laplacian((8, 8))
