@@ -416,6 +416,25 @@ int psfun_talbot(int N, double t, double *xi, double *c, double *eta, double *si
  * 42-55, 114-142); this is the apply its documentation describes
  * (docs/pages/algorithms.md:106-125). */
 enum { PSFUN_QUAD_RE_SUM = 0, PSFUN_QUAD_A_IM_SUM = 1 };
+/* Preconditioned forms of psfun_mscg and psfun_quad_apply: the setting
+ * psfun_z_quadrature%set(prec) stores for the poles' solves
+ * (quadratures/psfun_z_quadrature_mod.F90:156-166), applied as
+ * krylov/shifted/psfun_dcg.F90:283 applies prec%apply.  A preconditioner of A
+ * is not shift-invariant, so the multi-shift solve runs first and every shift
+ * it leaves above eps after itmax iterations continues from its iterate by
+ * preconditioned COCG (all such shifts in one launch sequence) for at most
+ * itmax more iterations, with M = I (P of kind NONE), diag(A) + sigma_s I
+ * (DIAG) or the ILU(0) of A (BJAC, applied to the real and imaginary parts).
+ * P = NULL is psfun_mscg / psfun_quad_apply.  iters = multi-shift plus COCG
+ * iterations; resid = the final ||r_s|| / ||b|| (recursive estimate for the
+ * shifts the multi-shift solve finished, true residual for the others).
+ * nshift <= 64.  The quadrature form keeps the shifted solutions in the
+ * context's outer workspace (16 n nshift bytes). */
+int psfun_mscg_prec(psfun_ctx_t ctx, psfun_csr_t A, psfun_prec_t P, int nshift, const double *shifts,
+                    const double *b, double *X, int64_t ldx, double eps, int itmax, int *iters, double *resid);
+int psfun_quad_apply_prec(psfun_ctx_t ctx, psfun_csr_t A, psfun_prec_t P, int nshift, const double *shifts,
+                          const double *weights, double gamma, int form, const double *x, double *y, double eps,
+                          int itmax, int *iters, double *resid);
 /* Sampled derivative for the adjoint of a resolvent sum: for every nonzero q
  * = (i, k) of the device CSR pattern (rowptr, colind; n rows),
  *   grad[q] += scale * sum_s Re(w_s U_s[i] V_s[k]),

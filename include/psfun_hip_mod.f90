@@ -298,6 +298,30 @@ module psfun_hip_mod
       integer(c_int) :: st
     end function
 
+    function psfun_mscg_prec(ctx, a, p, nshift, shifts, b, x, ldx, eps, itmax, iters, resid) &
+         & bind(C, name='psfun_mscg_prec') result(st)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, a, p, b, x
+      integer(c_int), value :: nshift, itmax
+      real(c_double), intent(in) :: shifts(*)
+      integer(c_int64_t), value :: ldx
+      real(c_double), value :: eps
+      integer(c_int), intent(out) :: iters(*)
+      real(c_double), intent(out) :: resid(*)
+      integer(c_int) :: st
+    end function
+    function psfun_quad_apply_prec(ctx, a, p, nshift, shifts, weights, gamma, form, x, y, eps, itmax, iters, resid) &
+         & bind(C, name='psfun_quad_apply_prec') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, a, p, x, y
+      integer(c_int), value :: nshift, form, itmax
+      real(c_double), intent(in) :: shifts(*), weights(*)
+      real(c_double), value :: gamma, eps
+      integer(c_int), intent(out) :: iters(*)
+      real(c_double), intent(out) :: resid(*)
+      integer(c_int) :: st
+    end function
+
     ! ---- quadrature rules and special functions (host) ---------------------
     function psfun_horner(nc, c, x) bind(C, name='psfun_horner') result(v)
       import :: c_int, c_double

@@ -29,6 +29,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.or_set_threads.argtypes = [C.c_int]
         L.or_get_max_threads.restype = C.c_int
+        L.or_get_threads.restype = C.c_int
         L.or_spmv.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, _dp, C.c_double, _dp]
         L.or_dot.argtypes = [C.c_int, _dp, _dp]
         L.or_dot.restype = C.c_double
@@ -92,12 +93,27 @@ def lib():
         L.or_quad_apply.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp,
                                     C.c_double, C.c_int, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.or_quad_apply.restype = C.c_int
+        _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.or_quad_apply_prec.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
+                                         _dp, _dp, C.c_double, C.c_int, _i32, _dp]
+        L.or_quad_apply_prec.restype = C.c_int
+        L.or_pcocg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_double,
+                               C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.or_pcocg.restype = C.c_int
+        L.or_mscg_prec.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                                   _i32, _dp]
+        L.or_mscg_prec.restype = C.c_int
         _lib = L
     return _lib
 
 
 def set_threads(t):
     lib().or_set_threads(int(t))
+
+
+def threads():
+    """The thread count the oracle's loops use (1 unless set_threads)."""
+    return lib().or_get_threads()
 
 
 def max_threads():
@@ -306,6 +322,37 @@ def mscg(A, b, shifts, eps=1e-8, itmax=1000):
     return dict(X=Xc[..., 0] + 1j * Xc[..., 1], iters=iters, resid=resid)
 
 
+def pcocg(A, b, sigma, x0=None, prec=0, eps=1e-8, itmax=1000):
+    """Preconditioned COCG on (A + sigma I) x = b (prec 0 none, 1 Jacobi of
+    the shifted matrix, 2 ILU(0) of A)."""
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    x = np.zeros(2 * n) if x0 is None else np.ascontiguousarray(
+        np.stack([np.real(x0), np.imag(x0)], 1).reshape(-1), np.float64)
+    it = C.c_int(0)
+    rs = C.c_double(0.0)
+    rc = lib().or_pcocg(n, rp, ci, va, prec, float(np.real(sigma)), float(np.imag(sigma)),
+                        np.ascontiguousarray(b, np.float64), x, eps, itmax, C.byref(it), C.byref(rs))
+    xc = x.reshape(n, 2)
+    return dict(info=rc, x=xc[:, 0] + 1j * xc[:, 1], iter=it.value, resid=rs.value)
+
+
+def mscg_prec(A, b, shifts, prec, eps=1e-8, itmax=1000):
+    """or_mscg followed by the preconditioned per-shift COCG fallback."""
+    rp, ci, va = _csr(A)
+    n = len(rp) - 1
+    shifts = np.asarray(shifts, dtype=np.complex128)
+    ns = len(shifts)
+    sg = np.ascontiguousarray(np.stack([shifts.real, shifts.imag], 1).reshape(-1))
+    X = np.zeros(2 * ns * n)
+    iters = np.zeros(ns, dtype=np.int32)
+    resid = np.zeros(ns)
+    rc = lib().or_mscg_prec(n, rp, ci, va, prec, ns, sg, np.ascontiguousarray(b, np.float64), X, eps, itmax,
+                            iters, resid)
+    Xc = X.reshape(ns, n, 2)
+    return dict(info=rc, X=Xc[..., 0] + 1j * Xc[..., 1], iters=iters, resid=resid)
+
+
 def ellipkkp(L):
     K = np.zeros(2)
     lib().or_ellipkkp(L, K)
@@ -345,7 +392,9 @@ def talbot(N, t=1.0):
     return s[0::2] + 1j * s[1::2], w[0::2] + 1j * w[1::2]
 
 
-def quad_apply(A, shifts, weights, gamma, form, x, eps=1e-8, itmax=2000):
+def quad_apply(A, shifts, weights, gamma, form, x, eps=1e-8, itmax=2000, prec=None):
+    """prec None: the multi-shift solve alone; 0, 1, 2: with the
+    preconditioned per-shift fallback (or_quad_apply_prec)."""
     rp, ci, va = _csr(A)
     n = len(rp) - 1
     shifts = np.asarray(shifts, np.complex128)
@@ -355,6 +404,7 @@ def quad_apply(A, shifts, weights, gamma, form, x, eps=1e-8, itmax=2000):
     ww = np.ascontiguousarray(np.stack([weights.real, weights.imag], 1).reshape(-1))
     y = np.zeros(n)
     iters = np.zeros(ns, dtype=np.int32)
-    lib().or_quad_apply(n, rp, ci, va, ns, sg, ww, gamma, form, np.ascontiguousarray(x, np.float64), y, eps,
-                        itmax, iters)
-    return dict(y=y, iters=iters)
+    resid = np.zeros(ns)
+    rc = lib().or_quad_apply_prec(n, rp, ci, va, -1 if prec is None else int(prec), ns, sg, ww, gamma, form,
+                                  np.ascontiguousarray(x, np.float64), y, eps, itmax, iters, resid)
+    return dict(info=rc, y=y, iters=iters, resid=resid)

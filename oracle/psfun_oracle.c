@@ -28,6 +28,9 @@
  *   or_ilu0 / or_ilu0_apply            -- L U equals A on A's pattern; exact LU (M^{-1} A = I) for
  *                                         tridiagonal and dense patterns; the breakdown on a zero pivot
  *   or_mscg                            -- dense complex solve per shift
+ *   or_pcocg / or_mscg_prec            -- dense complex solve per shift for each preconditioner; with
+ *                                         M = I one COCG run equals or_mscg's iterate for one shift
+ *                                         (the shared Krylov space); warm start from a solution
  *   or_ellipkkp / or_ellipj / horner  -- the control values of examples/utils/utiltest.f90,
  *                                         mpmath elliptic functions
  *   or_hhtmethod1 / or_quad_apply      -- scipy sqrtm / expm on small SPD matrices
@@ -52,6 +55,7 @@ void or_set_threads(int t) {
   omp_set_num_threads(g_threads);
 #endif
 }
+int or_get_threads(void) { return g_threads; }
 int or_get_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -1076,6 +1080,7 @@ int or_mscg(int n, const int *rp, const int *ci, const double *va, int nshift, c
     zp[s] = 1.0;
     act[s] = 1;
     iters[s] = 0;
+    if (resid) resid[s] = (bn == 0.0) ? 0.0 : 1.0;   /* |zeta_0| ||r_0|| / ||b|| */
     for (int i = 0; i < n; ++i) {
       Xs[(size_t)s * n + i] = 0.0;
       Ps[(size_t)s * n + i] = b[i];
@@ -1122,6 +1127,156 @@ int or_mscg(int n, const int *rp, const int *ci, const double *va, int nshift, c
     if (rr == 0.0) break;
   }
   free(sg); free(Ps); free(r); free(p); free(q); free(zc); free(zp); free(act);
+  return 0;
+}
+
+/* Preconditioned COCG (van der Vorst and Melissen 1990: CG with the
+ * bilinear form x^T y in place of x^H y, valid for complex symmetric
+ * systems) for (A + sigma I) x = b, b real, x complex: the initial guess on
+ * entry, the solution on exit.  M is complex symmetric:
+ *   prec 0: M = I
+ *   prec 1: M = diag(A) + sigma I, the Jacobi matrix of the shifted system
+ *           (an entry equal to 0 is replaced by 1)
+ *   prec 2: M = L U, the ILU(0) of the real A (L D L^T for a symmetric A),
+ *           applied to the real and the imaginary part.
+ *   r = b - (A + sigma I) x;  ||r|| <= eps ||b|| already: no iteration
+ *   z = M^{-1} r; p = z; rho = r^T z
+ *   repeat: q = (A + sigma I) p; mu = p^T q; alpha = rho / mu
+ *           x += alpha p; r -= alpha q; it++
+ *           stop when ||r||_2 <= eps ||b||_2 or it == itmax
+ *           z = M^{-1} r; rho' = r^T z; p = z + (rho'/rho) p; rho = rho'
+ * mu == 0 or rho == 0 ends the run (a breakdown; the iterate is kept).
+ * b = 0 gives x = 0.  iter = iterations done, resid = ||r||_2 / ||b||_2 of
+ * the last residual.  Returns 10 on a zero ILU(0) pivot. */
+static int pcocg_run(int n, const int *rp, const int *ci, const double *va, int prec, const or_ilu_t *ilu,
+                     zdouble sigma, const double *b, zdouble *x, double eps, int itmax, int *iter, double *resid) {
+  const double bn = or_nrm2(n, b);
+  *iter = 0;
+  if (bn == 0.0) {
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    *resid = 0.0;
+    return 0;
+  }
+  zdouble *r = (zdouble *)malloc(sizeof(zdouble) * n), *z = (zdouble *)malloc(sizeof(zdouble) * n),
+          *p = (zdouble *)malloc(sizeof(zdouble) * n), *q = (zdouble *)malloc(sizeof(zdouble) * n),
+          *dm = NULL;
+  double *tr = (double *)malloc(sizeof(double) * n), *ti = (double *)malloc(sizeof(double) * n),
+         *ur = (double *)malloc(sizeof(double) * n), *ui = (double *)malloc(sizeof(double) * n);
+  if (prec == 1) {
+    dm = (zdouble *)malloc(sizeof(zdouble) * n);
+    for (int i = 0; i < n; ++i) {
+      double dv = 0.0;
+      for (int j = rp[i]; j < rp[i + 1]; ++j)
+        if (ci[j] == i) dv += va[j];
+      const zdouble m = dv + sigma;
+      dm[i] = (m != 0.0) ? m : 1.0;
+    }
+  }
+#define PC_AX(src, dst)                                                             \
+  for (int i = 0; i < n; ++i) {                                                     \
+    zdouble t = 0.0;                                                                \
+    for (int j = rp[i]; j < rp[i + 1]; ++j) t += va[j] * src[ci[j]];                \
+    dst[i] = t + sigma * src[i];                                                    \
+  }
+#define PC_PREC()                                                                   \
+  if (prec == 1) {                                                                  \
+    for (int i = 0; i < n; ++i) z[i] = r[i] / dm[i];                                \
+  } else if (prec == 2) {                                                           \
+    for (int i = 0; i < n; ++i) { tr[i] = creal(r[i]); ti[i] = cimag(r[i]); }       \
+    ilu0_solve(ilu, tr, ur);                                                        \
+    ilu0_solve(ilu, ti, ui);                                                        \
+    for (int i = 0; i < n; ++i) z[i] = ur[i] + I * ui[i];                           \
+  } else {                                                                          \
+    for (int i = 0; i < n; ++i) z[i] = r[i];                                        \
+  }
+  PC_AX(x, q);
+  double rn2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    r[i] = b[i] - q[i];
+    rn2 += creal(r[i]) * creal(r[i]) + cimag(r[i]) * cimag(r[i]);
+  }
+  *resid = sqrt(rn2) / bn;
+  if (sqrt(rn2) > eps * bn && itmax > 0) {
+    PC_PREC();
+    zdouble rho = 0.0;
+    for (int i = 0; i < n; ++i) {
+      rho += r[i] * z[i];
+      p[i] = z[i];
+    }
+    int it = 0;
+    while (rho != 0.0) {
+      PC_AX(p, q);
+      zdouble mu = 0.0;
+      for (int i = 0; i < n; ++i) mu += p[i] * q[i];
+      if (mu == 0.0) break;
+      const zdouble alpha = rho / mu;
+      rn2 = 0.0;
+      for (int i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+        rn2 += creal(r[i]) * creal(r[i]) + cimag(r[i]) * cimag(r[i]);
+      }
+      it++;
+      *resid = sqrt(rn2) / bn;
+      if (sqrt(rn2) <= eps * bn || it >= itmax) break;
+      PC_PREC();
+      zdouble rho_new = 0.0;
+      for (int i = 0; i < n; ++i) rho_new += r[i] * z[i];
+      const zdouble beta = rho_new / rho;
+      for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+      rho = rho_new;
+    }
+    *iter = it;
+  }
+#undef PC_AX
+#undef PC_PREC
+  free(r); free(z); free(p); free(q); free(dm); free(tr); free(ti); free(ur); free(ui);
+  return 0;
+}
+
+int or_pcocg(int n, const int *rp, const int *ci, const double *va, int prec, double sre, double sim,
+             const double *b, double *x, double eps, int itmax, int *iter, double *resid) {
+  or_ilu_t ilu;
+  int have = 0;
+  if (prec == 2) {
+    if (ilu0_build(n, rp, ci, va, &ilu)) {
+      ilu0_free(&ilu);
+      return 10;
+    }
+    have = 1;
+  }
+  const int rc = pcocg_run(n, rp, ci, va, prec, have ? &ilu : NULL, sre + I * sim, b, (zdouble *)x, eps, itmax,
+                           iter, resid);
+  if (have) ilu0_free(&ilu);
+  return rc;
+}
+
+/* or_mscg, then each shift it left above eps (resid > eps) continues from
+ * its multi-shift iterate by preconditioned COCG (prec as in or_pcocg) for
+ * at most itmax more iterations: iters = multi-shift + COCG iterations,
+ * resid = COCG's final relative residual for those shifts. */
+int or_mscg_prec(int n, const int *rp, const int *ci, const double *va, int prec, int nshift, const double *shifts,
+                 const double *b, double *X, double eps, int itmax, int *iters, double *resid) {
+  or_ilu_t ilu;
+  int have = 0;
+  if (prec == 2) {
+    if (ilu0_build(n, rp, ci, va, &ilu)) {
+      ilu0_free(&ilu);
+      return 10;
+    }
+    have = 1;
+  }
+  or_mscg(n, rp, ci, va, nshift, shifts, b, X, eps, itmax, iters, resid);
+  for (int s = 0; s < nshift; ++s) {
+    if (!(resid[s] > eps)) continue;
+    int it = 0;
+    double rs = 0.0;
+    pcocg_run(n, rp, ci, va, prec, have ? &ilu : NULL, shifts[2 * s] + I * shifts[2 * s + 1], b,
+              (zdouble *)X + (size_t)s * n, eps, itmax, &it, &rs);
+    iters[s] += it;
+    resid[s] = rs;
+  }
+  if (have) ilu0_free(&ilu);
   return 0;
 }
 
@@ -1270,14 +1425,25 @@ int or_talbot(int N, double t, double *sigma, double *w) {
   return 0;
 }
 
-/* Quadrature apply.  v_j = (A + sigma_j I)^{-1} x by or_mscg, then
+/* Quadrature apply.  v_j = (A + sigma_j I)^{-1} x by or_mscg (prec < 0) or
+ * by or_mscg_prec (prec 0, 1, 2: the preconditioned per-shift fallback), then
  *   form 0 (resolvent sum): y = gamma * Re( sum_j w_j v_j )
  *   form 1 (HHT method 1):  y = gamma * A * Im( sum_j w_j v_j ). */
-int or_quad_apply(int n, const int *rp, const int *ci, const double *va, int nshift, const double *sigma,
-                  const double *w, double gamma, int form, const double *x, double *y, double eps, int itmax,
-                  int *iters) {
+int or_quad_apply_prec(int n, const int *rp, const int *ci, const double *va, int prec, int nshift,
+                       const double *sigma, const double *w, double gamma, int form, const double *x, double *y,
+                       double eps, int itmax, int *iters, double *resid) {
   double *X = (double *)malloc(sizeof(double) * 2 * (size_t)nshift * n);
-  or_mscg(n, rp, ci, va, nshift, sigma, x, X, eps, itmax, iters, NULL);
+  double *rs = resid ? resid : (double *)malloc(sizeof(double) * (nshift > 0 ? nshift : 1));
+  int rc = 0;
+  if (prec < 0)
+    or_mscg(n, rp, ci, va, nshift, sigma, x, X, eps, itmax, iters, rs);
+  else
+    rc = or_mscg_prec(n, rp, ci, va, prec, nshift, sigma, x, X, eps, itmax, iters, rs);
+  if (!resid) free(rs);
+  if (rc) {
+    free(X);
+    return rc;
+  }
   double *acc = (double *)calloc((size_t)n, sizeof(double));
   for (int s = 0; s < nshift; ++s) {
     const zdouble ws = w[2 * s] + I * w[2 * s + 1];
@@ -1295,4 +1461,10 @@ int or_quad_apply(int n, const int *rp, const int *ci, const double *va, int nsh
   free(X);
   free(acc);
   return 0;
+}
+
+int or_quad_apply(int n, const int *rp, const int *ci, const double *va, int nshift, const double *sigma,
+                  const double *w, double gamma, int form, const double *x, double *y, double eps, int itmax,
+                  int *iters) {
+  return or_quad_apply_prec(n, rp, ci, va, -1, nshift, sigma, w, gamma, form, x, y, eps, itmax, iters, NULL);
 }

@@ -112,6 +112,8 @@ SIGNATURES = {
     "psfun_prec_info": (_i, [_vp, _pi, _pi]),
     "psfun_prec_apply": (_i, [_vp, _vp, _vp, _vp]),
     "psfun_cg_shifted_prec": (_i, [_vp, _vp, _vp, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
+    "psfun_mscg_prec": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
+    "psfun_quad_apply_prec": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _d, _i, _vp, _vp, _d, _i, _vp, _vp]),
     "psfun_mscg": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i64, _d, _i, _vp, _vp]),
     "psfun_horner": (_d, [_i, _vp, _d]),
     "psfun_ellipkkp": (_i, [_d, _vp]),

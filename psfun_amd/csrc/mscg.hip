@@ -196,7 +196,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
     int ux = 0, up = 0;
     cplx zn = {0.0, 0.0}, al = {0.0, 0.0}, be = {0.0, 0.0};
     double res = 0.0;
-    if (t < nshift && st->active[t]) {
+    if (t < nshift && st->active[t] && !(ctl->bnorm > 0.0)) {
+      ux = 1;  // b = 0: x^s = 0 (alpha^s = 0 on the zero start), no iteration
+    } else if (t < nshift && st->active[t]) {
       const cplx zc = {st->zc[t][0], st->zc[t][1]}, zp = {st->zp[t][0], st->zp[t][1]};
       const cplx sg = {ctl->sig[t][0], ctl->sig[t][1]};
       // denominator: a_k b_{k-1} (zp - zc) + zp a_{k-1} (1 + s a_k)
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
         nx->zp[t][1] = st->zc[t][1];
         nx->zc[t][0] = zn.re;
         nx->zc[t][1] = zn.im;
-        ctl->iters[t] = k + 1;
+        ctl->iters[t] = (ctl->bnorm > 0.0) ? k + 1 : 0;
         ctl->resid[t] = (ctl->bnorm > 0.0) ? res / ctl->bnorm : 0.0;
       } else {
         nx->zp[t][0] = st->zp[t][0];
@@ -372,6 +374,13 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
     if (resid) resid[s] = h->resid[s];
   }
   return PSFUN_SUCCESS;
+}
+
+int psfun_mscg_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *shifts, const double *b, double *X,
+                   int64_t ldx, double eps, int itmax, int *iters, double *resid) {
+  MsWork w;
+  PSFUN_TRY(ms_workspace(c, A->n, nshift, false, &w));
+  return ms_run(c, A, nshift, shifts, b, (double2 *)X, ldx, w, eps, itmax, iters, resid);
 }
 
 extern "C" int psfun_mscg(psfun_ctx_t c, psfun_csr_t A, int nshift, const double *shifts, const double *b,

@@ -110,6 +110,64 @@ __global__ __launch_bounds__(PSFUN_BS) void k_trsv_levels(int l0, int l1, const 
   }
 }
 
+// The same solves for S complex right-hand sides stored shift-major with
+// leading dimension ld, for the systems s with run[s] != 0 (read from device
+// memory, the same value for every thread): the real and the imaginary part
+// take the real solve's operations one by one, so each equals the real solve
+// of that part bit for bit.
+template <bool LOWER>
+__global__ __launch_bounds__(PSFUN_BS) void k_trsv_levels_c(int l0, int l1, const int *__restrict__ lvl_ptr,
+                                                            const int *__restrict__ rows, const int *__restrict__ tp,
+                                                            const int *__restrict__ tc, const double *__restrict__ tv,
+                                                            const double *__restrict__ diag,
+                                                            const double2 *__restrict__ rhs, double2 *__restrict__ out,
+                                                            int64_t ld, int S, const int *__restrict__ run) {
+  for (int l = l0; l < l1; ++l) {
+    const int b = lvl_ptr[l], e = lvl_ptr[l + 1];
+    for (int idx = b + (int)(blockIdx.x * PSFUN_BS + threadIdx.x); idx < e; idx += (int)(gridDim.x * PSFUN_BS)) {
+      const int i = rows[idx];
+      const int q0 = tp[i], q1 = tp[i + 1];
+      for (int s = 0; s < S; ++s) {
+        if (!run[s]) continue;
+        const double2 *o = out + (size_t)s * ld;
+        double2 t = rhs[(size_t)s * ld + i];
+        for (int q = q0; q < q1; ++q) {
+          const double v = tv[q];
+          const double2 u = o[tc[q]];
+          t.x = fma(-v, u.x, t.x);
+          t.y = fma(-v, u.y, t.y);
+        }
+        if (!LOWER) {
+          const double d = diag[i];
+          t.x = t.x / d;
+          t.y = t.y / d;
+        }
+        out[(size_t)s * ld + i] = t;
+      }
+    }
+    if (l + 1 < l1) __syncthreads();
+  }
+}
+
+int psfun_prec_launch_c(psfun_ctx *c, const psfun_prec *P, const double2 *r, double2 *z, double2 *ybuf, int64_t ld,
+                        int S, const int *run) {
+  if (P->kind != PSFUN_PREC_BJAC) return PSFUN_ERR_INVALID_ARG;
+  for (const auto &g : P->lgroups) {
+    const int rows = P->lptr[g.second] - P->lptr[g.first];
+    const int G = (g.second - g.first > 1) ? 1 : std::min((rows + PSFUN_BS - 1) / PSFUN_BS, 8 * c->num_cu);
+    hipLaunchKernelGGL(k_trsv_levels_c<true>, dim3(G), dim3(PSFUN_BS), 0, c->stream, g.first, g.second, P->d_lptr,
+                       P->lrows, P->Lp, P->Lc, P->Lv, (const double *)nullptr, r, ybuf, ld, S, run);
+  }
+  for (const auto &g : P->ugroups) {
+    const int rows = P->uptr[g.second] - P->uptr[g.first];
+    const int G = (g.second - g.first > 1) ? 1 : std::min((rows + PSFUN_BS - 1) / PSFUN_BS, 8 * c->num_cu);
+    hipLaunchKernelGGL(k_trsv_levels_c<false>, dim3(G), dim3(PSFUN_BS), 0, c->stream, g.first, g.second, P->d_uptr,
+                       P->urows, P->Up, P->Uc, P->Uv, P->ud, (const double2 *)ybuf, z, ld, S, run);
+  }
+  PSFUN_HIP_TRY(hipGetLastError());
+  return PSFUN_SUCCESS;
+}
+
 // z = M^{-1} r on the context stream (r and z distinct device vectors).
 int psfun_prec_launch(psfun_ctx *c, const psfun_prec *P, const double *r, double *z) {
   if (P->kind != PSFUN_PREC_BJAC) return PSFUN_ERR_INVALID_ARG;

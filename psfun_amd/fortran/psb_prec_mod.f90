@@ -17,7 +17,7 @@ module psb_prec_mod
     character(len=20) :: ptype = ''
     integer(c_int) :: code = PSFUN_PREC_NONE
     logical :: built = .false.
-    type(c_ptr) :: p = c_null_ptr       ! the device object of a built BJAC
+    type(c_ptr) :: p = c_null_ptr       ! the device object once built
     integer(c_int) :: levels = 0
   contains
     procedure, pass(prec) :: init => psb_dprec_init
@@ -67,8 +67,8 @@ contains
     prec%levels = 0
   end subroutine release
 
-  ! The device CG forms the inverse diagonal of the matrix it is given (the
-  ! unshifted A) itself; BJAC factors A here (a zero pivot is an error).
+  ! Creates the device object: NONE and DIAG keep no state (the solvers form
+  ! the diagonal they need), BJAC factors A here (a zero pivot is an error).
   subroutine psb_dprec_build(prec, a, desc_a, info)
     class(psb_dprec_type), intent(inout) :: prec
     type(psb_dspmat_type), intent(in) :: a
@@ -82,16 +82,14 @@ contains
       return
     end if
     call release(prec)
-    if (prec%code == PSFUN_PREC_BJAC) then
-      st = psfun_prec_create(a%ctxt%ctx, a%a, PSFUN_PREC_BJAC, prec%p)
-      if (st /= PSFUN_SUCCESS) then
-        prec%p = c_null_ptr
-        info = psb_err_from_subroutine_
-        call psb_errpush(info, 'psb_precbld', a_err='ILU(0) factorization')
-        return
-      end if
-      st = psfun_prec_info(prec%p, kind, prec%levels)
+    st = psfun_prec_create(a%ctxt%ctx, a%a, prec%code, prec%p)
+    if (st /= PSFUN_SUCCESS) then
+      prec%p = c_null_ptr
+      info = psb_err_from_subroutine_
+      call psb_errpush(info, 'psb_precbld', a_err='ILU(0) factorization')
+      return
     end if
+    st = psfun_prec_info(prec%p, kind, prec%levels)
     prec%built = .true.
   end subroutine psb_dprec_build
 
@@ -111,7 +109,7 @@ contains
     type(psb_desc_type), intent(in) :: desc_a
     integer(psb_ipk_), intent(out) :: info
     info = psb_success_
-    if (.not. c_associated(prec%p)) then
+    if (.not. c_associated(prec%p) .or. prec%code /= PSFUN_PREC_BJAC) then
       info = psb_err_invalid_args_combination_
       call psb_errpush(info, 'psb_precaply', a_err='no built BJAC object')
       return

@@ -64,7 +64,7 @@ contains
     cnd = dzero
     pc = c_null_ptr
     if (want_cond) pc = c_loc(cnd)
-    if (c_associated(prec%p)) then      ! a built BJAC object
+    if (c_associated(prec%p)) then      ! a built preconditioner object
       st = psfun_cg_shifted_prec(a%ctxt%ctx, a%a, prec%p, b%v, eta, zeta, x%v, eps, itmax_, istop_, it, derr, pc)
     else
       st = psfun_cg_shifted(a%ctxt%ctx, a%a, prec%get_code(), b%v, eta, zeta, x%v, eps, itmax_, istop_, it, &

@@ -17,7 +17,9 @@
 !
 ! The stored rule (xi, c, eta, sign) means the shifted systems
 ! (xi_j I + sign A) v_j = x.  apply solves all of them with one multi-shift
-! CG on the device (psfun_quad_apply) and assembles
+! CG on the device (psfun_quad_apply; with a built preconditioner set, the
+! poles left above eps continue by preconditioned COCG, psfun_quad_apply_prec)
+! and assembles
 !   hhtmethod1:  y = eta * A * Im(sum_j c_j v_j)
 !   other rules: y = eta * Re(sum_j c_j v_j)     (complex-conjugate pole pairs
 !                                                 folded into one solve)
@@ -311,7 +313,17 @@ contains
       ww(2 * i - 1) = real(w(i), c_double)
       ww(2 * i) = aimag(w(i))
     end do
-    st = psfun_quad_apply(quad%a%ctxt%ctx, quad%a%a, ns, sh, ww, gamma, form, x%v, y%v, eps_, itmax_, it, rr)
+    if (associated(quad%prec)) then
+      if (.not. c_associated(quad%prec%p)) then
+        info = psb_err_invalid_args_combination_
+        call psb_errpush(info, 'psfun_z_quadrature_apply', a_err='preconditioner not built')
+        return
+      end if
+      st = psfun_quad_apply_prec(quad%a%ctxt%ctx, quad%a%a, quad%prec%p, ns, sh, ww, gamma, form, x%v, y%v, &
+           & eps_, itmax_, it, rr)
+    else
+      st = psfun_quad_apply(quad%a%ctxt%ctx, quad%a%a, ns, sh, ww, gamma, form, x%v, y%v, eps_, itmax_, it, rr)
+    end if
     if (st /= PSFUN_SUCCESS) then
       info = psb_status_to_info(st)
       call psb_errpush(info, 'psfun_z_quadrature_apply', a_err=trim(psb_status_text(st)))

@@ -45,12 +45,11 @@ class psb_dprec_type:
         if self.ptype is None:
             return psb.psb_err_invalid_args_combination_
         self.free()
-        if self.ptype == "BJAC":
-            h = C.c_void_p()
-            st = _lib.lib().psfun_prec_create(a.ctxt.handle, a.handle, _lib.PSFUN_PREC_BJAC, C.byref(h))
-            if st != _lib.PSFUN_SUCCESS:
-                return psb.psb_err_from_subroutine_
-            self.handle = h
+        h = C.c_void_p()
+        st = _lib.lib().psfun_prec_create(a.ctxt.handle, a.handle, self.code(), C.byref(h))
+        if st != _lib.PSFUN_SUCCESS:
+            return psb.psb_err_from_subroutine_
+        self.handle = h
         self.n = a.n
         self.built = True
         return psb.psb_success_
@@ -60,7 +59,7 @@ class psb_dprec_type:
 
     def levels(self):
         """Level-scheduled steps of one BJAC application (0 otherwise)."""
-        if self.handle is None:
+        if self.handle is None or self.ptype != "BJAC":
             return 0
         k, lv = C.c_int(), C.c_int()
         _lib.check(_lib.lib().psfun_prec_info(self.handle, C.byref(k), C.byref(lv)))
@@ -105,7 +104,7 @@ def psb_krylov(method, a, prec, b, eta, zeta, x, eps, desc_a, itmax=None, itrace
     it = C.c_int(0)
     err = C.c_double(0.0)
     cnd = C.c_double(0.0)
-    if prec is not None and getattr(prec, "handle", None) is not None:   # a built BJAC object
+    if prec is not None and getattr(prec, "handle", None) is not None:   # a built object
         st = _lib.lib().psfun_cg_shifted_prec(ctx.handle, a.handle, prec.handle, C.c_void_p(b.v.data_ptr()),
                                               float(eta), float(zeta), C.c_void_p(x.v.data_ptr()), float(eps),
                                               itmax_, istop_, C.byref(it), C.byref(err),
@@ -119,9 +118,12 @@ def psb_krylov(method, a, prec, b, eta, zeta, x, eps, desc_a, itmax=None, itrace
     return psb.psb_success_, it.value, err.value, (cnd.value if want_cond else 0.0)
 
 
-def mscg(a, b, shifts, eps=1e-8, itmax=1000):
+def mscg(a, b, shifts, eps=1e-8, itmax=1000, prec=None):
     """Solve (A + sigma_s I) X[s] = b for every complex shift on the GPU.
-    Returns (X as a complex128 torch tensor of shape (S, n), iters, resid)."""
+    With a built preconditioner object, the shifts the multi-shift solve
+    leaves above eps after itmax iterations continue by preconditioned COCG
+    (psfun_mscg_prec).  Returns (X as a complex128 torch tensor of shape
+    (S, n), iters, resid)."""
     import torch
     shifts = np.asarray(shifts, dtype=np.complex128).reshape(-1)
     S = len(shifts)
@@ -132,8 +134,17 @@ def mscg(a, b, shifts, eps=1e-8, itmax=1000):
     resid = np.zeros(S)
     ctx = a.ctxt
     ctx.bind_stream()
-    st = _lib.lib().psfun_mscg(ctx.handle, a.handle, S, sh.ctypes.data_as(C.c_void_p), C.c_void_p(b.v.data_ptr()),
-                               C.c_void_p(X.data_ptr()), n, float(eps), int(itmax), iters.ctypes.data_as(C.c_void_p),
-                               resid.ctypes.data_as(C.c_void_p))
-    _lib.check(st, "psfun_mscg")
+    if prec is not None:
+        if getattr(prec, "handle", None) is None:
+            raise ValueError("mscg: the preconditioner is not built (prec.build(a))")
+        st = _lib.lib().psfun_mscg_prec(ctx.handle, a.handle, prec.handle, S, sh.ctypes.data_as(C.c_void_p),
+                                        C.c_void_p(b.v.data_ptr()), C.c_void_p(X.data_ptr()), n, float(eps),
+                                        int(itmax), iters.ctypes.data_as(C.c_void_p),
+                                        resid.ctypes.data_as(C.c_void_p))
+        _lib.check(st, "psfun_mscg_prec")
+    else:
+        st = _lib.lib().psfun_mscg(ctx.handle, a.handle, S, sh.ctypes.data_as(C.c_void_p),
+                                   C.c_void_p(b.v.data_ptr()), C.c_void_p(X.data_ptr()), n, float(eps), int(itmax),
+                                   iters.ctypes.data_as(C.c_void_p), resid.ctypes.data_as(C.c_void_p))
+        _lib.check(st, "psfun_mscg")
     return X, iters, resid

@@ -8,10 +8,12 @@
 !                                              overridden as lines 211-215 do)
 !   psfun_typed shifted <deck>                 examples/shiftedkrylov/shiftedtest.f90
 !   psfun_typed serial  <deck>                 examples/serial/serialtest.f90
-!   psfun_typed quad <matrix> HHT|TALBOT <N> <p1> [p2]
+!   psfun_typed quad <matrix> HHT|TALBOT <N> <p1> [p2] [PREC=<ptype>] [ITMAX=<n>]
 !                                              psfun_z_quadrature: HHT sqrt(z) on
 !                                              [p1, p2], or Talbot exp(-p1 z),
-!                                              applied to x = 1
+!                                              applied to x = 1; with PREC, a
+!                                              preconditioner built on A and
+!                                              given by quad%set(prec)
 ! Decks are read as the reference's get_parms reads them (list-directed, the
 ! line count first).  Machine-readable results follow lines starting '@'.
 ! Functions handed to the quadrature rules as procedure pointers (module
@@ -312,6 +314,9 @@ contains
     procedure(zquadrule), pointer :: quadformula
     real(psb_dpk_), allocatable :: xg(:), y(:), resid(:)
     integer(psb_ipk_), allocatable :: iters(:)
+    type(psb_dprec_type), target :: prec
+    character(len=20) :: ptype
+    integer(psb_ipk_) :: itmax, k
 
     call get_command_argument(2, mname)
     call get_command_argument(3, rule)
@@ -320,10 +325,18 @@ contains
     call get_command_argument(5, arg)
     read(arg, *) p1
     p2 = p1
-    if (command_argument_count() >= 6) then
-      call get_command_argument(6, arg)
-      read(arg, *) p2
-    end if
+    ptype = ''
+    itmax = 2000
+    do k = 6, command_argument_count()
+      call get_command_argument(k, arg)
+      if (arg(1:5) == 'PREC=') then
+        ptype = arg(6:)
+      else if (arg(1:6) == 'ITMAX=') then
+        read(arg(7:), *) itmax
+      else
+        read(arg, *) p2
+      end if
+    end do
     call mm_mat_read(aux_a, info, iunit=12, filename=mname)
     call psb_matdist(aux_a, a, ctxt, desc_a, info, fmt='CSR', parts=part_block)
     allocate(xg(desc_a%get_local_rows()))
@@ -348,9 +361,19 @@ contains
     end do
     write(psb_out_unit, '(a,2(1x,es24.16))') '@ETA_SIGN', quad%eta, quad%sign
     call quad%set(a)
-    call quad%apply(x_col, y_col, info, eps=1.0e-12_psb_dpk_, itmax=2000, iters=iters, resid=resid)
+    if (len_trim(ptype) > 0) then
+      call prec%init(ctxt, ptype, info)
+      call prec%build(a, desc_a, info)
+      call quad%set(prec)
+    end if
+    call quad%apply(x_col, y_col, info, eps=1.0e-12_psb_dpk_, itmax=itmax, iters=iters, resid=resid)
     write(psb_out_unit, '(a,1x,i0)') '@INFO', info
     write(psb_out_unit, '(a,1x,i0)') '@NSOLVE', size(iters)
+    if (info == 0) then
+      write(psb_out_unit, '(a,1x,i0)') '@MAXITER', maxval(iters)
+      write(psb_out_unit, '(a,1x,es24.16)') '@MAXRESID', maxval(resid)
+    end if
+    if (len_trim(ptype) > 0) call prec%free(info)
     y = y_col%get_vect()
     call print_vec('@Y', y)
     call psb_gefree(x_col, desc_a, info)

@@ -187,3 +187,27 @@ def test_typed_quadrature(gpu_ctx, tmp_path):
     y = _vec(out, "Y")
     ex = sla.expm(-Ad) @ np.ones(n)
     assert np.linalg.norm(y - ex) <= 1e-10 * np.linalg.norm(ex)
+
+
+def test_typed_quadrature_with_preconditioner(gpu_ctx, tmp_path):
+    """quad%set(prec) with a BJAC object built on A: sqrt(A)1 by 20 HHT poles
+    on an anisotropic operator (kappa ~ 1.7e3) with itmax 60, which the
+    multi-shift solve alone cannot meet; the poles it leaves continue by
+    preconditioned COCG.  Against scipy's sqrtm and the oracle."""
+    A = matgen.anisotropic_2d(20, seed=6)
+    n = 400
+    _write_mtx(tmp_path / "aniso.mtx", A)
+    Ad = matgen.to_scipy(A).toarray()
+    ev = np.linalg.eigvalsh(Ad)
+    ex = np.real(sla.sqrtm(Ad)) @ np.ones(n)
+    args = ("quad", "aniso.mtx", "HHT", 20, repr(float(ev[0])), repr(float(ev[-1])), "ITMAX=60")
+    out, _ = _run(tmp_path, *args, "PREC=BJAC")
+    assert int(out["INFO"][0][0]) == 0
+    assert int(out["MAXITER"][0][0]) > 60 and float(out["MAXRESID"][0][0]) <= 1e-12
+    y = _vec(out, "Y")
+    assert np.linalg.norm(y - ex) <= 1e-8 * np.linalg.norm(ex)
+    xi, c, eta, sign = O.hhtmethod1(np.sqrt, 20, [ev[0], ev[-1]])
+    ref = O.quad_apply(A, sign * xi, c, eta * sign, 1, np.ones(n), eps=1e-12, itmax=60, prec=2)
+    assert np.linalg.norm(y - ref["y"]) <= 1e-9 * np.linalg.norm(ref["y"])
+    out, _ = _run(tmp_path, *args)        # no preconditioner: the budget is not met
+    assert int(out["INFO"][0][0]) == 0 and float(out["MAXRESID"][0][0]) > 1e-12

@@ -250,3 +250,85 @@ def test_bjac_cg_parity(gpu_ctx, istop, fmt):
     info2, it2, err2, _ = psb_krylov("cg", M, q, _vec(b), eta, zeta, x2, 1e-10, psb.psb_desc_type(n),
                                      itmax=2000, istop=istop)
     assert info2 == 0 and it2 == it and np.array_equal(x2.get_vect(), x.get_vect())
+
+
+# ---- preconditioned per-shift fallback of the multi-shift solve ------------
+def _stalled_case():
+    """sqrt(A) x by HHT method 1 (20 poles) on an anisotropic operator with
+    kappa ~ 1.7e3 at eps 1e-12: a short multi-shift budget leaves the poles
+    nearest the spectrum unconverged."""
+    A = matgen.anisotropic_2d(20, seed=6)
+    Ad = matgen.to_scipy(A).toarray()
+    ev = np.linalg.eigvalsh(Ad)
+    xi, c, eta, sign = O.hhtmethod1(np.sqrt, 20, [ev[0], ev[-1]])
+    return A, Ad, sign * xi, c, eta * sign
+
+
+def test_mscg_zero_rhs(gpu_ctx):
+    A = matgen.laplacian((10, 10))
+    M = psb.psb_dspmat_type.from_csr(A)
+    X, iters, resid = mscg(M, _vec(np.zeros(100)), [0.5 + 1j, 2.0], eps=1e-10, itmax=50)
+    assert torch.all(X == 0) and np.all(iters == 0) and np.all(resid == 0)
+
+
+@pytest.mark.parametrize("fmt", ["CSR", "SELL"])
+@pytest.mark.parametrize("kind,itmax", [("NONE", 150), ("DIAG", 150), ("BJAC", 60)])
+def test_mscg_prec_parity(gpu_ctx, kind, itmax, fmt):
+    """psfun_mscg_prec against the oracle's or_mscg_prec: the same shifts
+    continue, their iteration counts agree within a few, every solution
+    matches the dense solve (or, for NONE, the oracle's unfinished iterate)."""
+    A, Ad, shifts, _, _ = _stalled_case()
+    n = len(Ad)
+    b = matgen.uniform_pm1(11, n)
+    M = psb.psb_dspmat_type.from_csr(A)
+    M.set_format(fmt)
+    p = _prec(kind)
+    assert p.build(M) == 0
+    X, iters, resid = mscg(M, _vec(b), shifts, eps=1e-12, itmax=itmax, prec=p)
+    code = {"NONE": 0, "DIAG": 1, "BJAC": 2}[kind]
+    ref = O.mscg_prec(A, b, shifts, code, eps=1e-12, itmax=itmax)
+    X = X.cpu().numpy()
+    # COCG in finite precision: the stopping index moves with the summation
+    # order of the SpMV and the dots (as for the real CG) by a few percent
+    assert np.all(np.abs(iters - ref["iters"]) <= np.maximum(3, 0.03 * ref["iters"])), (iters, ref["iters"])
+    assert np.all((iters > itmax) == (ref["iters"] > itmax))
+    for s in range(len(shifts)):
+        if ref["resid"][s] <= 1e-12:
+            ex = np.linalg.solve(Ad + shifts[s] * np.eye(n), b)
+            assert resid[s] <= 1e-12
+            assert np.linalg.norm(X[s] - ex) <= 1e-9 * np.linalg.norm(ex), s
+        else:   # NONE leaves some shifts unfinished: compare the iterates
+            assert np.linalg.norm(X[s] - ref["X"][s]) <= 1e-6 * np.linalg.norm(ref["X"][s]), s
+    if kind != "NONE":
+        assert np.all(resid <= 1e-12)
+
+
+def test_quad_apply_prec_sqrt(gpu_ctx):
+    """The quadrature object with set(prec): sqrt(A) x by HHT method 1 with a
+    budget the multi-shift solve alone cannot meet, finished by BJAC- and
+    DIAG-preconditioned COCG, against scipy's sqrtm and the oracle."""
+    import scipy.linalg as sla
+    A, Ad, shifts, c, gamma = _stalled_case()
+    n = len(Ad)
+    x = matgen.uniform_pm1(11, n)
+    ysq = np.real(sla.sqrtm(Ad)) @ x
+    ev = np.linalg.eigvalsh(Ad)
+    M = psb.psb_dspmat_type.from_csr(A)
+    for kind, itmax in (("BJAC", 60), ("DIAG", 150)):
+        q = Q.psfun_z_quadrature()
+        assert q.computepoles(Q.hhtmethod1, np.sqrt, 20, rparams=[ev[0], ev[-1]]) == 0
+        q.set(M)
+        q.set(_prec(kind))          # built on the quadrature's matrix by apply
+        y = _vec(np.zeros(n))
+        info, iters, resid = q.apply(_vec(x), y, eps=1e-12, itmax=itmax)
+        assert info == 0 and np.all(resid <= 1e-12) and iters.max() > itmax
+        ref = O.quad_apply(A, shifts, c, gamma, 1, x, eps=1e-12, itmax=itmax, prec={"DIAG": 1, "BJAC": 2}[kind])
+        assert np.linalg.norm(y.get_vect() - ref["y"]) <= 1e-9 * np.linalg.norm(ref["y"])
+        assert np.linalg.norm(y.get_vect() - ysq) <= 1e-8 * np.linalg.norm(ysq)
+    # the same budget without a preconditioner misses the rule's accuracy
+    q = Q.psfun_z_quadrature()
+    q.computepoles(Q.hhtmethod1, np.sqrt, 20, rparams=[ev[0], ev[-1]])
+    q.set(M)
+    y = _vec(np.zeros(n))
+    info, iters, resid = q.apply(_vec(x), y, eps=1e-12, itmax=60)
+    assert info == 0 and np.linalg.norm(y.get_vect() - ysq) > 1e-5 * np.linalg.norm(ysq)

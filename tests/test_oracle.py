@@ -203,7 +203,7 @@ def test_lanczos_batch_is_or_lanczos_per_instance():
     X = matgen.normal(8, sum(sizes) * K).reshape(sum(sizes), K)
     X[off[3]:off[4], 1] = np.sin(np.arange(1, 31) * 2 * np.pi / 31)
     X[off[2]:off[3], 2] = 0.0
-    saved = O.max_threads()
+    saved = O.threads()
     try:
         O.set_threads(4)
         Y, it, er = O.lanczos_batch(off, (rp, ci, va), X, 1e-10, 40, 1, opscale=-0.5)
@@ -530,3 +530,115 @@ def test_arnoldi_and_sai_userf_exp_equal_exp():
     assert abs(a["iter"] - b["iter"]) <= 1
     assert np.linalg.norm(a["y"] - b["y"]) <= 1e-9 * np.linalg.norm(a["y"])
     assert np.linalg.norm(b["y"] - matgen.dst_expm_laplacian((10, 10), x, 10.0)) <= 1e-8 * np.linalg.norm(b["y"])
+
+
+# ---- preconditioned COCG: the per-shift fallback of the multi-shift solve ---
+def _aniso(m, seed):
+    A = matgen.anisotropic_2d(m, seed=seed)
+    return A, matgen.to_scipy(A).toarray()
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("sigma", [0.3 + 0.2j, -0.5 + 0.01j, 4.0 - 3.0j])
+def test_pcocg_dense_solve(prec, sigma):
+    """Each preconditioner solves (A + sigma I) x = b, also for an
+    indefinite shift (-0.5 lies inside the spectrum [0.35, 221]); resid is
+    the true relative residual."""
+    A, Ad = _aniso(12, 4)
+    n = 144
+    b = matgen.uniform_pm1(2, n)
+    K = Ad + sigma * np.eye(n)
+    r = O.pcocg(A, b, sigma, prec=prec, eps=1e-12, itmax=3000)
+    ex = np.linalg.solve(K, b)
+    assert r["info"] == 0 and r["resid"] <= 1e-12 and 0 < r["iter"] < 3000
+    assert np.linalg.norm(r["x"] - ex) <= 1e-9 * np.linalg.norm(ex)
+    true = np.linalg.norm(b - K @ r["x"]) / np.linalg.norm(b)
+    assert abs(true - r["resid"]) <= 1e-13
+
+
+def test_pcocg_without_preconditioner_is_the_multishift_iterate():
+    """With M = I and x0 = 0, COCG on one shift and the multi-shift CG build
+    the same Krylov space under the same Galerkin condition, so after k
+    steps their iterates agree: a wrong sign or index in either recurrence
+    breaks it."""
+    A = matgen.laplacian((9, 8))
+    b = matgen.uniform_pm1(6, 72)
+    shifts = [0.5 + 1.0j, 2.0 - 0.3j, 0.05 + 0.0j]
+    for k in (1, 4, 9):
+        ms = O.mscg(A, b, shifts, eps=1e-30, itmax=k)
+        for s, xs in zip(shifts, ms["X"]):
+            r = O.pcocg(A, b, s, prec=0, eps=1e-30, itmax=k)
+            assert r["iter"] == k
+            assert np.linalg.norm(r["x"] - xs) <= 1e-11 * np.linalg.norm(xs), (k, s)
+
+
+def test_pcocg_warm_start_and_edge_cases():
+    A, Ad = _aniso(8, 1)
+    n = 64
+    b = matgen.uniform_pm1(3, n)
+    s = 0.7 + 0.4j
+    ex = np.linalg.solve(Ad + s * np.eye(n), b)
+    r = O.pcocg(A, b, s, x0=ex, prec=2, eps=1e-10)
+    assert r["iter"] == 0 and r["resid"] <= 1e-10 and np.array_equal(r["x"], ex)
+    half = 0.5 * ex
+    r = O.pcocg(A, b, s, x0=half, prec=1, eps=1e-12)
+    assert r["iter"] > 0 and np.linalg.norm(r["x"] - ex) <= 1e-9 * np.linalg.norm(ex)
+    r = O.pcocg(A, np.zeros(n), s, x0=half, prec=1)
+    assert r["iter"] == 0 and r["resid"] == 0.0 and np.all(r["x"] == 0)
+    # a zero entry of diag(A) + sigma is replaced by 1 (sigma = -diag cancels it)
+    D = (np.arange(3, dtype=np.int32), np.arange(2, dtype=np.int32), np.array([2.0, 5.0]))
+    r = O.pcocg(D, np.array([1.0, 1.0]), -2.0, prec=1, eps=1e-14, itmax=10)
+    assert r["info"] == 0 and np.all(np.isfinite(r["x"]))
+
+
+def test_bjac_is_symmetric_for_a_symmetric_matrix():
+    """COCG needs a (complex) symmetric M: for a symmetric A the ILU(0)
+    factors satisfy U = D L^T, so M = L U = L D L^T."""
+    A, Ad = _aniso(9, 2)
+    rc, rp, ci, lu = O.ilu0(A)
+    L, U = _ilu_factors(rp, ci, lu)
+    M = L @ U
+    assert rc == 0 and np.max(np.abs(M - M.T)) <= 1e-13 * np.max(np.abs(M))
+
+
+def _stalled_case():
+    """sqrt(A) x by HHT method 1 (20 poles) on an anisotropic operator with
+    kappa ~ 1.7e3 at eps 1e-12: 60 multi-shift iterations leave the poles
+    nearest the spectrum unconverged."""
+    A, Ad = _aniso(20, 6)
+    ev = np.linalg.eigvalsh(Ad)
+    xi, c, eta, sign = O.hhtmethod1(np.sqrt, 20, [ev[0], ev[-1]])
+    return A, Ad, sign * xi, c, eta * sign
+
+
+@pytest.mark.parametrize("prec,itmax", [(1, 150), (2, 60)])
+def test_mscg_prec_finishes_the_stalled_shifts(prec, itmax):
+    A, Ad, shifts, _, _ = _stalled_case()
+    n = len(Ad)
+    b = matgen.uniform_pm1(11, n)
+    plain = O.mscg(A, b, shifts, eps=1e-12, itmax=itmax)
+    stalled = plain["resid"] > 1e-12
+    assert stalled.sum() >= 5 and (~stalled).sum() >= 5
+    r = O.mscg_prec(A, b, shifts, prec, eps=1e-12, itmax=itmax)
+    assert r["info"] == 0 and np.all(r["resid"] <= 1e-12)
+    # without a preconditioner the same budget leaves shifts unfinished
+    assert np.any(O.mscg_prec(A, b, shifts, 0, eps=1e-12, itmax=itmax)["resid"] > 1e-12)
+    # the shifts the multi-shift solve finished are its own result, unchanged
+    assert np.array_equal(r["X"][~stalled], plain["X"][~stalled])
+    assert np.array_equal(r["iters"][~stalled], plain["iters"][~stalled])
+    assert np.all(r["iters"][stalled] > itmax)
+    for s, xs in zip(shifts, r["X"]):
+        ex = np.linalg.solve(Ad + s * np.eye(n), b)
+        assert np.linalg.norm(xs - ex) <= 1e-9 * np.linalg.norm(ex)
+
+
+def test_quad_apply_prec_sqrt_against_sqrtm():
+    A, Ad, shifts, c, gamma = _stalled_case()
+    x = matgen.uniform_pm1(11, len(Ad))
+    ysq = np.real(sla.sqrtm(Ad)) @ x
+    plain = O.quad_apply(A, shifts, c, gamma, 1, x, eps=1e-12, itmax=60)
+    assert np.linalg.norm(plain["y"] - ysq) > 1e-5 * np.linalg.norm(ysq)
+    for prec, itmax in ((1, 150), (2, 60)):
+        r = O.quad_apply(A, shifts, c, gamma, 1, x, eps=1e-12, itmax=itmax, prec=prec)
+        # the 20-pole rule's own error on this spectrum is 5.4e-9
+        assert r["info"] == 0 and np.linalg.norm(r["y"] - ysq) <= 1e-8 * np.linalg.norm(ysq)
