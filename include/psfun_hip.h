@@ -101,6 +101,15 @@ int psfun_ctx_use_own_stream(psfun_ctx_t ctx);
  * from the calling thread inside the apply, once per eigenvalue of each f(H)
  * step; USERF applies without a function return PSFUN_ERR_INVALID_ARG. */
 int psfun_set_userf(psfun_ctx_t ctx, psfun_userf_t f, void *user);
+/* The degree of the diagonal Pade approximant of the GENPADE variant
+ * (fun%padedegree, set by psfun_d_serial%set('PADE_DEGREE', v) and passed to
+ * DGPADM at serial/psfun_d_serial_mod.F90:271; default 6), for the serial
+ * apply and the Arnoldi and shift-and-invert f(H) steps of this context.
+ * Degree 6 evaluates exp(H) to the unit roundoff by the Taylor scheme (as
+ * DGPADM(6) does, to roundoff); any other degree evaluates DGPADM's rational
+ * approximant with its scaling and squaring, whose truncation error the
+ * result then carries.  2 <= degree <= 30. */
+int psfun_set_pade_degree(psfun_ctx_t ctx, int degree);
 int psfun_ctx_synchronize(psfun_ctx_t ctx);
 /* Number of compute units of the context's device (for launch-shape tuning). */
 int psfun_ctx_num_cus(psfun_ctx_t ctx, int *num_cus);

@@ -51,6 +51,12 @@ module psfun_hip_mod
       type(c_ptr), value :: user
       integer(c_int) :: st
     end function
+    function psfun_set_pade_degree(ctx, degree) bind(C, name='psfun_set_pade_degree') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: degree
+      integer(c_int) :: st
+    end function
 
     ! ---- psb_dspmat_type (CSR) ---------------------------------------------
     function psfun_csr_create(ctx, n, nnz, rowptr, colind, val, index_base, mem, a) &

@@ -93,6 +93,9 @@ def lib():
         L.or_quad_apply.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp,
                                     C.c_double, C.c_int, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.or_quad_apply.restype = C.c_int
+        L.or_pade_expm.argtypes = [C.c_int, C.c_int, _dp, C.c_double, _dp]
+        L.or_pade_expm.restype = C.c_int
+        L.or_set_pade_degree.argtypes = [C.c_int]
         _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         L.or_quad_apply_prec.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
                                          _dp, _dp, C.c_double, C.c_int, _i32, _dp]
@@ -320,6 +323,21 @@ def mscg(A, b, shifts, eps=1e-8, itmax=1000):
     lib().or_mscg(n, rp, ci, va, ns, sg, np.ascontiguousarray(b, np.float64), X, eps, itmax, iters, resid)
     Xc = X.reshape(ns, n, 2)
     return dict(X=Xc[..., 0] + 1j * Xc[..., 1], iters=iters, resid=resid)
+
+
+def pade_expm(H, t, degree):
+    """exp(t H) by the degree-p diagonal Pade approximant with scaling and
+    squaring (EXPOKIT DGPADM's method)."""
+    H = np.asarray(H, np.float64)
+    k = H.shape[0]
+    E = np.zeros(k * k)
+    rc = lib().or_pade_expm(int(degree), k, np.ascontiguousarray(H.T).reshape(-1), float(t), E)
+    return rc, E.reshape(k, k).T.copy()
+
+
+def set_pade_degree(p):
+    """The GENPADE degree the restatements' dense exp steps use (6: exact)."""
+    lib().or_set_pade_degree(int(p))
 
 
 def pcocg(A, b, sigma, x0=None, prec=0, eps=1e-8, itmax=1000):

@@ -14,6 +14,8 @@
  * Pinning (what checks each routine; see tests/):
  *   or_tridiag_eig / or_exp_tridiag   -- scipy.linalg.expm fixtures, closed-form 2x2
  *   or_dense_expm                      -- scipy.linalg.expm fixtures, diagonal/nilpotent cases
+ *   or_pade_expm                       -- scalar closed form r_p(x/2^ns)^(2^ns) from the factorial
+ *                                         Pade coefficients, diagonal matrices, scipy expm at p = 6..13
  *   or_sym_eig / or_userf_sym_apply    -- numpy.linalg.eigh (LAPACK) eigenvalues and reconstruction,
  *                                         scipy expm / sqrtm of the upper-triangle matrix
  *   or_lanczos_f / or_arnoldi_f /
@@ -403,10 +405,80 @@ int or_dense_expm(int k, const double *H, double s, double *E) {
   return rc;
 }
 
+/* E = exp(t*H), H k x k column-major, by the diagonal Pade approximant
+ * r_p = N/D of degree p = ideg >= 2 with scaling and squaring, the method of
+ * EXPOKIT's DGPADM (Sidje, ACM TOMS 24 (1998) 130-156; the reference calls it
+ * with fun%padedegree, serial/psfun_d_serial_mod.F90:271):
+ *   ns = max(0, int(log2(|t| ||H||_inf)) + 2), X = (t / 2^ns) H,
+ *   c_0 = 1, c_j = c_{j-1} (p+1-j) / (j (2p+1-j)),  N(X) = sum_j c_j X^j,
+ *   D(X) = N(-X);  E = (D^{-1} N)^(2^ns).
+ * Evaluated here as N = Ev + X Od, D = Ev - X Od with Ev, Od polynomials in
+ * X^2 (Horner), one solve D E = N, then ns squarings.  Returns 1 on a
+ * singular D, 2 for ideg < 2; H = 0 gives I (where DGPADM stops). */
+int or_pade_expm(int ideg, int k, const double *H, double t, double *E) {
+  if (ideg < 2) return 2;
+  const size_t kk = (size_t)k * k;
+  double hn = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double rs = 0.0;
+    for (int j = 0; j < k; ++j) rs += fabs(H[i + (size_t)j * k]);
+    hn = fmax(hn, rs);
+  }
+  hn = fabs(t * hn);
+  if (hn == 0.0) {
+    for (size_t i = 0; i < kk; ++i) E[i] = (i % (size_t)(k + 1) == 0) ? 1.0 : 0.0;
+    return 0;
+  }
+  int ns = (int)(log(hn) / log(2.0)) + 2;
+  if (ns < 0) ns = 0;
+  double *c = (double *)malloc(sizeof(double) * (ideg + 1));
+  c[0] = 1.0;
+  for (int j = 1; j <= ideg; ++j) c[j] = c[j - 1] * (double)(ideg + 1 - j) / (double)(j * (2 * ideg + 1 - j));
+  double *X = (double *)malloc(sizeof(double) * kk), *X2 = (double *)malloc(sizeof(double) * kk),
+         *Ev = (double *)calloc(kk, sizeof(double)), *Od = (double *)calloc(kk, sizeof(double)),
+         *T = (double *)malloc(sizeof(double) * kk), *D = (double *)malloc(sizeof(double) * kk);
+  const double sc = ldexp(t, -ns);
+  for (size_t i = 0; i < kk; ++i) X[i] = sc * H[i];
+  mm(k, X, X, X2);
+  /* Ev = sum_{j even} c_j X^j and Od = sum_{j odd} c_j X^(j-1), Horner in X^2 */
+  const int je = (ideg % 2 == 0) ? ideg : ideg - 1, jo = (ideg % 2 == 1) ? ideg : ideg - 1;
+  for (int i = 0; i < k; ++i) {
+    Ev[i + (size_t)i * k] = c[je];
+    Od[i + (size_t)i * k] = c[jo];
+  }
+  for (int j = je - 2; j >= 0; j -= 2) {
+    mm(k, Ev, X2, T);
+    for (int i = 0; i < k; ++i) T[i + (size_t)i * k] += c[j];
+    memcpy(Ev, T, sizeof(double) * kk);
+  }
+  for (int j = jo - 2; j >= 1; j -= 2) {
+    mm(k, Od, X2, T);
+    for (int i = 0; i < k; ++i) T[i + (size_t)i * k] += c[j];
+    memcpy(Od, T, sizeof(double) * kk);
+  }
+  mm(k, X, Od, T); /* the odd part X Od */
+  for (size_t i = 0; i < kk; ++i) {
+    D[i] = Ev[i] - T[i];
+    E[i] = Ev[i] + T[i];
+  }
+  int rc = gesv(k, k, D, E) ? 1 : 0;
+  for (int q = 0; q < ns && rc == 0; ++q) {
+    mm(k, E, E, T);
+    memcpy(E, T, sizeof(double) * kk);
+  }
+  free(c); free(X); free(X2); free(Ev); free(Od); free(T); free(D);
+  return rc;
+}
+
+/* The dense exp step of the restatements below follows the GENPADE degree
+ * when one other than 6 is set (or_set_pade_degree), else or_dense_expm. */
+static int g_pade = 0;
+void or_set_pade_degree(int p) { g_pade = p; }
+
 /* y = exp(s*H) x for a dense k x k column-major H. */
 int or_dense_expm_apply(int k, const double *H, double s, const double *x, double *y) {
   double *E = (double *)malloc(sizeof(double) * (size_t)k * k);
-  int rc = or_dense_expm(k, H, s, E);
+  int rc = (g_pade >= 2 && g_pade != 6) ? or_pade_expm(g_pade, k, H, s, E) : or_dense_expm(k, H, s, E);
   if (rc == 0)
     for (int i = 0; i < k; ++i) {
       double a = 0.0;

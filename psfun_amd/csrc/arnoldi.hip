@@ -49,7 +49,8 @@ __device__ void ar_control(LzCtl *ctl, double *H, int ldh, int j, double sigma, 
 // (order <= 64); otherwise one block.
 template <bool WAVE>
 __global__ __launch_bounds__(WAVE ? 64 : PSFUN_BS) void k_ar_est(LzCtl *ctl, const double *H, int ldh, int model,
-                                                                 int j, int nl, double eps, double fscale, int *hf) {
+                                                                 int j, int nl, double eps, double fscale, int *hf,
+                                                                 int pade, double *S) {
   if (ctl->done) return;
   const int tid = threadIdx.x;
   const int k = j - 1;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(WAVE ? 64 : PSFUN_BS) void k_ar_est(LzCtl *ctl, con
     __shared__ DenseLds W;
     for (int i = tid; i < k; i += PSFUN_BS) W.y[i] = (i == 0) ? 1.0 : 0.0;
     __syncthreads();
-    block_expmv(k, H, ldh, model, fscale, &W);
+    block_expmv_var(k, H, ldh, model, fscale, pade, S, &W);
     double ss = 0.0;
     for (int i = tid; i < k; i += PSFUN_BS) {
       const double v = W.y[i];
@@ -94,13 +95,16 @@ __global__ __launch_bounds__(WAVE ? 64 : PSFUN_BS) void k_ar_est(LzCtl *ctl, con
   }
 }
 
+// pade > 0: the GENPADE step by the Pade approximant of that degree (always
+// the block kernel), with its 5 nl^2 work doubles at S.
 static int ar_launch_est(psfun_ctx *c, hipStream_t st, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl,
-                         double eps, double fscale) {
-  if (j - 1 <= PSFUN_WDENSE_K)
-    hipLaunchKernelGGL(k_ar_est<true>, dim3(1), dim3(64), 0, st, ctl, H, ldh, model, j, nl, eps, fscale, c->d_flags);
+                         double eps, double fscale, int pade, double *S) {
+  if (j - 1 <= PSFUN_WDENSE_K && pade == 0)
+    hipLaunchKernelGGL(k_ar_est<true>, dim3(1), dim3(64), 0, st, ctl, H, ldh, model, j, nl, eps, fscale, c->d_flags,
+                       pade, S);
   else
     hipLaunchKernelGGL(k_ar_est<false>, dim3(1), dim3(PSFUN_BS), 0, st, ctl, H, ldh, model, j, nl, eps, fscale,
-                       c->d_flags);
+                       c->d_flags, pade, S);
   PSFUN_HIP_TRY(hipGetLastError());
   return PSFUN_SUCCESS;
 }
@@ -113,10 +117,10 @@ static int ar_launch_est(psfun_ctx *c, hipStream_t st, LzCtl *ctl, const double 
 // control block may have advanced itx past the stop index; the host then
 // takes itx = m (the reference exits with itx = m on convergence).
 static int ar_fork_est(psfun_ctx *c, LzCtl *ctl, const double *H, int ldh, int model, int j, int nl, double eps,
-                       double fscale) {
+                       double fscale, int pade, double *S) {
   PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
   PSFUN_HIP_TRY(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
-  PSFUN_TRY(ar_launch_est(c, c->aux_stream, ctl, H, ldh, model, j, nl, eps, fscale));
+  PSFUN_TRY(ar_launch_est(c, c->aux_stream, ctl, H, ldh, model, j, nl, eps, fscale, pade, S));
   PSFUN_HIP_TRY(hipEventRecord(c->ev_join, c->aux_stream));
   return PSFUN_SUCCESS;
 }
@@ -472,7 +476,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_ctrl(const double *__restrict__
 // yk = f(H(1:itx-1,1:itx-1)) e1 (psfun_d_arnoldi.f90:259-296); the 1x1 block
 // stands in for itx-1 = 0.
 __global__ __launch_bounds__(PSFUN_BS) void k_ar_final(LzCtl *ctl, const double *H, int ldh, int model,
-                                                       double fscale) {
+                                                       double fscale, int pade, double *S) {
   __shared__ DenseLds W;
   const int done = ctl->done;
   if (done == LZ_DONE_CONVERGED || done == LZ_DONE_ZERO) return;
@@ -480,7 +484,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ar_final(LzCtl *ctl, const double 
   if (k < 1) k = 1;
   for (int i = threadIdx.x; i < k; i += PSFUN_BS) W.y[i] = (i == 0) ? 1.0 : 0.0;
   __syncthreads();
-  block_expmv(k, H, ldh, model, fscale, &W);
+  block_expmv_var(k, H, ldh, model, fscale, pade, S, &W);
   for (int i = threadIdx.x; i < k; i += PSFUN_BS) ctl->yk[i] = W.y[i];
   if (threadIdx.x == 0) ctl->m = k;
 }
@@ -574,8 +578,11 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
   const size_t ctl_bytes = align_up(sizeof(LzCtl), 256);
   const size_t h_bytes = align_up(sizeof(double) * (size_t)ldh * ldh, 256);
   const size_t vbytes = align_up(sizeof(double) * (size_t)ldv * (size_t)(nl + 2), 256);  // v_1..v_{nl-1}, u0, u1
+  const int pade = (!userf && variant == PSFUN_VAR_GENPADE && c->pade_degree != 6) ? c->pade_degree : 0;
+  const size_t pade_bytes = pade ? align_up(sizeof(double) * 5 * (size_t)nl * nl, 256) : 0;
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace(c, ctl_bytes + h_bytes + vbytes + ar_ortho_bytes(c, n, nl), &ws));
+  PSFUN_TRY(psfun_workspace(c, ctl_bytes + h_bytes + vbytes + ar_ortho_bytes(c, n, nl) + pade_bytes, &ws));
+  double *S = pade ? (double *)((char *)ws + ctl_bytes + h_bytes + vbytes + ar_ortho_bytes(c, n, nl)) : nullptr;
   LzCtl *ctl = (LzCtl *)ws;
   double *H = (double *)((char *)ws + ctl_bytes);
   double *V = (double *)((char *)ws + ctl_bytes + h_bytes);
@@ -638,7 +645,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
       est_pending = false;
     }
     if (istop_k == 1 && j >= 2) {
-      PSFUN_TRY(ar_fork_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale));
+      PSFUN_TRY(ar_fork_est(c, ctl, H, ldh, model, j, nl_k, eps, fscale, pade, S));
       est_pending = true;
     }
     if (fsync && j >= 2) {
@@ -671,7 +678,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
     hipLaunchKernelGGL(k_ar_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[cur], gcur, nl, ctl, H, ldh, model,
                        nl_k, istop_k, eps, fscale, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
-    if (istop_k == 1) PSFUN_TRY(ar_launch_est(c, c->stream, ctl, H, ldh, model, nl, nl_k, eps, fscale));
+    if (istop_k == 1) PSFUN_TRY(ar_launch_est(c, c->stream, ctl, H, ldh, model, nl, nl_k, eps, fscale, pade, S));
     if (fsync) {
       bool stopped = false;
       PSFUN_TRY(userf_eval(c, nl - 1, SYM_FROM_UPPER, H, nullptr, ldh, fscale, sc, &stopped));
@@ -689,7 +696,7 @@ extern "C" int psfun_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, int 
       PSFUN_TRY(userf_launch_final(c, ctl, k, sc));
     }
   } else {
-    hipLaunchKernelGGL(k_ar_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, fscale);
+    hipLaunchKernelGGL(k_ar_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, fscale, pade, S);
     PSFUN_HIP_TRY(hipGetLastError());
   }
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);
@@ -742,19 +749,20 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sai_normalize(int n, const double 
 
 // yk = f(H_k^{-1} - tau I) e1 into W->y; false when H_k is singular.
 __device__ bool sai_fproj(int k, const double *H, int ldh, int model, double *M, double *Hi, double tau,
-                          double fscale, DenseLds *W) {
+                          double fscale, int pade, double *S, DenseLds *W) {
   if (!block_gj_inverse(k, H, ldh, M, Hi, W)) return false;
   for (int d = threadIdx.x; d < k; d += PSFUN_BS) Hi[d * k + d] -= tau;
   for (int i = threadIdx.x; i < k; i += PSFUN_BS) W->y[i] = (i == 0) ? 1.0 : 0.0;
   __syncthreads();
-  block_expmv(k, Hi, k, model, fscale, W);
+  block_expmv_var(k, Hi, k, model, fscale, pade, S, W);
   return true;
 }
 
 // Reference control at loop index i (h(i,i-1) = ||w|| from the partials).
 __global__ __launch_bounds__(PSFUN_BS) void k_sai_ctrl(const double *__restrict__ p_in, int g_in, int i, LzCtl *ctl,
                                                        double *H, int ldh, int model, double *M, double *Hi,
-                                                       double tau, int nl, int istop, double eps, double fscale) {
+                                                       double tau, int nl, int istop, double eps, double fscale,
+                                                       int pade, double *S) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   __shared__ DenseLds W;
   if (ctl->done) return;
@@ -766,7 +774,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sai_ctrl(const double *__restrict_
   }
   if (istop == 1) {
     const int k = i - 1;
-    if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, &W)) {
+    if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, pade, S, &W)) {
       if (tid == 0) ctl->done = LZ_DONE_BREAKDOWN;
       return;
     }
@@ -796,13 +804,13 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sai_ctrl(const double *__restrict_
 // Coefficients when the loop did not end by convergence: the same projection
 // f(H_{itx-1}^{-1} - tau I) e1 (the 1x1 block stands in for itx-1 = 0).
 __global__ __launch_bounds__(PSFUN_BS) void k_sai_final(LzCtl *ctl, const double *H, int ldh, int model, double *M,
-                                                        double *Hi, double tau, double fscale) {
+                                                        double *Hi, double tau, double fscale, int pade, double *S) {
   __shared__ DenseLds W;
   const int done = ctl->done;
   if (done == LZ_DONE_CONVERGED || done == LZ_DONE_ZERO || done == LZ_DONE_BREAKDOWN) return;
   int k = ctl->itx - 1;
   if (k < 1) k = 1;
-  if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, &W)) {
+  if (!sai_fproj(k, H, ldh, model, M, Hi, tau, fscale, pade, S, &W)) {
     if (threadIdx.x == 0) ctl->done = LZ_DONE_BREAKDOWN;
     return;
   }
@@ -868,9 +876,15 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
   const size_t m_bytes = align_up(sizeof(double) * (size_t)nl * nl, 256);
   const size_t vbytes = align_up(sizeof(double) * (size_t)ldv * (size_t)(nl + 1), 256);  // v_1..v_nl, w
   const size_t u_bytes = align_up(userf ? userf_scratch_bytes(nl) : 0, 256);
+  const int pade = (!userf && variant == PSFUN_VAR_GENPADE && c->pade_degree != 6) ? c->pade_degree : 0;
+  const size_t pade_bytes = pade ? 5 * m_bytes : 0;
   // the basis lives in the second workspace: the inner CG takes the first
   void *ws = nullptr;
-  PSFUN_TRY(psfun_workspace2(c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes + ar_ortho_bytes(c, n, nl), &ws));
+  PSFUN_TRY(psfun_workspace2(
+      c, ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes + ar_ortho_bytes(c, n, nl) + pade_bytes, &ws));
+  double *S = pade ? (double *)((char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes +
+                                ar_ortho_bytes(c, n, nl))
+                   : nullptr;
   UserfScratch sc;
   if (userf) sc = userf_scratch_carve((char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes, nl);
   const ArOrtho orth = ar_ortho_carve(c, (char *)ws + ctl_bytes + h_bytes + 2 * m_bytes + vbytes + u_bytes, n, nl);
@@ -911,7 +925,7 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
       hipLaunchKernelGGL(k_sai_normalize, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, w, V + (size_t)ldv * j,
                          P[0], orth.Gl, ctl, 0);
       hipLaunchKernelGGL(k_sai_ctrl, dim3(1), dim3(PSFUN_BS), 0, c->stream, P[0], orth.Gl, i, ctl, H, ldh, model, M,
-                         Hi, tau, nl_k, istop_k, eps, fscale);
+                         Hi, tau, nl_k, istop_k, eps, fscale, pade, S);
       PSFUN_HIP_TRY(hipGetLastError());
       if (fsync) {
         // yk = f(H_{i-1}^{-1} - tau I) e1 with f on the host, rs(i) from h(i, i-1)
@@ -943,7 +957,8 @@ extern "C" int psfun_sai_arnoldi_apply(psfun_ctx_t c, psfun_csr_t A, int fname, 
       }
     }
   } else {
-    hipLaunchKernelGGL(k_sai_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, M, Hi, tau, fscale);
+    hipLaunchKernelGGL(k_sai_final, dim3(1), dim3(PSFUN_BS), 0, c->stream, ctl, H, ldh, model, M, Hi, tau, fscale,
+                       pade, S);
     PSFUN_HIP_TRY(hipGetLastError());
   }
   const int Ga = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 8);

@@ -18,6 +18,7 @@
 #include "../../include/psfun_hip.h"
 
 #define PSFUN_BS 256           // threads per block for the streaming kernels
+#define PSFUN_PADE_MAX 30       // highest degree of the GENPADE approximant
 #define PSFUN_MAX_GRID 4096    // upper bound on reduction grids (partials slots)
 #define PSFUN_PINNED_BYTES (1 << 20)
 #define PSFUN_PROF_NK 16       // kernel ids for psfun_prof_*
@@ -65,6 +66,8 @@ struct psfun_ctx {
   size_t ws_bytes = 0;
   void *ws2 = nullptr;
   size_t ws2_bytes = 0;
+  // degree of the Pade approximant of the GENPADE variant (psfun_set_pade_degree)
+  int pade_degree = 6;
   // user function of the USERF f(H) step (psfun_set_userf), called on the host
   psfun_userf_t userf = nullptr;
   void *userf_data = nullptr;

@@ -109,6 +109,12 @@ extern "C" int psfun_set_userf(psfun_ctx_t c, psfun_userf_t f, void *user) {
   return PSFUN_SUCCESS;
 }
 
+extern "C" int psfun_set_pade_degree(psfun_ctx_t c, int degree) {
+  if (!c || degree < 2 || degree > PSFUN_PADE_MAX) return PSFUN_ERR_INVALID_ARG;
+  c->pade_degree = degree;
+  return PSFUN_SUCCESS;
+}
+
 extern "C" int psfun_ctx_use_own_stream(psfun_ctx_t c) {
   if (!c) return PSFUN_ERR_INVALID_ARG;
   c->stream = c->own_stream;

@@ -287,3 +287,144 @@ __device__ bool block_gj_inverse(int k, const double *H, int ldh, double *M, dou
   }
   return true;
 }
+
+// C = alpha * A * B for k x k column-major matrices (A, B with leading
+// dimensions lda, ldb; C with k), one block; C distinct from A and B.
+__device__ inline void block_gemm(int k, double alpha, const double *A, int lda, const double *B, int ldb,
+                                  double *C) {
+  for (int idx = threadIdx.x; idx < k * k; idx += PSFUN_BS) {
+    const int r = idx % k, c = idx / k;
+    double acc = 0.0;
+    for (int l = 0; l < k; ++l) acc = fma(A[(int64_t)l * lda + r], B[(int64_t)c * ldb + l], acc);
+    C[idx] = alpha * acc;
+  }
+  __syncthreads();
+}
+
+// exp(t*H) (k x k) by the irreducible diagonal Pade approximant of degree
+// 2 <= ideg <= PSFUN_PADE_MAX with scaling and squaring, the method of
+// EXPOKIT's DGPADM (Sidje, ACM TOMS 24 (1998) 130-156; called by
+// serial/psfun_d_serial_mod.F90:271 with fun%padedegree):
+//   ns = max(0, int(log2(|t| ||H||_inf)) + 2),  x = (t / 2^ns) H
+//   c_0 = 1, c_j = c_{j-1} (p + 1 - j) / (j (2p + 1 - j)):  N(x) = sum c_j x^j
+//   and D(x) = N(-x), split into even and odd parts Ev(x^2), x Od(x^2)
+//   (Horner in x^2 from c_p and c_{p-1}); r = N/D = 1 + 2 (Ev - x Od)^{-1} x Od,
+//   and the same identity with the roles swapped gives -r when the odd part
+//   lands in the other Horner polynomial; then r^(2^ns) by squaring (which
+//   also removes that sign), the sign flipped explicitly when ns = 0.
+// S holds 5 k*k doubles; returns the pointer to the result inside S, or
+// nullptr when the denominator is singular (DGPADM stops there).  H = 0
+// gives the identity (DGPADM stops there too).
+__device__ const double *block_pade_expm(int k, const double *H, int ldh, double t, int ideg, double *S,
+                                         DenseLds *W) {
+  const int tid = threadIdx.x;
+  const int kk = k * k;
+  double rmax = 0.0;
+  for (int r = tid; r < k; r += PSFUN_BS) {
+    double rs = 0.0;
+    for (int c = 0; c < k; ++c) rs += fabs(H[(int64_t)c * ldh + r]);
+    rmax = fmax(rmax, rs);
+  }
+  const double hnorm = fabs(t * dense_block_max(rmax, W));
+  double *buf[5] = {S, S + kk, S + 2 * (size_t)kk, S + 3 * (size_t)kk, S + 4 * (size_t)kk};
+  if (hnorm == 0.0) {
+    for (int idx = tid; idx < kk; idx += PSFUN_BS) buf[0][idx] = (idx % k == idx / k) ? 1.0 : 0.0;
+    __syncthreads();
+    return buf[0];
+  }
+  const int ns = max(0, (int)(log(hnorm) / log(2.0)) + 2);
+  const double scale = ldexp(t, -ns);
+  if (tid == 0) {  // Pade coefficients c_0..c_p in LDS
+    W->part[0] = 1.0;
+    for (int j = 1; j <= ideg; ++j)
+      W->part[j] = (W->part[j - 1] * (double)(ideg + 1 - j)) / (double)(j * (2 * ideg + 1 - j));
+  }
+  __syncthreads();
+  const double *cf = W->part;
+  double *H2 = buf[0], *P = buf[1], *Q = buf[2], *F = buf[3], *G = buf[4];
+  block_gemm(k, scale * scale, H, ldh, H, ldh, H2);
+  for (int idx = tid; idx < kk; idx += PSFUN_BS) {
+    const bool dg = idx % k == idx / k;
+    P[idx] = dg ? cf[ideg - 1] : 0.0;
+    Q[idx] = dg ? cf[ideg] : 0.0;
+  }
+  __syncthreads();
+  int odd = 1;  // the polynomial the next Horner step extends: Q when odd, else P
+  for (int j = ideg - 1; j > 0; --j) {
+    double *used = odd ? Q : P;
+    block_gemm(k, 1.0, used, k, H2, k, F);
+    for (int d = tid; d < k; d += PSFUN_BS) F[d * k + d] += cf[j - 1];
+    __syncthreads();
+    if (odd)
+      Q = F;
+    else
+      P = F;
+    F = used;
+    odd ^= 1;
+  }
+  // the polynomial whose turn would come next holds the odd part: multiply it by x
+  if (odd) {
+    block_gemm(k, scale, Q, k, H, ldh, F);
+    double *tq = Q;
+    Q = F;
+    F = tq;
+  } else {
+    block_gemm(k, scale, P, k, H, ldh, F);
+    double *tp = P;
+    P = F;
+    F = tp;
+  }
+  for (int idx = tid; idx < kk; idx += PSFUN_BS) Q[idx] -= P[idx];
+  __syncthreads();
+  // X = (Q - P)^{-1} P, then 2 X + I (F: elimination work, G: the inverse)
+  if (!block_gj_inverse(k, Q, k, F, G, W)) return nullptr;
+  block_gemm(k, 1.0, G, k, P, k, F);
+  for (int idx = tid; idx < kk; idx += PSFUN_BS) F[idx] = 2.0 * F[idx] + ((idx % k == idx / k) ? 1.0 : 0.0);
+  __syncthreads();
+  if (ns == 0) {
+    if (odd) {
+      for (int idx = tid; idx < kk; idx += PSFUN_BS) F[idx] = -F[idx];
+      __syncthreads();
+    }
+    return F;
+  }
+  double *cur = F, *nxt = G;
+  for (int q = 0; q < ns; ++q) {
+    block_gemm(k, 1.0, cur, k, cur, k, nxt);
+    double *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  return cur;
+}
+
+// W->y <- exp(t*H) W->y by block_pade_expm; false on a singular denominator.
+__device__ inline bool block_pade_expmv(int k, const double *H, int ldh, double t, int ideg, double *S,
+                                        DenseLds *W) {
+  const double *E = block_pade_expm(k, H, ldh, t, ideg, S, W);
+  if (!E) return false;
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) W->t0[i] = W->y[i];
+  __syncthreads();
+  for (int r = threadIdx.x; r < k; r += PSFUN_BS) {
+    double acc = 0.0;
+    for (int c = 0; c < k; ++c) acc = fma(E[(int64_t)c * k + r], W->t0[c], acc);
+    W->y[r] = acc;
+  }
+  __syncthreads();
+  return true;
+}
+
+// The dense exp step of the variants: the Taylor scheme, or with pade > 0 the
+// degree-`pade` Pade approximant (S: 5 k*k doubles).  A singular Pade
+// denominator leaves NaN in W->y.
+__device__ inline void block_expmv_var(int k, const double *H, int ldh, int model, double s, int pade, double *S,
+                                       DenseLds *W) {
+  if (pade > 0) {
+    if (!block_pade_expmv(k, H, ldh, s, pade, S, W)) {
+      for (int i = threadIdx.x; i < k; i += PSFUN_BS) W->y[i] = NAN;
+      __syncthreads();
+    }
+  } else {
+    block_expmv(k, H, ldh, model, s, W);
+  }
+}

@@ -27,6 +27,19 @@ __global__ __launch_bounds__(PSFUN_BS) void k_serial_dense(int k, const double *
   for (int i = threadIdx.x; i < k; i += PSFUN_BS) y[i] = W.y[i];
 }
 
+// y = exp(s*H) x by the degree-`ideg` Pade approximant (GENPADE with a
+// degree other than 6), one block; S: 5 k*k doubles; *bad = 1 on a singular
+// Pade denominator.
+__global__ __launch_bounds__(PSFUN_BS) void k_serial_pade(int k, const double *H, double s, int ideg, double *S,
+                                                         const double *x, double *y, int *bad) {
+  __shared__ DenseLds W;
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) W.y[i] = x[i];
+  __syncthreads();
+  const bool ok = block_pade_expmv(k, H, k, s, ideg, S, &W);
+  if (threadIdx.x == 0) *bad = ok ? 0 : 1;
+  for (int i = threadIdx.x; i < k; i += PSFUN_BS) y[i] = W.y[i];
+}
+
 // The same for k <= 64 by one wavefront (wave_expmv).
 __global__ __launch_bounds__(64) void k_serial_dense_wave(int k, const double *H, int model, double s,
                                                          const double *x, double *y) {
@@ -96,6 +109,22 @@ extern "C" int psfun_serial_apply_host(psfun_ctx_t c, int fname, int variant, do
     PSFUN_HIP_TRY(hipMemcpyAsync(dd, buf.data(), sizeof(double) * 3 * k, hipMemcpyHostToDevice, c->stream));
     PSFUN_TRY(launch_serial_tridiag(c, k, dd, dd + k, scaling, dd + 2 * k, dd + 3 * k));
     PSFUN_HIP_TRY(hipMemcpyAsync(y, dd + 3 * k, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+  } else if (variant == PSFUN_VAR_GENPADE && c->pade_degree != 6) {
+    const size_t hk = (size_t)k * k;
+    PSFUN_TRY(psfun_workspace(c, sizeof(double) * (6 * hk + 2 * (size_t)k) + 256, &ws));
+    double *dH = (double *)ws;
+    double *dx = dH + hk, *dy = dx + k, *S = dy + k;
+    int *dbad = (int *)(S + 5 * hk);
+    PSFUN_HIP_TRY(hipMemcpyAsync(dH, H, sizeof(double) * hk, hipMemcpyHostToDevice, c->stream));
+    PSFUN_HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_serial_pade, dim3(1), dim3(PSFUN_BS), 0, c->stream, k, (const double *)dH, scaling,
+                       c->pade_degree, S, (const double *)dx, dy, dbad);
+    PSFUN_HIP_TRY(hipGetLastError());
+    int *hb = (int *)c->h_pinned;
+    PSFUN_HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipMemcpyAsync(hb, dbad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
+    return *hb ? PSFUN_ERR_BREAKDOWN : PSFUN_SUCCESS;
   } else {
     const int model = variant == PSFUN_VAR_CHBSYM   ? DENSE_SYM_UPPER
                       : variant == PSFUN_VAR_CHBHES ? DENSE_HESSENBERG

@@ -240,7 +240,7 @@ contains
     call psfun_bind_userf(fun, a%ctxt%ctx, box)
     st = psfun_lanczos_apply(a%ctxt%ctx, a%a, fn, 1.0_c_double, 1.0_c_double, x%v, y%v, eps, nl, istop_, it, &
          & derr, rbuf, nres)
-    if (fn == PSFUN_FN_USERF) call psfun_unbind_userf(a%ctxt%ctx)
+    call psfun_unbind_userf(a%ctxt%ctx)
     call krylov_outputs('LANCZOS', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
   end subroutine psfun_d_lanczos
 
@@ -274,7 +274,7 @@ contains
     call psfun_bind_userf(fun, a%ctxt%ctx, box)
     st = psfun_arnoldi_apply(a%ctxt%ctx, a%a, fn, var, 1.0_c_double, fun%scaling, x%v, y%v, eps, nl, istop_, &
          & it, derr, rbuf, nres)
-    if (fn == PSFUN_FN_USERF) call psfun_unbind_userf(a%ctxt%ctx)
+    call psfun_unbind_userf(a%ctxt%ctx)
     call krylov_outputs('ARNOLDI', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
   end subroutine psfun_d_arnoldi
 
@@ -316,7 +316,7 @@ contains
     call psfun_bind_userf(fun, a%ctxt%ctx, box)
     st = psfun_sai_arnoldi_apply(a%ctxt%ctx, a%a, fn, var, 1.0_c_double, tau, prec%get_code(), initmax, inistop, &
          & x%v, y%v, eps, nl, istop_, it, derr, rbuf, nres)
-    if (fn == PSFUN_FN_USERF) call psfun_unbind_userf(a%ctxt%ctx)
+    call psfun_unbind_userf(a%ctxt%ctx)
     call krylov_outputs('SAIARNOLDI', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
   end subroutine psfun_d_saiarnoldi
 

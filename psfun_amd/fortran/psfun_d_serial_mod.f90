@@ -68,21 +68,26 @@ contains
   end function psfun_userf_trampoline
 
   ! Make fun%f_ptr the library context's USERF function (the box must stay
-  ! alive, at a fixed address, during the calls that may use it).
+  ! alive, at a fixed address, during the calls that may use it), and
+  ! fun%padedegree its GENPADE degree (DGPADM's ideg, serial_mod:271).
   subroutine psfun_bind_userf(fun, ctx, box)
     type(psfun_d_serial), intent(in) :: fun
     type(c_ptr), intent(in) :: ctx
     type(psfun_userf_box), intent(inout), target :: box
     integer(c_int) :: st
+    if (trim(fun%fname) == 'EXP' .and. trim(fun%variant) == 'GENPADE') &
+         & st = psfun_set_pade_degree(ctx, int(fun%padedegree, c_int))
     if (trim(fun%fname) /= 'USERF' .or. .not. associated(fun%f_ptr)) return
     box%f => fun%f_ptr
     st = psfun_set_userf(ctx, c_funloc(psfun_userf_trampoline), c_loc(box))
   end subroutine psfun_bind_userf
 
+  ! Undo psfun_bind_userf: no USERF function, GENPADE degree 6.
   subroutine psfun_unbind_userf(ctx)
     type(c_ptr), intent(in) :: ctx
     integer(c_int) :: st
     st = psfun_set_userf(ctx, c_null_funptr, c_null_ptr)
+    st = psfun_set_pade_degree(ctx, 6_c_int)
   end subroutine psfun_unbind_userf
 
   subroutine psfun_d_setstring(fun, what, val, info)
@@ -161,6 +166,7 @@ contains
         var = PSFUN_VAR_CHBGEN
       case ('GENPADE')
         var = PSFUN_VAR_GENPADE
+        if (fun%padedegree < 2 .or. fun%padedegree > 30) st = PSFUN_ERR_INVALID_ARG
       case ('TAYLOR')
         var = PSFUN_VAR_TAYLOR
       case ('SASQ')
@@ -206,7 +212,7 @@ contains
     allocate(yy(k))
     call psfun_bind_userf(fun, psb_default_ctxt%ctx, box)
     st = psfun_serial_apply_host(psb_default_ctxt%ctx, fn, var, fun%scaling, k, h, xx, yy)
-    if (trim(fun%fname) == 'USERF') call psfun_unbind_userf(psb_default_ctxt%ctx)
+    call psfun_unbind_userf(psb_default_ctxt%ctx)
     if (st /= PSFUN_SUCCESS) then
       info = psb_err_from_subroutine_
       call psb_errpush(info, 'psb_krylov', a_err=trim(psb_status_text(st)))

@@ -26,6 +26,8 @@ from . import psb
 
 # Variants of fname 'EXP' that the reference accepts (serial_mod:216-390).
 EXP_VARIANTS = ("TAYLOR", "SASQ", "GENPADE", "CHBHES", "CHBGEN", "CHBSYM")
+# Degrees of the GENPADE approximant the device evaluates (psfun_set_pade_degree).
+PADE_MAX = 30
 # Variants of fname 'USERF' (serial_mod:420-498).
 USERF_VARIANTS = ("SYM",)
 
@@ -74,6 +76,8 @@ class psfun_d_serial:
         if self.fname == "EXP":
             if self.variant not in EXP_VARIANTS:
                 return _lib.PSFUN_ERR_INVALID_ARG, None, None
+            if self.variant == "GENPADE" and not 2 <= self.padedegree <= PADE_MAX:
+                return _lib.PSFUN_ERR_INVALID_ARG, None, None
             return _lib.PSFUN_SUCCESS, _lib.PSFUN_FN_EXP, _lib.VARIANTS[self.variant]
         if self.fname == "USERF":
             if self.variant not in USERF_VARIANTS or self.f_ptr is None:
@@ -81,12 +85,20 @@ class psfun_d_serial:
             return _lib.PSFUN_SUCCESS, _lib.PSFUN_FN_USERF, _lib.PSFUN_VAR_SYM
         return _lib.PSFUN_ERR_UNSUPPORTED, None, None
 
+    @contextlib.contextmanager
     def bound(self, ctx):
         """Context manager that makes this object's f_ptr the context's USERF
-        function for the calls inside it (nothing to do for EXP)."""
-        if self.fname == "USERF" and self.f_ptr is not None:
-            return ctx.userf(self.f_ptr)
-        return contextlib.nullcontext()
+        function, and its Pade degree the context's GENPADE degree
+        (psfun_set_pade_degree; DGPADM takes fun%padedegree,
+        serial_mod:271), for the calls inside it."""
+        with contextlib.ExitStack() as stack:
+            if self.fname == "USERF" and self.f_ptr is not None:
+                stack.enter_context(ctx.userf(self.f_ptr))
+            if self.fname == "EXP" and self.variant == "GENPADE" and self.padedegree != 6:
+                _lib.check(_lib.lib().psfun_set_pade_degree(ctx.handle, int(self.padedegree)),
+                           "psfun_set_pade_degree")
+                stack.callback(_lib.lib().psfun_set_pade_degree, ctx.handle, 6)
+            yield
 
     def apply(self, a, y, x, ctxt=None):
         """y[:] = f(scaling*a) x for a k x k matrix `a`: a dense host array

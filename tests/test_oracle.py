@@ -642,3 +642,55 @@ def test_quad_apply_prec_sqrt_against_sqrtm():
         r = O.quad_apply(A, shifts, c, gamma, 1, x, eps=1e-12, itmax=itmax, prec=prec)
         # the 20-pole rule's own error on this spectrum is 5.4e-9
         assert r["info"] == 0 and np.linalg.norm(r["y"] - ysq) <= 1e-8 * np.linalg.norm(ysq)
+
+
+# ---- GENPADE with fun%padedegree: DGPADM's Pade approximant ----------------
+def _pade_scalar(x, p):
+    from math import factorial
+    c = [factorial(2 * p - j) * factorial(p) / (factorial(2 * p) * factorial(j) * factorial(p - j))
+         for j in range(p + 1)]
+    return sum(c[j] * x ** j for j in range(p + 1)) / sum(c[j] * (-x) ** j for j in range(p + 1))
+
+
+def _pade_ns(hnorm):
+    return max(0, int(np.log(hnorm) / np.log(2.0)) + 2)    # int() truncates toward zero, as Fortran INT
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 6, 9])
+@pytest.mark.parametrize("h", [0.3, -0.7, 5.0, -40.0, 1e-3])
+def test_pade_expm_scalar_closed_form(p, h):
+    """1x1: r_p(h / 2^ns)^(2^ns) with the Pade coefficients written from
+    factorials, independent of the even/odd Horner split."""
+    ref = _pade_scalar(h / 2.0 ** _pade_ns(abs(h)), p) ** (2 ** _pade_ns(abs(h)))
+    rc, E = O.pade_expm([[h]], 1.0, p)
+    assert rc == 0 and abs(E[0, 0] - ref) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("p", [2, 5])
+def test_pade_expm_diagonal_shares_one_scaling(p):
+    d = np.array([0.5, -3.0, 1.25, 7.0])
+    t = -0.5
+    ns = _pade_ns(abs(t) * 7.0)
+    rc, E = O.pade_expm(np.diag(d), t, p)
+    ref = [_pade_scalar(t * v / 2.0 ** ns, p) ** (2 ** ns) for v in d]
+    assert rc == 0 and np.allclose(np.diag(E), ref, rtol=1e-13, atol=0) and np.all(E - np.diag(np.diag(E)) == 0)
+
+
+@pytest.mark.parametrize("p", [6, 8, 13])
+def test_pade_expm_matches_scipy_at_full_degree(p):
+    rng = np.random.default_rng(p)
+    for k in (6, 20):
+        H = rng.standard_normal((k, k)) / np.sqrt(k) * 2.0
+        rc, E = O.pade_expm(H, 1.3, p)
+        ex = sla.expm(1.3 * H)
+        assert rc == 0 and np.linalg.norm(E - ex) <= 1e-13 * np.linalg.norm(ex) * k
+
+
+def test_pade_expm_low_degree_carries_its_truncation_and_zero_is_identity():
+    rng = np.random.default_rng(3)
+    H = rng.standard_normal((10, 10))
+    rc, E2 = O.pade_expm(H, 1.0, 2)
+    ex = sla.expm(H)
+    assert rc == 0 and np.linalg.norm(E2 - ex) > 1e-7 * np.linalg.norm(ex)   # 5.3e-7 here
+    assert O.pade_expm(np.zeros((3, 3)), 1.0, 4)[1].tolist() == np.eye(3).tolist()
+    assert O.pade_expm(H, 1.0, 1)[0] == 2
