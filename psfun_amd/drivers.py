@@ -11,8 +11,9 @@ examples/shiftedkrylov/shifted.inp):
 As the Fortran programs do, the deck comes from the file named as the first
 argument or else from standard input: a count line, then one value per line
 (Fortran list-directed: the first item of the line, quotes and 'd'
-exponents allowed, T/F logicals).  Matrices and vectors are MatrixMarket
-files (the HB format is refused); the partitioning and storage-format
+exponents allowed, T/F logicals).  Matrices are MatrixMarket (MM) or
+Harwell-Boeing (HB) files, vectors MatrixMarket arrays (an HB file's own
+right-hand side, when it holds one, is the vector); the partitioning and storage-format
 entries are read and reported but a single device holds the whole matrix.
 
 Kept from the reference drivers: the Lanczos and Arnoldi programs run with
@@ -88,15 +89,30 @@ def _vec(a):
 
 
 def _read_matrix(name, filefmt, ctxt):
-    if str(filefmt).upper() != "MM":
-        raise DeckError(f"file format {filefmt!r}: only MatrixMarket (MM) input is read")
-    a, info = mmio.mm_mat_read(name, ctxt)
+    """(matrix, right-hand sides held in the file or None): MM reads the
+    matrix file alone, HB the matrix and its right-hand sides
+    (lanczostest.F90:107-127)."""
+    f = str(filefmt).upper()
+    if f == "MM":
+        a, info = mmio.mm_mat_read(name, ctxt)
+        b = None
+    elif f == "HB":
+        a, b, info = mmio.hb_read(name, ctxt)
+    else:
+        raise DeckError(f"file format {filefmt!r}: MM or HB")
     if info != 0:
         raise DeckError(f"could not read the matrix {name!r}")
-    return a
+    return a, b
 
 
-def _rhs(name, n):
+def _rhs(name, n, file_rhs=None, filefmt="MM"):
+    """The first right-hand side of an HB file when it holds one of length n
+    (the deck's vector name is not read then), the deck's MM vector file, or
+    (1, ..., 1) (lanczostest.F90:136-155)."""
+    if str(filefmt).upper() == "HB":
+        if file_rhs is not None and file_rhs.shape[0] == n:
+            return file_rhs[:, 0].copy()
+        return np.ones(n)
     if str(name).upper() == "NONE":
         return np.ones(n)
     x, info = mmio.mm_array_read(name)
@@ -114,9 +130,9 @@ def krylov_driver(kname, deck, out=sys.stdout):
     scaling = _real(v[7])
     ctx = psb.default_context()
     t0 = time.perf_counter()
-    a = _read_matrix(mname, filefmt, ctx)
+    a, frhs = _read_matrix(mname, filefmt, ctx)
     n = a.get_nrows()
-    x = _vec(_rhs(rhs_file, n))
+    x = _vec(_rhs(rhs_file, n, frhs, filefmt))
     y = _vec(np.zeros(n))
     t_read = time.perf_counter() - t0
     fun = psfun_d_serial()
@@ -181,9 +197,9 @@ def shifted_driver(deck, out=sys.stdout):
     istopc, itmax, itrace, irst = (int(t) for t in v[9:13])
     eps = _real(v[13])
     ctx = psb.default_context()
-    a = _read_matrix(mname, filefmt, ctx)
+    a, frhs = _read_matrix(mname, filefmt, ctx)
     n = a.get_nrows()
-    bh = _rhs(bname, n)
+    bh = _rhs(bname, n, frhs, filefmt)
     b = _vec(bh)
     x = _vec(np.zeros(n))
     desc = psb.psb_desc_type(n, ctx)

@@ -4,6 +4,9 @@
 !   mm_mat_read(a, info, iunit, filename)    MatrixMarket coordinate file, into
 !                                            the host global matrix or straight
 !                                            into a device matrix
+!   hb_read(a, info, iunit, b, filename, mtitle)
+!                                            Harwell-Boeing file and the
+!                                            right-hand sides it holds
 !   mm_array_read(b, info, iunit, filename)  MatrixMarket array file
 !   mm_array_write(y, title, info, iunit, filename)
 !   psb_matdist(aux_a, a, ctxt, desc_a, info, fmt, parts)
@@ -170,6 +173,106 @@ contains
     a%m = nr
     a%nz = q
   end subroutine mm_lmat_read
+
+  ! Harwell-Boeing file (the drivers' filefmt 'HB', lanczostest.F90:119-122;
+  ! format of Duff, Grimes and Lewis, 1992): a 4-line header (5 with
+  ! right-hand sides), then the column pointers, row indices, values and
+  ! right-hand sides, each block on its own cards read with the format the
+  ! header gives.  Types R (real) or P (pattern: values 1) with U, S or Z
+  ! (one triangle stored, the other generated) and A (assembled).  b receives
+  ! the full ('F') right-hand sides, when the file holds any.
+  subroutine hb_read(a, info, iunit, b, filename, mtitle)
+    type(psb_ldspmat_type), intent(out) :: a
+    integer(psb_ipk_), intent(out) :: info
+    integer(psb_ipk_), intent(in), optional :: iunit
+    real(psb_dpk_), allocatable, intent(inout), optional :: b(:, :)
+    character(len=*), intent(in), optional :: filename
+    character(len=72), intent(out), optional :: mtitle
+    character(len=72) :: title
+    character(len=8) :: key
+    character(len=3) :: mxtype, rhstyp
+    character(len=16) :: ptrfmt, indfmt
+    character(len=20) :: valfmt, rhsfmt
+    integer :: u, ios
+    integer(psb_lpk_) :: totcrd, ptrcrd, indcrd, valcrd, rhscrd, nrow, ncol, nnz, neltvl, nrhs, nrhsix
+    integer(psb_lpk_) :: j, k, q
+    integer(psb_lpk_), allocatable :: colptr(:), rowind(:)
+    real(psb_dpk_), allocatable :: vals(:)
+    logical :: sym, skew
+    info = psb_success_
+    u = 12
+    if (present(iunit)) u = iunit
+    if (.not. present(filename)) then
+      info = psb_err_invalid_args_combination_
+      return
+    end if
+    open(u, file=filename, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      info = psb_err_from_subroutine_
+      call psb_errpush(info, 'hb_read', a_err=trim(filename))
+      return
+    end if
+    rhscrd = 0
+    read(u, '(a72,a8)', iostat=ios) title, key
+    if (ios == 0) read(u, '(5i14)', iostat=ios) totcrd, ptrcrd, indcrd, valcrd, rhscrd
+    if (ios == 0) read(u, '(a3,11x,4i14)', iostat=ios) mxtype, nrow, ncol, nnz, neltvl
+    if (ios == 0) read(u, '(2a16,2a20)', iostat=ios) ptrfmt, indfmt, valfmt, rhsfmt
+    nrhs = 0
+    rhstyp = ' '
+    if (ios == 0 .and. rhscrd > 0) read(u, '(a3,11x,2i14)', iostat=ios) rhstyp, nrhs, nrhsix
+    mxtype = psb_toupper(mxtype)
+    rhstyp = psb_toupper(rhstyp)
+    if (ios /= 0 .or. scan(mxtype(1:1), 'RP') == 0 .or. scan(mxtype(2:2), 'USZR') == 0 .or. mxtype(3:3) /= 'A' &
+         & .or. nrow /= ncol) then
+      info = psb_err_from_subroutine_
+      call psb_errpush(info, 'hb_read', a_err='bad header or unsupported type '//mxtype)
+      close(u)
+      return
+    end if
+    allocate(colptr(ncol + 1), rowind(nnz), vals(nnz))
+    read(u, ptrfmt, iostat=ios) colptr
+    if (ios == 0) read(u, indfmt, iostat=ios) rowind
+    if (ios == 0) then
+      if (mxtype(1:1) == 'R') then
+        read(u, valfmt, iostat=ios) vals
+      else
+        vals = done
+      end if
+    end if
+    if (ios == 0 .and. nrhs > 0 .and. rhstyp(1:1) == 'F' .and. present(b)) then
+      if (allocated(b)) deallocate(b)
+      allocate(b(nrow, nrhs))
+      if (len_trim(rhsfmt) == 0) rhsfmt = valfmt
+      read(u, rhsfmt, iostat=ios) b
+    end if
+    close(u)
+    if (ios /= 0) then
+      info = psb_err_from_subroutine_
+      call psb_errpush(info, 'hb_read', a_err='short or malformed data cards')
+      return
+    end if
+    sym = mxtype(2:2) == 'S'
+    skew = mxtype(2:2) == 'Z'
+    allocate(a%ia(2 * nnz), a%ja(2 * nnz), a%val(2 * nnz))
+    q = 0
+    do j = 1, ncol
+      do k = colptr(j), colptr(j + 1) - 1
+        q = q + 1
+        a%ia(q) = rowind(k)
+        a%ja(q) = j
+        a%val(q) = vals(k)
+        if ((sym .or. skew) .and. rowind(k) /= j) then
+          q = q + 1
+          a%ia(q) = j
+          a%ja(q) = rowind(k)
+          a%val(q) = merge(-vals(k), vals(k), skew)
+        end if
+      end do
+    end do
+    a%m = nrow
+    a%nz = q
+    if (present(mtitle)) mtitle = title
+  end subroutine hb_read
 
   ! The same file assembled as a device matrix on the context psb_init
   ! created (serial drivers read straight into psb_dspmat_type,

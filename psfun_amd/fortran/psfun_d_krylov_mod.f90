@@ -170,8 +170,9 @@ contains
   end subroutine krylov_args
 
   ! Outputs of a device run: iter, err, res(1:nres) allocated here, the trace.
-  subroutine krylov_outputs(methdname, st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
+  subroutine krylov_outputs(methdname, st, it, derr, rbuf, nres, eps, itrace, estimates, info, iter, err, res)
     character(len=*), intent(in) :: methdname
+    logical, intent(in) :: estimates
     integer(c_int), intent(in) :: st, it, nres
     real(c_double), intent(in) :: derr, rbuf(:), eps
     integer(psb_ipk_), optional, intent(in) :: itrace
@@ -198,12 +199,15 @@ contains
     if (tr > 0) then
       ! log_header / log_conv / log_end of the reference's loop
       ! (psfun_d_lanczos.f90:180, 304-305, 365): the estimate at loop index i
-      ! is rs(i) = res(i) for itx = i - 1, then the final err.
+      ! is rs(i) = res(i) for itx = i - 1 (res(it+1) is the final err); with
+      ! istop 2 there are no estimates, and log_conv is not called.
       write(psb_out_unit, '(a,a)') trim(methdname), ':   Iteration   Error estimate   Tolerance'
-      do i = 2, it
-        if (mod(i - 1, tr) == 0) write(psb_out_unit, '(a,a,i6,2(2x,es15.8))') trim(methdname), ':', i - 1, &
-             & rbuf(i), eps
-      end do
+      if (estimates) then
+        do i = 2, min(it + 1, nres)
+          if (mod(i - 1, tr) == 0) write(psb_out_unit, '(a,a,i6,2(2x,es15.8))') trim(methdname), ':', i - 1, &
+               & rbuf(i), eps
+        end do
+      end if
       write(psb_out_unit, '(a,a,i6,a,es15.8)') trim(methdname), ': iterations ', it, ', error estimate ', derr
     end if
   end subroutine krylov_outputs
@@ -241,7 +245,8 @@ contains
     st = psfun_lanczos_apply(a%ctxt%ctx, a%a, fn, 1.0_c_double, 1.0_c_double, x%v, y%v, eps, nl, istop_, it, &
          & derr, rbuf, nres)
     call psfun_unbind_userf(a%ctxt%ctx)
-    call krylov_outputs('LANCZOS', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
+    call krylov_outputs('LANCZOS+'//trim(fun%fname)//'+'//trim(fun%variant), st, it, derr, rbuf, nres, eps, &
+         & itrace, istop_ == 1, info, iter, err, res)
   end subroutine psfun_d_lanczos
 
   ! psfun_d_arnoldi (krylov/polynomial/psfun_d_arnoldi.f90:37-352): the SpMV
@@ -275,7 +280,8 @@ contains
     st = psfun_arnoldi_apply(a%ctxt%ctx, a%a, fn, var, 1.0_c_double, fun%scaling, x%v, y%v, eps, nl, istop_, &
          & it, derr, rbuf, nres)
     call psfun_unbind_userf(a%ctxt%ctx)
-    call krylov_outputs('ARNOLDI', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
+    call krylov_outputs('ARNOLDI+'//trim(fun%fname)//'+'//trim(fun%variant), st, it, derr, rbuf, nres, eps, &
+         & itrace, istop_ == 1, info, iter, err, res)
   end subroutine psfun_d_arnoldi
 
   ! psfun_d_saiarnoldi (krylov/rational/psfun_d_sai.f90:37-419): Arnoldi on
@@ -317,7 +323,8 @@ contains
     st = psfun_sai_arnoldi_apply(a%ctxt%ctx, a%a, fn, var, 1.0_c_double, tau, prec%get_code(), initmax, inistop, &
          & x%v, y%v, eps, nl, istop_, it, derr, rbuf, nres)
     call psfun_unbind_userf(a%ctxt%ctx)
-    call krylov_outputs('SAIARNOLDI', st, it, derr, rbuf, nres, eps, itrace, info, iter, err, res)
+    call krylov_outputs('SAIARNOLDI+'//trim(fun%fname)//'+'//trim(fun%variant), st, it, derr, rbuf, nres, eps, &
+         & itrace, istop_ == 1, info, iter, err, res)
   end subroutine psfun_d_saiarnoldi
 
 end module psfun_d_krylov_mod

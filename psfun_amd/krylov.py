@@ -94,4 +94,6 @@ def lanczos_apply(fun, a, desc_a, y, x, eps, itmax=None, itrace=None, istop=None
                                             res.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nres))
     if st != _lib.PSFUN_SUCCESS:
         return psb.status_to_info(st), 0, 0.0, None
+    psb.log_trace(f"LANCZOS+{fun.fname}+{fun.variant}", itrace, it.value, err.value, res[: nres.value], float(eps),
+                  estimates=istop_ == 1)
     return psb.psb_success_, it.value, err.value, res[: nres.value].copy()

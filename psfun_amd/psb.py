@@ -105,6 +105,28 @@ class Context:
 _contexts = {}
 
 
+def log_trace(methdname, itrace, it, err, res, eps, estimates=True, out=None):
+    """The convergence log of the reference's Krylov loops (log_header,
+    log_conv every itrace-th estimate, log_end; psfun_d_lanczos.f90:180,
+    304-305, 365), printed after the device run from its history: the
+    estimate at loop index i is rs(i) = res[i-1], logged for itx = i - 1
+    (itx = it is the final err).  Without estimates (istop 2) only the
+    header and the end line appear, as log_conv is not called then.
+    PSBLAS's own line format is not part of the reference sources; this one
+    is the Fortran layer's."""
+    if itrace is None or int(itrace) <= 0:
+        return
+    import sys
+    w = (out or sys.stdout).write
+    tr = int(itrace)
+    w(f"{methdname}:   Iteration   Error estimate   Tolerance\n")
+    if estimates:
+        for i in range(2, it + 2):
+            if (i - 1) % tr == 0 and i - 1 < len(res):
+                w(f"{methdname}:{i - 1:6d}  {res[i - 1]:15.8E}  {eps:15.8E}\n")
+    w(f"{methdname}: iterations {it:6d}, error estimate {err:15.8E}\n")
+
+
 def default_context(device=0):
     if device not in _contexts:
         _contexts[device] = Context(device)

@@ -115,6 +115,10 @@ def psb_krylov(method, a, prec, b, eta, zeta, x, eps, desc_a, itmax=None, itrace
                                          C.byref(it), C.byref(err), C.byref(cnd) if want_cond else None)
     if st != _lib.PSFUN_SUCCESS:
         return psb.psb_err_from_subroutine_, 0, 0.0, 0.0
+    # PSBLAS's check_conv logs every itrace-th iteration inside the loop; the
+    # device loop keeps no per-iteration history, so the log has its first
+    # and last lines (the header and the final error)
+    psb.log_trace("CG", itrace, it.value, err.value, [], float(eps), estimates=False)
     return psb.psb_success_, it.value, err.value, (cnd.value if want_cond else 0.0)
 
 

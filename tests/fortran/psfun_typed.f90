@@ -131,13 +131,19 @@ contains
       istop = 1
     end if
 
-    call mm_mat_read(aux_a, info, iunit=12, filename=mname)
+    if (psb_toupper(trim(filefmt)) == 'HB') then
+      call hb_read(aux_a, info, iunit=12, b=aux_x, filename=mname)
+    else
+      call mm_mat_read(aux_a, info, iunit=12, filename=mname)
+    end if
     if (info /= psb_success_) then
       call psb_error(ctxt)
       error stop 3
     end if
     m_problem = int(aux_a%get_nrows())
-    if (trim(rhs_file) /= 'NONE') then
+    if (allocated(aux_x)) then          ! the HB file's own right-hand side
+      x_glob = aux_x(:, 1)
+    else if (trim(rhs_file) /= 'NONE') then
       call mm_array_read(aux_x, info, iunit=12, filename=rhs_file)
       x_glob = aux_x(:, 1)
     else

@@ -74,7 +74,28 @@ def test_typed_lanczostest_deck(gpu_ctx, tmp_path):
     res = _vec(out, "RES")
     assert len(res) == it + 1 and abs(res[0] - np.sqrt(50.0)) <= 1e-12 * np.sqrt(50.0)
     assert np.allclose(res, ref["res"], rtol=1e-8, atol=0)
-    assert "LANCZOS:" in text and "iterations" in text      # itrace = 1 log
+    assert "LANCZOS+EXP+CHBSYM:" in text and "iterations" in text      # itrace = 1 log
+    # one log line per estimate, the last one the final err
+    log = [ln for ln in text.splitlines() if ln.startswith("LANCZOS+EXP+CHBSYM:") and ln.split(":")[1].split()
+           and ln.split(":")[1].split()[0].isdigit()]
+    assert len(log) == it and float(log[-1].split()[-2]) == pytest.approx(res[-1], rel=1e-7)
+
+
+def test_typed_lanczostest_harwell_boeing(gpu_ctx, tmp_path):
+    """lanczostest with filefmt HB (lanczostest.F90:119-122): hb_read takes the
+    matrix and the right-hand side the file holds, which replaces x = 1."""
+    from psfun_amd import mmio
+    A = _laplacian50(tmp_path)
+    rhs = np.linspace(0.5, 1.5, 50)
+    assert mmio.hb_write(str(tmp_path / "laplacian.rsa"), *A, "laplacian", "LAP50", "RSA", rhs=rhs) == 0
+    deck = LANCZOS.replace("laplacian.mtx", "laplacian.rsa").replace("MM            ! Format", "HB            ! Format")
+    (tmp_path / "lanczos.inp").write_text(deck)
+    out, _ = _run(tmp_path, "krylov", "lanczos.inp")
+    assert int(out["INFO"][0][0]) == 0
+    ref = O.lanczos(A, rhs, 1e-6, 100, 1)
+    assert int(out["ITER"][0][0]) == ref["iter"]
+    y = _vec(out, "Y")
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
 
 
 def test_typed_arnoldi_and_errors(gpu_ctx, tmp_path):

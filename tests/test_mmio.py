@@ -85,8 +85,78 @@ def test_mm_errors(tmp_path):
         mmio.mm_mat_read_csr(_write(tmp_path, "t.mtx", "%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1\n"))
     x, info = mmio.mm_array_read(str(tmp_path / "missing.mtx"))
     assert x is None and info != 0
-    a, info = mmio.hb_read(str(tmp_path / "a.rsa"))
-    assert a is None and info != 0
+    a, b, info = mmio.hb_read(str(tmp_path / "missing.rsa"))
+    assert a is None and b is None and info != 0
+
+
+# ---- Harwell-Boeing ---------------------------------------------------------
+# A hand-made RUA file: 4 x 4, 6 entries by columns, an (8I3) pointer card, a
+# (7I3) index card, values in (3D12.4) with D exponents (one with a sign in
+# place of its exponent letter), and one full RHS in (4F10.3).
+#   [[ 1.5  0    0    2 ]
+#    [ 0   -3    0    0 ]
+#    [ 4    0    0.25 0 ]
+#    [ 0    1e3  0    0 ]]
+HB_RUA = (
+    f"{'hand-made 4x4':<72}{'HAND4':<8}\n"
+    f"{5:14d}{1:14d}{1:14d}{2:14d}{1:14d}\n"
+    f"{'RUA':<3}{'':11}{4:14d}{4:14d}{6:14d}{0:14d}\n"
+    f"{'(8I3)':<16}{'(7I3)':<16}{'(3D12.4)':<20}{'(4F10.3)':<20}\n"
+    f"{'F':<3}{'':11}{1:14d}{0:14d}\n"
+    "  1  3  5  6  7\n"
+    "  1  3  2  4  3  1\n"
+    "  0.1500D+01  0.4000D+01 -0.3000D+01\n"
+    "  0.1000D+04   0.2500-00  0.2000D+01\n"
+    "     1.000     2.000    -3.500     4.000\n"
+)
+HB_RUA_DENSE = np.array([[1.5, 0, 0, 2.0], [0, -3.0, 0, 0], [4.0, 0, 0.25, 0], [0, 1e3, 0, 0]])
+
+
+def test_hb_hand_made_rua(tmp_path):
+    h = mmio.hb_read_csr(_write(tmp_path, "hand.rua", HB_RUA))
+    assert h["title"] == "hand-made 4x4" and h["key"] == "HAND4" and h["mxtype"] == "RUA"
+    D = _dense(h["rowptr"], h["colind"], h["val"], 4, 4)
+    assert np.array_equal(D, HB_RUA_DENSE)
+    assert np.array_equal(h["rhs"][:, 0], [1.0, 2.0, -3.5, 4.0])
+
+
+def test_hb_symmetric_pattern_and_touching_fields(tmp_path):
+    """RSA keeps one triangle (the other is generated), PSA has no values
+    (1 each); (2E9.2) fields touch with no blank between them."""
+    rsa = (f"{'sym':<72}{'S':<8}\n{3:14d}{1:14d}{1:14d}{1:14d}{0:14d}\n"
+           f"{'RSA':<3}{'':11}{3:14d}{3:14d}{4:14d}{0:14d}\n"
+           f"{'(4I2)':<16}{'(4I2)':<16}{'(4E9.2)':<20}{'':<20}\n"
+           " 1 3 4 5\n 1 2 2 3\n 2.00E+00-1.00E+00 3.00E+00 4.00E+00\n")
+    h = mmio.hb_read_csr(_write(tmp_path, "s.rsa", rsa))
+    assert np.array_equal(_dense(h["rowptr"], h["colind"], h["val"], 3, 3),
+                          [[2.0, -1.0, 0], [-1.0, 3.0, 0], [0, 0, 4.0]]) and h["rhs"] is None
+    psa = rsa.replace("RSA", "PSA").replace(f"{1:14d}{0:14d}\n", f"{0:14d}{0:14d}\n", 1)
+    psa = psa.replace(f"{3:14d}{1:14d}{1:14d}{0:14d}{0:14d}", f"{2:14d}{1:14d}{1:14d}{0:14d}{0:14d}")
+    psa = psa.replace(" 2.00E+00-1.00E+00 3.00E+00 4.00E+00\n", "")
+    h = mmio.hb_read_csr(_write(tmp_path, "s.psa", psa))
+    assert np.array_equal(_dense(h["rowptr"], h["colind"], h["val"], 3, 3), [[1, 1, 0], [1, 1, 0], [0, 0, 1]])
+
+
+def test_hb_round_trip_example_matrix(tmp_path):
+    """laplacian.mtx written as RSA (lower triangle) and RUA reads back as the
+    MatrixMarket reader reads it, bit for bit, with its right-hand side."""
+    text, (rp, ci, va) = _laplacian50_text()
+    rpm, cim, vam, m, n = mmio.mm_mat_read_csr(_write(tmp_path, "l.mtx", text))
+    rhs = np.linspace(-1.0, 1.0, 50) / 3.0
+    for kind in ("RSA", "RUA"):
+        f = str(tmp_path / f"l.{kind.lower()}")
+        assert mmio.hb_write(f, rpm, cim, vam, "laplacian", "LAP50", kind, rhs=rhs) == 0
+        h = mmio.hb_read_csr(f)
+        assert np.array_equal(_dense(h["rowptr"], h["colind"], h["val"], 50, 50), _dense(rpm, cim, vam, 50, 50))
+        assert np.array_equal(h["rhs"][:, 0], rhs)
+
+
+def test_hb_errors(tmp_path):
+    for bad in (HB_RUA.replace("RUA", "CUA", 1), HB_RUA.replace("RUA", "RUE", 1), "\n".join(HB_RUA.splitlines()[:6])):
+        with pytest.raises(mmio.MMError):
+            mmio.hb_read_csr(_write(tmp_path, "bad.rua", bad))
+        a, b, info = mmio.hb_read(str(tmp_path / "bad.rua"))
+        assert a is None and info != 0
 
 
 @pytest.mark.gpu
