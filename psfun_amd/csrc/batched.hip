@@ -26,6 +26,7 @@
 #include "fexp.hpp"
 
 #include <map>
+#include <type_traits>
 #include <tuple>
 
 #define BT_KMAX 64
@@ -422,8 +423,12 @@ __global__ __launch_bounds__(64) void k_bt_beta(const int *__restrict__ gchunk, 
   }
 }
 
+// SMALL: every order this launch sequence reaches is <= 64 (itmax <= 65), so
+// only the register path runs and the LDS holds the coefficient table alone
+// (more waves per CU beside the SpMM this kernel overlaps).
+template <bool SMALL>
 __global__ __launch_bounds__(64) void k_bt_est(int j, BtInst in, BtCtl *bctl, int nl, double eps, double fscale) {
-  __shared__ ChebLds W;
+  __shared__ typename std::conditional<SMALL, ChebCoefLds, ChebLds>::type W;
   const int I = blockIdx.x;
   if (in.done[I] != BT_RUN) return;
   const int64_t base = (int64_t)I * in.ld;
@@ -431,7 +436,10 @@ __global__ __launch_bounds__(64) void k_bt_est(int j, BtInst in, BtCtl *bctl, in
   const int lane = threadIdx.x;
   int fin = BT_RUN;
   double nrm, last;
-  wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+  if constexpr (SMALL)
+    wave_exp_tridiag_e1_small(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+  else
+    wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
   const double errnum = in.rs1[I] * beta * fabs(last);
   const double rsj = errnum / nrm;
   if (lane == 0) {
@@ -804,7 +812,12 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     if (istop == 1) {  // the estimate of index j+1 beside the SpMM of that index
       PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
       PSFUN_HIP_TRY(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
-      hipLaunchKernelGGL(k_bt_est, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps, fscale);
+      if (itmax <= 65)
+        hipLaunchKernelGGL(k_bt_est<true>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
+                           fscale);
+      else
+        hipLaunchKernelGGL(k_bt_est<false>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
+                           fscale);
       PSFUN_HIP_TRY(hipEventRecord(c->ev_join, c->aux_stream));
     }
     if (j + 1 <= itmax - 1) {

@@ -24,6 +24,13 @@
 #define PSFUN_CHEB_RMAX 400.0   // largest half-width per sub-step
 #define PSFUN_FT_KMAX 512       // largest order of T
 
+// The coefficient table alone: what the register path (k <= 64) needs, so a
+// kernel that only takes that path holds 5.6 KB of LDS instead of 18 KB.
+struct ChebCoefLds {
+  double coef[PSFUN_CHEB_CAP];
+  int M;
+};
+
 struct ChebLds {
   double coef[PSFUN_CHEB_CAP];
   double y[PSFUN_FT_KMAX];
@@ -144,8 +151,8 @@ __device__ void wave_exp_tridiag(int k, const double *d, const double *e, double
 // come from cross-lane shuffles, so the recurrence needs no LDS traffic or
 // fences.  Same formulas, in the same order, as wave_exp_tridiag.  Returns
 // y[lane] (0 for lane >= k).
-__device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const double *e, double s,
-                                                       ChebLds *W) {
+template <typename Lds>
+__device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const double *e, double s, Lds *W) {
   const int lane = threadIdx.x & 63;
   const bool in = lane < k;
   const double dl = in ? d[lane] : 0.0;
@@ -197,7 +204,8 @@ __device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const d
 
 // exp(s*T) e1 for k <= 64 through the register routine; returns ||y||_2 and
 // y[k-1] in every lane and, if yout is not null, stores y[0..k-1].
-__device__ inline void wave_exp_tridiag_e1_small(int k, const double *d, const double *e, double s, ChebLds *W,
+template <typename Lds>
+__device__ inline void wave_exp_tridiag_e1_small(int k, const double *d, const double *e, double s, Lds *W,
                                                  double *yout, double &nrm, double &last) {
   const int lane = threadIdx.x & 63;
   const double y = wave_exp_tridiag_e1_reg(k, d, e, s, W);
