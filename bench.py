@@ -20,6 +20,9 @@ The same JSON line also carries:
                    A's nonzeros, vs the same algorithm looped over the 32
                    columns one at a time (and, beside it, the plain GPU path
                    with a multi-shift CG gradient per column)
+  irregular     -- Lanczos exp(A')x on a 2M-node power-law (Chung-Lu) graph
+                   Laplacian, rows of 1 to ~150k entries: CSR row blocks
+                   (CSRS) against the vector-CSR kernel and sliced ELL
   quadrature    -- exp(-A)x on the 96^3 Laplacian by the 16-pole Talbot rule
                    with one multi-shift CG over all poles (end-to-end error
                    <= 1e-8 against the closed form)
@@ -63,9 +66,9 @@ def parse():
     p.add_argument("--copy-probe", action="store_true", help="also run the 1 GiB copy that scales PMC counters")
     p.add_argument("--prof-reps", type=int, default=4, help="back-to-back launches per timed SpMV step")
     p.add_argument("--no-extra", action="store_true", help="headline only (no blas1/batched/adjoint/quadrature)")
-    p.add_argument("--sections", default="blas1,batched,adjoint,quadrature,cg,arnoldi",
+    p.add_argument("--sections", default="blas1,batched,adjoint,irregular,quadrature,cg,arnoldi",
                    help="comma-separated extra sections to run after the headline")
-    p.add_argument("--format", default="", help="SELL, HDIA, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
+    p.add_argument("--format", default="", help="SELL, HDIA, CSRS, L1, L2, L4 or L8 (CSR lanes per row); default: autotune")
     p.add_argument("--bt-chunks", default="64,128,256,512,1024",
                    help="rows-per-chunk candidates of the batched autotuner")
     return p.parse_args()
@@ -146,7 +149,7 @@ def headline(args):
     # storage autotune for this matrix: sliced ELL, or vector CSR with 1..8
     # lanes per row (timed separately, not in `value`)
     def use(choice):
-        if choice in ("SELL", "HDIA"):
+        if choice in ("SELL", "HDIA", "CSRS"):
             M.set_format(choice)
         else:
             M.set_format("CSR")
@@ -156,7 +159,7 @@ def headline(args):
     if args.format:
         use(args.format)
     else:
-        for choice in ("SELL", "HDIA", "L1", "L2", "L4", "L8"):
+        for choice in ("SELL", "HDIA", "CSRS", "L1", "L2", "L4", "L8"):
             try:
                 use(choice)
             except _lib.PsfunError:   # diagonal slices refused (too many diagonals per slice)
@@ -224,7 +227,8 @@ def headline(args):
     csr_equiv = per_launch_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     upd_us = 1e3 * ku["ms"] / max(ku["launches"], 1)
     upd_bytes = ku["bytes"] / max(ku["launches"], 1)
-    kname = {"SELL": "k_lz_spmv_sell", "HDIA": "k_lz_spmv_hdia"}.get(fmt, f"k_lz_spmv<{lanes}>")
+    kname = {"SELL": "k_lz_spmv_sell", "HDIA": "k_lz_spmv_hdia", "CSRS": "k_lz_spmv_csrs"}.get(
+        fmt, f"k_lz_spmv<{lanes}>")
     per_unit = {"SELL": f"{idx_bits // 8 + 8}*nnz (values + {idx_bits}-bit column offsets) + 16n",
                 "HDIA": "(8*64 + 4)*slice_diagonals + 4*(slices + 1) + 16n (no column indices)"}.get(
                     fmt, "12*nnz + 4(n+1) + 16n")
@@ -468,6 +472,76 @@ def adjoint_section(args):
             "lanczos_iters_min_max": [int(iters.min()), int(iters.max())], "quad_poles": 16}
 
 
+def irregular_section(args):
+    """Lanczos exp(-A)x on a power-law graph Laplacian (Chung-Lu, n = 2M,
+    ~17M nonzeros, rows of 1 to ~150k entries): the case no sliced or
+    diagonal storage serves.  Each storage timed; the CSR row blocks (CSRS)
+    against the vector-CSR kernel at its best lanes-per-row."""
+    import ctypes as C
+    import torch
+    from psfun_amd import _lib, matgen, psb
+    from psfun_amd.krylov import psfun_d_krylov
+    from psfun_amd.serial import psfun_d_serial
+
+    n = 2_000_000
+    tg = time.perf_counter()
+    G = matgen.chung_lu_laplacian(n, 7)
+    G = (G[0], G[1], (-10.0 / float(np.max(np.abs(G[2])))) * G[2])   # spectrum of the operator in [-20, 0]
+    nnz = len(G[1])
+    lens = np.diff(G[0])
+    log(f"irregular: Chung-Lu graph n={n} nnz={nnz} row lengths {int(lens.min())}..{int(lens.max())} "
+        f"(median {int(np.median(lens))}) in {time.perf_counter() - tg:.1f}s")
+    ctx = psb.default_context(0)
+    M = psb.psb_dspmat_type.from_csr(G, ctx)
+    fun = psfun_d_serial()
+    fun.set("VARIANT", "CHBSYM")
+    km = psfun_d_krylov()
+    km.set("KNAME", "LANCZOS")
+    x = matgen.uniform_pm1(5, n)
+    xv = psb.psb_d_vect_type(torch.as_tensor(x, device="cuda:0"))
+    yv = psb.psb_d_vect_type(torch.zeros(n, dtype=torch.float64, device="cuda:0"))
+    L = _lib.lib()
+    out, ys = {}, {}
+    for f in ("CSRS", "L4", "L8", "SELL"):
+        try:
+            if f.startswith("L"):
+                M.set_format("CSR")
+                M.set_lanes(int(f[1:]))
+            else:
+                M.set_format(f)
+        except _lib.PsfunError:
+            continue
+        km.apply(fun, M, None, yv, xv, args.eps, itmax=80, istop=1)
+        torch.cuda.synchronize()
+        L.psfun_prof_reset(ctx.handle)
+        L.psfun_prof_enable(ctx.handle, 1)
+        steps = 3
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            info, it, err, res = km.apply(fun, M, None, yv, xv, args.eps, itmax=80, istop=1)
+        torch.cuda.synchronize()
+        ta = (time.perf_counter() - t0) / steps
+        k = read_prof(L, _lib, ctx, "lz_spmv")
+        L.psfun_prof_enable(ctx.handle, 0)
+        L.psfun_prof_reset(ctx.handle)
+        us = 1e3 * k["ms"] / max(k["launches"], 1)
+        by = k["bytes"] / max(k["launches"], 1)
+        ys[f] = yv.get_vect()
+        out[f] = {"f(A)x/s": round(1.0 / ta, 2), "ms_per_apply": round(ta * 1e3, 3), "iterations": it,
+                  "spmv_us": round(us, 2), "spmv_GBs": round(by / (us * 1e-6) / 1e9, 1),
+                  "spmv_frac": round(by / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"irregular: {f}: {1.0 / ta:.2f} f(A)x/s, SpMV {us:.1f} us ({out[f]['spmv_GBs']} GB/s), iter {it}")
+    ref = ys["CSRS"]
+    vec_best = min((v for k_, v in out.items() if k_.startswith("L")), key=lambda v: v["ms_per_apply"])
+    return {"workload": "Lanczos exp(A')x, A' = -(10/max|a_ij|) L, L the Laplacian of a Chung-Lu power-law graph "
+                        "(exponent 2.2, mean degree 8), n = 2M, eps 1e-10",
+            "n": n, "nnz": nnz, "row_len_max": int(lens.max()), "by_format": out,
+            "csrs_speedup_vs_vector_csr": round(vec_best["ms_per_apply"] / out["CSRS"]["ms_per_apply"], 2),
+            "max_rel_diff_between_formats": float(max(np.linalg.norm(y - ref) / np.linalg.norm(ref)
+                                                      for y in ys.values())),
+            "bytes_per_unit": "12*nnz + 4(n+1) + 16n per SpMV step"}
+
+
 def quadrature_section(args):
     import torch
     from psfun_amd import _lib, matgen, psb
@@ -690,6 +764,7 @@ def main():
     if not args.no_extra:
         wanted = set(args.sections.split(","))
         for name, fn in (("blas1", blas1_section), ("batched", batched_section), ("adjoint", adjoint_section),
+                         ("irregular", irregular_section),
                          ("quadrature", quadrature_section), ("cg", cg_section), ("arnoldi", arnoldi_section)):
             if name not in wanted:
                 continue

@@ -76,7 +76,8 @@ typedef struct psfun_csr *psfun_csr_t;
 enum { PSFUN_MEM_HOST = 0, PSFUN_MEM_DEVICE = 1 };
 
 /* Storage the SpMV kernels read (psfun_csr_set_format). */
-enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, PSFUN_FORMAT_HDIA = 3 };
+enum { PSFUN_FORMAT_AUTO = 0, PSFUN_FORMAT_CSR = 1, PSFUN_FORMAT_SELL = 2, PSFUN_FORMAT_HDIA = 3,
+       PSFUN_FORMAT_CSRS = 4 };
 
 /* ---- context / errors --------------------------------------------------- */
 const char *psfun_strerror(int status);
@@ -163,7 +164,10 @@ int psfun_csr_info(psfun_csr_t A, int64_t *n, int64_t *nnz, int *lanes_per_row);
  * on first choice, PSFUN_ERR_UNSUPPORTED when its rows would store more than
  * 1.5x nnz values; the Lanczos, Arnoldi, CG and multi-shift CG steps and
  * psfun_spmv read it) or PSFUN_FORMAT_AUTO (SELL when its padding stays within
- * 25% of nnz).  Results differ only in rounding order.  (PSBLAS's
+ * 25% of nnz), or PSFUN_FORMAT_CSRS (CSR row blocks of at most 256 rows and
+ * 1024 entries, a longer row alone: the Lanczos step streams each block's
+ * entries with coalesced loads and sums its rows from LDS; the other kernels
+ * read it as vector CSR).  Results differ only in rounding order.  (PSBLAS's
  * psb_d_hdia_sparse_mat is the reference ecosystem's counterpart.) */
 int psfun_csr_set_format(psfun_csr_t A, int format);
 /* Slice diagonals the diagonal-slice copy stores (values = 64 x that), 0 when

@@ -33,6 +33,7 @@ PSFUN_FORMAT_AUTO = 0
 PSFUN_FORMAT_CSR = 1
 PSFUN_FORMAT_SELL = 2
 PSFUN_FORMAT_HDIA = 3
+PSFUN_FORMAT_CSRS = 4
 PSFUN_PREC_NONE = 0
 PSFUN_PREC_DIAG = 1
 PSFUN_PREC_BJAC = 2

@@ -133,9 +133,36 @@ struct psfun_csr {
   int *hdia_ptr = nullptr;  // nullptr: not built
   int *hdia_off = nullptr;
   double *hdia_val = nullptr;
+  // Optional CSR row blocks (built when PSFUN_FORMAT_CSRS is first chosen),
+  // four ints per block {a, b, c, kind}:
+  //   kind 0: rows [a, b), at most PSFUN_BS rows and PSFUN_CSRS_CAP entries;
+  //           the Lanczos step streams their entries with coalesced loads
+  //           into LDS and sums each row there (CSR-stream, Greathouse and
+  //           Daga, SC14)
+  //   kind 1: row a alone (more than PSFUN_CSRS_CAP entries), summed by the
+  //           whole block
+  //   kind 2 + s: entries [b, c) of row a, one segment of split row s (rows
+  //           longer than PSFUN_CSRS_SPLIT, at most PSFUN_CSRS_MAXSPLIT of
+  //           them); the block that completes a row's segments (an arrival
+  //           counter) adds them in segment order
+  int *csrs_blk = nullptr;  // nullptr: not built
+  int csrs_nblk = 0;
+  int csrs_nsplit = 0;
+  int *csrs_split = nullptr;   // [2 * nsplit]: first block, segment count
+  int *csrs_cnt = nullptr;     // [nsplit] arrival counters (0 between launches)
+  double *csrs_seg = nullptr;  // [nblk] segment sums
   int format = PSFUN_FORMAT_CSR;  // format the kernels use
 };
 #define PSFUN_SELL_C 64
+#ifndef PSFUN_CSRS_CAP
+#define PSFUN_CSRS_CAP 1024      // entries per block of short rows
+#endif
+#ifndef PSFUN_CSRS_PER_CU
+#define PSFUN_CSRS_PER_CU 12     // CSR-stream blocks per CU in the Lanczos grid (12 measured best of 4..16)
+#endif
+#define PSFUN_CSRS_SPLIT 8192
+#define PSFUN_CSRS_SEG 4096
+#define PSFUN_CSRS_MAXSPLIT 1024
 // lanes-per-row template value that selects the diagonal-slice row sum
 #define PSFUN_L_HDIA (-1)
 

@@ -407,6 +407,70 @@ int psfun_sell_refill(psfun_csr *A) {
   return hdia_launch_fill(A);
 }
 
+static void csrs_free(psfun_csr *A) {
+  for (void *p : {(void *)A->csrs_blk, (void *)A->csrs_split, (void *)A->csrs_cnt, (void *)A->csrs_seg})
+    if (p) hipFree(p);
+  A->csrs_blk = A->csrs_split = A->csrs_cnt = nullptr;
+  A->csrs_seg = nullptr;
+  A->csrs_nblk = A->csrs_nsplit = 0;
+}
+
+// Blocks of the CSR-stream step, greedy over the rows: a block of short rows
+// closes before it would pass PSFUN_BS rows or PSFUN_CSRS_CAP entries; a row
+// with more entries is a block of its own, and a row with more than
+// PSFUN_CSRS_SPLIT (the first PSFUN_CSRS_MAXSPLIT such rows) is cut into
+// segments of PSFUN_CSRS_SEG entries, one block each.
+static int csrs_build(psfun_csr *A) {
+  const int n = A->n;
+  std::vector<int> rp(n + 1);
+  PSFUN_HIP_TRY(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  std::vector<int> blk, split;
+  blk.reserve(4 * (n / 64 + 16));
+  auto add = [&](int a, int b, int c, int kind) {
+    blk.push_back(a);
+    blk.push_back(b);
+    blk.push_back(c);
+    blk.push_back(kind);
+  };
+  int r = 0;
+  while (r < n) {
+    const int e0 = rp[r], len = rp[r + 1] - e0;
+    if (len <= PSFUN_CSRS_CAP) {
+      const int start = r;
+      while (r < n && r - start < PSFUN_BS && rp[r + 1] - e0 <= PSFUN_CSRS_CAP) ++r;
+      add(start, r, 0, 0);
+    } else if (len <= PSFUN_CSRS_SPLIT || (int)split.size() / 2 >= PSFUN_CSRS_MAXSPLIT) {
+      add(r, r + 1, 0, 1);
+      ++r;
+    } else {
+      const int s = (int)split.size() / 2, first = (int)blk.size() / 4;
+      int nseg = 0;
+      for (int q = e0; q < rp[r + 1]; q += PSFUN_CSRS_SEG, ++nseg) add(r, q, std::min(q + PSFUN_CSRS_SEG, rp[r + 1]), 2 + s);
+      split.push_back(first);
+      split.push_back(nseg);
+      ++r;
+    }
+  }
+  const int nblk = (int)blk.size() / 4, nsplit = (int)split.size() / 2;
+  hipError_t e = hipMalloc(&A->csrs_blk, sizeof(int) * std::max<size_t>(blk.size(), 4));
+  if (e == hipSuccess && !blk.empty())
+    e = hipMemcpy(A->csrs_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&A->csrs_split, sizeof(int) * std::max<size_t>(split.size(), 2));
+  if (e == hipSuccess && !split.empty())
+    e = hipMemcpy(A->csrs_split, split.data(), sizeof(int) * split.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&A->csrs_cnt, sizeof(int) * std::max(nsplit, 1));
+  if (e == hipSuccess) e = hipMemset(A->csrs_cnt, 0, sizeof(int) * std::max(nsplit, 1));
+  if (e == hipSuccess) e = hipMalloc(&A->csrs_seg, sizeof(double) * std::max(nblk, 1));
+  if (e != hipSuccess) {
+    psfun_record_hip_error(e, "csrs_build", __FILE__, __LINE__);
+    csrs_free(A);
+    return PSFUN_ERR_ALLOC;
+  }
+  A->csrs_nblk = nblk;
+  A->csrs_nsplit = nsplit;
+  return PSFUN_SUCCESS;
+}
+
 static int auto_format(const psfun_csr *A) {
   if (A->sell_nnz > 0 && (double)A->sell_nnz <= 1.25 * (double)std::max<int64_t>(A->nnz, 1))
     return PSFUN_FORMAT_SELL;
@@ -505,10 +569,11 @@ extern "C" int psfun_csr_destroy(psfun_csr_t A) {
   // The context may already be gone (a caller that destroys it first): only
   // the device recorded at creation is used here, and a device-wide wait
   // stands in for the context stream's.
-  if (A->owns || A->sell_nnz || A->sell_ptr || A->hdia_ptr) {
+  if (A->owns || A->sell_nnz || A->sell_ptr || A->hdia_ptr || A->csrs_blk) {
     hipSetDevice(A->device);
     hipDeviceSynchronize();
   }
+  csrs_free(A);
   if (A->owns) {
     if (A->rowptr) hipFree(A->rowptr);
     if (A->colind) hipFree(A->colind);
@@ -568,6 +633,13 @@ extern "C" int psfun_csr_set_format(psfun_csr_t A, int format) {
       PSFUN_TRY(hdia_build(A));
     }
     A->format = PSFUN_FORMAT_HDIA;
+  } else if (format == PSFUN_FORMAT_CSRS) {
+    if (!A->csrs_blk && A->n > 0) {
+      PSFUN_HIP_TRY(hipSetDevice(A->ctx->device));
+      PSFUN_HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+      PSFUN_TRY(csrs_build(A));
+    }
+    A->format = PSFUN_FORMAT_CSRS;
   } else {
     return PSFUN_ERR_INVALID_ARG;
   }

@@ -32,6 +32,7 @@
 // loop index and stop state to coherent pinned host memory, on which the host
 // paces its launches (no flag copies or events in the loop).
 #include "common.hpp"
+#include <type_traits>
 #include "kernels.hpp"
 #include "fexp.hpp"
 #include "lanczos.hpp"
@@ -47,9 +48,14 @@ __device__ __forceinline__ void lz_stop(LzCtl *ctl, int *hf, int state) {
 // SMALLT: every T this loop can meet has order <= 64 (nl <= 65), so only the
 // register f(T) routine is compiled in, which keeps the streaming kernel
 // that hosts this control block within 64 VGPRs.
+// (Lds: the coefficient table alone suffices for SMALLT, which keeps the
+// streaming kernels that host the control block at 5.6 KB of LDS for it.)
+template <bool SMALLT>
+using LzCtlLds = typename std::conditional<SMALLT, ChebCoefLds, ChebLds>::type;
+
 template <bool SMALLT>
 __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, double eps, double fscale,
-                           ChebLds *W, int *hf) {
+                           LzCtlLds<SMALLT> *W, int *hf) {
   const int lane = threadIdx.x;
   if (lane == 0) {
     host_flag_store(hf, j);
@@ -63,7 +69,7 @@ __device__ void lz_control(LzCtl *ctl, int j, double beta, int nl, int istop, do
   if (istop == 1) {
     // yk = f(T(1:j-1,1:j-1)) e1; rs(j) = rs1 * beta * |yk(j-1)| / ||yk||  (lines 276-315)
     double nrm, last;
-    if (SMALLT)
+    if constexpr (SMALLT)
       wave_exp_tridiag_e1_small(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
     else
       wave_exp_tridiag_e1_lds(j - 1, ctl->alpha + 1, ctl->beta + 1, fscale, W, ctl->yk, nrm, last);
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
       }
       return;
     }
-    __shared__ ChebLds W;
+    __shared__ LzCtlLds<SMALLT> W;
     lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
@@ -128,6 +134,110 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
     if (row < n && lane == 0) {
+      const double w = src[row];
+      const double u = opscale * sum;
+      if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
+      u_out[row] = u;
+      dot = fma(u, w * inv, dot);
+    }
+  }
+  dot = block_sum<PSFUN_BS>(dot, lds);
+  if (threadIdx.x == 0) p_out[blockIdx.x] = dot;
+}
+
+// Same step on the CSR row blocks (A->csrs_blk, kinds as documented with the
+// matrix): a block of short rows puts its entries' products a_k v(col_k) in
+// LDS by coalesced loads of the whole block, then L lanes per row (the
+// largest power of two with rows * L <= PSFUN_BS) sum each row from LDS; a
+// long row alone is summed by the whole block; a segment of a split row is
+// summed by the whole block into seg[b], and the block whose arrival
+// completes the row (counter cnt[s]) adds the row's segments in order and
+// puts the row's alpha term in partial slot G + s (the consumers fold G +
+// nsplit partials), so the result does not depend on the arrival order.
+template <bool SMALLT>
+__global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *__restrict__ blk,
+                                                           const int *__restrict__ split, int *__restrict__ cnt,
+                                                           double *__restrict__ seg, const int *__restrict__ rowptr,
+                                                           const int *__restrict__ col,
+                                                           const double *__restrict__ val, double opscale,
+                                                           const double *__restrict__ src, int j, int G,
+                                                           const double *__restrict__ p_in, int g_in,
+                                                           double *__restrict__ v_out, double *__restrict__ u_out,
+                                                           double *__restrict__ p_out, LzCtl *ctl, int nl, int istop,
+                                                           double eps, double fscale, int *hf) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  // the streaming blocks' products and the control block's coefficient
+  // table share one LDS buffer (a block is one or the other)
+  constexpr size_t SH = sizeof(LzCtlLds<SMALLT>) > sizeof(double) * PSFUN_CSRS_CAP ? sizeof(LzCtlLds<SMALLT>)
+                                                                                   : sizeof(double) * PSFUN_CSRS_CAP;
+  __shared__ __align__(16) unsigned char shbuf[SH];
+  double *prod = reinterpret_cast<double *>(shbuf);
+  if (__syncthreads_or(ctl->done)) return;
+  const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
+  if (blockIdx.x == G) {  // control block
+    if (threadIdx.x >= 64) return;
+    if (j == 1) {
+      if (threadIdx.x == 0) {
+        host_flag_store(hf, 1);
+        ctl->rs1 = sigma;
+        ctl->rs[1] = sigma;
+        if (sigma == 0.0) lz_stop(ctl, hf, LZ_DONE_ZERO);
+      }
+      return;
+    }
+    lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, reinterpret_cast<LzCtlLds<SMALLT> *>(shbuf), hf);
+    return;
+  }
+  if (sigma == 0.0 && j == 1) return;
+  const double inv = 1.0 / sigma;
+  const XcdRange xr = xcd_range_g(nblk, G);
+  double dot = 0.0;
+  for (int64_t b = xr.begin; b < xr.end; b += xr.stride) {
+    const int ba = blk[4 * b], bb = blk[4 * b + 1], bc = blk[4 * b + 2], kind = blk[4 * b + 3];
+    int row = -1;
+    double sum = 0.0;
+    bool lead = false;
+    if (kind == 0) {  // short rows [ba, bb)
+      const int e0 = rowptr[ba], ne = rowptr[bb] - e0;
+      for (int t = threadIdx.x; t < ne; t += PSFUN_BS) prod[t] = val[e0 + t] * (src[col[e0 + t]] * inv);
+      __syncthreads();
+      const int nr = bb - ba;
+      int L = 1;
+      while (L < 64 && nr * L * 2 <= PSFUN_BS) L *= 2;
+      const int rr = threadIdx.x / L, ln = threadIdx.x % L;
+      if (rr < nr) {
+        row = ba + rr;
+        const int s = rowptr[row] - e0, e = rowptr[row + 1] - e0;
+        for (int q = s + ln; q < e; q += L) sum += prod[q];
+      }
+      for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+      lead = rr < nr && ln == 0;
+      __syncthreads();  // the next block of rows overwrites prod
+    } else {  // one row, or a segment of one
+      const int q0 = kind == 1 ? rowptr[ba] : bb, q1 = kind == 1 ? rowptr[ba + 1] : bc;
+      for (int q = q0 + (int)threadIdx.x; q < q1; q += PSFUN_BS) sum = fma(val[q], src[col[q]] * inv, sum);
+      sum = block_sum<PSFUN_BS>(sum, lds);
+      if (kind == 1) {
+        row = ba;
+        lead = threadIdx.x == 0;
+      } else if (threadIdx.x == 0) {
+        const int sp = kind - 2, first = split[2 * sp], nseg = split[2 * sp + 1];
+        __hip_atomic_store(seg + b, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        if (atomicAdd(cnt + sp, 1) == nseg - 1) {  // the row's last segment to arrive
+          __threadfence();
+          double t = 0.0;
+          for (int k = 0; k < nseg; ++k) t += __hip_atomic_load(seg + first + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const double w = src[ba];
+          const double u = opscale * t;
+          if (j == 1) v_out[ba] = w;
+          u_out[ba] = u;
+          p_out[G + sp] = u * (w * inv);
+          __hip_atomic_store(cnt + sp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    if (lead) {
       const double w = src[row];
       const double u = opscale * sum;
       if (j == 1) v_out[row] = w;  // the basis keeps w_0 = x (v_1 = x / rs1)
@@ -186,7 +296,7 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_sell(int n, const int *
       }
       return;
     }
-    __shared__ ChebLds W;
+    __shared__ LzCtlLds<SMALLT> W;
     lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
@@ -253,7 +363,7 @@ __global__ __launch_bounds__(PSFUN_BS, 8) void k_lz_spmv_hdia(int n, const int *
       }
       return;
     }
-    __shared__ ChebLds W;
+    __shared__ LzCtlLds<SMALLT> W;
     lz_control<SMALLT>(ctl, j, sigma, nl, istop, eps, fscale, &W, hf);
     return;
   }
@@ -447,6 +557,23 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
     if (prof) PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;
   }
+  if (A->format == PSFUN_FORMAT_CSRS && A->csrs_blk) {
+    for (int r = 0; r < reps; ++r) {
+      if (nl <= 65)
+        hipLaunchKernelGGL((k_lz_spmv_csrs<true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->csrs_nblk,
+                           A->csrs_blk, A->csrs_split, A->csrs_cnt, A->csrs_seg, A->rowptr, A->colind, A->val,
+                           opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, istop, eps, fscale,
+                           c->d_flags);
+      else
+        hipLaunchKernelGGL((k_lz_spmv_csrs<false>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->csrs_nblk,
+                           A->csrs_blk, A->csrs_split, A->csrs_cnt, A->csrs_seg, A->rowptr, A->colind, A->val,
+                           opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, istop, eps, fscale,
+                           c->d_flags);
+    }
+    PSFUN_HIP_TRY(hipGetLastError());
+    if (prof) PSFUN_TRY(psfun_prof_end(c));
+    return PSFUN_SUCCESS;
+  }
   if (psfun_reads_sell(A)) {
 #define LZ_SELL(SM, D)                                                                                           \
   hipLaunchKernelGGL((k_lz_spmv_sell<SM, D>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->n, A->sell_ptr,      \
@@ -529,13 +656,18 @@ static int lz_apply(psfun_ctx *c, psfun_csr *A, bool userf, double fscale, doubl
   // sliced ELL and diagonal slices: one thread per row, chunks of PSFUN_BS rows
   const bool sell = psfun_row_per_thread(A);
   const int L = sell ? 1 : A->lanes;
-  const int64_t chunks = ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
+  // CSR row blocks: one block of the grid per row block at a time
+  const bool csrs = A->format == PSFUN_FORMAT_CSRS && A->csrs_blk;
+  const int64_t chunks = csrs ? (int64_t)A->csrs_nblk : ((int64_t)n + PSFUN_BS / L - 1) / (PSFUN_BS / L);
+  // alpha partials: one per streaming block, and with split rows one more per row
+  const int nsplit = csrs ? A->csrs_nsplit : 0;
   // SpMV streaming blocks; with sliced ELL 8 blocks per CU are resident, and
   // the grid (control block + Gs) is kept within one residency round.  On
   // diagonal slices 6 per CU (5-6 row chunks per block at 128^3) measured
   // about 1% faster per f(A)x than 8 per CU.
-  int Gs = psfun_grid(c, chunks, psfun_reads_hdia(A) ? 6 : (sell ? 8 : 4));
+  int Gs = psfun_grid(c, chunks, psfun_reads_hdia(A) ? 6 : (sell ? 8 : (csrs ? PSFUN_CSRS_PER_CU : 4)));
   if (sell && Gs + 1 > 8 * c->num_cu) Gs = (8 * c->num_cu - 1) / 8 * 8;
+  if (Gs + nsplit > PSFUN_MAX_GRID) Gs = (PSFUN_MAX_GRID - nsplit) / 8 * 8;
   const int Gu = psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4);  // update blocks
 
   c->h_flags[0] = 0;
@@ -558,7 +690,8 @@ static int lz_apply(psfun_ctx *c, psfun_csr *A, bool userf, double fscale, doubl
     const double *vjm1 = (j > 1) ? V + (size_t)ldv * (j - 2) : nullptr;
     // algorithmic bytes: u read + w written, v_j (and v_{j-1}) read
     if (prof) PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_LZ_UPDATE, (j > 1 ? 32.0 : 24.0) * n, j));
-    hipLaunchKernelGGL(k_lz_update, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, Pa, Gs, Pw, ctl);
+    hipLaunchKernelGGL(k_lz_update, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, u, vj, vjm1, j, Pa, Gs + nsplit, Pw,
+                       ctl);
     PSFUN_HIP_TRY(hipGetLastError());
     if (prof) PSFUN_TRY(psfun_prof_end(c));
     if (j + 1 <= nl - 1) {

@@ -153,6 +153,33 @@ def rgg_laplacian(n, seed, degree=8.0, wlo=0.5, whi=1.5):
     return _coo_to_csr(n, rows, cols, vals)
 
 
+def chung_lu_laplacian(n, seed, degree=8.0, exponent=2.2, wlo=0.5, whi=1.5):
+    """Graph with a power-law degree profile (Chung-Lu: about n*degree/2
+    edges whose ends are drawn with probability proportional to the weights
+    w_i = (i + 1)^(-1/(exponent - 1)), repeated pairs merged, self loops
+    dropped), edge weights U[wlo, whi], nodes numbered by decreasing weight.
+    Returns the Laplacian L = D - W: rows of very uneven length (a few
+    thousand entries at the top, 1-2 at the tail), the irregular case no
+    sliced or diagonal storage serves."""
+    w = (np.arange(n, dtype=np.float64) + 1.0) ** (-1.0 / (exponent - 1.0))
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    m = int(n * degree / 2)
+    u = uniform01(seed, 2 * m)
+    i = np.minimum(np.searchsorted(cdf, u[:m], side="right"), n - 1)
+    j = np.minimum(np.searchsorted(cdf, u[m:], side="right"), n - 1)
+    keep = i != j
+    a, b = np.minimum(i[keep], j[keep]), np.maximum(i[keep], j[keep])
+    key = np.unique(a.astype(np.int64) * n + b)
+    a, b = key // n, key % n
+    ww = wlo + (whi - wlo) * uniform01(seed ^ 0x2545F491, len(a))
+    deg = np.bincount(a, weights=ww, minlength=n) + np.bincount(b, weights=ww, minlength=n)
+    rows = np.concatenate([np.arange(n), a, b])
+    cols = np.concatenate([np.arange(n), b, a])
+    vals = np.concatenate([deg, -ww, -ww])
+    return _coo_to_csr(n, rows, cols, vals)
+
+
 def rgg_sizes(count, seed, nmin=1000, nmax=64000):
     """Graph orders drawn log-uniform in [nmin, nmax]."""
     u = uniform01(seed, count)

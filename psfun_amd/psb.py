@@ -221,7 +221,7 @@ class psb_dspmat_type:
         self._src = self._src[:2] + (v, self._src[3])
 
     _FORMATS = {"AUTO": _lib.PSFUN_FORMAT_AUTO, "CSR": _lib.PSFUN_FORMAT_CSR, "SELL": _lib.PSFUN_FORMAT_SELL,
-                "HDIA": _lib.PSFUN_FORMAT_HDIA}
+                "HDIA": _lib.PSFUN_FORMAT_HDIA, "CSRS": _lib.PSFUN_FORMAT_CSRS}
 
     def set_format(self, fmt):
         """'CSR' (vector CSR), 'SELL' (sliced ELL, 64-row slices), 'HDIA'

@@ -302,7 +302,7 @@ def _lanczos_gpu(A, x, eps, itmax, istop=1, fmt="AUTO"):
     return info, it, err, res, y.get_vect()
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA", "CSRS"])
 @pytest.mark.parametrize("dims,itmax", [((100, 100), 30), ((12, 10, 9), 50), ((64, 64, 64), 50)])
 def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     A = matgen.laplacian(dims, scale=-1.0)
@@ -319,7 +319,7 @@ def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):
     assert len(res) == it + 1 and err < 1e-10
 
 
-@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA"])
+@pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA", "CSRS"])
 def test_lanczos_long_run_lds_fT(gpu_ctx, fmt):
     """exp(-30 L) on 16^3 takes ~90 iterations: the loop runs with itmax > 65,
     so the control block's f(T) step is the LDS routine for orders past 64."""
@@ -504,3 +504,46 @@ def test_lanczos_itrace_log(gpu_ctx, capsys):
     assert [int(b[0]) for b in body] == list(range(1, it + 1))
     assert np.allclose([float(b[1]) for b in body], res[1:], rtol=1e-8, atol=0)
     assert float(body[-1][1]) == pytest.approx(err, rel=1e-8) and err < 1e-10
+
+
+@pytest.mark.parametrize("fmt", ["CSRS", "CSR"])
+def test_lanczos_power_law_rows(gpu_ctx, fmt):
+    """A power-law graph Laplacian (Chung-Lu, rows of 1 to ~2000 entries):
+    the CSR row blocks take the long rows alone (whole-block sums) and up to
+    256 short rows per block, against the oracle."""
+    A = matgen.chung_lu_laplacian(20000, 11)
+    d = np.diff(A[0])
+    assert d.max() > 1024 and d.min() <= 2
+    A = (A[0], A[1], -0.02 * A[2])
+    x = matgen.uniform_pm1(12, 20000)
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, 120, fmt=fmt)
+    ref = O.lanczos(A, x, 1e-10, 120, 1)
+    assert info == 0 and abs(it - ref["iter"]) <= 1
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+
+
+def _arrow(n, seed):
+    """Symmetric arrow matrix: diagonal 2 + U[0, 1), first row and column
+    U[-1, 1) * 0.5 / sqrt(n): row 0 holds all n entries."""
+    d = 2.0 + matgen.uniform01(seed, n)
+    a = 0.5 / np.sqrt(n) * matgen.uniform_pm1(seed + 1, n - 1)
+    rows = np.concatenate([np.arange(n), np.zeros(n - 1, np.int64), np.arange(1, n)])
+    cols = np.concatenate([np.arange(n), np.arange(1, n), np.zeros(n - 1, np.int64)])
+    vals = np.concatenate([d, a, a])
+    return matgen._coo_to_csr(n, rows, cols, vals)
+
+
+@pytest.mark.parametrize("n", [9000, 40000])
+def test_lanczos_csrs_split_row(gpu_ctx, n):
+    """A row longer than the split length (8192 entries) is cut into 4096-entry
+    segments summed by different blocks and added in segment order by the
+    last to arrive; results against the oracle, and bitwise repeatable."""
+    A = _arrow(n, 7)
+    A = (A[0], A[1], -A[2])
+    x = matgen.uniform_pm1(9, n)
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, 60, fmt="CSRS")
+    ref = O.lanczos(A, x, 1e-10, 60, 1)
+    assert info == 0 and abs(it - ref["iter"]) <= 1
+    assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"])
+    info2, it2, err2, res2, y2 = _lanczos_gpu(A, x, 1e-10, 60, fmt="CSRS")
+    assert it2 == it and np.array_equal(y2, y) and np.array_equal(res2, res)
