@@ -635,7 +635,7 @@ def cg_section(args):
             "iterations": int(it), "us_per_iteration": round(us_it, 2), "relative_residual": rel,
             "format": fmt, "tune_ms": tune,
             "spmv_kernel": "k_cg_a (p update + (eta A + zeta I) p + sigma partials; "
-                           "12*nnz + 4(n+1) + 40n bytes)",
+                           "12*nnz + 4(n+1) + 32n bytes: z and p_old read, p and q written)",
             "spmv_kernel_GBs": round(ka["bytes"] / max(ka["ms"], 1e-9) / 1e6, 1),
             "spmv_kernel_us": round(1e3 * ka["ms"] / max(ka["launches"], 1), 2)}
 

@@ -9,14 +9,15 @@
 //   k_cg_a(it): every block folds the rho partials of the previous k_cg_b
 //               (of k_cg_resid, which runs on the SpMV grid, when it == 1)
 //               (rho_it = r.z), beta = rho_it / rho_{it-1}; p_new = z + beta
-//               p_old for its rows (z = M^{-1} r formed on the fly, p_old
-//               gathered on the fly), q = eta*A p_new + zeta*p_new, partials
-//               of sigma = p.q.  A control block runs the reference's checks
+//               p_old for its rows (z = M^{-1} r and p_old gathered, p_new
+//               formed on the fly at every column), q = eta*A p_new +
+//               zeta*p_new, partials of sigma = p.q.  A control block runs the reference's checks
 //               for the previous iteration: psb_check_conv (from the
 //               ||r||/||x|| partials) and the rho_old == 0 breakdown.
 //   k_cg_b(it): sigma from partials; sigma == 0 is a breakdown; alpha =
-//               rho/sigma; x += alpha p; r -= alpha q; partials of rho_next =
-//               r.M^{-1}r and of the convergence norms.
+//               rho/sigma; x += alpha p; r -= alpha q; z = M^{-1} r for DIAG
+//               (the preconditioner object writes z for BJAC); partials of
+//               rho_next = r.z and of the convergence norms.
 // A breakdown ends the device loop; the host then restarts the way the
 // reference does (r = b - (eta A + zeta I) x, it = 0) while itx < itmax.
 #include "common.hpp"
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_a(int n, const int *__restrict_
   if (it > 1 && rho_old == 0.0) return;
   const double beta = (it > 1) ? rho / rho_old : 0.0;
   // direction p = M^{-1} r + beta p_old, formed on the fly at every gathered
-  // column; z = M^{-1} r is read from zv when a preconditioner object formed it
+  // column; z = M^{-1} r is read from zv where it is stored (DIAG, BJAC)
   auto zk = [&](int c) -> double { return zv ? zv[c] : prec_apply(dinv, c, r[c]); };
   auto pk = [&](int c) -> double { return (it > 1) ? fma(beta, p_old[c], zk(c)) : zk(c); };
   constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
                                                    const double *__restrict__ dinv, int it,
                                                    const double *__restrict__ p_sig, int g_sig,
                                                    double *__restrict__ p_rho, double *__restrict__ p_conv,
-                                                   CgCtl *ctl, int istop, int *hf) {
+                                                   CgCtl *ctl, int istop, int *hf, double *__restrict__ zout) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   if (ctl->done) return;
   const double sigma = block_sum_partials<PSFUN_BS>(p_sig, g_sig, lds);
@@ -152,7 +153,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
     const double ri = fma(-alpha, q[i], r[i]);
     x[i] = xi;
     r[i] = ri;
-    arho = fma(ri, prec_apply(dinv, i, ri), arho);
+    const double zi = prec_apply(dinv, i, ri);
+    if (zout) zout[i] = zi;
+    arho = fma(ri, zi, arho);
     if (istop == 1) {
       a1 = fmax(a1, fabs(ri));
       a2 = fmax(a2, fabs(xi));
@@ -192,7 +195,8 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_resid(int n, const int *__restr
                                                        const int *__restrict__ col, const double *__restrict__ val,
                                                        double eta, double zeta, const double *__restrict__ b,
                                                        const double *__restrict__ x, double *__restrict__ r,
-                                                       const double *__restrict__ dinv, double *__restrict__ p_rho) {
+                                                       const double *__restrict__ dinv, double *__restrict__ p_rho,
+                                                       double *__restrict__ zout) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
   constexpr int RPB = SELL ? PSFUN_BS : PSFUN_BS / L;
   const int64_t nchunks = (n + RPB - 1) / RPB;
@@ -205,7 +209,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_resid(int n, const int *__restr
     if (row < n && lane == 0) {
       const double rr = fma(-zeta, x[row], fma(-eta, sum, b[row]));
       r[row] = rr;
-      acc = fma(rr, prec_apply(dinv, row, rr), acc);
+      const double zr = prec_apply(dinv, row, rr);
+      if (zout) zout[row] = zr;
+      acc = fma(rr, zr, acc);
     }
   }
   acc = block_sum<PSFUN_BS>(acc, lds);
@@ -350,7 +356,10 @@ static int cg_run(psfun_ctx *c, psfun_csr *A, int prec, const psfun_prec *P, con
   double *r = (double *)((char *)ws + ctl_bytes);
   double *pA = r + vec, *pB = pA + vec, *q = pB + vec, *dinv = q + vec, *zv = dinv + vec;
   if (prec != PSFUN_PREC_DIAG) dinv = nullptr;
-  if (!zmode) zv = nullptr;
+  // z = M^{-1} r is stored for DIAG (by the residual updates) and for BJAC
+  // (by the preconditioner object), so the SpMV step gathers z and p_old only
+  double *zdiag = dinv ? zv : nullptr;
+  if (!zmode && !zdiag) zv = nullptr;
 
   double *Psig = c->d_partials, *Prho = c->d_partials + PSFUN_MAX_GRID,
          *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID,
@@ -371,7 +380,7 @@ static int cg_run(psfun_ctx *c, psfun_csr *A, int prec, const psfun_prec *P, con
     if (itx_host >= itmax) break;  // psfun_dcg.F90:256
     // r = b - (eta A + zeta I) x, rho partials
     PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_cg_resid<LL, SS>), dim3(Gs), dim3(PSFUN_BS), 0, c->stream, n,
-                                                 MRP, MCOL, MVAL, eta, zeta, b, x, r, dinv, Prho));
+                                                 MRP, MCOL, MVAL, eta, zeta, b, x, r, dinv, Prho, zdiag));
     PSFUN_HIP_TRY(hipGetLastError());
     if (restart > 0) {
       hipLaunchKernelGGL(k_cg_initconv, dim3(1), dim3(PSFUN_BS), 0, c->stream, Pani, Pb, Gu, ctl, 0);
@@ -387,15 +396,17 @@ static int cg_run(psfun_ctx *c, psfun_csr *A, int prec, const psfun_prec *P, con
     int it;
     double *pold = pA, *pnew = pB;
     for (it = 1; it <= itmax - itx_host + 1; ++it) {
-      PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_CG, 12.0 * A->nnz + 4.0 * (n + 1) + (it > 1 ? 40.0 : 32.0) * n));
+      // algorithmic bytes: val+colind, rowptr, z (and p_old) read, p_new and q written
+      PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_CG, 12.0 * A->nnz + 4.0 * (n + 1) + (it > 1 ? 32.0 : 24.0) * n));
       PSFUN_DISPATCH_STORAGE(A, hipLaunchKernelGGL((k_cg_a<LL, SS>), dim3(Gs + 1), dim3(PSFUN_BS), 0, c->stream, n,
-                                                   MRP, MCOL, MVAL, eta, zeta, r, dinv, pold, pnew, q, it, Gs, Prho,
+                                                   MRP, MCOL, MVAL, eta, zeta, r, zv ? nullptr : dinv, pold, pnew, q,
+                                                   it, Gs, Prho,
                                                    (it == 1 && !zmode) ? Gs : Gu, Pconv, Gu, Psig, ctl, istop, eps,
                                                    itmax, c->d_flags, zv));
       PSFUN_HIP_TRY(hipGetLastError());
       PSFUN_TRY(psfun_prof_end(c));
       hipLaunchKernelGGL(k_cg_b, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, x, r, pnew, q, dinv, it, Psig, Gs, Prho,
-                         Pconv, ctl, istop, c->d_flags);
+                         Pconv, ctl, istop, c->d_flags, zdiag);
       PSFUN_HIP_TRY(hipGetLastError());
       if (zmode) {  // the next iteration's z = M^{-1} r and rho = r.z
         PSFUN_TRY(psfun_prec_launch(c, P, r, zv));
