@@ -11,6 +11,34 @@ from psfun_amd import matgen
 
 
 # ---- examples/utils/utiltest.f90:63-87 control values ----------------------
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-0.5, 2.0), (3.0, -1.0), (0.0, 0.5)])
+def test_spmv_against_scipy(alpha, beta):
+    """or_spmv (y = alpha A x + beta y) against scipy's CSR product on a ragged
+    matrix with empty rows, unsorted columns and repeated (row, col) entries,
+    which a CSR product sums."""
+    import scipy.sparse as sps
+    rng = np.random.default_rng(11)
+    n, nnz = 57, 400
+    rows = np.sort(rng.integers(0, n, nnz))
+    rows[rows == 7] = 8                      # row 7 stays empty
+    cols = rng.integers(0, n, nnz)
+    cols[:40] = cols[40:80]                  # repeat some columns ...
+    rows[:40] = rows[40:80]                  # ... in the same rows
+    order = np.argsort(rows, kind="stable")
+    rows, cols = rows[order], cols[order]
+    vals = rng.standard_normal(nnz)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    S = sps.csr_matrix((vals, cols.astype(np.int32), rp), shape=(n, n))
+    assert not S.has_canonical_format        # duplicates are present
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    got = O.spmv((rp, cols, vals), x, alpha=alpha, beta=beta, y=y0)
+    want = alpha * (S @ x) + beta * y0
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+    assert got[7] == beta * y0[7]
+
+
 def test_horner_kat():
     assert O.horner([132.0, 42.0, 14.0, 5.0, 2.0, 1.0, 0.0], 1.0) == 196.0
     # brute-force polynomial on another point
