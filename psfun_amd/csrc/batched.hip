@@ -153,8 +153,11 @@ static inline int bt_spmm_grid(int nc) { return BT_XCD ? 8 * ((nc + 7) / 8) : nc
 #endif
 #define BT_SR 64    // rows per LDS-staged sub-chunk of the SpMM
 #define BT_CAP 768  // entries staged per sub-chunk (9 KiB with the row pointers)
+// At least 7 waves per SIMD: the kernel waits on gather latency, and the
+// bound holds it at 72 VGPRs (73 unbounded, 6 waves) with no spills; 8 waves
+// would spill.  Measured on the 256-graph batch: +2% f(A)x/s.
 template <int KP>
-__global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm(const int *__restrict__ rowptr, const int *__restrict__ col,
+__global__ __launch_bounds__(PSFUN_BS, 7) void k_bt_spmm(const int *__restrict__ rowptr, const int *__restrict__ col,
                                                       const double *__restrict__ val, double opscale,
                                                       const int *__restrict__ chunk, const int *__restrict__ gchunk,
                                                       int K, int j, const double *__restrict__ src,
