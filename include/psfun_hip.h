@@ -356,6 +356,21 @@ int psfun_krylov_resolvents(psfun_ctx_t ctx, int64_t nrows, int K, int mmax, con
  * of psfun_krylov_resolvents; host weights w (nshift <= 32). */
 int psfun_sddmm_cols(psfun_ctx_t ctx, int n, const int32_t *rowptr, const int32_t *colind, int K, int nshift,
                      const double *weights, const double *U, const double *V, double scale, double *grad);
+/* One-call vector-Jacobian product of Y = exp(-tA) X for a symmetric A held
+ * as the single graph of a batch in natural order (no reference counterpart:
+ * the differentiable form of the f(A)x path for C and Fortran callers).  For
+ * dL/dY = G (device, nrows x K row-major like X and Y, K <= 64):
+ *   gX    (device, nrows x K)  = exp(-tA) G                  (may be NULL)
+ *   gt    (host scalar)        = -<G, A Y>; needs Y          (may be NULL)
+ *   gvals (device, nnz)        = dL/d(stored nonzeros of A), overwritten; needs
+ *          X and t > 0; from the npoles-point modified Talbot resolvent form
+ *          with conjugate poles folded, resolvents taken from the Lanczos
+ *          bases of G and of X (the forward basis is recomputed) (may be NULL)
+ * Lanczos runs use eps, itmax and istop 1.  Allocates its bases and the
+ * resolvents ((itmax-1)*nrows*K + 4*S*nrows*K doubles, S <= 32 folded poles)
+ * and returns after the stream has finished. */
+int psfun_expv_vjp(psfun_ctx_t ctx, psfun_batch_t B, int K, double t, const double *X, const double *Y,
+                   const double *G, double eps, int itmax, int npoles, double *gX, double *gt, double *gvals);
 
 /* ---- shifted solves (psb_krylov extension) -------------------------------- */
 /* Preconditioners of the shifted CG (psb_dprec_type%init(ctxt, ptype, info),

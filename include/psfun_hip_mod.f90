@@ -364,5 +364,45 @@ module psfun_hip_mod
       real(c_double), intent(out) :: xi(*), c(*), eta, sign
       integer(c_int) :: st
     end function
+    ! batched Lanczos over block-diagonal graphs (host CSR, 0-based, copied)
+    function psfun_batch_create(ctx, ngraph, row_offsets, rowptr, colind, val, b) &
+         & bind(C, name='psfun_batch_create') result(st)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngraph
+      integer(c_int64_t), intent(in) :: row_offsets(*)
+      integer(c_int32_t), intent(in) :: rowptr(*), colind(*)
+      real(c_double), intent(in) :: val(*)
+      type(c_ptr), intent(out) :: b
+      integer(c_int) :: st
+    end function
+    function psfun_batch_destroy(b) bind(C, name='psfun_batch_destroy') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: st
+    end function
+    function psfun_batch_set_values(b, val) bind(C, name='psfun_batch_set_values') result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, val          ! val: device array of nnz doubles
+      integer(c_int) :: st
+    end function
+    function psfun_batch_lanczos(ctx, b, k, fscale, opscale, x, y, eps, itmax, istop, iters, errs) &
+         & bind(C, name='psfun_batch_lanczos') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, b, x, y
+      integer(c_int), value :: k, itmax, istop
+      real(c_double), value :: fscale, opscale, eps
+      type(c_ptr), value :: iters, errs     ! host arrays of ngraph*k, or c_null_ptr
+      integer(c_int) :: st
+    end function
+    ! vector-Jacobian product of Y = exp(-tA) X (one symmetric matrix in b)
+    function psfun_expv_vjp(ctx, b, k, t, x, y, g, eps, itmax, npoles, gx, gt, gvals) &
+         & bind(C, name='psfun_expv_vjp') result(st)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, b, x, y, g, gx, gt, gvals   ! gt: host scalar or c_null_ptr
+      integer(c_int), value :: k, itmax, npoles
+      real(c_double), value :: t, eps
+      integer(c_int) :: st
+    end function
   end interface
 end module psfun_hip_mod

@@ -107,6 +107,7 @@ SIGNATURES = {
     "psfun_batch_lanczos_basis": (_i, [_vp, _vp, _i, _d, _d, _vp, _vp, _d, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "psfun_krylov_resolvents": (_i, [_vp, _i64, _i, _i, _vp, _vp, _vp, _i, _d, _vp, _vp, _i, _vp, _vp]),
     "psfun_sddmm_cols": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _d, _vp]),
+    "psfun_expv_vjp": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp, _d, _i, _i, _vp, _pd, _vp]),
     "psfun_cg_shifted": (_i, [_vp, _vp, _i, _vp, _d, _d, _vp, _d, _i, _i, _pi, _pd, _pd]),
     "psfun_set_pade_degree": (_i, [_vp, _i]),
     "psfun_prec_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),

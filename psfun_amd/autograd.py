@@ -152,6 +152,25 @@ class ExpmvOperator:
         _lib.check(st, "psfun_sddmm_cols")
         return grad
 
+    def vjp(self, X, Y, G, t, need_x=True, need_t=True, need_a=True):
+        """(dL/dX, dL/dt, dL/dvals) for dL/dY = G in one C-ABI call
+        (psfun_expv_vjp: the backward below with the forward basis
+        recomputed); entries not asked for are None."""
+        K = 1 if X.dim() == 1 else int(X.shape[1])
+        X, Y, G = X.contiguous(), Y.contiguous(), G.contiguous()
+        gX = torch.empty_like(G) if need_x else None
+        gv = torch.empty(self.nnz, dtype=torch.float64, device=G.device) if need_a else None
+        gt = C.c_double(0.0)
+        self.ctxt.bind_stream()
+        st = _lib.lib().psfun_expv_vjp(self.ctxt.handle, self.batch.handle, K, float(t), C.c_void_p(X.data_ptr()),
+                                       C.c_void_p(Y.data_ptr()), C.c_void_p(G.data_ptr()), float(self.eps),
+                                       int(self.itmax), int(self.quad_poles),
+                                       C.c_void_p(gX.data_ptr()) if need_x else None,
+                                       C.byref(gt) if need_t else None,
+                                       C.c_void_p(gv.data_ptr()) if need_a else None)
+        _lib.check(st, "psfun_expv_vjp")
+        return gX, (gt.value if need_t else None), gv
+
     # ---- exact-solve variant (cross-check) --------------------------------
     def grad_values_mscg(self, X, G, t):
         S, sh, ww = self._rule(t)

@@ -88,3 +88,50 @@ def test_basis_memory_decision(gpu_ctx):
     K_huge = int(free // op.basis_bytes(1)) + 1
     assert not op.keep_basis(K_huge, "cuda:0")
 
+
+
+def test_one_call_vjp_matches_autograd_and_oracle(gpu_ctx):
+    """psfun_expv_vjp (C-ABI, forward basis recomputed) against the autograd
+    backward that recomputes its basis: the same kernels in the same order, so
+    dL/dX and dL/dvals are bitwise equal; dL/dt differs only in the order of
+    its dot product.  Both against the dense Frechet fixture."""
+    g, op, vals, X, t, G = _setup("krylov", False)
+    Y = expmv(vals, X, t, op)
+    (G * Y).sum().backward()
+    with torch.no_grad():
+        op.set_values(vals)
+        gX, gt, gv = op.vjp(X.detach(), Y.detach(), G, float(t))
+    assert torch.equal(gX, X.grad)
+    assert torch.equal(gv, vals.grad)
+    assert abs(gt - t.grad.item()) <= 1e-13 * abs(t.grad.item())
+    assert _rel(gX.cpu().numpy(), g["grad_X"]) <= 1e-10
+    assert _rel(gv.cpu().numpy(), g["grad_val"]) <= 1e-9
+    assert abs(gt - float(g["grad_t"])) <= 1e-10 * abs(float(g["grad_t"]))
+    # partial requests: each output alone equals the full call's
+    with torch.no_grad():
+        gX2, gt2, gv2 = op.vjp(X.detach(), Y.detach(), G, float(t), need_t=False, need_a=False)
+        assert gt2 is None and gv2 is None and torch.equal(gX2, gX)
+        gX3, gt3, gv3 = op.vjp(X.detach(), Y.detach(), G, float(t), need_x=False, need_t=False)
+        assert gX3 is None and gt3 is None and torch.equal(gv3, gv)
+
+
+def test_one_call_vjp_argument_errors(gpu_ctx):
+    import ctypes as C
+    from psfun_amd import _lib
+    g, op, vals, X, t, G = _setup("krylov", False)
+    L = _lib.lib()
+    x = X.detach().contiguous()
+    gv = torch.empty(op.nnz, dtype=torch.float64, device=x.device)
+    gt = C.c_double(0.0)
+    p = lambda a: C.c_void_p(a.data_ptr())
+    K = int(x.shape[1])
+    h, b = op.ctxt.handle, op.batch.handle
+    # dL/dt without Y, dL/dvals without X or with t <= 0, K out of range
+    assert L.psfun_expv_vjp(h, b, K, 0.1, p(x), None, p(G), 1e-12, 200, 24, None, C.byref(gt), None) == \
+        _lib.PSFUN_ERR_INVALID_ARG
+    assert L.psfun_expv_vjp(h, b, K, 0.1, None, p(x), p(G), 1e-12, 200, 24, None, None, p(gv)) == \
+        _lib.PSFUN_ERR_INVALID_ARG
+    assert L.psfun_expv_vjp(h, b, K, 0.0, p(x), p(x), p(G), 1e-12, 200, 24, None, None, p(gv)) == \
+        _lib.PSFUN_ERR_INVALID_ARG
+    assert L.psfun_expv_vjp(h, b, 65, 0.1, p(x), p(x), p(G), 1e-12, 200, 24, None, None, p(gv)) == \
+        _lib.PSFUN_ERR_INVALID_ARG
