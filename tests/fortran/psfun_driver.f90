@@ -7,8 +7,12 @@
 ! and one shifted solve as in examples/shiftedkrylov/shifted.inp:
 !   CG       ((-A + I) x = 1, DIAG, eps 1e-7, istop 2),
 ! and the LANCZOS deck again with fname USERF/SYM and a Fortran user function
-! f(x) = exp(x) handed over with c_funloc (USERF).
-! Prints "<METHOD> <iter> <err>" followed by the n entries of the result.
+! f(x) = exp(x) handed over with c_funloc (USERF).  Last, the batched and
+! differentiable forms on B = -A (SPD) with K = 2 columns, t = 0.1:
+!   BATCH    Y = exp(-tB) X                        (n x 2, row-major)
+!   VJPGX    dL/dX = exp(-tB) G for L = <G, Y>     (err field: dL/dt)
+!   VJPGV    dL/d(stored nonzeros of B)             (err field: dL/dt)
+! Prints "<METHOD> <iter> <err>" followed by the entries of the result.
 module driver_userf
   use iso_c_binding
   implicit none
@@ -89,12 +93,63 @@ program psfun_driver
   call check(psfun_vec_download(ctx, y, dy, int(n, c_int64_t)), 'psfun_vec_download')
   call emit('USERF', iter, err, y)
 
+  call vjp_section(ctx, k)
+
   call check(psfun_vec_free(ctx, dx), 'psfun_vec_free')
   call check(psfun_vec_free(ctx, dy), 'psfun_vec_free')
   call check(psfun_csr_destroy(a), 'psfun_csr_destroy')
   call check(psfun_ctx_destroy(ctx), 'psfun_ctx_destroy')
 
 contains
+
+  subroutine vjp_section(ctx, nnz)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nnz
+    integer, parameter :: kk = 2
+    integer(c_int32_t) :: rp0(n + 1), ci0(nnz)
+    integer(c_int64_t) :: offs(2)
+    real(c_double) :: nv(nnz), xh(n * kk), gh(n * kk), out(n * kk), gv(nnz)
+    real(c_double), target :: gt
+    type(c_ptr) :: b, bx, by, bg, bgx, bgv
+    integer :: i
+    integer(c_int64_t) :: nk
+    ! 0-based copy of the pattern, values of B = -A
+    rp0 = rowptr - 1
+    ci0 = colind(1:nnz) - 1
+    nv = -val(1:nnz)
+    offs = [0_c_int64_t, int(n, c_int64_t)]
+    do i = 1, n
+      xh(kk * (i - 1) + 1) = 1.0d0
+      xh(kk * (i - 1) + 2) = dble(i) / dble(n)
+      gh(kk * (i - 1) + 1) = sin(0.3d0 * i)
+      gh(kk * (i - 1) + 2) = cos(0.7d0 * i)
+    end do
+    nk = int(n * kk, c_int64_t)
+    call check(psfun_batch_create(ctx, 1_c_int, offs, rp0, ci0, nv, b), 'psfun_batch_create')
+    call check(psfun_vec_alloc(ctx, nk, bx), 'psfun_vec_alloc')
+    call check(psfun_vec_alloc(ctx, nk, by), 'psfun_vec_alloc')
+    call check(psfun_vec_alloc(ctx, nk, bg), 'psfun_vec_alloc')
+    call check(psfun_vec_alloc(ctx, nk, bgx), 'psfun_vec_alloc')
+    call check(psfun_vec_alloc(ctx, int(nnz, c_int64_t), bgv), 'psfun_vec_alloc')
+    call check(psfun_vec_upload(ctx, bx, xh, nk), 'psfun_vec_upload')
+    call check(psfun_vec_upload(ctx, bg, gh, nk), 'psfun_vec_upload')
+    call check(psfun_batch_lanczos(ctx, b, int(kk, c_int), 1.0d0, -0.1d0, bx, by, 1.0d-12, 50_c_int, 1_c_int, &
+         & c_null_ptr, c_null_ptr), 'psfun_batch_lanczos')
+    call check(psfun_expv_vjp(ctx, b, int(kk, c_int), 0.1d0, bx, by, bg, 1.0d-12, 50_c_int, 24_c_int, bgx, &
+         & c_loc(gt), bgv), 'psfun_expv_vjp')
+    call check(psfun_vec_download(ctx, out, by, nk), 'psfun_vec_download')
+    call emit('BATCH', 0_c_int, 0.0d0, out)
+    call check(psfun_vec_download(ctx, out, bgx, nk), 'psfun_vec_download')
+    call emit('VJPGX', 0_c_int, gt, out)
+    call check(psfun_vec_download(ctx, gv, bgv, int(nnz, c_int64_t)), 'psfun_vec_download')
+    call emit('VJPGV', 0_c_int, gt, gv)
+    call check(psfun_vec_free(ctx, bx), 'psfun_vec_free')
+    call check(psfun_vec_free(ctx, by), 'psfun_vec_free')
+    call check(psfun_vec_free(ctx, bg), 'psfun_vec_free')
+    call check(psfun_vec_free(ctx, bgx), 'psfun_vec_free')
+    call check(psfun_vec_free(ctx, bgv), 'psfun_vec_free')
+    call check(psfun_batch_destroy(b), 'psfun_batch_destroy')
+  end subroutine
 
   subroutine check(status, what)
     integer(c_int), intent(in) :: status

@@ -1,8 +1,9 @@
 """The iso_c_binding boundary from Fortran: tests/fortran/psfun_driver (built
 by tests/fortran/build.py against include/psfun_hip_mod.f90) runs the
 reference example decks lanczos.inp / arnoldi.inp / shifted.inp on the 50x50
-example matrix, and the Lanczos deck with a Fortran USERF function; its
-results are checked against the oracle and the fixture."""
+example matrix, the Lanczos deck with a Fortran USERF function, and the
+batched exp(-tB)X with its one-call vector-Jacobian product; its results are
+checked against the oracle, the fixture and dense scipy (expm, expm_frechet)."""
 import os
 import subprocess
 
@@ -53,3 +54,29 @@ def test_fortran_driver_matches_oracle(gpu_ctx):
     x = np.array(got["CG"]["y"])
     assert np.linalg.norm(x - g["x_shift"]) <= 1e-6 * np.linalg.norm(g["x_shift"])
     assert got["CG"]["err"] <= 1e-7
+    # batched exp(-tB)X and its vector-Jacobian product for B = -A (SPD),
+    # against dense scipy: expm, and expm_frechet per stored nonzero
+    import scipy.linalg as sla
+    n, t = 50, 0.1
+    rp, ci, va = A
+    B = np.zeros((n, n))
+    for i in range(n):
+        for q in range(rp[i], rp[i + 1]):
+            B[i, ci[q]] += -va[q]
+    i1 = np.arange(1, n + 1)
+    X = np.stack([np.ones(n), i1 / n], 1)
+    G = np.stack([np.sin(0.3 * i1), np.cos(0.7 * i1)], 1)
+    E = sla.expm(-t * B)
+    Y = E @ X
+    assert np.linalg.norm(np.array(got["BATCH"]["y"]) - Y.reshape(-1)) <= 1e-10 * np.linalg.norm(Y)
+    gX = E @ G
+    assert np.linalg.norm(np.array(got["VJPGX"]["y"]) - gX.reshape(-1)) <= 1e-10 * np.linalg.norm(gX)
+    gt = -np.sum(G * (B @ Y))
+    assert abs(got["VJPGX"]["err"] - gt) <= 1e-10 * abs(gt)
+    gv = np.empty(len(va))
+    for i in range(n):
+        for q in range(rp[i], rp[i + 1]):
+            Eq = np.zeros((n, n))
+            Eq[i, ci[q]] = 1.0
+            gv[q] = np.sum(G * (sla.expm_frechet(-t * B, -t * Eq, compute_expm=False) @ X))
+    assert np.linalg.norm(np.array(got["VJPGV"]["y"]) - gv) <= 1e-8 * np.linalg.norm(gv)
