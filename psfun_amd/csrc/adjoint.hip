@@ -166,6 +166,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nro
   }
 }
 
+#define SD_QMAX 8  // row length up to which the SDDMM keeps a row's sums in registers
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_sddmm_cols(int n, const int *__restrict__ rowptr,
                                                          const int *__restrict__ col, int K, int nshift,
@@ -184,6 +185,40 @@ __global__ __launch_bounds__(PSFUN_BS) void k_sddmm_cols(int n, const int *__res
   // rows are handled group-by-group; each KP-lane group owns one row at a time
   for (int64_t i = (int64_t)blockIdx.x * GPB + grp; i < n; i += (int64_t)gridDim.x * GPB) {
     const int q0 = rowptr[i], q1 = rowptr[i + 1];
+    if (q1 - q0 <= SD_QMAX) {
+      // rows of up to SD_QMAX entries: U_s[i,:] is loaded once per shift and
+      // serves all of the row's entries; each entry's sum still runs over s
+      // in order, as below
+      int kq[SD_QMAX];
+      double acc[SD_QMAX];
+#pragma unroll
+      for (int m = 0; m < SD_QMAX; ++m) {
+        kq[m] = (q0 + m < q1) ? col[q0 + m] : 0;
+        acc[m] = 0.0;
+      }
+      if (lane < K) {
+        for (int s = 0; s < nshift; ++s) {
+          const double2 a = U[(int64_t)s * tot + i * K + lane];
+          const double wr = sw[s][0], wi = sw[s][1];
+#pragma unroll
+          for (int m = 0; m < SD_QMAX; ++m) {
+            if (q0 + m < q1) {
+              const double2 b = V[(int64_t)s * tot + (int64_t)kq[m] * K + lane];
+              const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+              acc[m] += wr * pr - wi * pi;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < SD_QMAX; ++m) {
+        double v = acc[m];
+#pragma unroll
+        for (int off = KP / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, KP);
+        if (lane == 0 && q0 + m < q1) grad[q0 + m] = fma(scale, v, grad[q0 + m]);
+      }
+      continue;
+    }
     for (int q = q0; q < q1; ++q) {
       const int k = col[q];
       double acc = 0.0;
