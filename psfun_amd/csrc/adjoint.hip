@@ -116,9 +116,10 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents(int64_t nrows, i
 
 // Same sum with R elements per thread that share one column: for K | PSFUN_BS
 // the elements e, e + PSFUN_BS, ... of a tile of R*PSFUN_BS have the same
-// column, so each coefficient load serves R rows (the coefficient table is
-// read through the caches, and one load per V value made it the limit).
+// column, so each coefficient read serves R rows (one coefficient load per V
+// value made the coefficient table the limit).
 // Each element's sum runs over l in the same order as k_krylov_resolvents.
+#define RS_LDS 2048  // double2 coefficients staged per block (32 KiB)
 template <int S, int R>
 __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nrows, int K, int mmax, int nshift,
                                                                      const double *__restrict__ V,
@@ -127,29 +128,46 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nro
   const int64_t tot = nrows * K;
   const int64_t vs = tot;
   const double2 *cf = reinterpret_cast<const double2 *>(coef);
+  // The coefficients of lt consecutive l (all shifts, all K columns) are
+  // staged in LDS by the whole block, so the four waves share one load of
+  // them instead of each lane fetching its column's through the L1 at every l
+  // (those loads were four fifths of the kernel's L1 accesses).
+  __shared__ double2 scoef[RS_LDS];
+  const int lt = RS_LDS / (S * K);  // l per staged chunk (>= 2: S*K <= 16*64)
+  const int col = (int)(threadIdx.x % K);
   for (int64_t t0 = (int64_t)blockIdx.x * (R * PSFUN_BS); t0 < tot; t0 += (int64_t)gridDim.x * (R * PSFUN_BS)) {
     const int64_t e0 = t0 + threadIdx.x;
-    const int col = (int)(e0 % K);
     double ar[R][S], ai[R][S];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int q = 0; q < S; ++q) ar[r][q] = ai[r][q] = 0.0;
-    for (int l = 0; l < mmax; ++l) {
-      double v[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int64_t e = e0 + (int64_t)r * PSFUN_BS;
-        v[r] = (e < tot) ? V[(int64_t)l * vs + e] : 0.0;
+    for (int l0 = 0; l0 < mmax; l0 += lt) {
+      const int nl = min(lt, mmax - l0);
+      __syncthreads();  // the previous chunk has been read
+      for (int idx = threadIdx.x; idx < nshift * nl * K; idx += PSFUN_BS) {
+        const int q = idx / (nl * K), rem = idx - q * (nl * K);
+        const int dl = rem / K, c = rem - dl * K;
+        scoef[(q * lt + dl) * K + c] = cf[((int64_t)q * mmax + l0 + dl) * K + c];
       }
+      __syncthreads();
+      for (int dl = 0; dl < nl; ++dl) {
+        const int l = l0 + dl;
+        double v[R];
 #pragma unroll
-      for (int q = 0; q < S; ++q) {
-        if (q < nshift) {
-          const double2 c = cf[((int64_t)q * mmax + l) * K + col];
+        for (int r = 0; r < R; ++r) {
+          const int64_t e = e0 + (int64_t)r * PSFUN_BS;
+          v[r] = (e < tot) ? V[(int64_t)l * vs + e] : 0.0;
+        }
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            ar[r][q] = fma(c.x, v[r], ar[r][q]);
-            ai[r][q] = fma(c.y, v[r], ai[r][q]);
+        for (int q = 0; q < S; ++q) {
+          if (q < nshift) {
+            const double2 c = scoef[(q * lt + dl) * K + col];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              ar[r][q] = fma(c.x, v[r], ar[r][q]);
+              ai[r][q] = fma(c.y, v[r], ai[r][q]);
+            }
           }
         }
       }
