@@ -1,8 +1,8 @@
 // The small f(T) step on the device: y = exp(s*T) v for a symmetric
 // tridiagonal T of order k <= PSFUN_FT_KMAX, evaluated by ONE wavefront.
 //
-// Method: Chebyshev expansion of exp on the Gershgorin interval [lo, hi] of
-// s*T, exp(c + r x) = e^{c+r} [b_0 + 2 sum_m b_m T_m(x)] with
+// Method: Chebyshev expansion of exp on an interval [lo, hi] holding the
+// spectrum of s*T, exp(c + r x) = e^{c+r} [b_0 + 2 sum_m b_m T_m(x)] with
 // b_m = e^{-r} I_m(r) (modified Bessel functions).  The b_m come from Miller's
 // backward recurrence I_{m-1} = (2m/r) I_m + I_{m+1}, normalised by the
 // identity e^{-r}(I_0 + 2 sum_m I_m) = 1.  The three-term recurrence
@@ -12,6 +12,14 @@
 // estimate come out with an error proportional to their own size.  Widths
 // above 2*PSFUN_CHEB_RMAX are split into equal sub-steps
 // exp(sT) = (exp(sT/p))^p.
+//
+// lo is the Gershgorin lower bound; hi is the largest eigenvalue of s*T to
+// within PSFUN_CHEB_HI_TOL (multisection on Sturm counts), not the Gershgorin
+// upper bound.  The bracketed sum is e^{-hi} exp(sT) e1, so an upper end d
+// above the largest eigenvalue makes it e^{-d} times smaller than its terms
+// and costs e^{d} in relative rounding error: the Gershgorin bound of a
+// Lanczos T with large off-diagonals can exceed the spectrum by tens (30 on a
+// 1024^2 anisotropic diffusion operator scaled by t = 0.1, an error of 1e-3).
 //
 // All vectors live in LDS (one wavefront, rows i = lane, lane+64, ...), so the
 // routine costs few registers and can sit in a control block of a streaming
@@ -23,6 +31,7 @@
 #define PSFUN_CHEB_CAP 704      // coefficient table capacity (doubles; r + 12 sqrt(r) + 42 at r = RMAX)
 #define PSFUN_CHEB_RMAX 400.0   // largest half-width per sub-step
 #define PSFUN_FT_KMAX 512       // largest order of T
+#define PSFUN_CHEB_HI_TOL 0.25  // bracket width of the largest eigenvalue (e^0.25 of roundoff at most)
 
 // The coefficient table alone: what the register path (k <= 64) needs, so a
 // kernel that only takes that path holds 5.6 KB of LDS instead of 18 KB.
@@ -78,6 +87,52 @@ __device__ inline int cheb_exp_coefficients(double r, double *coef, int cap) {
   return M;
 }
 
+// Number of eigenvalues of a symmetric tridiagonal below x (Sturm count of
+// the LDL^T pivots of the matrix minus x I, a zero pivot replaced by -1e-300
+// as in LAPACK's dstebz).  diag(i) and off2(i) return the i-th diagonal entry
+// and the square of the off-diagonal entry (i-1, i) for i >= 1.
+template <typename Diag, typename Off2>
+__device__ inline int tridiag_count_below(int k, Diag diag, Off2 off2, double x) {
+  int cnt = 0;
+  double q = 1.0;
+#pragma unroll 1
+  for (int i = 0; i < k; ++i) {
+    double v = diag(i) - x;
+    if (i > 0) v -= off2(i) / q;
+    if (fabs(v) < 1e-300) v = -1e-300;
+    q = v;
+    cnt += (q < 0.0);
+  }
+  return cnt;
+}
+
+// The largest eigenvalue from above: the bracket [glo, ghi] (Gershgorin) is
+// cut at 64 interior points per round, one Sturm count per lane, and narrowed
+// to the cell where the count reaches k, until it is no wider than
+// PSFUN_CHEB_HI_TOL (at most 6 rounds).  Every lane of the wavefront computes
+// the same points and returns the same upper end, which is >= the largest
+// eigenvalue up to the count's rounding.
+template <typename Diag, typename Off2>
+__device__ __attribute__((noinline)) double wave_tridiag_upper(int k, Diag diag, Off2 off2, double glo, double ghi) {
+  const int lane = threadIdx.x & 63;
+  double lo = glo, hi = ghi;
+#pragma unroll 1
+  for (int round = 0; round < 6 && hi - lo > PSFUN_CHEB_HI_TOL; ++round) {
+    const double step = (hi - lo) / 65.0;
+    const double x = fma((double)(lane + 1), step, lo);
+    const unsigned long long all = __ballot(tridiag_count_below(k, diag, off2, x) == k);
+    if (all == 0ull) {
+      lo = fma(64.0, step, lo);
+    } else {
+      const int L = __ffsll((long long)all) - 1;
+      const double nlo = (L > 0) ? fma((double)L, step, lo) : lo;
+      hi = fma((double)(L + 1), step, lo);
+      lo = nlo;
+    }
+  }
+  return hi;
+}
+
 // y <- exp(s*T) y, y = W->y[0..k-1].  d (k) and e (k-1) are the diagonal and
 // off-diagonal of T in global memory.  Called by all 64 lanes of one wave.
 __device__ void wave_exp_tridiag(int k, const double *d, const double *e, double s, ChebLds *W) {
@@ -92,13 +147,25 @@ __device__ void wave_exp_tridiag(int k, const double *d, const double *e, double
   }
   glo = wave_min(glo);
   ghi = wave_max(ghi);
-  const double width = ghi - glo;
-  if (!(width > 1e-300)) {  // s*T is a multiple of the identity
+  if (!(ghi - glo > 1e-300)) {  // s*T is a multiple of the identity
     const double f = exp(ghi);
     for (int i = lane; i < k; i += 64) W->y[i] *= f;
     wave_lds_sync();
     return;
   }
+  {  // s*d and (s*e)^2 staged in t0/t1 (free until the recurrence) for the Sturm counts
+    double *sdv = W->t0, *se2 = W->t1;
+    for (int i = lane; i < k; i += 64) {
+      sdv[i] = s * d[i];
+      const double ei = (i > 0) ? s * e[i - 1] : 0.0;
+      se2[i] = ei * ei;
+    }
+    wave_lds_sync();
+    ghi = wave_tridiag_upper(
+        k, [&](int i) { return sdv[i]; }, [&](int i) { return se2[i]; }, glo, ghi);
+    wave_lds_sync();
+  }
+  const double width = ghi - glo;
   int nsub = (int)ceil(0.5 * width / PSFUN_CHEB_RMAX);
   if (nsub < 1) nsub = 1;
   const double ssub = 1.0 / nsub;
@@ -161,10 +228,18 @@ __device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const d
   const double di = s * dl;
   const double al = fabs(s * elo), ah = fabs(s * ehi);
   const double glo = wave_min(in ? di - al - ah : 1e308);
-  const double ghi = wave_max(in ? di + al + ah : -1e308);
+  double ghi = wave_max(in ? di + al + ah : -1e308);
   double y = (lane == 0) ? 1.0 : 0.0;
+  if (!(ghi - glo > 1e-300)) return y * exp(ghi);  // s*T is a multiple of the identity
+  // the Sturm counts read row i's entries from lane i (no memory traffic)
+  ghi = wave_tridiag_upper(
+      k, [&](int i) { return wave_read_f64(di, i); },
+      [&](int i) {
+        const double ei = wave_read_f64(al, i);
+        return ei * ei;
+      },
+      glo, ghi);
   const double width = ghi - glo;
-  if (!(width > 1e-300)) return y * exp(ghi);  // s*T is a multiple of the identity
   int nsub = (int)ceil(0.5 * width / PSFUN_CHEB_RMAX);
   if (nsub < 1) nsub = 1;
   const double ssub = 1.0 / nsub;
