@@ -1,0 +1,185 @@
+"""A second sweep, over the f(H) step and the rational paths:
+
+- the serial f(H) apply (serial/psfun_d_serial_mod.F90:172-523) for every EXP
+  variant on matrices whose norms reach 300, read under the variant's matrix
+  model (CHBSYM: symmetric from the upper triangle, tridiagonal or dense;
+  CHBHES: the Hessenberg part; the others: all entries), against scipy's
+  expm: relative 1e-11;
+- shift-and-invert Arnoldi (krylov/rational/psfun_d_sai.f90) on the
+  wide-spectrum anisotropic operator against the oracle and scipy (1e-8,
+  the inner CG tolerance bounds both);
+- the quadrature apply (Talbot exp, HHT sqrt) on the anisotropic operator and
+  a random SPD matrix against the oracle (1e-10) and scipy (the rule's own
+  accuracy);
+- multi-shift CG on a random SPD matrix with complex shifts against dense
+  solves."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import torch
+from scipy.sparse.linalg import expm_multiply
+
+from oracle import oracle as O
+from psfun_amd import matgen, psb
+from psfun_amd import quadrature as Q
+from psfun_amd.arnoldi import saiarnoldi_apply
+from psfun_amd.serial import psfun_d_serial
+from psfun_amd.shifted import mscg
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _vec(v):
+    return psb.psb_d_vect_type(torch.as_tensor(np.ascontiguousarray(v, np.float64), device="cuda:0"))
+
+
+def _random_spd(n, seed):
+    rng = np.random.default_rng(seed)
+    R = sp.random(n, n, density=6.0 / n, random_state=rng, format="csr")
+    S = R + R.T
+    d = np.asarray(abs(S).sum(1)).ravel() + rng.uniform(0.1, 5.0, n)
+    S = sp.csr_matrix(S + sp.diags(d))
+    S.sort_indices()
+    return S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64)
+
+
+def _lanczos_T(k, seed):
+    """A Lanczos tridiagonal of the 48^2 anisotropic operator scaled by -1
+    (||T|| ~ 800, Gershgorin bound far above its spectrum), order k."""
+    A = matgen.to_scipy(matgen.anisotropic_2d(48, seed=3))
+    x = matgen.normal(seed, A.shape[0])
+    V = [x / np.linalg.norm(x)]
+    al, be = [], []
+    vold, b = np.zeros_like(x), 0.0
+    for _ in range(k):
+        w = -(A @ V[-1])
+        a = w @ V[-1]
+        w = w - a * V[-1] - b * vold
+        for v in V:                       # full reorthogonalisation: a clean T
+            w -= (w @ v) * v
+        al.append(a)
+        b = np.linalg.norm(w)
+        be.append(b)
+        vold = V[-1]
+        V.append(w / b)
+    return np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+
+
+def _model(H, variant):
+    if variant == "CHBSYM":
+        U = np.triu(H)
+        return U + np.triu(H, 1).T
+    if variant == "CHBHES":
+        return np.triu(H, -1)
+    return H
+
+
+@pytest.mark.parametrize("variant", ["TAYLOR", "SASQ", "GENPADE", "CHBHES", "CHBGEN", "CHBSYM"])
+@pytest.mark.parametrize("k,norm", [(12, 1.0), (40, 50.0), (90, 300.0)])
+def test_serial_variants_large_norm(gpu_ctx, variant, k, norm):
+    rng = np.random.default_rng(k)
+    B = rng.normal(size=(k, k))
+    H = -(B @ B.T)                            # negative semidefinite (DSCHBV's domain)
+    H *= norm / np.abs(np.linalg.eigvalsh(H)).max()
+    if variant != "CHBSYM":
+        # a non-normal, dissipative matrix: symmetric part <= 0, so ||exp(H)|| <= 1
+        C = rng.normal(size=(k, k))
+        H = H + 0.25 * norm * (C - C.T) / np.linalg.norm(C - C.T, 2)
+        if variant == "CHBHES":
+            H = sla.hessenberg(H)             # orthogonal similarity keeps the numerical range
+            H = np.triu(H, -1)
+    x = rng.normal(size=k)
+    fun = psfun_d_serial()
+    fun.set("VARIANT", variant)
+    y = np.zeros(k)
+    assert fun.apply(H, y, x) == 0
+    ref = sla.expm(_model(H, variant)) @ x
+    assert _rel(y, ref) <= 1e-11, (variant, k, norm, _rel(y, ref))
+
+
+@pytest.mark.parametrize("k", [30, 64, 65, 120])
+def test_serial_chbsym_tridiagonal_wide_spectrum(gpu_ctx, k):
+    """The tridiagonal CHBSYM path (the Chebyshev step of the Lanczos loop) on
+    Lanczos tridiagonals whose Gershgorin bound lies far above the spectrum."""
+    T = _lanczos_T(k, 11)
+    x = np.zeros(k)
+    x[0] = 1.0
+    fun = psfun_d_serial()
+    fun.set("VARIANT", "CHBSYM")
+    y = np.zeros(k)
+    assert fun.apply(T, y, x) == 0
+    ref = sla.expm(T)[:, 0]
+    assert _rel(y, ref) <= 1e-12, _rel(y, ref)
+
+
+def test_sai_wide_spectrum(gpu_ctx):
+    A = matgen.anisotropic_2d(48, seed=3)
+    n = 48 * 48
+    x = matgen.normal(8, n)
+    t = 0.3
+    Amt = (A[0], A[1], -t * A[2])             # negative definite: inner CG on (-tA + tau I), tau <= 0
+    fun = psfun_d_serial()
+    fun.set("VARIANT", "GENPADE")
+    M = psb.psb_dspmat_type.from_csr(Amt)
+    y = _vec(np.zeros(n))
+    info, it, err, res = saiarnoldi_apply(fun, M, "CG", None, -1.0, psb.psb_desc_type(n), 6000, 0, 2, y, _vec(x),
+                                          1e-10, itmax=150, istop=1)
+    assert info == 0
+    ref = O.sai_arnoldi(Amt, x, 1e-10, 150, tau=-1.0, istop=1, in_itmax=6000, in_istop=2)
+    ys = expm_multiply(matgen.to_scipy(Amt), x)
+    assert _rel(ref["y"], ys) <= 1e-8
+    assert abs(it - ref["iter"]) <= 1
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-8
+    assert _rel(y.get_vect(), ys) <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["aniso", "spd"])
+def test_quadrature_exp(gpu_ctx, which):
+    if which == "aniso":
+        A, t, n = matgen.anisotropic_2d(48, seed=3), 0.3, 48 * 48
+    else:
+        A, t, n = _random_spd(2500, 5), 0.5, 2500
+    x = matgen.normal(9, n)
+    q = Q.psfun_z_quadrature()
+    assert q.computepoles(Q.talbot, None, 32, rparams=[t]) == 0
+    q.set(psb.psb_dspmat_type.from_csr(A))
+    y = _vec(np.zeros(n))
+    info, iters, resid = q.apply(_vec(x), y, eps=1e-13, itmax=20000)
+    assert info == 0
+    ref = O.quad_apply(A, q.sign * q.xi, q.c, q.eta * q.sign, 0, x, eps=1e-13, itmax=20000)
+    ys = expm_multiply(-t * matgen.to_scipy(A), x)
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-10
+    assert _rel(y.get_vect(), ys) <= 1e-9         # Talbot-32 on [0, inf)
+
+
+def test_quadrature_sqrt_spd(gpu_ctx):
+    A = _random_spd(900, 12)
+    S = matgen.to_scipy(A).toarray()
+    ev = np.linalg.eigvalsh(S)
+    x = matgen.normal(10, 900)
+    q = Q.psfun_z_quadrature()
+    assert q.computepoles(Q.hhtmethod1, np.sqrt, 32, rparams=[ev[0], ev[-1]]) == 0
+    q.set(psb.psb_dspmat_type.from_csr(A))
+    y = _vec(np.zeros(900))
+    info, iters, resid = q.apply(_vec(x), y, eps=1e-14, itmax=5000)
+    assert info == 0
+    ys = np.real(sla.sqrtm(S)) @ x
+    assert _rel(y.get_vect(), ys) <= 1e-9
+
+
+def test_mscg_spd_shifts(gpu_ctx):
+    A = _random_spd(1500, 13)
+    S = matgen.to_scipy(A).toarray()
+    b = matgen.normal(11, 1500)
+    shifts = np.array([0.0, 0.5 + 2.0j, -0.05 + 0.3j, 3.0 - 1.0j, 10.0 + 10.0j])
+    M = psb.psb_dspmat_type.from_csr(A)
+    X, iters, resid = mscg(M, _vec(b), shifts, eps=1e-12, itmax=5000)
+    Xh = X.cpu().numpy()
+    for s, sig in enumerate(shifts):
+        ref = np.linalg.solve(S + sig * np.eye(1500), b)
+        assert _rel(Xh[s], ref) <= 1e-9, (sig, _rel(Xh[s], ref))
