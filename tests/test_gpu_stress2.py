@@ -12,7 +12,9 @@
   a random SPD matrix against the oracle (1e-10) and scipy (the rule's own
   accuracy);
 - multi-shift CG on a random SPD matrix with complex shifts against dense
-  solves."""
+  solves, and the shifted PCG with every preconditioner;
+- USERF/SYM through a long Lanczos run and GENPADE at degrees 3 and 10 with
+  norm 300 against the oracle's restatements."""
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -183,3 +185,70 @@ def test_mscg_spd_shifts(gpu_ctx):
     for s, sig in enumerate(shifts):
         ref = np.linalg.solve(S + sig * np.eye(1500), b)
         assert _rel(Xh[s], ref) <= 1e-9, (sig, _rel(Xh[s], ref))
+
+
+def test_lanczos_userf_wide_spectrum(gpu_ctx):
+    """USERF/SYM (f = exp, a host function on the eigenvalues of T) through
+    106 Lanczos steps on the wide-spectrum operator."""
+    from psfun_amd.krylov import lanczos_apply
+    A = matgen.anisotropic_2d(48, seed=3)
+    n = 48 * 48
+    x = matgen.normal(14, n)
+    fun = psfun_d_serial()
+    fun.set("FNAME", "USERF")
+    fun.set("VARIANT", "SYM")
+    fun.set("FPOINTER", np.exp)
+    y = _vec(np.zeros(n))
+    info, it, err, res = lanczos_apply(fun, psb.psb_dspmat_type.from_csr(A), psb.psb_desc_type(n), y, _vec(x),
+                                       1e-10, itmax=250, opscale=-1.0)
+    assert info == 0
+    ref = O.lanczos(A, x, 1e-10, 250, 1, opscale=-1.0, f=np.exp)
+    ys = expm_multiply(-matgen.to_scipy(A), x)
+    assert abs(it - ref["iter"]) <= 2
+    assert _rel(y.get_vect(), ref["y"]) <= 1e-9
+    assert _rel(y.get_vect(), ys) <= 1e-9
+
+
+@pytest.mark.parametrize("p", [3, 10])
+def test_serial_genpade_degree_large_norm(gpu_ctx, p):
+    """DGPADM's degree-p approximant with scaling and squaring at norm 300
+    (ns ~ 10 squarings) against the oracle's restatement at the same degree."""
+    k = 60
+    rng = np.random.default_rng(90 + p)
+    B = rng.normal(size=(k, k))
+    H = -(B @ B.T)
+    H *= 300.0 / np.abs(np.linalg.eigvalsh(H)).max()
+    C = rng.normal(size=(k, k))
+    H = H + 75.0 * (C - C.T) / np.linalg.norm(C - C.T, 2)
+    x = rng.normal(size=k)
+    fun = psfun_d_serial()
+    fun.set("VARIANT", "GENPADE")
+    fun.set("PADE_DEGREE", p)
+    out = np.zeros(k)
+    assert fun.apply(H, out, x) == 0
+    rc, E = O.pade_expm(H, 1.0, p)
+    assert rc == 0 and _rel(out, E @ x) <= 1e-11
+    if p == 10:
+        assert _rel(out, sla.expm(H) @ x) <= 1e-11
+
+
+@pytest.mark.parametrize("prec", ["NONE", "DIAG", "BJAC"])
+@pytest.mark.parametrize("zeta", [0.0, 0.3])
+def test_shifted_cg_random_spd(gpu_ctx, prec, zeta):
+    """psb_krylov('CG') on (A + zeta I) x = b for a random sparse SPD matrix
+    with every preconditioner, against a dense solve."""
+    from psfun_amd.shifted import psb_dprec_type, psb_krylov
+    A = _random_spd(2000, 15)
+    n = 2000
+    S = matgen.to_scipy(A).toarray()
+    b = matgen.normal(16, n)
+    M = psb.psb_dspmat_type.from_csr(A)
+    desc = psb.psb_desc_type(n)
+    P = psb_dprec_type()
+    P.init(M.ctxt, prec)
+    P.build(M, desc)
+    xv = _vec(np.zeros(n))
+    info, it, err, _ = psb_krylov("CG", M, P, _vec(b), 1.0, zeta, xv, 1e-12, desc, itmax=3000, istop=2)
+    assert info == 0 and err <= 1e-12
+    ref = np.linalg.solve(S + zeta * np.eye(n), b)
+    assert _rel(xv.get_vect(), ref) <= 1e-9
