@@ -20,10 +20,13 @@
 // everywhere; per-shift state is double-buffered by the parity of k):
 //   k_ms_spmv(k):  b_{k-1} = rr_k/rr_{k-1}; p_k = r_k + b_{k-1} p_{k-1} (own rows
 //                  written, neighbours formed on the fly); q = A p_k; p.q partials.
-//   k_ms_seed(k):  a_k = rr_k / (p.q); r_{k+1} = r_k - a_k q; rr_{k+1} partials.
-//   k_ms_shift(k): every shift's zeta/alpha/beta; x^s += alpha^s p^s;
-//                  p^s = zeta r + beta^s p^s for shifts still running.  The
-//                  control block commits the new shift state and the stop flag.
+//   k_ms_step(k):  a_k = rr_k / (p.q); r_{k+1} = r_k - a_k q and rr_{k+1} partials;
+//                  the stop test of iteration k-1 for every shift (it needs
+//                  ||r_k||); for the shifts still running p^s_k = zeta_k r_k +
+//                  beta^s_{k-1} p^s_{k-1}, zeta_{k+1}, alpha^s_k and x^s +=
+//                  alpha^s_k p^s_k.  The control block commits the new shift
+//                  state and the stop flag.
+//   k_ms_finish:   after a loop that reached itmax, the last test's bookkeeping.
 // Complex vectors are stored shift-major, n (re, im) pairs per shift.
 #include "common.hpp"
 #include "kernels.hpp"
@@ -143,128 +146,152 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_spmv(int n, const int *__restri
   if (threadIdx.x == 0) p_pq[blockIdx.x] = pq;
 }
 
-__global__ __launch_bounds__(PSFUN_BS) void k_ms_seed(int n, double *__restrict__ r, const double *__restrict__ q,
-                                                      int k, const double *__restrict__ p_pq, int g_pq,
-                                                      double *__restrict__ p_rr, MsCtl *ctl) {
-  __shared__ double lds[PSFUN_BS / 64 + 1];
-  if (ctl->done) return;
-  const double pq = block_sum_partials<PSFUN_BS>(p_pq, g_pq, lds);
-  const double a = ctl->rr[k & 1] / pq;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->a[k & 1] = a;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
-    const double v = fma(-a, q[i], r[i]);
-    r[i] = v;
-    acc = fma(v, v, acc);
-  }
-  acc = block_sum<PSFUN_BS>(acc, lds);
-  if (threadIdx.x == 0) p_rr[blockIdx.x] = acc;
-}
-
-// x^s += alpha^s p^s;  p^s = zeta_{k+1} r_{k+1} + beta^s p^s.  At k = 0 the
-// starting values x^s = 0, p^s = b are used without being read.
-// ACC: the x^s are not kept; instead acc += sum_s Re(wq_s alpha^s p^s)
-// (form 0) or Im(...) (form 1), the quadrature's combination of the x^s
-// accumulated one iteration at a time (acc is not read at k = 0).  A shift
-// whose x^s stops changing stops contributing, as its x^s would.
+// One seed iteration after its SpMV, in one pass over the rows:
+//   a_k = rr_k / (p_k.q_k);  r_{k+1} = r_k - a_k q_k and the rr_{k+1} partials;
+//   for every shift still running (the test of iteration k-1 below):
+//     p^s_k = zeta_k r_k + beta^s_{k-1} p^s_{k-1}   (p^s_0 = b, never stored)
+//     zeta_{k+1}, alpha^s_k = a_k zeta_{k+1}/zeta_k;  x^s += alpha^s_k p^s_k
+// The test of iteration k-1 (|zeta_k| ||r_k|| <= eps ||b||, or the iteration
+// limit) needs ||r_k||, which the launch before produced, so each shift's
+// direction update comes one launch after the classical place (at the step
+// that uses it); every value is the same formula of the same inputs, so the
+// iterates do not change.  A shift that stops contributes its last x^s
+// update at the step it stops after, as its x^s would.
+// ACC: the x^s are not kept; acc += sum_s Re(wq_s alpha^s p^s) (form 0) or
+// Im(...) (form 1), the quadrature's combination accumulated one iteration
+// at a time (acc is not read at k = 0).
 template <bool ACC>
-__global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__restrict__ r,
-                                                       const double *__restrict__ b, double2 *__restrict__ X,
-                                                       int64_t ldx, double *__restrict__ acc, const double *__restrict__ wq,
-                                                       int form, double2 *__restrict__ P, int64_t ldp, int k, int G,
-                                                       const double *__restrict__ p_rr, int g_rr, MsCtl *ctl,
-                                                       double eps, int itmax, int *hf) {
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_step(int n, double *__restrict__ r, const double *__restrict__ q,
+                                                      const double *__restrict__ b, double2 *__restrict__ X,
+                                                      int64_t ldx, double *__restrict__ acc,
+                                                      const double *__restrict__ wq, int form,
+                                                      double2 *__restrict__ P, int64_t ldp, int k, int G,
+                                                      const double *__restrict__ p_pq, int g_pq,
+                                                      double *__restrict__ p_rr, MsCtl *ctl, double eps, int itmax,
+                                                      int *hf) {
   __shared__ double lds[PSFUN_BS / 64 + 1];
-  __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zn[MS_MAX][2], s_w[MS_MAX][2];
-  __shared__ int s_upd_x[MS_MAX], s_upd_p[MS_MAX];
-  if (ACC && threadIdx.x < MS_MAX) {
-    s_w[threadIdx.x][0] = threadIdx.x < ctl->nshift ? wq[2 * threadIdx.x] : 0.0;
-    s_w[threadIdx.x][1] = threadIdx.x < ctl->nshift ? wq[2 * threadIdx.x + 1] : 0.0;
+  __shared__ double s_alpha[MS_MAX][2], s_beta[MS_MAX][2], s_zc[MS_MAX][2], s_w[MS_MAX][2];
+  __shared__ int s_x[MS_MAX];
+  const int t = threadIdx.x;
+  if (ACC && t < MS_MAX) {
+    s_w[t][0] = t < ctl->nshift ? wq[2 * t] : 0.0;
+    s_w[t][1] = t < ctl->nshift ? wq[2 * t + 1] : 0.0;
   }
   // block-uniform: the control block of this same grid may set the flag
   if (__syncthreads_or(ctl->done)) return;
-  const double rr1 = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
+  const double pq = block_sum_partials<PSFUN_BS>(p_pq, g_pq, lds);
   const int nshift = ctl->nshift;
   const int par = k & 1;
-  const MsState *st = &ctl->st[par];
+  const MsState *st = &ctl->st[par];  // zc = zeta_k, zp = zeta_{k-1}; active: updated at k-1 (running at k = 0)
   const double rrk = ctl->rr[par];
-  const double ak = ctl->a[par], akm1 = ctl->a[par ^ 1], bkm1 = ctl->b[par ^ 1];
-  const double bk = rr1 / rrk;
-  const double rnorm = sqrt(rr1);
-  const int t = threadIdx.x;
+  const double ak = rrk / pq;
+  const double akm1 = ctl->a[par ^ 1];                        // a_{k-1} (a_{-1} = 1)
+  const double bkm1 = (k > 0) ? rrk / ctl->rr[par ^ 1] : 0.0;  // b_{k-1} (b_{-1} = 0)
+  const double bnorm = ctl->bnorm;
+  const double rnorm = sqrt(rrk);
   if (t < MS_MAX) {
-    int ux = 0, up = 0;
+    int xk = 0;
     cplx zn = {0.0, 0.0}, al = {0.0, 0.0}, be = {0.0, 0.0};
+    const cplx zc = {st->zc[t][0], st->zc[t][1]}, zp = {st->zp[t][0], st->zp[t][1]};
+    bool tested = false;
     double res = 0.0;
-    if (t < nshift && st->active[t] && !(ctl->bnorm > 0.0)) {
-      ux = 1;  // b = 0: x^s = 0 (alpha^s = 0 on the zero start), no iteration
-    } else if (t < nshift && st->active[t]) {
-      const cplx zc = {st->zc[t][0], st->zc[t][1]}, zp = {st->zp[t][0], st->zp[t][1]};
-      const cplx sg = {ctl->sig[t][0], ctl->sig[t][1]};
-      // denominator: a_k b_{k-1} (zp - zc) + zp a_{k-1} (1 + s a_k)
-      const cplx d1 = {ak * bkm1 * (zp.re - zc.re), ak * bkm1 * (zp.im - zc.im)};
-      const cplx one_sa = {1.0 + sg.re * ak, sg.im * ak};
-      const cplx d2 = cmul(zp, one_sa);
-      const cplx den = {d1.re + akm1 * d2.re, d1.im + akm1 * d2.im};
-      const cplx num0 = cmul(zc, zp);
-      const cplx num = {num0.re * akm1, num0.im * akm1};
-      zn = cdiv(num, den);
-      const cplx ratio = cdiv(zn, zc);
-      al = {ak * ratio.re, ak * ratio.im};
-      const cplx r2 = cmul(ratio, ratio);
-      be = {bk * r2.re, bk * r2.im};
-      res = hypot(zn.re, zn.im) * rnorm;
-      ux = 1;
-      up = (res <= eps * ctl->bnorm || k + 1 >= itmax) ? 0 : 1;
+    if (t < nshift && st->active[t]) {
+      if (!(bnorm > 0.0)) {
+        xk = (k == 0);  // b = 0: x^s = 0 (alpha^s = 0 on the zero start), no iteration
+      } else {
+        bool run = true;
+        if (k > 0) {  // the test of iteration k-1
+          res = hypot(zc.re, zc.im) * rnorm;
+          tested = true;
+          run = !(res <= eps * bnorm || k >= itmax);
+          if (run) {  // beta^s_{k-1} = b_{k-1} (zeta_k / zeta_{k-1})^2
+            const cplx ratio = cdiv(zc, zp);
+            const cplx r2 = cmul(ratio, ratio);
+            be = {bkm1 * r2.re, bkm1 * r2.im};
+          }
+        }
+        if (run) {
+          // zeta_{k+1}: denominator a_k b_{k-1} (zp - zc) + zp a_{k-1} (1 + s a_k)
+          const cplx sg = {ctl->sig[t][0], ctl->sig[t][1]};
+          const cplx d1 = {ak * bkm1 * (zp.re - zc.re), ak * bkm1 * (zp.im - zc.im)};
+          const cplx one_sa = {1.0 + sg.re * ak, sg.im * ak};
+          const cplx d2 = cmul(zp, one_sa);
+          const cplx den = {d1.re + akm1 * d2.re, d1.im + akm1 * d2.im};
+          const cplx num0 = cmul(zc, zp);
+          const cplx num = {num0.re * akm1, num0.im * akm1};
+          zn = cdiv(num, den);
+          const cplx ratio = cdiv(zn, zc);
+          al = {ak * ratio.re, ak * ratio.im};
+          xk = 1;
+        }
+      }
     }
     s_alpha[t][0] = al.re;
     s_alpha[t][1] = al.im;
     s_beta[t][0] = be.re;
     s_beta[t][1] = be.im;
-    s_zn[t][0] = zn.re;
-    s_zn[t][1] = zn.im;
-    s_upd_x[t] = ux;
-    s_upd_p[t] = up;
+    s_zc[t][0] = zc.re;
+    s_zc[t][1] = zc.im;
+    s_x[t] = xk;
     if (blockIdx.x == G) {  // control block: commit the state for iteration k+1
       MsState *nx = &ctl->st[par ^ 1];
-      if (ux) {
-        nx->zp[t][0] = st->zc[t][0];
-        nx->zp[t][1] = st->zc[t][1];
+      if (tested) {  // the bookkeeping of iteration k-1
+        ctl->iters[t] = k;
+        ctl->resid[t] = res / bnorm;
+      } else if (xk && !(bnorm > 0.0)) {
+        ctl->iters[t] = 0;
+        ctl->resid[t] = 0.0;
+      }
+      if (xk && bnorm > 0.0) {
+        nx->zp[t][0] = zc.re;
+        nx->zp[t][1] = zc.im;
         nx->zc[t][0] = zn.re;
         nx->zc[t][1] = zn.im;
-        ctl->iters[t] = (ctl->bnorm > 0.0) ? k + 1 : 0;
-        ctl->resid[t] = (ctl->bnorm > 0.0) ? res / ctl->bnorm : 0.0;
       } else {
-        nx->zp[t][0] = st->zp[t][0];
-        nx->zp[t][1] = st->zp[t][1];
-        nx->zc[t][0] = st->zc[t][0];
-        nx->zc[t][1] = st->zc[t][1];
+        nx->zp[t][0] = zp.re;
+        nx->zp[t][1] = zp.im;
+        nx->zc[t][0] = zc.re;
+        nx->zc[t][1] = zc.im;
       }
-      nx->active[t] = up;
+      nx->active[t] = (xk && bnorm > 0.0) ? 1 : 0;
     }
   }
   __syncthreads();
+  int any = 0;
+  for (int s = 0; s < nshift; ++s) any |= s_x[s];
   if (blockIdx.x == G) {
     if (t == 0) {
-      ctl->b[par] = bk;
-      int any = 0;
-      for (int s = 0; s < nshift; ++s) any |= s_upd_p[s];
+      ctl->a[par] = ak;
       host_flag_store(hf, k + 1);
-      if (!any || rr1 == 0.0) {
+      // nothing runs at k, or b = 0 (its only step is this one)
+      if (!any || !(bnorm > 0.0)) {
         ctl->done = MS_DONE;
         host_flag_store(hf + 1, MS_DONE);
       }
     }
     return;
   }
+  if (!any) return;
+  double arr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + t; i < n; i += (int64_t)G * PSFUN_BS) {
     const double ri = r[i];
-    const double bi = (k == 0) ? b[i] : 0.0;
+    const double rn = fma(-ak, q[i], ri);
+    r[i] = rn;
+    arr = fma(rn, rn, arr);
+    const double bi = (k <= 1) ? b[i] : 0.0;
     double a = (ACC && k > 0) ? acc[i] : 0.0;
     for (int s = 0; s < nshift; ++s) {
-      if (!s_upd_x[s]) continue;
-      double2 *pp = P + (int64_t)s * ldp + i;
-      const double2 pv = (k == 0) ? make_double2(bi, 0.0) : *pp;
+      if (!s_x[s]) continue;
+      double2 pv;
+      if (k == 0) {
+        pv = make_double2(bi, 0.0);
+      } else {
+        double2 *pp = P + (int64_t)s * ldp + i;
+        const double2 po = (k == 1) ? make_double2(bi, 0.0) : *pp;
+        const double br = s_beta[s][0], bim = s_beta[s][1];
+        pv = make_double2(fma(s_zc[s][0], ri, br * po.x - bim * po.y), fma(s_zc[s][1], ri, br * po.y + bim * po.x));
+        *pp = pv;
+      }
       const double ar = s_alpha[s][0], ai = s_alpha[s][1];
       const double dxr = ar * pv.x - ai * pv.y, dxi = ar * pv.y + ai * pv.x;  // alpha^s p^s
       if (ACC) {
@@ -274,12 +301,26 @@ __global__ __launch_bounds__(PSFUN_BS) void k_ms_shift(int n, const double *__re
         const double2 xv = (k == 0) ? make_double2(0.0, 0.0) : *xp;
         *xp = make_double2(xv.x + dxr, xv.y + dxi);
       }
-      if (s_upd_p[s]) {
-        const double br = s_beta[s][0], bim = s_beta[s][1];
-        *pp = make_double2(fma(s_zn[s][0], ri, br * pv.x - bim * pv.y), fma(s_zn[s][1], ri, br * pv.y + bim * pv.x));
-      }
     }
     if (ACC) acc[i] = a;
+  }
+  arr = block_sum<PSFUN_BS>(arr, lds);
+  if (t == 0) p_rr[blockIdx.x] = arr;
+}
+
+// The bookkeeping of the last iteration after the loop ran to itmax: the
+// shifts that were updated at k = itmax-1 get iters = itmax and their
+// residual |zeta_itmax| ||r_itmax|| / ||b||.
+__global__ __launch_bounds__(PSFUN_BS) void k_ms_finish(int k, const double *__restrict__ p_rr, int g_rr,
+                                                        MsCtl *ctl) {
+  __shared__ double lds[PSFUN_BS / 64 + 1];
+  if (ctl->done) return;
+  const double rr = block_sum_partials<PSFUN_BS>(p_rr, g_rr, lds);
+  const int t = threadIdx.x;
+  const MsState *st = &ctl->st[k & 1];
+  if (t < ctl->nshift && st->active[t]) {
+    ctl->iters[t] = k;
+    ctl->resid[t] = hypot(st->zc[t][0], st->zc[t][1]) * sqrt(rr) / ctl->bnorm;
   }
 }
 
@@ -341,18 +382,16 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
                                                  MRP, MCOL, MVAL, r, pold, pnew, q, k, Gs, Prr, Gu, Ppq, ctl));
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
-    hipLaunchKernelGGL(k_ms_seed, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, r, q, k, Ppq, Gs, Prr, ctl);
-    PSFUN_HIP_TRY(hipGetLastError());
-    // algorithmic bytes of the shift kernel: r read, p^s read+written per running shift, and
-    // x^s read+written per shift, or the one real accumulator read+written
+    // algorithmic bytes of the step kernel: r and q read, r written (the seed), p^s read+written
+    // per running shift, and x^s read+written per shift, or the one real accumulator read+written
     PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_MS_UPDATE,
-                               acc ? 8.0 * n + 32.0 * n * nshift + 16.0 * n : 8.0 * n + 64.0 * n * nshift, k));
+                               acc ? 24.0 * n + 32.0 * n * nshift + 16.0 * n : 24.0 * n + 64.0 * n * nshift, k));
     if (acc)
-      hipLaunchKernelGGL(k_ms_shift<true>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, acc, wq, form,
-                         P, w.ld, k, Gu, Prr, Gu, ctl, eps, itmax, c->d_flags);
+      hipLaunchKernelGGL(k_ms_step<true>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, q, b, X, ldx, acc, wq,
+                         form, P, w.ld, k, Gu, Ppq, Gs, Prr, ctl, eps, itmax, c->d_flags);
     else
-      hipLaunchKernelGGL(k_ms_shift<false>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, b, X, ldx, acc, wq, form,
-                         P, w.ld, k, Gu, Prr, Gu, ctl, eps, itmax, c->d_flags);
+      hipLaunchKernelGGL(k_ms_step<false>, dim3(Gu + 1), dim3(PSFUN_BS), 0, c->stream, n, r, q, b, X, ldx, acc, wq,
+                         form, P, w.ld, k, Gu, Ppq, Gs, Prr, ctl, eps, itmax, c->d_flags);
     PSFUN_HIP_TRY(hipGetLastError());
     PSFUN_TRY(psfun_prof_end(c));
     std::swap(pold, pnew);
@@ -363,6 +402,9 @@ static int ms_run(psfun_ctx *c, const psfun_csr *A, int nshift, const double *sh
       if (c->h_flags[1] != 0) break;
     }
   }
+  // a loop that ran to itmax still owes its last iteration's bookkeeping
+  hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(PSFUN_BS), 0, c->stream, itmax, Prr, Gu, ctl);
+  PSFUN_HIP_TRY(hipGetLastError());
   MsCtl *h = (MsCtl *)((char *)c->h_pinned + 256);
   PSFUN_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, c->stream));
   PSFUN_HIP_TRY(hipStreamSynchronize(c->stream));
