@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 #include "fexp.hpp"
 
+#include <algorithm>
 #include <map>
 #include <type_traits>
 #include <tuple>
@@ -58,6 +59,9 @@ struct psfun_batch {
   // original position of stored entry q'
   int order = PSFUN_ORDER_NATURAL;
   int *d_perm = nullptr, *d_nzperm = nullptr;
+  // per row: its place in its 64-row window (windows start at each graph's
+  // first row) when the window's rows are taken longest first
+  unsigned char *d_sched = nullptr;
   std::map<std::tuple<const void *, void *, const void *, int, int, int, double, double, double>, hipGraphExec_t>
       graphs;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -311,6 +315,175 @@ __global__ __launch_bounds__(PSFUN_BS, 7) void k_bt_spmm(const int *__restrict__
   }
 }
 
+// Wide-load SpMM for even K (two columns per lane, H = KP / 2 lanes per row).
+// Per 64-row window of a chunk the block stages in LDS the row pointers, the
+// window's row schedule (its rows longest first, built at batch creation) and,
+// for every entry, its value and the byte offset col * K * 8 of the row of src
+// it gathers.  A wavefront takes W = 64 / H rows of the schedule at a time and
+// walks their entries together up to the length of the group's first (longest)
+// row, four entries per step: one 16-byte buffer load per lane and entry, with
+// no per-entry branches.  An entry past its row's end loads from an offset
+// beyond the buffer, which the hardware returns as 0, and adds 0 * 0.  Each
+// column's sum runs over its row's entries in order with the same operations
+// as k_bt_spmm, so u is the same to the bit; the alpha partials are summed in
+// another order.  Against k_bt_spmm this issues about half the vector memory
+// instructions and a third of the VALU work per nonzero, which is what bounds
+// that kernel (its gathers wait on the texture-address path, not on HBM).
+#define BTW_SENT 0x80000000u  // byte offset past every buffer this kernel reads
+#define BTW_CAP 768           // entries staged per window
+typedef double btw_d2 __attribute__((ext_vector_type(2)));
+template <int KP>
+__global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm_wide(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                           const double *__restrict__ val, double opscale,
+                                                           const int *__restrict__ chunk,
+                                                           const unsigned char *__restrict__ sched, int K, int j,
+                                                           const double *__restrict__ src, unsigned int src_bytes,
+                                                           double *__restrict__ v_out, double *__restrict__ u_out,
+                                                           double *__restrict__ p_out, BtInst in, BtCtl *bctl, int nc) {
+  constexpr int H = KP / 2, W = 64 / H;
+  __shared__ double s_inv[KP];
+  __shared__ int s_act[KP];
+  __shared__ int s_any;
+  __shared__ double red[2 * PSFUN_BS];
+  __shared__ int s_rp[BT_SR + 1];
+  __shared__ unsigned char s_sc[BT_SR];
+  __shared__ unsigned int s_off[BTW_CAP + 64];
+  __shared__ double s_val[BTW_CAP + 64];
+  if (bctl->remaining == 0) return;
+  const int c = bt_block_chunk(nc);
+  if (c < 0) return;
+  const int r0 = chunk[3 * c], r1 = chunk[3 * c + 1], g = chunk[3 * c + 2];
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    const int k = threadIdx.x;
+    int act = 0;
+    double inv = 0.0;
+    if (k < K) {
+      const int I = g * K + k;
+      if (j == 1) {
+        const double sg = in.rs1[I];  // folded by k_bt_fold<KP, 1>
+        act = (sg != 0.0);
+        inv = act ? 1.0 / sg : 0.0;
+      } else {
+        act = (in.done[I] == BT_RUN);
+        if (act) inv = 1.0 / in.beta[(int64_t)I * in.ld + j - 1];
+      }
+    }
+    s_inv[k] = inv;
+    s_act[k] = act;
+    if (act) s_any = 1;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int h = lane % H, jr = lane / H;
+  const int K2 = K / 2;
+  const bool act = s_any && h < K2 && (s_act[2 * h] || s_act[2 * h + 1]);
+  const double inv0 = act ? s_inv[2 * h] : 0.0, inv1 = act ? s_inv[2 * h + 1] : 0.0;
+  const unsigned int kb = (unsigned int)K * 8u, h16 = 16u * (unsigned int)h;
+  const __amdgpu_buffer_rsrc_t xs = __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)src_bytes, 0x00020000);
+  const double2 *src2 = reinterpret_cast<const double2 *>(src);
+  double2 *u2 = reinterpret_cast<double2 *>(u_out);
+  double2 *v2 = reinterpret_cast<double2 *>(v_out);
+  double dot0 = 0.0, dot1 = 0.0;
+  for (int sr0 = r0; s_any && sr0 < r1; sr0 += BT_SR) {
+    const int nr = min(r1 - sr0, BT_SR);
+    __syncthreads();  // the previous window is done with the LDS
+    if (threadIdx.x <= nr) s_rp[threadIdx.x] = rowptr[sr0 + threadIdx.x];
+    if (threadIdx.x < nr) s_sc[threadIdx.x] = sched[sr0 + threadIdx.x];
+    __syncthreads();
+    const int e0 = s_rp[0], ne = s_rp[nr] - e0;
+    if (ne <= BTW_CAP) {
+      // every load of a thread in flight before its LDS stores
+      constexpr int NI = BTW_CAP / PSFUN_BS;
+      unsigned int to[NI];
+      double tv[NI];
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + it * PSFUN_BS;
+        to[it] = (i < ne) ? (unsigned int)col[e0 + i] * kb : BTW_SENT;
+        tv[it] = (i < ne) ? val[e0 + i] : 0.0;
+      }
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        s_off[threadIdx.x + it * PSFUN_BS] = to[it];
+        s_val[threadIdx.x + it * PSFUN_BS] = tv[it];
+      }
+      if (threadIdx.x < 64) {
+        s_off[BTW_CAP + threadIdx.x] = BTW_SENT;
+        s_val[BTW_CAP + threadIdx.x] = 0.0;
+      }
+      __syncthreads();
+      for (int g0 = wv * W; g0 < nr; g0 += 4 * W) {
+        const int slot = g0 + jr;
+        const bool rok = act && slot < nr;
+        const int lr = s_sc[slot < nr ? slot : g0];
+        const int es = s_rp[lr] - e0, len = rok ? s_rp[lr + 1] - s_rp[lr] : 0;
+        const int lf = s_sc[g0];
+        const int M = __builtin_amdgcn_readfirstlane(s_rp[lf + 1] - s_rp[lf]);  // longest row of the group
+        double sum0 = 0.0, sum1 = 0.0;
+        for (int m = 0; m < M; m += 4) {
+          btw_d2 x[4];
+          double a[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const bool ok = m + q < len;
+            const int ix = min(es + m + q, BTW_CAP + 63);
+            x[q] = __builtin_bit_cast(
+                btw_d2, __builtin_amdgcn_raw_buffer_load_b128(xs, ok ? s_off[ix] + h16 : BTW_SENT, 0, 0));
+            a[q] = ok ? s_val[ix] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            sum0 = fma(a[q], x[q].x * inv0, sum0);
+            sum1 = fma(a[q], x[q].y * inv1, sum1);
+          }
+        }
+        if (rok) {
+          const int64_t i2 = (int64_t)(sr0 + lr) * K2 + h;
+          const double2 xv = src2[i2];
+          const double2 uu = make_double2(opscale * sum0, opscale * sum1);
+          if (j == 1) v2[i2] = xv;  // basis slot 0 keeps x (v_1 = x / rs1)
+          u2[i2] = uu;
+          dot0 = fma(uu.x, xv.x * inv0, dot0);
+          dot1 = fma(uu.y, xv.y * inv1, dot1);
+        }
+      }
+    } else {
+      // more entries than the stage holds: the window's rows in order, their
+      // entries straight from memory
+      for (int lr = wv * W + jr; lr - jr < nr; lr += 4 * W) {
+        if (!act || lr >= nr) continue;
+        const int64_t i2 = (int64_t)(sr0 + lr) * K2 + h;
+        const int s = s_rp[lr], e = s_rp[lr + 1];
+        double sum0 = 0.0, sum1 = 0.0;
+        for (int q = s; q < e; ++q) {
+          const btw_d2 x = __builtin_bit_cast(
+              btw_d2, __builtin_amdgcn_raw_buffer_load_b128(xs, (unsigned int)col[q] * kb + h16, 0, 0));
+          const double a = val[q];
+          sum0 = fma(a, x.x * inv0, sum0);
+          sum1 = fma(a, x.y * inv1, sum1);
+        }
+        const double2 xv = src2[i2];
+        const double2 uu = make_double2(opscale * sum0, opscale * sum1);
+        if (j == 1) v2[i2] = xv;
+        u2[i2] = uu;
+        dot0 = fma(uu.x, xv.x * inv0, dot0);
+        dot1 = fma(uu.y, xv.y * inv1, dot1);
+      }
+    }
+  }
+  red[2 * threadIdx.x] = dot0;
+  red[2 * threadIdx.x + 1] = dot1;
+  __syncthreads();
+  if (threadIdx.x < KP) {
+    const int hh = threadIdx.x >> 1, odd = threadIdx.x & 1;
+    double s = 0.0;
+    for (int q = 0; q < PSFUN_BS / H; ++q) s += red[2 * (q * H + hh) + odd];
+    p_out[(int64_t)c * KP + threadIdx.x] = s;
+  }
+}
+
 template <int KP>
 __global__ __launch_bounds__(PSFUN_BS) void k_bt_update(const int *__restrict__ chunk, const int *__restrict__ gchunk,
                                                         int K, int j, const double *__restrict__ u,
@@ -426,12 +599,13 @@ __global__ __launch_bounds__(64) void k_bt_beta(const int *__restrict__ gchunk, 
   }
 }
 
-// SMALL: every order this launch sequence reaches is <= 64 (itmax <= 65), so
-// only the register path runs and the LDS holds the coefficient table alone
-// (more waves per CU beside the SpMM this kernel overlaps).
-template <bool SMALL>
+// ORD: the largest order of T this launch sequence reaches.  0: <= 64 (itmax
+// <= 65), 1: <= 128 (itmax <= 129): only the register paths run (one or two
+// rows of T per lane) and the LDS holds the coefficient table alone, so more
+// waves fit per CU beside the SpMM this kernel overlaps.  2: any order.
+template <int ORD>
 __global__ __launch_bounds__(64) void k_bt_est(int j, BtInst in, BtCtl *bctl, int nl, double eps, double fscale) {
-  __shared__ typename std::conditional<SMALL, ChebCoefLds, ChebLds>::type W;
+  __shared__ typename std::conditional<ORD < 2, ChebCoefLds, ChebLds>::type W;
   const int I = blockIdx.x;
   if (in.done[I] != BT_RUN) return;
   const int64_t base = (int64_t)I * in.ld;
@@ -439,9 +613,14 @@ __global__ __launch_bounds__(64) void k_bt_est(int j, BtInst in, BtCtl *bctl, in
   const int lane = threadIdx.x;
   int fin = BT_RUN;
   double nrm, last;
-  if constexpr (SMALL)
+  if constexpr (ORD == 0)
     wave_exp_tridiag_e1_small(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
-  else
+  else if constexpr (ORD == 1) {
+    if (j - 1 <= 64)
+      wave_exp_tridiag_e1_small(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+    else
+      wave_exp_tridiag_e1_mid(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
+  } else
     wave_exp_tridiag_e1(j - 1, in.alpha + base + 1, in.beta + base + 1, fscale, &W, in.yk + base, nrm, last);
   const double errnum = in.rs1[I] * beta * fabs(last);
   const double rsj = errnum / nrm;
@@ -673,6 +852,28 @@ static int bt_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, cons
     psfun_batch_destroy(B);
     return PSFUN_ERR_HIP;
   }
+  // row schedule of every 64-row window: local rows by decreasing length
+  // (ties in row order), read by the wide-load SpMM
+  {
+    std::vector<unsigned char> sc((size_t)std::max<int64_t>(nrows, 1));
+    std::vector<int> idx(BT_SR);
+    for (int g = 0; g < ngraph; ++g)
+      for (int64_t a = row_offsets[g]; a < row_offsets[g + 1]; a += BT_SR) {
+        const int nw = (int)std::min<int64_t>(BT_SR, row_offsets[g + 1] - a);
+        for (int i = 0; i < nw; ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.begin() + nw, [&](int x, int y) {
+          return rowptr[a + x + 1] - rowptr[a + x] > rowptr[a + y + 1] - rowptr[a + y];
+        });
+        for (int i = 0; i < nw; ++i) sc[(size_t)(a + i)] = (unsigned char)idx[i];
+      }
+    e = hipMalloc(&B->d_sched, sc.size());
+    if (e == hipSuccess) e = hipMemcpy(B->d_sched, sc.data(), sc.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      psfun_record_hip_error(e, "psfun_batch_create", __FILE__, __LINE__);
+      psfun_batch_destroy(B);
+      return PSFUN_ERR_HIP;
+    }
+  }
   const int st = bt_build_chunks(B);
   if (st != PSFUN_SUCCESS) {
     psfun_batch_destroy(B);
@@ -750,6 +951,7 @@ extern "C" int psfun_batch_destroy(psfun_batch_t B) {
   if (B->d_part) hipFree(B->d_part);
   if (B->d_perm) hipFree(B->d_perm);
   if (B->d_nzperm) hipFree(B->d_nzperm);
+  if (B->d_sched) hipFree(B->d_sched);
   delete B;
   return PSFUN_SUCCESS;
 }
@@ -775,6 +977,31 @@ extern "C" int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, in
   return PSFUN_SUCCESS;
 }
 
+// The SpMM step u = opscale * A (src / sigma) with the alpha partials: the
+// wide-load kernel where it applies (even K >= 4, chunks made of whole 64-row
+// windows, the gathered operand under 2 GiB so every byte offset and the
+// out-of-range sentinel fit 32 bits), else k_bt_spmm.
+static bool bt_wide(const psfun_batch *B, int K) {
+  return K >= 4 && K % 2 == 0 && B->rows_per_chunk % BT_SR == 0 && B->d_sched &&
+         (double)B->nrows * K * 8.0 < (double)BTW_SENT;
+}
+template <int KP>
+static void bt_spmm_launch(psfun_ctx *c, psfun_batch *B, double opscale, int K, int j, const double *src,
+                           double *v_out, double *u_out, const double *p_in, double *p_out, const BtInst &in,
+                           BtCtl *bctl) {
+  const int NC = B->nchunks;
+  if constexpr (KP >= 4) {
+    if (bt_wide(B, K)) {
+      hipLaunchKernelGGL((k_bt_spmm_wide<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr,
+                         B->colind, B->val, opscale, B->d_chunk, B->d_sched, K, j, src,
+                         (unsigned int)(B->nrows * K * 8), v_out, u_out, p_out, in, bctl, NC);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind,
+                     B->val, opscale, B->d_chunk, B->d_gchunk, K, j, src, v_out, u_out, p_in, p_out, in, bctl, NC);
+}
+
 template <int KP>
 static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double opscale, const double *X, double *Y,
                       double eps, int itmax, int istop, const BtInst &in, BtCtl *bctl, double *V, double *U0,
@@ -795,8 +1022,7 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
   // src read and u written (the first step also copies x into basis slot 0)
   const double spmm_bytes = 12.0 * B->nnz + 4.0 * (B->nrows + 1) + 16.0 * B->nrows * K;
   PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes + 8.0 * B->nrows * K));
-  hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr, B->colind,
-                     B->val, opscale, B->d_chunk, B->d_gchunk, K, 1, X, V, U0, P0, P1, in, bctl, NC);
+  bt_spmm_launch<KP>(c, B, opscale, K, 1, X, V, U0, P0, P1, in, bctl);
   PSFUN_TRY(psfun_prof_end(c));
   PSFUN_HIP_TRY(hipGetLastError());
   volatile int *hflag = (volatile int *)c->h_pinned;
@@ -810,27 +1036,30 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
     double *w_out = (j + 1 <= itmax - 1) ? V + (size_t)vs * j : u;
     hipLaunchKernelGGL((k_bt_update<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, B->d_gchunk, K, j, u,
                        w_out, vj, vjm1, P1, P0, in, bctl);
+    // the estimate of index j (forked in the previous step) is joined here,
+    // before the beta step that reads the stop state it sets
+    if (istop == 1 && j > 1) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     hipLaunchKernelGGL((k_bt_beta<KP>), dim3(ninst), dim3(64), 0, c->stream, B->d_gchunk, K, j + 1, P0, in, bctl,
                        itmax, istop);
     if (istop == 1) {  // the estimate of index j+1 beside the SpMM of that index
       PSFUN_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
       PSFUN_HIP_TRY(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
       if (itmax <= 65)
-        hipLaunchKernelGGL(k_bt_est<true>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
+        hipLaunchKernelGGL(k_bt_est<0>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
+                           fscale);
+      else if (itmax <= 129)
+        hipLaunchKernelGGL(k_bt_est<1>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
                            fscale);
       else
-        hipLaunchKernelGGL(k_bt_est<false>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
+        hipLaunchKernelGGL(k_bt_est<2>, dim3(ninst), dim3(64), 0, c->aux_stream, j + 1, in, bctl, itmax, eps,
                            fscale);
       PSFUN_HIP_TRY(hipEventRecord(c->ev_join, c->aux_stream));
     }
     if (j + 1 <= itmax - 1) {
       PSFUN_TRY(psfun_prof_begin(c, PSFUN_PROF_BATCH, spmm_bytes));
-      hipLaunchKernelGGL((k_bt_spmm<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr,
-                         B->colind, B->val, opscale, B->d_chunk, B->d_gchunk, K, j + 1, V + (size_t)vs * j,
-                         nullptr, (j & 1) ? U1 : U0, P0, P1, in, bctl, NC);
+      bt_spmm_launch<KP>(c, B, opscale, K, j + 1, V + (size_t)vs * j, nullptr, (j & 1) ? U1 : U0, P0, P1, in, bctl);
       PSFUN_TRY(psfun_prof_end(c));
     }
-    if (istop == 1) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     PSFUN_HIP_TRY(hipGetLastError());
     if (poll_host && j % 2 == 0) {
       const int slot = (j / 2) & 1;
@@ -844,6 +1073,7 @@ static int bt_enqueue(psfun_ctx *c, psfun_batch *B, int K, double fscale, double
       pending = slot;
     }
   }
+  if (istop == 1 && itmax >= 2) PSFUN_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));  // the last estimate
   hipLaunchKernelGGL(k_bt_final, dim3(ninst), dim3(64), 0, c->stream, in, fscale);
   hipLaunchKernelGGL((k_bt_assemble<KP>), dim3(NC), dim3(PSFUN_BS), 0, c->stream, B->d_chunk, K, V, vs, in,
                      (const int *)B->d_perm, Y);

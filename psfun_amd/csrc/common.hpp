@@ -283,6 +283,12 @@ __device__ __forceinline__ double wave_max_dpp(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// v of lane - 1 (0 in lane 0) and v of lane + 1 (0 in lane 63), by DPP
+// wavefront shifts: register moves, no LDS permute.  Every lane of the
+// wavefront must be active.
+__device__ __forceinline__ double wave_from_prev(double v) { return dpp_mov_f64<0x138, 0xf, 0xf>(0.0, v); }  // wave_shr:1
+__device__ __forceinline__ double wave_from_next(double v) { return dpp_mov_f64<0x130, 0xf, 0xf>(0.0, v); }  // wave_shl:1
+
 // Sum over the wavefront by the same DPP row moves (lanes with no source
 // add 0), read from lane 63: the result is wavefront-uniform.
 __device__ __forceinline__ double wave_sum_dpp(double v) {

@@ -215,8 +215,9 @@ __device__ void wave_exp_tridiag(int k, const double *d, const double *e, double
 
 // exp(s*T) e1 for k <= 64: row i = lane lives in registers (its entries of
 // the scaled S and of the Chebyshev vectors); the neighbours t[i-1], t[i+1]
-// come from cross-lane shuffles, so the recurrence needs no LDS traffic or
-// fences.  Same formulas, in the same order, as wave_exp_tridiag.  Returns
+// come from DPP wavefront shifts, so the recurrence needs no LDS traffic or
+// fences (a wavefront that runs beside LDS-heavy kernels on its CU is not
+// held up by their LDS queue).  Same formulas, in the same order, as wave_exp_tridiag.  Returns
 // y[lane] (0 for lane >= k).
 template <typename Lds>
 __device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const double *e, double s, Lds *W) {
@@ -258,23 +259,136 @@ __device__ inline double wave_exp_tridiag_e1_reg(int k, const double *d, const d
     double t0 = y;
     y = b0 * t0;
     double t1 = sdi * t0;
-    t1 = fma(slo, __shfl_up(t0, 1, 64), t1);
-    t1 = fma(shi, __shfl_down(t0, 1, 64), t1);
+    t1 = fma(slo, wave_from_prev(t0), t1);
+    t1 = fma(shi, wave_from_next(t0), t1);
     if (M >= 1) y = fma(b1, t1, y);
+    // the coefficients 64 at a time, one per lane; term m reads its own by
+    // readlane, so the recurrence waits on no LDS load
+    for (int mb = 2; mb <= M; mb += 64) {
+      const double cb = (mb + lane <= M) ? W->coef[mb + lane] : 0.0;
+      const int me = min(M, mb + 63);
 #pragma unroll 2
-    for (int m = 2; m <= M; ++m) {
-      const double bm = W->coef[m];
-      double v = sdi * t1;
-      v = fma(slo, __shfl_up(t1, 1, 64), v);
-      v = fma(shi, __shfl_down(t1, 1, 64), v);
-      const double t2 = fma(2.0, v, -t0);
-      y = fma(bm, t2, y);
-      t0 = t1;
-      t1 = t2;
+      for (int m = mb; m <= me; ++m) {
+        const double bm = wave_read_f64(cb, m - mb);
+        double v = sdi * t1;
+        v = fma(slo, wave_from_prev(t1), v);
+        v = fma(shi, wave_from_next(t1), v);
+        const double t2 = fma(2.0, v, -t0);
+        y = fma(bm, t2, y);
+        t0 = t1;
+        t1 = t2;
+      }
     }
     y *= escale;
   }
   return in ? y : 0.0;
+}
+
+// exp(s*T) e1 for 64 < k <= 128: lane l holds rows 2l and 2l+1 in registers.
+// Row 2l's upper neighbour and row 2l+1's lower one are in the same lane; the
+// other two come from one wavefront shift each, as many as the one-row
+// routine needs.  Each row is updated by the same formulas, in the same
+// order, as in wave_exp_tridiag.  Returns the two components in ya (row 2l)
+// and yb (row 2l+1), 0 past k.
+template <typename Lds>
+__device__ inline void wave_exp_tridiag_e1_reg2(int k, const double *d, const double *e, double s, Lds *W,
+                                                double &ya, double &yb) {
+  const int lane = threadIdx.x & 63;
+  const int ia = 2 * lane, ib = ia + 1;
+  const bool ina = ia < k, inb = ib < k;
+  const double da = ina ? d[ia] : 0.0, db = inb ? d[ib] : 0.0;
+  const double ea_lo = (ina && ia > 0) ? e[ia - 1] : 0.0;  // T(ia, ia-1)
+  const double eab = (ib < k) ? e[ia] : 0.0;               // T(ia, ib) = T(ib, ia)
+  const double eb_hi = (ib < k - 1) ? e[ib] : 0.0;         // T(ib, ib+1)
+  const double dia = s * da, dib = s * db;
+  const double ala = fabs(s * ea_lo), aab = fabs(s * eab), ahb = fabs(s * eb_hi);
+  const double glo = wave_min(fmin(ina ? dia - ala - aab : 1e308, inb ? dib - aab - ahb : 1e308));
+  double ghi = wave_max(fmax(ina ? dia + ala + aab : -1e308, inb ? dib + aab + ahb : -1e308));
+  ya = (lane == 0) ? 1.0 : 0.0;
+  yb = 0.0;
+  if (!(ghi - glo > 1e-300)) {  // s*T is a multiple of the identity
+    ya *= exp(ghi);
+    return;
+  }
+  // the Sturm counts read row i's entries from lane i / 2 (no memory traffic)
+  ghi = wave_tridiag_upper(
+      k, [&](int i) { return (i & 1) ? wave_read_f64(dib, i >> 1) : wave_read_f64(dia, i >> 1); },
+      [&](int i) {
+        const double ei = (i & 1) ? wave_read_f64(aab, i >> 1) : wave_read_f64(ala, i >> 1);
+        return ei * ei;
+      },
+      glo, ghi);
+  const double width = ghi - glo;
+  int nsub = (int)ceil(0.5 * width / PSFUN_CHEB_RMAX);
+  if (nsub < 1) nsub = 1;
+  const double ssub = 1.0 / nsub;
+  const double c = 0.5 * (glo + ghi) * ssub, r = 0.5 * width * ssub, hs = ghi * ssub;
+  if (lane == 0) W->M = cheb_exp_coefficients(r, W->coef, PSFUN_CHEB_CAP);
+  wave_lds_sync();
+  const int M = W->M;
+  const double rinv = 1.0 / r;
+  const double sd = s * ssub * rinv, cd = c * rinv;
+  const double sda = ina ? sd * da - cd : 0.0, sdb = inb ? sd * db - cd : 0.0;  // S_ii
+  const double sla = ina ? sd * ea_lo : 0.0;                                  // S(ia, ia-1)
+  const double sha = ina ? sd * eab : 0.0, slb = inb ? sd * eab : 0.0;        // S(ia, ib), S(ib, ia)
+  const double shb = inb ? sd * eb_hi : 0.0;                                  // S(ib, ib+1)
+  const double escale = exp(hs);
+  const double b0 = W->coef[0], b1 = (M >= 1) ? W->coef[1] : 0.0;
+  for (int sub = 0; sub < nsub; ++sub) {
+    double t0a = ya, t0b = yb;
+    ya = b0 * t0a;
+    yb = b0 * t0b;
+    double t1a = sda * t0a, t1b = sdb * t0b;
+    t1a = fma(sla, wave_from_prev(t0b), t1a);
+    t1a = fma(sha, t0b, t1a);
+    t1b = fma(slb, t0a, t1b);
+    t1b = fma(shb, wave_from_next(t0a), t1b);
+    if (M >= 1) {
+      ya = fma(b1, t1a, ya);
+      yb = fma(b1, t1b, yb);
+    }
+    for (int mb = 2; mb <= M; mb += 64) {  // coefficients as in the one-row routine
+      const double cb = (mb + lane <= M) ? W->coef[mb + lane] : 0.0;
+      const int me = min(M, mb + 63);
+#pragma unroll 2
+      for (int m = mb; m <= me; ++m) {
+        const double bm = wave_read_f64(cb, m - mb);
+        double va = sda * t1a, vb = sdb * t1b;
+        va = fma(sla, wave_from_prev(t1b), va);
+        va = fma(sha, t1b, va);
+        vb = fma(slb, t1a, vb);
+        vb = fma(shb, wave_from_next(t1a), vb);
+        const double t2a = fma(2.0, va, -t0a), t2b = fma(2.0, vb, -t0b);
+        ya = fma(bm, t2a, ya);
+        yb = fma(bm, t2b, yb);
+        t0a = t1a;
+        t0b = t1b;
+        t1a = t2a;
+        t1b = t2b;
+      }
+    }
+    ya *= escale;
+    yb *= escale;
+  }
+  if (!ina) ya = 0.0;
+  if (!inb) yb = 0.0;
+}
+
+// exp(s*T) e1 for 64 < k <= 128 through the two-row register routine; returns
+// ||y||_2 and y[k-1] in every lane and, if yout is not null, stores y[0..k-1].
+template <typename Lds>
+__device__ inline void wave_exp_tridiag_e1_mid(int k, const double *d, const double *e, double s, Lds *W,
+                                               double *yout, double &nrm, double &last) {
+  const int lane = threadIdx.x & 63;
+  double ya, yb;
+  wave_exp_tridiag_e1_reg2(k, d, e, s, W, ya, yb);
+  if (yout) {
+    if (2 * lane < k) yout[2 * lane] = ya;
+    if (2 * lane + 1 < k) yout[2 * lane + 1] = yb;
+  }
+  nrm = sqrt(wave_sum(fma(ya, ya, yb * yb)));
+  const double la = __shfl(ya, (k - 1) >> 1, 64), lb = __shfl(yb, (k - 1) >> 1, 64);
+  last = ((k - 1) & 1) ? lb : la;
 }
 
 // exp(s*T) e1 for k <= 64 through the register routine; returns ||y||_2 and
@@ -306,11 +420,13 @@ __device__ void wave_exp_tridiag_e1_lds(int k, const double *d, const double *e,
   last = W->y[k - 1];
 }
 
-// exp(s*T) e1 for any k <= PSFUN_FT_KMAX (registers up to 64, LDS beyond).
+// exp(s*T) e1 for any k <= PSFUN_FT_KMAX (registers up to 128, LDS beyond).
 __device__ inline void wave_exp_tridiag_e1(int k, const double *d, const double *e, double s, ChebLds *W,
                                            double *yout, double &nrm, double &last) {
   if (k <= 64)
     wave_exp_tridiag_e1_small(k, d, e, s, W, yout, nrm, last);
+  else if (k <= 128)
+    wave_exp_tridiag_e1_mid(k, d, e, s, W, yout, nrm, last);
   else
     wave_exp_tridiag_e1_lds(k, d, e, s, W, yout, nrm, last);
 }

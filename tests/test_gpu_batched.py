@@ -16,7 +16,7 @@ def _graphs(sizes, seed):
     return [matgen.rgg_laplacian(int(n), seed + i) for i, n in enumerate(sizes)]
 
 
-@pytest.mark.parametrize("K", [1, 3, 16, 64])
+@pytest.mark.parametrize("K", [1, 3, 6, 16, 64])
 def test_batched_matches_oracle_per_instance(gpu_ctx, K):
     sizes = [50, 300, 1000, 37, 2000, 5]
     mats = _graphs(sizes, 100)
@@ -65,11 +65,12 @@ def test_batched_graph_replay_and_tuning_are_bitwise_stable(gpu_ctx):
                 assert (torch.linalg.norm(Y0 - Y1) / torch.linalg.norm(Y0)).item() <= 1e-10
 
 
-def test_batched_zero_column_and_autotune(gpu_ctx):
+@pytest.mark.parametrize("K", [2, 4])
+def test_batched_zero_column_and_autotune(gpu_ctx, K):
     sizes = [500, 800]
     B = psfun_batch(_graphs(sizes, 11))
     N = sum(sizes)
-    X = matgen.normal(5, N * 2).reshape(N, 2)
+    X = matgen.normal(5, N * K).reshape(N, K)
     X[:500, 1] = 0.0
     Xd = torch.as_tensor(X, device="cuda:0")
     Y, iters, errs = B.expm(Xd, t=0.5)
@@ -78,6 +79,52 @@ def test_batched_zero_column_and_autotune(gpu_ctx):
     assert len(table) == 4 and best in [c for c, _ in table]
     Y2, _, _ = B.expm(Xd, t=0.5)
     assert (torch.linalg.norm(Y - Y2) / torch.linalg.norm(Y)).item() <= 1e-10
+
+
+def _star_path(n, hub, seed):
+    """A path of n nodes plus edges from node `hub` to every other node: one
+    row of n entries, the rest of 3 to 4 (windows with more entries than an
+    LDS stage holds, beside short ones)."""
+    rng = np.random.default_rng(seed)
+    i = list(range(n - 1)) + [hub] * (n - 1)
+    j = list(range(1, n)) + [q for q in range(n) if q != hub]
+    keep = [(a, b) for a, b in zip(i, j) if a != b]
+    a = np.array([p[0] for p in keep]); b = np.array([p[1] for p in keep])
+    key = np.unique(np.minimum(a, b) * n + np.maximum(a, b))
+    a, b = key // n, key % n
+    w = 0.5 + rng.random(len(a))
+    deg = np.bincount(a, weights=w, minlength=n) + np.bincount(b, weights=w, minlength=n)
+    rows = np.concatenate([np.arange(n), a, b]); cols = np.concatenate([np.arange(n), b, a])
+    vals = np.concatenate([deg, -w, -w])
+    import scipy.sparse as sp
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return (A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+
+
+@pytest.mark.parametrize("K", [4, 8])
+def test_batched_long_rows_match_oracle(gpu_ctx, K):
+    """Graphs whose hub row alone has more entries than a window's LDS stage
+    holds, beside graphs of short rows, on the wide-load SpMM (even K) with
+    whole and partial 64-row windows: every instance against the oracle."""
+    mats = [_star_path(3000, 5, 1), _graphs([700], 3)[0], _star_path(1500, 1400, 2), _graphs([90], 4)[0]]
+    sizes = [len(m[0]) - 1 for m in mats]
+    B = psfun_batch(mats)
+    N = sum(sizes)
+    X = matgen.normal(9, N * K).reshape(N, K)
+    t = 0.05
+    Xd = torch.as_tensor(X, device="cuda:0")
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for rpc in (128, 64):
+        B.set_tuning(rpc, False)
+        Y, iters, errs = B.expm(Xd, t=t, eps=1e-10, itmax=60)
+        Y = Y.cpu().numpy()
+        for g, A in enumerate(mats):
+            for k in range(K):
+                ref = O.lanczos(A, X[off[g]:off[g + 1], k], 1e-10, 60, 1, fscale=1.0, opscale=-t)
+                y = Y[off[g]:off[g + 1], k]
+                assert np.linalg.norm(y - ref["y"]) <= 1e-10 * np.linalg.norm(ref["y"]), (rpc, g, k)
+                assert abs(int(iters[g, k]) - ref["iter"]) <= 1
 
 
 def test_batched_rcm_order_matches_natural(gpu_ctx):
