@@ -472,25 +472,16 @@ def adjoint_section(args):
             "lanczos_iters_min_max": [int(iters.min()), int(iters.max())], "quad_poles": 16}
 
 
-def irregular_section(args):
-    """Lanczos exp(-A)x on a power-law graph Laplacian (Chung-Lu, n = 2M,
-    ~17M nonzeros, rows of 1 to ~150k entries): the case no sliced or
-    diagonal storage serves.  Each storage timed; the CSR row blocks (CSRS)
-    against the vector-CSR kernel at its best lanes-per-row."""
-    import ctypes as C
+def _lanczos_by_format(args, G, formats, tag):
+    """Time the plain Lanczos exp(A)x on the CSR matrix G in each storage of
+    `formats` ('L4'/'L8': vector CSR with that many lanes per row); returns
+    the per-format figures and the results."""
     import torch
     from psfun_amd import _lib, matgen, psb
     from psfun_amd.krylov import psfun_d_krylov
     from psfun_amd.serial import psfun_d_serial
 
-    n = 2_000_000
-    tg = time.perf_counter()
-    G = matgen.chung_lu_laplacian(n, 7)
-    G = (G[0], G[1], (-10.0 / float(np.max(np.abs(G[2])))) * G[2])   # spectrum of the operator in [-20, 0]
-    nnz = len(G[1])
-    lens = np.diff(G[0])
-    log(f"irregular: Chung-Lu graph n={n} nnz={nnz} row lengths {int(lens.min())}..{int(lens.max())} "
-        f"(median {int(np.median(lens))}) in {time.perf_counter() - tg:.1f}s")
+    n = len(G[0]) - 1
     ctx = psb.default_context(0)
     M = psb.psb_dspmat_type.from_csr(G, ctx)
     fun = psfun_d_serial()
@@ -502,7 +493,7 @@ def irregular_section(args):
     yv = psb.psb_d_vect_type(torch.zeros(n, dtype=torch.float64, device="cuda:0"))
     L = _lib.lib()
     out, ys = {}, {}
-    for f in ("CSRS", "L4", "L8", "SELL"):
+    for f in formats:
         try:
             if f.startswith("L"):
                 M.set_format("CSR")
@@ -530,16 +521,64 @@ def irregular_section(args):
         out[f] = {"f(A)x/s": round(1.0 / ta, 2), "ms_per_apply": round(ta * 1e3, 3), "iterations": it,
                   "spmv_us": round(us, 2), "spmv_GBs": round(by / (us * 1e-6) / 1e9, 1),
                   "spmv_frac": round(by / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
-        log(f"irregular: {f}: {1.0 / ta:.2f} f(A)x/s, SpMV {us:.1f} us ({out[f]['spmv_GBs']} GB/s), iter {it}")
-    ref = ys["CSRS"]
+        log(f"irregular: {tag}: {f}: {1.0 / ta:.2f} f(A)x/s, SpMV {us:.1f} us ({out[f]['spmv_GBs']} GB/s), iter {it}")
+    ref = ys[formats[0]]
+    diff = float(max(np.linalg.norm(y - ref) / np.linalg.norm(ref) for y in ys.values()))
+    return out, diff
+
+
+def irregular_section(args):
+    """Lanczos exp(A')x on two general (non-stencil) matrices, each storage
+    timed, the SpMV step charged the CSR count 12*nnz + 4(n+1) + 16n:
+    - a power-law graph Laplacian (Chung-Lu, n = 2M, ~17M nonzeros, rows of 1
+      to ~150k entries): the case no sliced or diagonal storage serves, its x
+      gathers at random columns;
+    - the 256 random-geometric-graph Laplacians of the batched section as one
+      block-diagonal CSR, each graph in reverse Cuthill-McKee order (n = 3.7M,
+      33M nonzeros, rows of ~9 entries): uneven rows with gather locality."""
+    from psfun_amd import matgen
+    from psfun_amd.batched import rcm_order
+
+    n = 2_000_000
+    tg = time.perf_counter()
+    G = matgen.chung_lu_laplacian(n, 7)
+    G = (G[0], G[1], (-10.0 / float(np.max(np.abs(G[2])))) * G[2])   # spectrum of the operator in [-20, 0]
+    nnz = len(G[1])
+    lens = np.diff(G[0])
+    log(f"irregular: Chung-Lu graph n={n} nnz={nnz} row lengths {int(lens.min())}..{int(lens.max())} "
+        f"(median {int(np.median(lens))}) in {time.perf_counter() - tg:.1f}s")
+    out, diff = _lanczos_by_format(args, G, ("CSRS", "L4", "L8", "SELL"), "power law")
     vec_best = min((v for k_, v in out.items() if k_.startswith("L")), key=lambda v: v["ms_per_apply"])
-    return {"workload": "Lanczos exp(A')x, A' = -(10/max|a_ij|) L, L the Laplacian of a Chung-Lu power-law graph "
-                        "(exponent 2.2, mean degree 8), n = 2M, eps 1e-10",
-            "n": n, "nnz": nnz, "row_len_max": int(lens.max()), "by_format": out,
-            "csrs_speedup_vs_vector_csr": round(vec_best["ms_per_apply"] / out["CSRS"]["ms_per_apply"], 2),
-            "max_rel_diff_between_formats": float(max(np.linalg.norm(y - ref) / np.linalg.norm(ref)
-                                                      for y in ys.values())),
-            "bytes_per_unit": "12*nnz + 4(n+1) + 16n per SpMV step"}
+    res = {"workload": "Lanczos exp(A')x, A' = -(10/max|a_ij|) L, L the Laplacian of a Chung-Lu power-law graph "
+                       "(exponent 2.2, mean degree 8), n = 2M, eps 1e-10",
+           "n": n, "nnz": nnz, "row_len_max": int(lens.max()), "by_format": out,
+           "csrs_speedup_vs_vector_csr": round(vec_best["ms_per_apply"] / out["CSRS"]["ms_per_apply"], 2),
+           "max_rel_diff_between_formats": diff,
+           "bytes_per_unit": "12*nnz + 4(n+1) + 16n per SpMV step"}
+    # the RGG batch as one CSR, each graph renumbered by RCM
+    tg = time.perf_counter()
+    sizes = matgen.rgg_sizes(256, 3)
+    rps, cis, vas, off = [np.zeros(1, np.int64)], [], [], 0
+    for i, nv in enumerate(sizes):
+        rp, ci, va = matgen.rgg_laplacian(int(nv), 1000 + i)
+        p = rcm_order(rp, ci)
+        ip = np.empty_like(p)
+        ip[p] = np.arange(len(p))
+        A = matgen.to_scipy((rp, ci, va))[p][:, p].tocsr()
+        A.sort_indices()
+        rps.append(A.indptr[1:].astype(np.int64) + rps[-1][-1])
+        cis.append(A.indices.astype(np.int64) + off)
+        vas.append(A.data)
+        off += int(nv)
+    R = (np.concatenate(rps).astype(np.int32), np.concatenate(cis).astype(np.int32), -0.5 * np.concatenate(vas))
+    lens = np.diff(R[0])
+    log(f"irregular: RGG batch as one CSR n={off} nnz={len(R[1])} in {time.perf_counter() - tg:.1f}s")
+    out2, diff2 = _lanczos_by_format(args, R, ("CSRS", "L4", "L8", "SELL"), "rgg")
+    res["rgg_one_csr"] = {"workload": "Lanczos exp(-0.5 L)x, L the 256 RGG Laplacians of the batched section as one "
+                                      "block-diagonal CSR (each graph in RCM order), eps 1e-10",
+                          "n": off, "nnz": int(len(R[1])), "row_len_max": int(lens.max()), "by_format": out2,
+                          "max_rel_diff_between_formats": diff2}
+    return res
 
 
 def quadrature_section(args):

@@ -155,7 +155,7 @@ struct psfun_csr {
 };
 #define PSFUN_SELL_C 64
 #ifndef PSFUN_CSRS_CAP
-#define PSFUN_CSRS_CAP 1024      // entries per block of short rows
+#define PSFUN_CSRS_CAP 2048      // entries per block of short rows
 #endif
 #ifndef PSFUN_CSRS_PER_CU
 #define PSFUN_CSRS_PER_CU 12     // CSR-stream blocks per CU in the Lanczos grid (12 measured best of 4..16)

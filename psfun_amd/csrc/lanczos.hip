@@ -199,15 +199,34 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *
     bool lead = false;
     if (kind == 0) {  // short rows [ba, bb)
       const int e0 = rowptr[ba], ne = rowptr[bb] - e0;
-      for (int t = threadIdx.x; t < ne; t += PSFUN_BS) prod[t] = val[e0 + t] * (src[col[e0 + t]] * inv);
-      __syncthreads();
       const int nr = bb - ba;
       int L = 1;
       while (L < 64 && nr * L * 2 <= PSFUN_BS) L *= 2;
       const int rr = threadIdx.x / L, ln = threadIdx.x % L;
+      // this thread's row bounds, and all of its entries' column and value
+      // loads, then all of its gathers, are in flight together (one round
+      // trip each instead of one per entry)
+      const int rs = (rr < nr) ? rowptr[ba + rr] : 0, re = (rr < nr) ? rowptr[ba + rr + 1] : 0;
+      constexpr int NI = PSFUN_CSRS_CAP / PSFUN_BS;
+      int cq[NI];
+      double aq[NI], xq[NI];
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const int t = threadIdx.x + it * PSFUN_BS;
+        cq[it] = (t < ne) ? col[e0 + t] : 0;
+        aq[it] = (t < ne) ? val[e0 + t] : 0.0;
+      }
+#pragma unroll
+      for (int it = 0; it < NI; ++it) xq[it] = (threadIdx.x + it * PSFUN_BS < ne) ? src[cq[it]] : 0.0;
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const int t = threadIdx.x + it * PSFUN_BS;
+        if (t < ne) prod[t] = aq[it] * (xq[it] * inv);
+      }
+      __syncthreads();
       if (rr < nr) {
         row = ba + rr;
-        const int s = rowptr[row] - e0, e = rowptr[row + 1] - e0;
+        const int s = rs - e0, e = re - e0;
         for (int q = s + ln; q < e; q += L) sum += prod[q];
       }
       for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
@@ -215,7 +234,23 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *
       __syncthreads();  // the next block of rows overwrites prod
     } else {  // one row, or a segment of one
       const int q0 = kind == 1 ? rowptr[ba] : bb, q1 = kind == 1 ? rowptr[ba + 1] : bc;
-      for (int q = q0 + (int)threadIdx.x; q < q1; q += PSFUN_BS) sum = fma(val[q], src[col[q]] * inv, sum);
+      // four strided entries per step, their loads in flight together; the
+      // thread's sum keeps its entry order
+      int q = q0 + (int)threadIdx.x;
+      for (; q + 3 * PSFUN_BS < q1; q += 4 * PSFUN_BS) {
+        int c4[4];
+        double a4[4], x4[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          c4[m] = col[q + m * PSFUN_BS];
+          a4[m] = val[q + m * PSFUN_BS];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) x4[m] = src[c4[m]];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) sum = fma(a4[m], x4[m] * inv, sum);
+      }
+      for (; q < q1; q += PSFUN_BS) sum = fma(val[q], src[col[q]] * inv, sum);
       sum = block_sum<PSFUN_BS>(sum, lds);
       if (kind == 1) {
         row = ba;
