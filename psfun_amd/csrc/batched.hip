@@ -338,9 +338,11 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm_wide(const int *__restrict
                                                            const int *__restrict__ chunk,
                                                            const unsigned char *__restrict__ sched, int K, int j,
                                                            const double *__restrict__ src, unsigned int src_bytes,
-                                                           double *__restrict__ v_out, double *__restrict__ u_out,
-                                                           double *__restrict__ p_out, BtInst in, BtCtl *bctl, int nc) {
+                                                           unsigned int nnz, double *__restrict__ v_out,
+                                                           double *__restrict__ u_out, double *__restrict__ p_out,
+                                                           BtInst in, BtCtl *bctl, int nc) {
   constexpr int H = KP / 2, W = 64 / H;
+  static_assert(2 * 2 * PSFUN_BS >= BTW_CAP + 1 && 4 * PSFUN_BS >= BTW_CAP + 3, "one staging pass per window");
   __shared__ double s_inv[KP];
   __shared__ int s_act[KP];
   __shared__ int s_any;
@@ -382,6 +384,10 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm_wide(const int *__restrict
   const double inv0 = act ? s_inv[2 * h] : 0.0, inv1 = act ? s_inv[2 * h + 1] : 0.0;
   const unsigned int kb = (unsigned int)K * 8u, h16 = 16u * (unsigned int)h;
   const __amdgpu_buffer_rsrc_t xs = __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)src_bytes, 0x00020000);
+  // the arrays are padded to whole 16-byte groups (bt_create)
+  const unsigned int nnz4 = (nnz + 3u) & ~3u;
+  const __amdgpu_buffer_rsrc_t cs = __builtin_amdgcn_make_buffer_rsrc((void *)col, (short)0, (int)(4u * nnz4), 0x00020000);
+  const __amdgpu_buffer_rsrc_t vsr = __builtin_amdgcn_make_buffer_rsrc((void *)val, (short)0, (int)(8u * nnz4), 0x00020000);
   const double2 *src2 = reinterpret_cast<const double2 *>(src);
   double2 *u2 = reinterpret_cast<double2 *>(u_out);
   double2 *v2 = reinterpret_cast<double2 *>(v_out);
@@ -394,24 +400,32 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_spmm_wide(const int *__restrict
     __syncthreads();
     const int e0 = s_rp[0], ne = s_rp[nr] - e0;
     if (ne <= BTW_CAP) {
-      // every load of a thread in flight before its LDS stores
-      constexpr int NI = BTW_CAP / PSFUN_BS;
-      unsigned int to[NI];
-      double tv[NI];
+      // The window's entries in 16-byte buffer loads from the aligned entry
+      // below e0 (four column indices, or two values, per lane; reads past
+      // the arrays return 0), all in flight before the LDS stores.  Slots
+      // past ne are never read as entries (the walk below masks entries past
+      // a row's end).
+      const int a0 = e0 & ~3, b0 = e0 & ~1;
+      const int4 cw = __builtin_bit_cast(
+          int4, __builtin_amdgcn_raw_buffer_load_b128(cs, (unsigned int)(a0 + 4 * (int)threadIdx.x) * 4u, 0, 0));
+      btw_d2 vw[2];
 #pragma unroll
-      for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + it * PSFUN_BS;
-        to[it] = (i < ne) ? (unsigned int)col[e0 + i] * kb : BTW_SENT;
-        tv[it] = (i < ne) ? val[e0 + i] : 0.0;
+      for (int it = 0; it < 2; ++it)
+        vw[it] = __builtin_bit_cast(
+            btw_d2, __builtin_amdgcn_raw_buffer_load_b128(
+                        vsr, (unsigned int)(b0 + 2 * ((int)threadIdx.x + it * PSFUN_BS)) * 8u, 0, 0));
+      {
+        const int i0 = a0 + 4 * (int)threadIdx.x - e0;
+        const int c4[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (i0 + m >= 0 && i0 + m < ne) s_off[i0 + m] = (unsigned int)c4[m] * kb;
       }
 #pragma unroll
-      for (int it = 0; it < NI; ++it) {
-        s_off[threadIdx.x + it * PSFUN_BS] = to[it];
-        s_val[threadIdx.x + it * PSFUN_BS] = tv[it];
-      }
-      if (threadIdx.x < 64) {
-        s_off[BTW_CAP + threadIdx.x] = BTW_SENT;
-        s_val[BTW_CAP + threadIdx.x] = 0.0;
+      for (int it = 0; it < 2; ++it) {
+        const int i0 = b0 + 2 * ((int)threadIdx.x + it * PSFUN_BS) - e0;
+        if (i0 >= 0 && i0 < ne) s_val[i0] = vw[it].x;
+        if (i0 + 1 >= 0 && i0 + 1 < ne) s_val[i0 + 1] = vw[it].y;
       }
       __syncthreads();
       for (int g0 = wv * W; g0 < nr; g0 += 4 * W) {
@@ -841,9 +855,15 @@ static int bt_create(psfun_ctx_t c, int ngraph, const int64_t *row_offsets, cons
   B->nrows = nrows;
   B->nnz = nnz;
   B->row_off.assign(row_offsets, row_offsets + ngraph + 1);
+  // column indices and values padded with zeros to whole 16-byte groups
+  // (four indices, two values), so the wide SpMM's 16-byte staging loads are
+  // either wholly inside the arrays or wholly past them
+  const int64_t nnz4 = std::max<int64_t>((nnz + 3) / 4 * 4, 4);
   hipError_t e = hipMalloc(&B->rowptr, sizeof(int) * (nrows + 1));
-  if (e == hipSuccess) e = hipMalloc(&B->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
-  if (e == hipSuccess) e = hipMalloc(&B->val, sizeof(double) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMalloc(&B->colind, sizeof(int) * nnz4);
+  if (e == hipSuccess) e = hipMalloc(&B->val, sizeof(double) * nnz4);
+  if (e == hipSuccess) e = hipMemset(B->colind, 0, sizeof(int) * nnz4);
+  if (e == hipSuccess) e = hipMemset(B->val, 0, sizeof(double) * nnz4);
   if (e == hipSuccess) e = hipMemcpy(B->rowptr, rowptr, sizeof(int) * (nrows + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess && nnz) e = hipMemcpy(B->colind, colind, sizeof(int) * nnz, hipMemcpyHostToDevice);
   if (e == hipSuccess && nnz) e = hipMemcpy(B->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice);
@@ -979,11 +999,11 @@ extern "C" int psfun_batch_info(psfun_batch_t B, int *ngraph, int64_t *nrows, in
 
 // The SpMM step u = opscale * A (src / sigma) with the alpha partials: the
 // wide-load kernel where it applies (even K >= 4, chunks made of whole 64-row
-// windows, the gathered operand under 2 GiB so every byte offset and the
-// out-of-range sentinel fit 32 bits), else k_bt_spmm.
+// windows, the gathered operand and the entry arrays under 2 GiB so every
+// byte offset and the out-of-range sentinel fit 32 bits), else k_bt_spmm.
 static bool bt_wide(const psfun_batch *B, int K) {
   return K >= 4 && K % 2 == 0 && B->rows_per_chunk % BT_SR == 0 && B->d_sched &&
-         (double)B->nrows * K * 8.0 < (double)BTW_SENT;
+         (double)B->nrows * K * 8.0 < (double)BTW_SENT && (double)(B->nnz + 4) * 8.0 < (double)BTW_SENT;
 }
 template <int KP>
 static void bt_spmm_launch(psfun_ctx *c, psfun_batch *B, double opscale, int K, int j, const double *src,
@@ -994,7 +1014,7 @@ static void bt_spmm_launch(psfun_ctx *c, psfun_batch *B, double opscale, int K, 
     if (bt_wide(B, K)) {
       hipLaunchKernelGGL((k_bt_spmm_wide<KP>), dim3(bt_spmm_grid(NC)), dim3(PSFUN_BS), 0, c->stream, B->rowptr,
                          B->colind, B->val, opscale, B->d_chunk, B->d_sched, K, j, src,
-                         (unsigned int)(B->nrows * K * 8), v_out, u_out, p_out, in, bctl, NC);
+                         (unsigned int)(B->nrows * K * 8), (unsigned int)B->nnz, v_out, u_out, p_out, in, bctl, NC);
       return;
     }
   }
