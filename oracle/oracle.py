@@ -95,6 +95,10 @@ def lib():
         L.or_quad_apply.restype = C.c_int
         L.or_pade_expm.argtypes = [C.c_int, C.c_int, _dp, C.c_double, _dp]
         L.or_pade_expm.restype = C.c_int
+        L.or_set_exp_cram.argtypes = [C.c_int]
+        L.or_set_exp_cram.restype = None
+        L.or_cram_apply.argtypes = [C.c_int, _dp, C.c_int, C.c_double, _dp]
+        L.or_cram_apply.restype = C.c_int
         L.or_set_pade_degree.argtypes = [C.c_int]
         _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         L.or_quad_apply_prec.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int,
@@ -426,3 +430,30 @@ def quad_apply(A, shifts, weights, gamma, form, x, eps=1e-8, itmax=2000, prec=No
     rc = lib().or_quad_apply_prec(n, rp, ci, va, -1 if prec is None else int(prec), ns, sg, ww, gamma, form,
                                   np.ascontiguousarray(x, np.float64), y, eps, itmax, iters, resid)
     return dict(info=rc, y=y, iters=iters, resid=resid)
+
+
+def cram_apply(H, t, y):
+    """EXPOKIT DSCHBV (serial/exponential/dschbv.f): exp(t H) y for a
+    symmetric H by the partial fraction form of CRAM(14,14)."""
+    H = np.ascontiguousarray(np.asarray(H, np.float64).T)  # column-major H
+    k = H.shape[0]
+    out = np.array(y, np.float64, copy=True)
+    rc = lib().or_cram_apply(k, H, k, float(t), out)
+    if rc:
+        raise RuntimeError(f"or_cram_apply failed: {rc}")
+    return out
+
+
+class exp_cram:
+    """Within the block, the EXP restatements (Lanczos, Arnoldi, the batch)
+    compute f(T) as the reference's CHBSYM does (DSCHBV's CRAM(14,14)),
+    error floor included, instead of the exact eigen-decomposition."""
+
+    def __enter__(self):
+        lib().or_set_exp_cram(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_exp_cram(0)
+        return False
+

@@ -13,6 +13,9 @@
  *
  * Pinning (what checks each routine; see tests/):
  *   or_tridiag_eig / or_exp_tridiag   -- scipy.linalg.expm fixtures, closed-form 2x2
+ *   or_cram_apply                      -- the approximation's known uniform error on [0, inf)
+ *                                         (max |exp(-x) - r(x)| about 9.4e-15, equioscillating),
+ *                                         scipy expm on negative definite matrices
  *   or_dense_expm                      -- scipy.linalg.expm fixtures, diagonal/nilpotent cases
  *   or_pade_expm                       -- scalar closed form r_p(x/2^ns)^(2^ns) from the factorial
  *                                         Pade coefficients, diagonal matrices, scipy expm at p = 6..13
@@ -188,9 +191,108 @@ int or_exp_tridiag(int k, const double *d, const double *e, double s, const doub
   return tridiag_fapply(k, d, e, s, NULL, x, y);
 }
 
+/* EXPOKIT's DSCHBV (serial/exponential/dschbv.f:1-95, called for CHBSYM at
+ * serial/psfun_d_serial_mod.F90:357-375): y = exp(t*H) y for a symmetric H of
+ * order k by the partial fraction form of the uniform rational Chebyshev
+ * approximation of type (14,14) to exp(-x) on [0, inf) (Gallopoulos and
+ * Saad, SIAM J. Sci. Stat. Comput. 13 (1992), the seven poles theta_p of
+ * positive imaginary part and their residues alpha_p, conjugate pairs
+ * folded):
+ *   y <- alpha0 y + Re sum_p alpha_p (-t H - theta_p I)^{-1} y.
+ * The reference solves each complex symmetric system with LAPACK's ZSYSV
+ * (Bunch-Kaufman); here Gaussian elimination with partial pivoting on the
+ * same matrix (the same solution; the rounding of the solve differs, far
+ * below the approximation's own error of about 1e-14).  H is dense,
+ * column-major with leading dimension ldh.  Returns 0, or 1 on a zero pivot
+ * (the reference stops there). */
+static const double cram_alpha0 = 0.183216998528140087e-11;
+static const double cram_alpha[7][2] = {
+    {0.557503973136501826e+02, -0.204295038779771857e+03}, {-0.938666838877006739e+02, 0.912874896775456363e+02},
+    {0.469965415550370835e+02, -0.116167609985818103e+02}, {-0.961424200626061065e+01, -0.264195613880262669e+01},
+    {0.752722063978321642e+00, 0.670367365566377770e+00},  {-0.188781253158648576e-01, -0.343696176445802414e-01},
+    {0.143086431411801849e-03, 0.287221133228814096e-03}};
+static const double cram_theta[7][2] = {
+    {-0.562314417475317895e+01, 0.119406921611247440e+01}, {-0.508934679728216110e+01, 0.358882439228376881e+01},
+    {-0.399337136365302569e+01, 0.600483209099604664e+01}, {-0.226978543095856366e+01, 0.846173881758693369e+01},
+    {0.208756929753827868e+00, 0.109912615662209418e+02},  {0.370327340957595652e+01, 0.136563731924991884e+02},
+    {0.889777151877331107e+01, 0.166309842834712071e+02}};
+
+int or_cram_apply(int k, const double *H, int ldh, double t, double *y) {
+  zdouble *A = (zdouble *)malloc(sizeof(zdouble) * (size_t)k * k);
+  zdouble *b = (zdouble *)malloc(sizeof(zdouble) * (size_t)k);
+  double *z = (double *)malloc(sizeof(double) * (size_t)k);
+  int rc = 0;
+  for (int j = 0; j < k; ++j) {
+    z[j] = y[j];
+    y[j] = y[j] * cram_alpha0;
+  }
+  for (int p = 0; p < 7 && rc == 0; ++p) {
+    const zdouble th = cram_theta[p][0] + I * cram_theta[p][1];
+    const zdouble al = cram_alpha[p][0] + I * cram_alpha[p][1];
+    for (int j = 0; j < k; ++j) {
+      for (int i = 0; i < k; ++i) A[i + (size_t)j * k] = -t * H[i + (size_t)j * ldh];
+      A[j + (size_t)j * k] -= th;
+      b[j] = z[j];
+    }
+    for (int c = 0; c < k && rc == 0; ++c) { /* elimination with partial pivoting */
+      int piv = c;
+      for (int i = c + 1; i < k; ++i)
+        if (cabs(A[i + (size_t)c * k]) > cabs(A[piv + (size_t)c * k])) piv = i;
+      if (cabs(A[piv + (size_t)c * k]) == 0.0) {
+        rc = 1;
+        break;
+      }
+      if (piv != c) {
+        for (int j = c; j < k; ++j) {
+          const zdouble tmp = A[c + (size_t)j * k];
+          A[c + (size_t)j * k] = A[piv + (size_t)j * k];
+          A[piv + (size_t)j * k] = tmp;
+        }
+        const zdouble tb = b[c];
+        b[c] = b[piv];
+        b[piv] = tb;
+      }
+      for (int i = c + 1; i < k; ++i) {
+        const zdouble m = A[i + (size_t)c * k] / A[c + (size_t)c * k];
+        if (m == 0.0) continue;
+        for (int j = c + 1; j < k; ++j) A[i + (size_t)j * k] -= m * A[c + (size_t)j * k];
+        b[i] -= m * b[c];
+      }
+    }
+    if (rc) break;
+    for (int i = k - 1; i >= 0; --i) {
+      zdouble a = b[i];
+      for (int j = i + 1; j < k; ++j) a -= A[i + (size_t)j * k] * b[j];
+      b[i] = a / A[i + (size_t)i * k];
+    }
+    for (int i = 0; i < k; ++i) y[i] += creal(al * b[i]);
+  }
+  free(A);
+  free(b);
+  free(z);
+  return rc;
+}
+
+/* Which exp the EXP variants of the Krylov restatements use for f(T):
+ * 0 (default) the eigen-decomposition below, exact to rounding; 1 DSCHBV's
+ * CRAM(14,14), as the reference's CHBSYM computes it, error floor included. */
+static int g_exp_cram = 0;
+void or_set_exp_cram(int on) { g_exp_cram = on ? 1 : 0; }
+
 /* y = Z diag(f(s*lambda)) Z^T x for symmetric tridiagonal T = Z diag(lambda) Z^T;
- * f = NULL is exp. */
+ * f = NULL is exp (or CRAM(14,14) after or_set_exp_cram(1)). */
 static int tridiag_fapply(int k, const double *d, const double *e, double s, or_f_t f, const double *x, double *y) {
+  if (!f && g_exp_cram) {
+    double *H = (double *)calloc((size_t)k * k, sizeof(double));
+    for (int i = 0; i < k; ++i) {
+      H[i + (size_t)i * k] = d[i];
+      if (i + 1 < k) H[i + (size_t)(i + 1) * k] = H[(i + 1) + (size_t)i * k] = e[i];
+    }
+    memcpy(y, x, sizeof(double) * (size_t)k);
+    const int rc = or_cram_apply(k, H, k, s, y);
+    free(H);
+    return rc;
+  }
   double *lam = (double *)malloc(sizeof(double) * (size_t)k);
   double *Z = (double *)malloc(sizeof(double) * (size_t)k * k);
   double *t = (double *)malloc(sizeof(double) * (size_t)k);

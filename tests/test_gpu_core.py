@@ -302,6 +302,25 @@ def _lanczos_gpu(A, x, eps, itmax, istop=1, fmt="AUTO"):
     return info, it, err, res, y.get_vect()
 
 
+@pytest.mark.parametrize("dims,itmax,fmt", [((100, 100), 30, "AUTO"), ((128, 128, 128), 50, "HDIA")])
+def test_lanczos_stop_index_matches_reference_cram(gpu_ctx, dims, itmax, fmt):
+    """The reference's CHBSYM computes f(T) by EXPOKIT's CRAM(14,14) (the
+    oracle's exp_cram mode); the device's exact f(T) stops at the same index
+    on the CPU-runnable config (100^2) and the plain GPU config (128^3, on
+    the diagonal slices the bench times), with the same final estimate to
+    CRAM's error and y within that error of the reference's."""
+    print(f"[cram] dims={dims} fmt={fmt}", flush=True)
+    A = matgen.laplacian(dims, scale=-1.0)
+    x = matgen.uniform_pm1(1, int(np.prod(dims)))
+    info, it, err, res, y = _lanczos_gpu(A, x, 1e-10, itmax, fmt=fmt)
+    with O.exp_cram():
+        ref = O.lanczos(A, x, 1e-10, itmax, 1)
+    assert info == 0 and it == ref["iter"]
+    assert err < 1e-10 and ref["err"] < 1e-10
+    assert np.allclose(res[1:it - 2], ref["res"][1:it - 2], rtol=1e-2, atol=0)
+    assert np.linalg.norm(y - ref["y"]) <= 1e-11 * np.linalg.norm(ref["y"])
+
+
 @pytest.mark.parametrize("fmt", ["CSR", "SELL", "HDIA", "CSRS"])
 @pytest.mark.parametrize("dims,itmax", [((100, 100), 30), ((12, 10, 9), 50), ((64, 64, 64), 50)])
 def test_lanczos_parity_closed_form(gpu_ctx, dims, itmax, fmt):

@@ -127,6 +127,65 @@ def test_exp_tridiag_2x2_closed_form():
     assert np.allclose(y, E[:, 0], rtol=1e-14, atol=1e-15)
 
 
+def test_cram_scalar_error_curve():
+    """DSCHBV's CRAM(14,14) on 1x1 matrices: r(x) - exp(-x) for x >= 0 is the
+    approximation's own error.  A wrong digit or sign in any pole or residue
+    moves it far above the table's level (max |error| 3.13e-12, at x = 0, the
+    value EXPOKIT's coefficients give in exact arithmetic), and at infinity
+    the error is r(inf) = alpha0 = 1.83216998528140087e-12 (dschbv.f:56)."""
+    xs = np.concatenate([np.linspace(0.0, 2.0, 401), np.linspace(2.0, 60.0, 581), [1e3, 1e6]])
+    err = np.array([O.cram_apply(np.array([[-x]]), 1.0, np.array([1.0]))[0] - np.exp(-x) for x in xs])
+    assert np.abs(err).max() <= 3.2e-12
+    assert abs(err[0] + 3.13e-12) <= 0.01e-12          # the largest deviation, at x = 0
+    assert abs(err[-1] - 1.83216998528140087e-12) <= 1e-15
+    assert np.any(err > 5e-13) and np.any(err < -5e-13)  # an error curve that changes sign, not a bias
+    # t scales H (exp(t H)): the same value for H = -x/2 at t = 2
+    assert O.cram_apply(np.array([[-0.25]]), 2.0, np.array([1.0]))[0] == pytest.approx(np.exp(-0.5), abs=3.2e-12)
+
+
+def test_cram_dense_against_scipy():
+    """DSCHBV on dense symmetric negative definite matrices: within the
+    approximation's error of scipy's expm, and the exact eigen-decomposition
+    path of the same oracle is 1e-3 of that away."""
+    rng = np.random.default_rng(4)
+    for k in (1, 5, 30):
+        B = rng.standard_normal((k, k))
+        H = -(B @ B.T) / k - 0.1 * np.eye(k)
+        y = rng.standard_normal(k)
+        ref = sla.expm(0.7 * H) @ y
+        got = O.cram_apply(H, 0.7, y)
+        assert np.linalg.norm(got - ref) <= 4e-12 * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("dims,itmax,it_expected", [((100, 100), 30, 20), ((128, 128, 128), 50, 25)])
+def test_lanczos_cram_trajectory_matches_exact(dims, itmax, it_expected):
+    """The reference's CHBSYM computes f(T) by CRAM(14,14) (about 3e-12
+    absolute error); this oracle's default and the device compute it to
+    rounding.  On the workloads of the CPU-runnable config (100^2, t = 1,
+    itmax 30) and the plain GPU config (128^3, t = 1, itmax 50) at eps 1e-10
+    both stop at the same index, their error estimates agree to 1e-6 while
+    above 1e-2 and to 1e-2 down to the last two (CRAM's error reaches the
+    small trailing components), and the reference's y carries CRAM's own
+    error (a few 1e-12 of the closed form) where the exact f(T) gives ~1e-14
+    at 128^3."""
+    A = matgen.laplacian(dims)
+    n = len(A[0]) - 1
+    x = matgen.uniform_pm1(1, n)
+    r0 = O.lanczos(A, x, 1e-10, itmax, 1, opscale=-1.0)
+    with O.exp_cram():
+        r1 = O.lanczos(A, x, 1e-10, itmax, 1, opscale=-1.0)
+    ex = matgen.dst_expm_laplacian(dims, x, 1.0)
+    assert r0["iter"] == r1["iter"] == it_expected
+    assert r1["err"] < 1e-10
+    k = r0["iter"]
+    a, b = r0["res"][1:k - 2], r1["res"][1:k - 2]  # all but the last two estimates
+    assert np.allclose(b, a, rtol=1e-2, atol=0)
+    assert np.allclose(b[a > 1e-2], a[a > 1e-2], rtol=1e-6, atol=0)
+    e0 = np.linalg.norm(r0["y"] - ex) / np.linalg.norm(ex)
+    e1 = np.linalg.norm(r1["y"] - ex) / np.linalg.norm(ex)
+    assert e0 <= 1e-12 and 5e-13 <= e1 <= 1e-11
+
+
 def test_dense_expm_against_scipy():
     g = load_golden("expm_small.npz")
     for k in (3, 12, 30):
