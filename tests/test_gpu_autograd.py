@@ -135,3 +135,43 @@ def test_one_call_vjp_argument_errors(gpu_ctx):
         _lib.PSFUN_ERR_INVALID_ARG
     assert L.psfun_expv_vjp(h, b, 65, 0.1, p(x), p(x), p(G), 1e-12, 200, 24, None, None, p(gv)) == \
         _lib.PSFUN_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("nrows,mmax", [(5000, 13), (40, 4)])
+def test_resolvent_kernels_agree_bitwise(gpu_ctx, nrows, mmax):
+    """psfun_krylov_resolvents takes k_krylov_resolvents_rows<8, 4> (four
+    rows per thread) for up to 8 shifts and <16, 2> past 8: for the same
+    data, the first 8 shifts' resolvents of a 9-shift call equal the 8-shift
+    call's bitwise (each element's sum runs over l and the shifts in the
+    same order).  A partial last tile, a basis length that is not a multiple
+    of the coefficient chunk, and a column with a shorter basis."""
+    import ctypes as C
+    from psfun_amd import _lib
+    K = 32
+    rng = np.random.default_rng(3)
+    dev = "cuda:0"
+    V = torch.as_tensor(rng.standard_normal((mmax, nrows, K)), device=dev)
+    ldt = mmax + 2
+    Ta = torch.as_tensor(np.concatenate([np.zeros((K, 1)), 1.0 + 2.0 * rng.random((K, ldt - 1))], 1), device=dev)
+    Tb = torch.as_tensor(np.concatenate([np.zeros((K, 1)), 0.1 + 0.4 * rng.random((K, ldt - 1))], 1), device=dev)
+    nb = np.full(K, mmax, np.int32)
+    nb[5] = max(1, mmax - 3)
+    nbd = torch.as_tensor(nb, device=dev)
+    rs1 = torch.as_tensor(1.0 + rng.random(K), device=dev)
+    sh = np.stack([0.5 + rng.random(9), rng.standard_normal(9)], 1).reshape(-1)
+    outs = []
+    ctx = _lib.lib()
+    from psfun_amd import psb
+    h = psb.default_context(0)
+    for ns in (8, 9):
+        out = torch.empty((ns, nrows, K), dtype=torch.complex128, device=dev)
+        h.bind_stream()
+        st = ctx.psfun_krylov_resolvents(h.handle, nrows, K, mmax, C.c_void_p(V.data_ptr()), C.c_void_p(Ta.data_ptr()),
+                                         C.c_void_p(Tb.data_ptr()), ldt, 0.5, C.c_void_p(nbd.data_ptr()),
+                                         C.c_void_p(rs1.data_ptr()), ns, sh.ctypes.data_as(C.c_void_p),
+                                         C.c_void_p(out.data_ptr()))
+        _lib.check(st, "psfun_krylov_resolvents")
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1][:8])
+    assert np.all(np.isfinite(outs[0])) and np.abs(outs[0]).max() > 0
