@@ -139,6 +139,8 @@ struct psfun_csr {
   //           the Lanczos step streams their entries with coalesced loads
   //           into LDS and sums each row there (CSR-stream, Greathouse and
   //           Daga, SC14)
+  //   kind -w: the same, with all of the block's columns in [c, c + w),
+  //           w <= PSFUN_CSRS_XT: that window of x is staged in LDS too
   //   kind 1: row a alone (more than PSFUN_CSRS_CAP entries), summed by the
   //           whole block
   //   kind 2 + s: entries [b, c) of row a, one segment of split row s (rows
@@ -148,6 +150,7 @@ struct psfun_csr {
   int *csrs_blk = nullptr;  // nullptr: not built
   int csrs_nblk = 0;
   int csrs_nsplit = 0;
+  int csrs_nxt = 0;  // blocks of short rows with a staged x window
   int *csrs_split = nullptr;   // [2 * nsplit]: first block, segment count
   int *csrs_cnt = nullptr;     // [nsplit] arrival counters (0 between launches)
   double *csrs_seg = nullptr;  // [nblk] segment sums
@@ -160,6 +163,7 @@ struct psfun_csr {
 #ifndef PSFUN_CSRS_PER_CU
 #define PSFUN_CSRS_PER_CU 12     // CSR-stream blocks per CU in the Lanczos grid (12 measured best of 4..16)
 #endif
+#define PSFUN_CSRS_XT 1024      // widest x window a block of short rows stages in LDS
 #define PSFUN_CSRS_SPLIT 8192
 #define PSFUN_CSRS_SEG 4096
 #define PSFUN_CSRS_MAXSPLIT 1024

@@ -412,7 +412,7 @@ static void csrs_free(psfun_csr *A) {
     if (p) hipFree(p);
   A->csrs_blk = A->csrs_split = A->csrs_cnt = nullptr;
   A->csrs_seg = nullptr;
-  A->csrs_nblk = A->csrs_nsplit = 0;
+  A->csrs_nblk = A->csrs_nsplit = A->csrs_nxt = 0;
 }
 
 // Blocks of the CSR-stream step, greedy over the rows: a block of short rows
@@ -420,10 +420,15 @@ static void csrs_free(psfun_csr *A) {
 // with more entries is a block of its own, and a row with more than
 // PSFUN_CSRS_SPLIT (the first PSFUN_CSRS_MAXSPLIT such rows) is cut into
 // segments of PSFUN_CSRS_SEG entries, one block each.
+// A block of short rows whose columns lie within PSFUN_CSRS_XT consecutive
+// values is marked kind = -span with its first column in c: the step stages
+// that window of x in LDS with coalesced loads instead of gathering it.
 static int csrs_build(psfun_csr *A) {
   const int n = A->n;
   std::vector<int> rp(n + 1);
   PSFUN_HIP_TRY(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  std::vector<int> ci((size_t)std::max<int64_t>(A->nnz, 1));
+  if (A->nnz) PSFUN_HIP_TRY(hipMemcpy(ci.data(), A->colind, sizeof(int) * A->nnz, hipMemcpyDeviceToHost));
   std::vector<int> blk, split;
   blk.reserve(4 * (n / 64 + 16));
   auto add = [&](int a, int b, int c, int kind) {
@@ -438,7 +443,16 @@ static int csrs_build(psfun_csr *A) {
     if (len <= PSFUN_CSRS_CAP) {
       const int start = r;
       while (r < n && r - start < PSFUN_BS && rp[r + 1] - e0 <= PSFUN_CSRS_CAP) ++r;
-      add(start, r, 0, 0);
+      int cmin = INT32_MAX, cmax = -1;
+      for (int q = e0; q < rp[r]; ++q) {
+        cmin = std::min(cmin, ci[q]);
+        cmax = std::max(cmax, ci[q]);
+      }
+      const int64_t span = (cmax >= cmin) ? (int64_t)cmax - cmin + 1 : 0;
+      if (span >= 1 && span <= PSFUN_CSRS_XT)
+        add(start, r, cmin, -(int)span);
+      else
+        add(start, r, 0, 0);
     } else if (len <= PSFUN_CSRS_SPLIT || (int)split.size() / 2 >= PSFUN_CSRS_MAXSPLIT) {
       add(r, r + 1, 0, 1);
       ++r;
@@ -468,6 +482,8 @@ static int csrs_build(psfun_csr *A) {
   }
   A->csrs_nblk = nblk;
   A->csrs_nsplit = nsplit;
+  A->csrs_nxt = 0;
+  for (int b = 0; b < nblk; ++b) A->csrs_nxt += (blk[4 * b + 3] < 0);
   return PSFUN_SUCCESS;
 }
 

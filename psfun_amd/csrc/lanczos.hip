@@ -154,7 +154,9 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv(int n, const int *__restri
 // completes the row (counter cnt[s]) adds the row's segments in order and
 // puts the row's alpha term in partial slot G + s (the consumers fold G +
 // nsplit partials), so the result does not depend on the arrival order.
-template <bool SMALLT>
+// XT: the matrix has blocks of short rows with a staged x window (kind < 0);
+// the kernel without it keeps the LDS and registers of a plain block.
+template <bool SMALLT, bool XT>
 __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *__restrict__ blk,
                                                            const int *__restrict__ split, int *__restrict__ cnt,
                                                            double *__restrict__ seg, const int *__restrict__ rowptr,
@@ -171,6 +173,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *
   constexpr size_t SH = sizeof(LzCtlLds<SMALLT>) > sizeof(double) * PSFUN_CSRS_CAP ? sizeof(LzCtlLds<SMALLT>)
                                                                                    : sizeof(double) * PSFUN_CSRS_CAP;
   __shared__ __align__(16) unsigned char shbuf[SH];
+  __shared__ double xt[XT ? PSFUN_CSRS_XT : 1];  // x window of a block of short rows (kind < 0)
   double *prod = reinterpret_cast<double *>(shbuf);
   if (__syncthreads_or(ctl->done)) return;
   const double sigma = sqrt(block_sum_partials<PSFUN_BS>(p_in, g_in, lds));
@@ -197,17 +200,18 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *
     int row = -1;
     double sum = 0.0;
     bool lead = false;
-    if (kind == 0) {  // short rows [ba, bb)
+    if (kind <= 0) {  // short rows [ba, bb)
       const int e0 = rowptr[ba], ne = rowptr[bb] - e0;
       const int nr = bb - ba;
       int L = 1;
       while (L < 64 && nr * L * 2 <= PSFUN_BS) L *= 2;
       const int rr = threadIdx.x / L, ln = threadIdx.x % L;
       // this thread's row bounds, and all of its entries' column and value
-      // loads, then all of its gathers, are in flight together (one round
-      // trip each instead of one per entry)
+      // loads (and its share of the x window), are in flight together, then
+      // all of its gathers (one round trip each instead of one per entry)
       const int rs = (rr < nr) ? rowptr[ba + rr] : 0, re = (rr < nr) ? rowptr[ba + rr + 1] : 0;
       constexpr int NI = PSFUN_CSRS_CAP / PSFUN_BS;
+      constexpr int NX = PSFUN_CSRS_XT / PSFUN_BS;
       int cq[NI];
       double aq[NI], xq[NI];
 #pragma unroll
@@ -216,8 +220,25 @@ __global__ __launch_bounds__(PSFUN_BS) void k_lz_spmv_csrs(int nblk, const int *
         cq[it] = (t < ne) ? col[e0 + t] : 0;
         aq[it] = (t < ne) ? val[e0 + t] : 0.0;
       }
+      if (XT && kind < 0) {
+        // the block's columns lie in [bc, bc + span): that window of x is
+        // staged in LDS with coalesced loads and the products read it there
+        const int span = -kind;
+        double xw[NX];
 #pragma unroll
-      for (int it = 0; it < NI; ++it) xq[it] = (threadIdx.x + it * PSFUN_BS < ne) ? src[cq[it]] : 0.0;
+        for (int it = 0; it < NX; ++it) {
+          const int t = threadIdx.x + it * PSFUN_BS;
+          xw[it] = (t < span) ? src[bc + t] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < NX; ++it) xt[threadIdx.x + it * PSFUN_BS] = xw[it];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < NI; ++it) xq[it] = (threadIdx.x + it * PSFUN_BS < ne) ? xt[cq[it] - bc] : 0.0;
+      } else {
+#pragma unroll
+        for (int it = 0; it < NI; ++it) xq[it] = (threadIdx.x + it * PSFUN_BS < ne) ? src[cq[it]] : 0.0;
+      }
 #pragma unroll
       for (int it = 0; it < NI; ++it) {
         const int t = threadIdx.x + it * PSFUN_BS;
@@ -593,18 +614,24 @@ static int lz_launch_spmv(psfun_ctx *c, const psfun_csr *A, double opscale, cons
     return PSFUN_SUCCESS;
   }
   if (A->format == PSFUN_FORMAT_CSRS && A->csrs_blk) {
+#define LZ_CSRS(SM, XTV)                                                                                         \
+  hipLaunchKernelGGL((k_lz_spmv_csrs<SM, XTV>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->csrs_nblk,        \
+                     A->csrs_blk, A->csrs_split, A->csrs_cnt, A->csrs_seg, A->rowptr, A->colind, A->val, opscale, \
+                     src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, istop, eps, fscale, c->d_flags)
     for (int r = 0; r < reps; ++r) {
-      if (nl <= 65)
-        hipLaunchKernelGGL((k_lz_spmv_csrs<true>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->csrs_nblk,
-                           A->csrs_blk, A->csrs_split, A->csrs_cnt, A->csrs_seg, A->rowptr, A->colind, A->val,
-                           opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, istop, eps, fscale,
-                           c->d_flags);
-      else
-        hipLaunchKernelGGL((k_lz_spmv_csrs<false>), dim3(G + 1), dim3(PSFUN_BS), 0, c->stream, A->csrs_nblk,
-                           A->csrs_blk, A->csrs_split, A->csrs_cnt, A->csrs_seg, A->rowptr, A->colind, A->val,
-                           opscale, src, j, G, p_in, g_in, v_out, u_out, p_out, ctl, nl_ctl, istop, eps, fscale,
-                           c->d_flags);
+      if (nl <= 65) {
+        if (A->csrs_nxt)
+          LZ_CSRS(true, true);
+        else
+          LZ_CSRS(true, false);
+      } else {
+        if (A->csrs_nxt)
+          LZ_CSRS(false, true);
+        else
+          LZ_CSRS(false, false);
+      }
     }
+#undef LZ_CSRS
     PSFUN_HIP_TRY(hipGetLastError());
     if (prof) PSFUN_TRY(psfun_prof_end(c));
     return PSFUN_SUCCESS;
