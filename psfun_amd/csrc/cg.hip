@@ -38,6 +38,7 @@ struct CgCtl {
 };
 
 #define CG_HIST_MAX PSFUN_MAX_GRID
+#define CG_UB 4  // elements per thread in flight in k_cg_b
 
 __device__ __forceinline__ double prec_apply(const double *__restrict__ dinv, int64_t i, double r) {
   return dinv ? r * dinv[i] : r;
@@ -148,7 +149,41 @@ __global__ __launch_bounds__(PSFUN_BS) void k_cg_b(int n, double *__restrict__ x
     }
   }
   double arho = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * PSFUN_BS) {
+  // CG_UB elements per thread at a time, their loads in flight together;
+  // each thread still visits its elements in the same order, so the
+  // partials are unchanged
+  const int64_t gs = (int64_t)gridDim.x * PSFUN_BS;
+  int64_t i = (int64_t)blockIdx.x * PSFUN_BS + threadIdx.x;
+  for (; i + (CG_UB - 1) * gs < n; i += CG_UB * gs) {
+    double xv[CG_UB], pv[CG_UB], rv[CG_UB], qv[CG_UB], dv[CG_UB];
+#pragma unroll
+    for (int u = 0; u < CG_UB; ++u) {
+      const int64_t k = i + u * gs;
+      xv[u] = x[k];
+      pv[u] = p[k];
+      rv[u] = r[k];
+      qv[u] = q[k];
+      dv[u] = dinv ? dinv[k] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < CG_UB; ++u) {
+      const int64_t k = i + u * gs;
+      const double xi = fma(alpha, pv[u], xv[u]);
+      const double ri = fma(-alpha, qv[u], rv[u]);
+      x[k] = xi;
+      r[k] = ri;
+      const double zi = dinv ? ri * dv[u] : ri;
+      if (zout) zout[k] = zi;
+      arho = fma(ri, zi, arho);
+      if (istop == 1) {
+        a1 = fmax(a1, fabs(ri));
+        a2 = fmax(a2, fabs(xi));
+      } else {
+        a1 = fma(ri, ri, a1);
+      }
+    }
+  }
+  for (; i < n; i += gs) {
     const double xi = fma(alpha, p[i], x[i]);
     const double ri = fma(-alpha, q[i], r[i]);
     x[i] = xi;
@@ -365,7 +400,8 @@ static int cg_run(psfun_ctx *c, psfun_csr *A, int prec, const psfun_prec *P, con
          *Pconv = c->d_partials + 2 * PSFUN_MAX_GRID, *Pani = c->d_partials + 4 * PSFUN_MAX_GRID,
          *Pb = c->d_partials + 7 * PSFUN_MAX_GRID;  // two lists of Gu (no BLAS-1 call runs inside)
   const int Gs = psfun_grid(c, psfun_spmv_chunks(A), psfun_row_per_thread(A) ? 8 : 4);
-  const int Gu = std::min(psfun_grid(c, (n + PSFUN_BS - 1) / PSFUN_BS, 4), PSFUN_MAX_GRID / 2);
+  // the update grid: CG_UB elements per thread where n allows (k_cg_b)
+  const int Gu = std::min(psfun_grid(c, (n + CG_UB * PSFUN_BS - 1) / (CG_UB * PSFUN_BS), 4), PSFUN_MAX_GRID / 2);
 
   hipLaunchKernelGGL(k_cg_rowstats, dim3(Gu), dim3(PSFUN_BS), 0, c->stream, n, A->rowptr, A->colind, A->val, b,
                      dinv, Pani, Pb);
