@@ -145,10 +145,13 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nro
     for (int l0 = 0; l0 < mmax; l0 += lt) {
       const int nl = min(lt, mmax - l0);
       __syncthreads();  // the previous chunk has been read
-      for (int idx = threadIdx.x; idx < nshift * nl * K; idx += PSFUN_BS) {
-        const int q = idx / (nl * K), rem = idx - q * (nl * K);
-        const int dl = rem / K, c = rem - dl * K;
-        scoef[(q * lt + dl) * K + c] = cf[((int64_t)q * mmax + l0 + dl) * K + c];
+      // for each shift the chunk is nl*K consecutive coefficients in both
+      // arrays: a straight copy (no index divisions, which cost about as many
+      // VALU instructions as the sums they feed)
+      for (int q = 0; q < nshift; ++q) {
+        const double2 *src = cf + ((int64_t)q * mmax + l0) * K;
+        double2 *dst = scoef + q * lt * K;
+        for (int idx = threadIdx.x; idx < nl * K; idx += PSFUN_BS) dst[idx] = src[idx];
       }
       __syncthreads();
       for (int dl = 0; dl < nl; ++dl) {
@@ -159,16 +162,23 @@ __global__ __launch_bounds__(PSFUN_BS) void k_krylov_resolvents_rows(int64_t nro
           const int64_t e = e0 + (int64_t)r * PSFUN_BS;
           v[r] = (e < tot) ? V[(int64_t)l * vs + e] : 0.0;
         }
+        // all S shifts unguarded (the sums of shifts >= nshift read stale LDS
+        // and are never stored), each coefficient read issued two shifts
+        // ahead of its sums: a branch per shift, or one register slot for
+        // the coefficient, waited for every read in turn
+        double2 cq[S];
+        const double2 *sc = scoef + dl * K + col;
+        cq[0] = sc[0];
+        cq[1] = sc[lt * K];
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-          if (q < nshift) {
-            const double2 c = scoef[(q * lt + dl) * K + col];
+          if (q + 2 < S) cq[q + 2] = sc[(q + 2) * lt * K];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              ar[r][q] = fma(c.x, v[r], ar[r][q]);
-              ai[r][q] = fma(c.y, v[r], ai[r][q]);
-            }
+          for (int r = 0; r < R; ++r) {
+            ar[r][q] = fma(cq[q].x, v[r], ar[r][q]);
+            ai[r][q] = fma(cq[q].y, v[r], ai[r][q]);
           }
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
