@@ -720,25 +720,7 @@ __global__ __launch_bounds__(PSFUN_BS) void k_bt_assemble(const int *__restrict_
 #pragma unroll
       for (int q = 0; q < BT_AR; ++q)
         acc[q] = y0 * (__builtin_nontemporal_load(V + (int64_t)(r + q * RP) * K + k) * b0);
-      // two slots per pass: their 2*BT_AR loads are in flight together (each
-      // row's sum still takes the slots in order)
-      int l = 1;
-      for (; l + 1 < m; l += 2) {
-        const double yl = yk[l], bl = ibk[l], yl1 = yk[l + 1], bl1 = ibk[l + 1];
-        const double *Vl = V + (int64_t)l * vstride;
-        double a0[BT_AR], a1[BT_AR];
-#pragma unroll
-        for (int q = 0; q < BT_AR; ++q) {
-          a0[q] = __builtin_nontemporal_load(Vl + (int64_t)(r + q * RP) * K + k);
-          a1[q] = __builtin_nontemporal_load(Vl + vstride + (int64_t)(r + q * RP) * K + k);
-        }
-#pragma unroll
-        for (int q = 0; q < BT_AR; ++q) {
-          acc[q] = fma(yl, a0[q] * bl, acc[q]);
-          acc[q] = fma(yl1, a1[q] * bl1, acc[q]);
-        }
-      }
-      if (l < m) {
+      for (int l = 1; l < m; ++l) {
         const double yl = yk[l], bl = ibk[l];
         const double *Vl = V + (int64_t)l * vstride;
 #pragma unroll
